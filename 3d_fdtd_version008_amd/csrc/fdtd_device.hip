@@ -1,0 +1,794 @@
+// fdtd_device.hip - C ABI of the gfx950 Yee-step engine (include/fdtd_hip.h).
+//
+// Host side: validates and repacks the reference-layout host arrays into the
+// device layout (material-id + exact LUT, CPML 1D profiles in the compute
+// precision, psi on the boundary slabs only, gain state in its bounding box),
+// owns one stream per handle, launches the fused H and E kernels in the
+// reference's step order, and applies the reference's stability policy to the
+// device reductions.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/fdtd_hip.h"
+#include "fdtd_kernels.hpp"
+
+using fdtdk::MonOut;
+using fdtdk::SourceBox;
+using fdtdk::StepArgs;
+using fdtdk::TLSCoef;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                                   \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            return fail(FDTD_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));            \
+    } while (0)
+
+struct Probe {
+    long id;
+    int comp, save_every;
+    long cap, count;
+    double* d_buf;
+};
+
+struct Dft {
+    int comp;
+    int box[6];
+    int nf;
+    double freqs[8];
+    long ncell;
+    double* d_acc;
+};
+
+struct Key12 {
+    double v[12];
+    bool operator==(const Key12& o) const { return std::memcmp(v, o.v, sizeof(v)) == 0; }
+};
+struct Key12Hash {
+    size_t operator()(const Key12& k) const {
+        uint64_t h = 1469598103934665603ull;
+        const uint64_t* w = reinterpret_cast<const uint64_t*>(k.v);
+        for (int q = 0; q < 12; ++q) {
+            h ^= w[q] + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2);
+        }
+        return (size_t)h;
+    }
+};
+
+}  // namespace
+
+struct fdtd_sim {
+    int nx = 0, ny = 0, nz = 0, np = 0, prec = 0, bc_type = 0;
+    long N = 0;
+    double dt = 0;
+    hipStream_t stream = nullptr;
+    size_t esz = 8;     // bytes per stored field value
+    size_t rsz = 8;     // bytes per compute value
+    std::vector<void*> allocs;
+    size_t bytes = 0;
+    void* F[6] = {};
+    void* psi[12] = {};
+    long psi_len[12] = {};
+    void* inv[3] = {};
+    void* cp[3] = {};
+    void* matid = nullptr;
+    int mid16 = 0, nmat = 0;
+    void* lut = nullptr;
+    bool lossless = false, h_uniform = false;
+    // gain
+    bool gain = false;
+    int g0[3] = {0, 0, 0}, gn[3] = {0, 0, 0};
+    long gbox = 0, ngain = 0;
+    int8_t* greg = nullptr;
+    double *gPz = nullptr, *gPp = nullptr, *gNu = nullptr, *gNg = nullptr;
+    TLSCoef* d_tls = nullptr;
+    std::vector<TLSCoef> h_tls;
+    // sources
+    SourceBox* d_src = nullptr;
+    int nsrc = 0;
+    double* d_srcv = nullptr;
+    long srcv_cap = 0;
+    // probes / dft
+    std::vector<Probe> probes;
+    std::vector<Dft> dfts;
+    // monitor
+    int mon_interval = 10;
+    MonOut* d_mon = nullptr;
+    int mon_cap = 0;
+    std::vector<fdtd_check_record> checks;
+    double prev_max_E = 0.0;
+    size_t inst_count = 0;
+    bool stable = true;
+    // FP16 scales: fields (6) and psi (12); value = stored * scale
+    double fsc[6] = {1, 1, 1, 1, 1, 1};
+    double psc[12] = {1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1};
+
+    ~fdtd_sim() {
+        if (stream) hipStreamSynchronize(stream);
+        for (void* p : allocs) hipFree(p);
+        if (stream) hipStreamDestroy(stream);
+    }
+
+    int alloc(void** p, size_t b) {
+        if (b == 0) b = 8;
+        hipError_t e = hipMalloc(p, b);
+        if (e != hipSuccess) return fail(FDTD_ENOMEM, "hipMalloc(" + std::to_string(b) + " B) failed: " + hipGetErrorString(e));
+        allocs.push_back(*p);
+        bytes += b;
+        return 0;
+    }
+    template <typename X>
+    int upload(X** dst, const X* src, size_t n) {
+        int rc = alloc(reinterpret_cast<void**>(dst), n * sizeof(X));
+        if (rc) return rc;
+        HIPCHK(hipMemcpy(*dst, src, n * sizeof(X), hipMemcpyHostToDevice));
+        return 0;
+    }
+};
+
+namespace {
+
+template <typename R>
+std::vector<R> to_r(const double* p, size_t n) {
+    std::vector<R> v(n);
+    for (size_t i = 0; i < n; ++i) v[i] = (R)p[i];
+    return v;
+}
+
+template <typename T, typename R>
+StepArgs<T, R> make_args(const fdtd_sim* s) {
+    StepArgs<T, R> a{};
+    a.nx = s->nx; a.ny = s->ny; a.nz = s->nz; a.np = s->np;
+    a.sI = (long)s->ny * s->nz;
+    a.sJ = s->nz;
+    a.Ex = (T*)s->F[0]; a.Ey = (T*)s->F[1]; a.Ez = (T*)s->F[2];
+    a.Hx = (T*)s->F[3]; a.Hy = (T*)s->F[4]; a.Hz = (T*)s->F[5];
+    a.idx = (const R*)s->inv[0]; a.idy = (const R*)s->inv[1]; a.idz = (const R*)s->inv[2];
+    a.cpml = s->bc_type == 1 && s->np > 0;
+    a.cpx = (const R*)s->cp[0]; a.cpy = (const R*)s->cp[1]; a.cpz = (const R*)s->cp[2];
+    for (int q = 0; q < 12; ++q) {
+        a.psi[q] = (T*)s->psi[q];
+        a.psc[q] = (R)s->psc[q];
+        a.ipsc[q] = (R)(1.0 / s->psc[q]);
+    }
+    for (int q = 0; q < 6; ++q) {
+        a.fsc[q] = (R)s->fsc[q];
+        a.ifsc[q] = (R)(1.0 / s->fsc[q]);
+    }
+    a.matid = s->matid;
+    a.mid16 = s->mid16;
+    a.lut = (const R*)s->lut;
+    a.nmat = s->nmat;
+    a.lossless = s->lossless;
+    a.h_uniform = s->h_uniform;
+    a.src = s->d_src;
+    a.nsrc = s->nsrc;
+    a.srcv = s->d_srcv;
+    a.gain = s->gain;
+    a.g0x = s->g0[0]; a.g0y = s->g0[1]; a.g0z = s->g0[2];
+    a.gnx = s->gn[0]; a.gny = s->gn[1]; a.gnz = s->gn[2];
+    a.greg = s->greg;
+    a.gPz = s->gPz; a.gPp = s->gPp; a.gNu = s->gNu; a.gNg = s->gNg;
+    a.tls = s->d_tls;
+    return a;
+}
+
+template <typename R>
+int setup_coeffs(fdtd_sim* s, const fdtd_grid_desc* g, const fdtd_material_desc* m, const fdtd_boundary_desc* bc) {
+    // spacing inverses
+    const int dims[3] = {s->nx, s->ny, s->nz};
+    const double* inv[3] = {g->inv_dx, g->inv_dy, g->inv_dz};
+    for (int a = 0; a < 3; ++a) {
+        auto v = to_r<R>(inv[a], (size_t)dims[a]);
+        R* d;
+        if (int rc = s->upload(&d, v.data(), v.size())) return rc;
+        s->inv[a] = d;
+    }
+    // CPML profiles [5][n]: kappa, bE, cE, bH, cH
+    for (int a = 0; a < 3; ++a) {
+        const int n = dims[a];
+        std::vector<R> v((size_t)5 * n);
+        for (int i = 0; i < n; ++i) {
+            if (s->bc_type == 1) {
+                const double* K[3] = {bc->kx, bc->ky, bc->kz};
+                const double* BE[3] = {bc->bEx, bc->bEy, bc->bEz};
+                const double* CE[3] = {bc->cEx, bc->cEy, bc->cEz};
+                const double* BH[3] = {bc->bHx, bc->bHy, bc->bHz};
+                const double* CH[3] = {bc->cHx, bc->cHy, bc->cHz};
+                v[i] = (R)K[a][i];
+                v[(size_t)n + i] = (R)BE[a][i];
+                v[(size_t)2 * n + i] = (R)CE[a][i];
+                v[(size_t)3 * n + i] = (R)BH[a][i];
+                v[(size_t)4 * n + i] = (R)CH[a][i];
+            } else {
+                v[i] = 1; v[(size_t)n + i] = 1; v[(size_t)2 * n + i] = 0; v[(size_t)3 * n + i] = 1; v[(size_t)4 * n + i] = 0;
+            }
+        }
+        R* d;
+        if (int rc = s->upload(&d, v.data(), v.size())) return rc;
+        s->cp[a] = d;
+    }
+    // Material tuples -> ids + exact LUT.
+    const double* arrs[12] = {m->aEx, m->bEx, m->aEy, m->bEy, m->aEz, m->bEz, m->aHx, m->bHx, m->aHy, m->bHy, m->aHz, m->bHz};
+    for (int q = 0; q < 12; ++q)
+        if (!arrs[q]) return fail(FDTD_EINVAL, "material array missing");
+    std::unordered_map<Key12, int, Key12Hash> ids;
+    std::vector<Key12> tuples;
+    std::vector<uint16_t> mid((size_t)s->N);
+    Key12 last{};
+    int last_id = -1;
+    for (long id = 0; id < s->N; ++id) {
+        Key12 k;
+        for (int q = 0; q < 12; ++q) k.v[q] = arrs[q][id];
+        int t;
+        if (last_id >= 0 && k == last) {
+            t = last_id;
+        } else {
+            auto it = ids.find(k);
+            if (it == ids.end()) {
+                t = (int)tuples.size();
+                if (t >= 65536) return fail(FDTD_EINVAL, "more than 65536 distinct material coefficient tuples");
+                ids.emplace(k, t);
+                tuples.push_back(k);
+            } else {
+                t = it->second;
+            }
+            last = k;
+            last_id = t;
+        }
+        mid[(size_t)id] = (uint16_t)t;
+    }
+    s->nmat = (int)tuples.size();
+    s->mid16 = s->nmat > 256;
+    if (s->mid16) {
+        uint16_t* d;
+        if (int rc = s->upload(&d, mid.data(), mid.size())) return rc;
+        s->matid = d;
+    } else {
+        std::vector<uint8_t> m8(mid.size());
+        for (size_t i = 0; i < mid.size(); ++i) m8[i] = (uint8_t)mid[i];
+        uint8_t* d;
+        if (int rc = s->upload(&d, m8.data(), m8.size())) return rc;
+        s->matid = d;
+    }
+    std::vector<R> lut((size_t)12 * s->nmat);
+    bool lossless = true, hu = true;
+    for (int t = 0; t < s->nmat; ++t) {
+        for (int q = 0; q < 12; ++q) lut[(size_t)q * s->nmat + t] = (R)tuples[t].v[q];
+        if (tuples[t].v[0] != 1.0 || tuples[t].v[2] != 1.0 || tuples[t].v[4] != 1.0) lossless = false;
+        for (int q = 6; q < 12; ++q)
+            if (tuples[t].v[q] != tuples[0].v[q]) hu = false;
+    }
+    s->lossless = lossless;
+    s->h_uniform = hu;
+    R* d;
+    if (int rc = s->upload(&d, lut.data(), lut.size())) return rc;
+    s->lut = d;
+    return 0;
+}
+
+int setup_gain(fdtd_sim* s, const fdtd_tls_desc* t) {
+    if (!t || t->n_regions <= 0 || !t->cell_region_map) return 0;
+    if (t->n_regions > 127) return fail(FDTD_EINVAL, "at most 127 TLS regions");
+    if (!t->Ndip || !t->Ng0 || !t->Nu || !t->Ng || !t->Pz || !t->Pz_prev)
+        return fail(FDTD_EINVAL, "TLS state array missing");
+    int lo[3] = {s->nx, s->ny, s->nz}, hi[3] = {-1, -1, -1};
+    long count = 0;
+    for (int i = 0; i < s->nx; ++i)
+        for (int j = 0; j < s->ny; ++j)
+            for (int k = 0; k < s->nz; ++k) {
+                const long id = ((long)i * s->ny + j) * s->nz + k;
+                const int r = t->cell_region_map[id];
+                if (r < 0 || r >= t->n_regions || !(t->Ndip[id] > 0)) continue;
+                const double N0 = t->regions[r].N0;
+                if (t->Ndip[id] != N0 || t->Ng0[id] != N0)
+                    return fail(FDTD_EINVAL, "TLS cell with Ndip/Ng0 different from its region's N0");
+                if (s->bc_type == 0 && (i == 0 || j == 0 || k == 0 || i == s->nx - 1 || j == s->ny - 1 || k == s->nz - 1))
+                    return fail(FDTD_EINVAL, "TLS cells on a PEC face are not supported");
+                const int c[3] = {i, j, k};
+                for (int a = 0; a < 3; ++a) {
+                    lo[a] = std::min(lo[a], c[a]);
+                    hi[a] = std::max(hi[a], c[a]);
+                }
+                ++count;
+            }
+    if (count == 0) return 0;
+    s->gain = true;
+    s->ngain = count;
+    for (int a = 0; a < 3; ++a) {
+        s->g0[a] = lo[a];
+        s->gn[a] = hi[a] - lo[a] + 1;
+    }
+    s->gbox = (long)s->gn[0] * s->gn[1] * s->gn[2];
+    std::vector<int8_t> reg((size_t)s->gbox, -1);
+    std::vector<double> Pz((size_t)s->gbox, 0.0), Pp((size_t)s->gbox, 0.0), Nu((size_t)s->gbox, 0.0), Ng((size_t)s->gbox, 0.0);
+    for (int gi = 0; gi < s->gn[0]; ++gi)
+        for (int gj = 0; gj < s->gn[1]; ++gj)
+            for (int gk = 0; gk < s->gn[2]; ++gk) {
+                const long g = ((long)gi * s->gn[1] + gj) * s->gn[2] + gk;
+                const long id = ((long)(gi + s->g0[0]) * s->ny + (gj + s->g0[1])) * s->nz + (gk + s->g0[2]);
+                const int r = t->cell_region_map[id];
+                if (r < 0 || r >= t->n_regions || !(t->Ndip[id] > 0)) continue;
+                reg[(size_t)g] = (int8_t)r;
+                Pz[(size_t)g] = t->Pz[id];
+                Pp[(size_t)g] = t->Pz_prev[id];
+                Nu[(size_t)g] = t->Nu[id];
+                Ng[(size_t)g] = t->Ng[id];
+            }
+    int rc;
+    if ((rc = s->upload(&s->greg, reg.data(), reg.size()))) return rc;
+    if ((rc = s->upload(&s->gPz, Pz.data(), Pz.size()))) return rc;
+    if ((rc = s->upload(&s->gPp, Pp.data(), Pp.size()))) return rc;
+    if ((rc = s->upload(&s->gNu, Nu.data(), Nu.size()))) return rc;
+    if ((rc = s->upload(&s->gNg, Ng.data(), Ng.size()))) return rc;
+    s->h_tls.resize((size_t)t->n_regions);
+    for (int r = 0; r < t->n_regions; ++r) {
+        const fdtd_tls_region& R = t->regions[r];
+        TLSCoef c{};
+        c.kapa = R.kapa; c.pa2 = R.pa2; c.pa3 = R.pa3; c.inv_dt = R.inv_dt;
+        c.inv_tau = 1.0 / R.tau;                        // src/two_level_system.hpp:637
+        c.inv_hbar_omega = 1.0 / (R.hbar * R.omega_a);  // :638
+        c.N0 = R.N0; c.dt = R.dt; c.clamp = R.clamp;
+        s->h_tls[(size_t)r] = c;
+    }
+    return s->upload(&s->d_tls, s->h_tls.data(), s->h_tls.size());
+}
+
+template <typename T, typename R>
+int launch_step(fdtd_sim* s, long n, int sidx) {
+    StepArgs<T, R> a = make_args<T, R>(s);
+    a.srcv = s->d_srcv + (long)sidx * s->nsrc;
+    const dim3 blk(64, 4, 1);
+    const dim3 grd((unsigned)((s->nz + 63) / 64), (unsigned)((s->ny + 3) / 4), (unsigned)s->nx);
+    hipLaunchKernelGGL((fdtdk::k_update_H<T, R>), grd, blk, 0, s->stream, a);
+    hipLaunchKernelGGL((fdtdk::k_update_E<T, R>), grd, blk, 0, s->stream, a);
+    if (s->bc_type == 0) {
+        const long faces = 2L * ((long)s->ny * s->nz + (long)s->nx * s->nz + (long)s->nx * s->ny);
+        hipLaunchKernelGGL((fdtdk::k_pec_faces<T>), dim3((unsigned)((faces + 255) / 256)), dim3(256), 0, s->stream,
+                           a.Ex, a.Ey, a.Ez, s->nx, s->ny, s->nz);
+    }
+    // probes: PointFieldDetector records when n % save_every == 0
+    fdtdk::ProbeArgs pa{};
+    for (auto& p : s->probes) {
+        if (n % p.save_every != 0 || p.count >= p.cap) continue;
+        pa.p[pa.n].id = p.id;
+        pa.p[pa.n].comp = p.comp;
+        pa.p[pa.n].out = p.d_buf + p.count;
+        p.count++;
+        if (++pa.n == 16) {
+            hipLaunchKernelGGL((fdtdk::k_probes<T, R>), dim3(1), dim3(64), 0, s->stream, a, pa);
+            pa.n = 0;
+        }
+    }
+    if (pa.n) hipLaunchKernelGGL((fdtdk::k_probes<T, R>), dim3(1), dim3(64), 0, s->stream, a, pa);
+    for (auto& d : s->dfts) {
+        fdtdk::DftArgs da{};
+        da.comp = d.comp;
+        da.i0 = d.box[0]; da.j0 = d.box[2]; da.k0 = d.box[4];
+        da.ny_b = d.box[3] - d.box[2];
+        da.nz_b = d.box[5] - d.box[4];
+        da.ncell = d.ncell;
+        da.nf = d.nf;
+        const double t = (d.comp < 3) ? ((double)n + 1.0) * s->dt : ((double)n + 0.5) * s->dt;
+        for (int f = 0; f < d.nf; ++f) {
+            const double w = 2.0 * 3.14159265358979323846 * d.freqs[f];
+            da.wr[f] = std::cos(w * t) * s->dt;
+            da.wi[f] = -std::sin(w * t) * s->dt;
+        }
+        da.acc = d.d_acc;
+        hipLaunchKernelGGL((fdtdk::k_dft<T, R>), dim3((unsigned)((d.ncell + 255) / 256)), dim3(256), 0, s->stream, a, da);
+    }
+    return 0;
+}
+
+template <typename T, typename R>
+void launch_monitor(fdtd_sim* s, MonOut* slot) {
+    StepArgs<T, R> a = make_args<T, R>(s);
+    hipLaunchKernelGGL((fdtdk::k_monitor<T, R>), dim3(1024), dim3(256), 0, s->stream, a, s->N, s->gain ? s->gbox : 0L, slot);
+}
+
+int dispatch_step(fdtd_sim* s, long n, int sidx) {
+    switch (s->prec) {
+    case FDTD_FP64: return launch_step<double, double>(s, n, sidx);
+    case FDTD_FP32: return launch_step<float, float>(s, n, sidx);
+    default: return launch_step<uint16_t, float>(s, n, sidx);
+    }
+}
+void dispatch_monitor(fdtd_sim* s, MonOut* slot) {
+    switch (s->prec) {
+    case FDTD_FP64: launch_monitor<double, double>(s, slot); break;
+    case FDTD_FP32: launch_monitor<float, float>(s, slot); break;
+    default: launch_monitor<uint16_t, float>(s, slot); break;
+    }
+}
+
+double bits_to_double(unsigned long long b) {
+    double d;
+    std::memcpy(&d, &b, sizeof(d));
+    return d;
+}
+
+// StabilityMonitor::check_stability policy (src/stability_monitor.hpp:42-144)
+// on one device record; returns false when the reference would terminate.
+bool apply_policy(fdtd_sim* s, const fdtd_check_record& r) {
+    if (r.nan_E || r.nan_H) { s->stable = false; return false; }
+    if (s->gain && (r.nan_P || r.nan_N)) { s->stable = false; return false; }
+    const double max_E = r.max_E;
+    if (max_E > 1e10) s->inst_count++;
+    if (s->gain && r.max_P > 1e5) s->inst_count++;
+    const size_t step = (size_t)r.step;
+    if (step > 0 && step % 100 == 0 && s->prev_max_E > 1e-20) {
+        s->prev_max_E = max_E;
+    } else if (step == 0 || s->prev_max_E < 1e-20) {
+        s->prev_max_E = std::max(max_E, 1e-20);
+    }
+    if (s->inst_count > 10) { s->stable = false; return false; }
+    return s->stable;
+}
+
+template <typename T>
+int download_raw(fdtd_sim* s, const void* src, size_t n, double scale, double* out) {
+    std::vector<T> tmp(n);
+    HIPCHK(hipMemcpyAsync(tmp.data(), src, n * sizeof(T), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    for (size_t i = 0; i < n; ++i) {
+        if constexpr (sizeof(T) == 2) {
+            __half h;
+            std::memcpy(&h, &tmp[i], 2);
+            out[i] = (double)__half2float(h) * scale;
+        } else {
+            out[i] = (double)tmp[i];
+        }
+    }
+    return 0;
+}
+
+int download_any(fdtd_sim* s, const void* src, size_t n, double scale, double* out) {
+    switch (s->prec) {
+    case FDTD_FP64: return download_raw<double>(s, src, n, scale, out);
+    case FDTD_FP32: return download_raw<float>(s, src, n, scale, out);
+    default: return download_raw<uint16_t>(s, src, n, scale, out);
+    }
+}
+
+int upload_any(fdtd_sim* s, void* dst, const double* in, size_t n, double scale) {
+    std::vector<unsigned char> tmp(n * s->esz);
+    for (size_t i = 0; i < n; ++i) {
+        if (s->prec == FDTD_FP64) {
+            std::memcpy(&tmp[i * 8], &in[i], 8);
+        } else if (s->prec == FDTD_FP32) {
+            float f = (float)in[i];
+            std::memcpy(&tmp[i * 4], &f, 4);
+        } else {
+            __half h = __float2half_rn((float)(in[i] / scale));
+            std::memcpy(&tmp[i * 2], &h, 2);
+        }
+    }
+    HIPCHK(hipMemcpyAsync(dst, tmp.data(), tmp.size(), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+// psi slab index of full-grid cell (i,j,k) for psi array q, or -1.
+long psi_slab_index(const fdtd_sim* s, int q, int i, int j, int k) {
+    const int np = s->np, nx = s->nx, ny = s->ny, nz = s->nz;
+    auto in = [np](int v, int n) { int d = v < n - 1 - v ? v : n - 1 - v; return d < np; };
+    static const int axis[12] = {1, 2, 2, 0, 0, 1, 1, 2, 2, 0, 0, 1};
+    switch (axis[q]) {
+    case 0:
+        if (!in(i, nx)) return -1;
+        return ((long)(i < np ? i : i - (nx - 2 * np)) * ny + j) * nz + k;
+    case 1:
+        if (!in(j, ny)) return -1;
+        return ((long)i * (2 * np) + (j < np ? j : j - (ny - 2 * np))) * nz + k;
+    default:
+        if (!in(k, nz)) return -1;
+        return ((long)i * ny + j) * (2 * np) + (k < np ? k : k - (nz - 2 * np));
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* fdtd_last_error(void) { return g_err.c_str(); }
+
+int fdtd_create(const fdtd_grid_desc* g, const fdtd_material_desc* m, const fdtd_boundary_desc* bc,
+                const fdtd_tls_desc* tls, int precision, fdtd_handle* out) {
+    if (!g || !m || !bc || !out) return fail(FDTD_EINVAL, "null argument");
+    *out = nullptr;
+    if (g->nx < 3 || g->ny < 3 || g->nz < 3) return fail(FDTD_EINVAL, "grid must be at least 3 cells per axis");
+    if (!g->inv_dx || !g->inv_dy || !g->inv_dz) return fail(FDTD_EINVAL, "spacing arrays missing");
+    if (precision < FDTD_FP64 || precision > FDTD_FP16) return fail(FDTD_EINVAL, "unknown precision");
+    if (bc->type != 0 && bc->type != 1) return fail(FDTD_EINVAL, "unknown boundary type");
+    if (bc->type == 1) {
+        if (bc->npml < 1 || 2 * bc->npml > g->nx || 2 * bc->npml > g->ny || 2 * bc->npml > g->nz)
+            return fail(FDTD_EINVAL, "npml out of range");
+        const double* need[] = {bc->kx, bc->ky, bc->kz, bc->bEx, bc->cEx, bc->bEy, bc->cEy, bc->bEz, bc->cEz,
+                                bc->bHx, bc->cHx, bc->bHy, bc->cHy, bc->bHz, bc->cHz};
+        for (const double* p : need)
+            if (!p) return fail(FDTD_EINVAL, "CPML profile array missing");
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(FDTD_EDEVICE, "no HIP device available");
+    fdtd_sim* s = new fdtd_sim();
+    s->nx = g->nx; s->ny = g->ny; s->nz = g->nz;
+    s->N = (long)g->nx * g->ny * g->nz;
+    s->dt = g->dt;
+    s->prec = precision;
+    s->bc_type = bc->type;
+    s->np = bc->type == 1 ? bc->npml : 0;
+    s->esz = precision == FDTD_FP64 ? 8 : (precision == FDTD_FP32 ? 4 : 2);
+    s->rsz = precision == FDTD_FP64 ? 8 : 4;
+    int rc = 0;
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete s;
+        return fail(FDTD_EDEVICE, "hipStreamCreate failed");
+    }
+    for (int q = 0; q < 6 && !rc; ++q) rc = s->alloc(&s->F[q], (size_t)s->N * s->esz);
+    if (!rc && s->bc_type == 1) {
+        const long lx = 2L * s->np * s->ny * s->nz, ly = (long)s->nx * 2 * s->np * s->nz, lz = (long)s->nx * s->ny * 2 * s->np;
+        static const int axis[12] = {1, 2, 2, 0, 0, 1, 1, 2, 2, 0, 0, 1};
+        for (int q = 0; q < 12 && !rc; ++q) {
+            s->psi_len[q] = axis[q] == 0 ? lx : (axis[q] == 1 ? ly : lz);
+            rc = s->alloc(&s->psi[q], (size_t)s->psi_len[q] * s->esz);
+        }
+    }
+    if (!rc) {
+        for (int q = 0; q < 6; ++q) hipMemsetAsync(s->F[q], 0, (size_t)s->N * s->esz, s->stream);
+        for (int q = 0; q < 12; ++q)
+            if (s->psi[q]) hipMemsetAsync(s->psi[q], 0, (size_t)s->psi_len[q] * s->esz, s->stream);
+    }
+    if (!rc) rc = precision == FDTD_FP64 ? setup_coeffs<double>(s, g, m, bc) : setup_coeffs<float>(s, g, m, bc);
+    if (!rc) rc = setup_gain(s, tls);
+    if (!rc) rc = s->alloc(reinterpret_cast<void**>(&s->d_src), sizeof(SourceBox) * 64);
+    if (!rc && hipStreamSynchronize(s->stream) != hipSuccess) rc = fail(FDTD_EDEVICE, "initialisation failed");
+    if (rc) {
+        delete s;
+        return rc;
+    }
+    *out = s;
+    return FDTD_OK;
+}
+
+int fdtd_destroy(fdtd_handle h) {
+    delete h;
+    return FDTD_OK;
+}
+
+int fdtd_device_bytes(fdtd_handle h, size_t* bytes) {
+    if (!h || !bytes) return fail(FDTD_EINVAL, "null argument");
+    *bytes = h->bytes;
+    return FDTD_OK;
+}
+
+int fdtd_set_sources(fdtd_handle h, int n, const fdtd_source_desc* src) {
+    if (!h || n < 0 || n > 64 || (n > 0 && !src)) return fail(FDTD_EINVAL, "bad source list (at most 64)");
+    std::vector<SourceBox> b((size_t)n);
+    for (int q = 0; q < n; ++q) {
+        const fdtd_source_desc& d = src[q];
+        if (d.component < 0 || d.component > 2) return fail(FDTD_EINVAL, "source component must be 0..2");
+        b[(size_t)q] = SourceBox{d.component, d.i0, d.i1, d.j0, d.j1, d.k0, d.k1};
+    }
+    if (n) HIPCHK(hipMemcpyAsync(h->d_src, b.data(), b.size() * sizeof(SourceBox), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->nsrc = n;
+    return FDTD_OK;
+}
+
+int fdtd_add_probe(fdtd_handle h, int i, int j, int k, int component, int save_every, long capacity) {
+    if (!h) return fail(FDTD_EINVAL, "null handle");
+    if (i < 0 || j < 0 || k < 0 || i >= h->nx || j >= h->ny || k >= h->nz) return fail(FDTD_EINVAL, "probe outside grid");
+    if (component < 0 || component > 11 || save_every < 1 || capacity < 1) return fail(FDTD_EINVAL, "bad probe arguments");
+    Probe p{};
+    p.id = ((long)i * h->ny + j) * h->nz + k;
+    p.comp = component;
+    p.save_every = save_every;
+    p.cap = capacity;
+    p.count = 0;
+    if (int rc = h->alloc(reinterpret_cast<void**>(&p.d_buf), (size_t)capacity * sizeof(double))) return rc;
+    h->probes.push_back(p);
+    return (int)h->probes.size() - 1;
+}
+
+int fdtd_add_dft(fdtd_handle h, int component, const int box[6], int n_freqs, const double* freqs) {
+    if (!h || !box || !freqs) return fail(FDTD_EINVAL, "null argument");
+    if (component < 0 || component > 5 || n_freqs < 1 || n_freqs > 8) return fail(FDTD_EINVAL, "bad DFT arguments");
+    if (box[0] < 0 || box[1] > h->nx || box[0] >= box[1] || box[2] < 0 || box[3] > h->ny || box[2] >= box[3] ||
+        box[4] < 0 || box[5] > h->nz || box[4] >= box[5])
+        return fail(FDTD_EINVAL, "DFT box outside grid");
+    Dft d{};
+    d.comp = component;
+    std::memcpy(d.box, box, sizeof(d.box));
+    d.nf = n_freqs;
+    for (int f = 0; f < n_freqs; ++f) d.freqs[f] = freqs[f];
+    d.ncell = (long)(box[1] - box[0]) * (box[3] - box[2]) * (box[5] - box[4]);
+    const size_t b = (size_t)d.ncell * n_freqs * 2 * sizeof(double);
+    if (int rc = h->alloc(reinterpret_cast<void**>(&d.d_acc), b)) return rc;
+    HIPCHK(hipMemsetAsync(d.d_acc, 0, b, h->stream));
+    h->dfts.push_back(d);
+    return (int)h->dfts.size() - 1;
+}
+
+int fdtd_set_monitor(fdtd_handle h, int interval) {
+    if (!h || interval < 0) return fail(FDTD_EINVAL, "bad monitor interval");
+    h->mon_interval = interval;
+    return FDTD_OK;
+}
+
+int fdtd_step(fdtd_handle h, long n0, int nsteps, const double* src_values) {
+    if (!h || nsteps < 0 || n0 < 0) return fail(FDTD_EINVAL, "bad step arguments");
+    if (nsteps == 0) return FDTD_OK;
+    if (h->nsrc > 0) {
+        if (!src_values) return fail(FDTD_EINVAL, "src_values required");
+        const long need = (long)nsteps * h->nsrc;
+        if (need > h->srcv_cap) {
+            void* p;
+            if (int rc = h->alloc(&p, (size_t)need * sizeof(double))) return rc;
+            h->d_srcv = (double*)p;
+            h->srcv_cap = need;
+        }
+        HIPCHK(hipMemcpyAsync(h->d_srcv, src_values, (size_t)need * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    }
+    // monitor slots for this call
+    std::vector<long> mon_steps;
+    if (h->mon_interval > 0)
+        for (int s = 0; s < nsteps; ++s)
+            if ((n0 + s) % h->mon_interval == 0) mon_steps.push_back(n0 + s);
+    if ((int)mon_steps.size() > h->mon_cap) {
+        void* p;
+        const int cap = std::max<int>((int)mon_steps.size(), 64);
+        if (int rc = h->alloc(&p, (size_t)cap * sizeof(MonOut))) return rc;
+        h->d_mon = (MonOut*)p;
+        h->mon_cap = cap;
+    }
+    if (!mon_steps.empty()) HIPCHK(hipMemsetAsync(h->d_mon, 0, mon_steps.size() * sizeof(MonOut), h->stream));
+    int slot = 0;
+    for (int s = 0; s < nsteps; ++s) {
+        const long n = n0 + s;
+        dispatch_step(h, n, s);
+        if (h->mon_interval > 0 && n % h->mon_interval == 0) dispatch_monitor(h, h->d_mon + slot++);
+    }
+    HIPCHK(hipGetLastError());
+    h->checks.clear();
+    if (!mon_steps.empty()) {
+        std::vector<MonOut> mo(mon_steps.size());
+        HIPCHK(hipMemcpyAsync(mo.data(), h->d_mon, mo.size() * sizeof(MonOut), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        bool ok = true;
+        long fail_step = -1;
+        for (size_t q = 0; q < mo.size(); ++q) {
+            fdtd_check_record r{};
+            const unsigned long long inf = 0x7ff0000000000000ull;
+            r.step = mon_steps[q];
+            r.nan_E = mo[q].maxE >= inf; r.nan_H = mo[q].maxH >= inf;
+            r.nan_P = mo[q].maxP >= inf; r.nan_N = mo[q].nanN >= inf;
+            r.max_E = bits_to_double(mo[q].maxE);
+            r.max_H = bits_to_double(mo[q].maxH);
+            r.max_P = bits_to_double(mo[q].maxP);
+            r.pop_total = mo[q].pop_total;
+            r.pop_initial = mo[q].pop_init;
+            h->checks.push_back(r);
+            if (ok && !apply_policy(h, r)) { ok = false; fail_step = r.step; }
+        }
+        if (!ok) return fail(FDTD_EUNSTABLE, "simulation unstable at step " + std::to_string(fail_step));
+    }
+    return FDTD_OK;
+}
+
+int fdtd_synchronize(fdtd_handle h) {
+    if (!h) return fail(FDTD_EINVAL, "null handle");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipGetLastError());
+    return FDTD_OK;
+}
+
+int fdtd_download(fdtd_handle h, int which, double* out) {
+    if (!h || !out) return fail(FDTD_EINVAL, "null argument");
+    if (which >= 0 && which < 6) return download_any(h, h->F[which], (size_t)h->N, h->fsc[which], out);
+    if (which >= 6 && which < 10) {
+        std::memset(out, 0, (size_t)h->N * sizeof(double));
+        if (!h->gain) return FDTD_OK;
+        double* srcs[4] = {h->gNu, h->gNg, h->gPz, h->gPp};
+        std::vector<double> box((size_t)h->gbox);
+        HIPCHK(hipMemcpyAsync(box.data(), srcs[which - 6], box.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        for (int gi = 0; gi < h->gn[0]; ++gi)
+            for (int gj = 0; gj < h->gn[1]; ++gj)
+                for (int gk = 0; gk < h->gn[2]; ++gk) {
+                    const long g = ((long)gi * h->gn[1] + gj) * h->gn[2] + gk;
+                    const long id = ((long)(gi + h->g0[0]) * h->ny + (gj + h->g0[1])) * h->nz + (gk + h->g0[2]);
+                    out[id] = box[(size_t)g];
+                }
+        return FDTD_OK;
+    }
+    if (which >= 10 && which < 22) {
+        const int q = which - 10;
+        std::memset(out, 0, (size_t)h->N * sizeof(double));
+        if (!h->psi[q]) return FDTD_OK;
+        std::vector<double> slab((size_t)h->psi_len[q]);
+        if (int rc = download_any(h, h->psi[q], slab.size(), h->psc[q], slab.data())) return rc;
+        for (int i = 0; i < h->nx; ++i)
+            for (int j = 0; j < h->ny; ++j)
+                for (int k = 0; k < h->nz; ++k) {
+                    const long p = psi_slab_index(h, q, i, j, k);
+                    if (p >= 0) out[((long)i * h->ny + j) * h->nz + k] = slab[(size_t)p];
+                }
+        return FDTD_OK;
+    }
+    return fail(FDTD_EINVAL, "unknown array id");
+}
+
+int fdtd_upload(fdtd_handle h, int which, const double* in) {
+    if (!h || !in) return fail(FDTD_EINVAL, "null argument");
+    if (which >= 0 && which < 6) return upload_any(h, h->F[which], in, (size_t)h->N, h->fsc[which]);
+    if (which >= 6 && which < 10) {
+        if (!h->gain) return FDTD_OK;
+        double* dsts[4] = {h->gNu, h->gNg, h->gPz, h->gPp};
+        std::vector<double> box((size_t)h->gbox);
+        for (int gi = 0; gi < h->gn[0]; ++gi)
+            for (int gj = 0; gj < h->gn[1]; ++gj)
+                for (int gk = 0; gk < h->gn[2]; ++gk) {
+                    const long g = ((long)gi * h->gn[1] + gj) * h->gn[2] + gk;
+                    const long id = ((long)(gi + h->g0[0]) * h->ny + (gj + h->g0[1])) * h->nz + (gk + h->g0[2]);
+                    box[(size_t)g] = in[id];
+                }
+        HIPCHK(hipMemcpyAsync(dsts[which - 6], box.data(), box.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        return FDTD_OK;
+    }
+    return fail(FDTD_EINVAL, "unknown array id");
+}
+
+int fdtd_read_probe(fdtd_handle h, int probe, double* out, long max, long* count) {
+    if (!h || probe < 0 || probe >= (int)h->probes.size()) return fail(FDTD_EINVAL, "bad probe id");
+    const Probe& p = h->probes[(size_t)probe];
+    if (count) *count = p.count;
+    const long n = std::min(max, p.count);
+    if (n > 0 && out) {
+        HIPCHK(hipMemcpyAsync(out, p.d_buf, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+    }
+    return FDTD_OK;
+}
+
+int fdtd_read_dft(fdtd_handle h, int m, double* out) {
+    if (!h || m < 0 || m >= (int)h->dfts.size() || !out) return fail(FDTD_EINVAL, "bad DFT id");
+    const Dft& d = h->dfts[(size_t)m];
+    HIPCHK(hipMemcpyAsync(out, d.d_acc, (size_t)d.ncell * d.nf * 2 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return FDTD_OK;
+}
+
+int fdtd_read_check(fdtd_handle h, fdtd_check_record* out, int max, int* count) {
+    if (!h) return fail(FDTD_EINVAL, "null handle");
+    const int n = (int)h->checks.size();
+    if (count) *count = n;
+    for (int q = 0; q < std::min(n, max); ++q) out[q] = h->checks[(size_t)q];
+    return FDTD_OK;
+}
+
+int fdtd_info(fdtd_handle h, long* info) {
+    if (!h || !info) return fail(FDTD_EINVAL, "null argument");
+    info[0] = h->nx; info[1] = h->ny; info[2] = h->nz; info[3] = h->prec;
+    info[4] = h->nmat; info[5] = h->mid16 ? 2 : 1; info[6] = h->ngain; info[7] = h->np;
+    return FDTD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,487 @@
+// fdtd_kernels.hpp - gfx950 kernels of the per-timestep Yee update.
+//
+// Layout in HBM: every field component is its own array in the reference's
+// idx3 order ((i*NyT + j)*NzT + k, k fastest), so a 64-lane wavefront reads
+// 64 consecutive k values of one (i, j) row: one fully coalesced
+// 256 B (FP32) / 512 B (FP64) / 128 B (FP16) request per operand.
+// CPML psi arrays exist only on the boundary slabs (x slabs [0,npml) and
+// [NxT-npml,NxT) over all j,k; likewise y and z), gain state only inside the
+// gain bounding box, and the per-cell material coefficients are a 1- or
+// 2-byte material id per cell into an exact lookup table of the distinct
+// (aE, bE, aH, bH) tuples of the baked MaterialGrids.
+//
+// Arithmetic restates, per cell, the reference's sequence of operations:
+// fdtd_update_H / fdtd_update_E (src/fdtd_stepper.hpp:8-89), the separate
+// CPML correction passes (src/boundary.hpp:150-300, same order of the six
+// corrections per cell), fdtd_update_E_with_gain and the two TLS passes
+// (src/two_level_system.hpp:299-342, 574-666).  Fusing the passes into one
+// thread per cell only removes memory round trips; each value goes through
+// the same operations in the same order (HIP contracts a*b+c into FMA, which
+// moves FP64 results by ~1e-16 relative per operation).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <stdint.h>
+
+namespace fdtdk {
+
+// Storage <-> compute conversion.  FP16 storage carries a power-of-two scale
+// per component (value = stored * scale), so scaling is exact.
+template <typename T, typename R>
+__device__ __forceinline__ R ld(const T* p, long i, R scale) {
+    if constexpr (sizeof(T) == 2) {
+        return (R)__half2float(reinterpret_cast<const __half*>(p)[i]) * scale;
+    } else {
+        (void)scale;
+        return (R)p[i];
+    }
+}
+template <typename T, typename R>
+__device__ __forceinline__ void st(T* p, long i, R v, R inv_scale) {
+    if constexpr (sizeof(T) == 2) {
+        reinterpret_cast<__half*>(p)[i] = __float2half_rn((float)(v * inv_scale));
+    } else {
+        (void)inv_scale;
+        p[i] = (T)v;
+    }
+}
+// The value a store followed by a load would return (rounding to storage).
+template <typename T, typename R>
+__device__ __forceinline__ double stored_value(R v, R scale, R inv_scale) {
+    if constexpr (sizeof(T) == 2) {
+        return (double)(__half2float(__float2half_rn((float)(v * inv_scale))) * scale);
+    } else {
+        (void)scale; (void)inv_scale;
+        return (double)(T)v;
+    }
+}
+
+struct SourceBox {
+    int comp, i0, i1, j0, j1, k0, k1;
+};
+
+struct TLSCoef {
+    double kapa, pa2, pa3, inv_dt, inv_tau, inv_hbar_omega, N0, dt;
+    int clamp;
+};
+
+template <typename T, typename R>
+struct StepArgs {
+    int nx, ny, nz, np;
+    long sI;
+    int sJ;
+    T *Ex, *Ey, *Ez, *Hx, *Hy, *Hz;
+    const R *idx, *idy, *idz;
+    // CPML 1D profiles, per axis [5][n]: kappa, bE, cE, bH, cH
+    const R *cpx, *cpy, *cpz;
+    int cpml;
+    // psi slabs: 0 Ex_y 1 Ex_z 2 Ey_z 3 Ey_x 4 Ez_x 5 Ez_y 6 Hx_y 7 Hx_z 8 Hy_z 9 Hy_x 10 Hz_x 11 Hz_y
+    T* psi[12];
+    R psc[12], ipsc[12];
+    R fsc[6], ifsc[6];
+    // materials: lut[12][nmat] = aEx bEx aEy bEy aEz bEz aHx bHx aHy bHy aHz bHz
+    const void* matid;
+    int mid16;
+    const R* lut;
+    int nmat;
+    int lossless;
+    int h_uniform;
+    // sources
+    const SourceBox* src;
+    int nsrc;
+    const double* srcv;  // values of this step
+    // gain box
+    int gain;
+    int g0x, g0y, g0z, gnx, gny, gnz;
+    const int8_t* greg;
+    double *gPz, *gPp, *gNu, *gNg;
+    const TLSCoef* tls;
+};
+
+__device__ __forceinline__ bool in_pml(int i, int n, int np) {
+    int d = i < n - 1 - i ? i : n - 1 - i;
+    return d < np;
+}
+// psi slab offsets
+__device__ __forceinline__ long psi_x(int i, int j, int k, int nx, int ny, int nz, int np) {
+    int ii = i < np ? i : i - (nx - 2 * np);
+    return ((long)ii * ny + j) * nz + k;
+}
+__device__ __forceinline__ long psi_y(int i, int j, int k, int nx, int ny, int nz, int np) {
+    (void)nx;
+    int jj = j < np ? j : j - (ny - 2 * np);
+    return ((long)i * (2 * np) + jj) * nz + k;
+}
+__device__ __forceinline__ long psi_z(int i, int j, int k, int nx, int ny, int nz, int np) {
+    (void)nx;
+    int kk = k < np ? k : k - (nz - 2 * np);
+    return ((long)i * ny + j) * (2 * np) + kk;
+}
+
+template <typename R>
+__device__ __forceinline__ int mat_of(const void* m, int mid16, long id) {
+    return mid16 ? (int)reinterpret_cast<const uint16_t*>(m)[id] : (int)reinterpret_cast<const uint8_t*>(m)[id];
+}
+
+// One CPML correction: psi = b*psi + c*d; field -= / += coef * (+-(corr - d)).
+// Returns the correction term (corr - d) with corr = d/kappa + psi_new.
+template <typename T, typename R>
+__device__ __forceinline__ R cpml_term(T* psi, long pi, R d, R kap, R b, R c, R sc, R isc) {
+    R corr = d / kap;
+    R p = b * ld<T, R>(psi, pi, sc) + c * d;
+    st<T, R>(psi, pi, p, isc);
+    corr += p;
+    return corr - d;
+}
+
+// H update with the CPML-H correction fused (fdtd_update_H, src/fdtd_stepper.hpp:8-47
+// + CPMLBoundary::apply_after_H, src/boundary.hpp:150-225).  Threads cover
+// i < NxT-1, j < NyT-1, k < NzT-1; blockDim = (64, BY).
+template <typename T, typename R>
+__global__ void __launch_bounds__(256) k_update_H(StepArgs<T, R> a) {
+    const int k = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int i = blockIdx.z;
+    if (k >= a.nz - 1 || j >= a.ny - 1 || i >= a.nx - 1) return;
+    const long id = ((long)i * a.ny + j) * a.nz + k;
+    const long sI = a.sI, sJ = a.sJ;
+    const R ex = ld<T, R>(a.Ex, id, a.fsc[0]), ex_j = ld<T, R>(a.Ex, id + sJ, a.fsc[0]), ex_k = ld<T, R>(a.Ex, id + 1, a.fsc[0]);
+    const R ey = ld<T, R>(a.Ey, id, a.fsc[1]), ey_i = ld<T, R>(a.Ey, id + sI, a.fsc[1]), ey_k = ld<T, R>(a.Ey, id + 1, a.fsc[1]);
+    const R ez = ld<T, R>(a.Ez, id, a.fsc[2]), ez_i = ld<T, R>(a.Ez, id + sI, a.fsc[2]), ez_j = ld<T, R>(a.Ez, id + sJ, a.fsc[2]);
+    const R idx = a.idx[i], idy = a.idy[j], idz = a.idz[k];
+    const R dEz_y = (ez_j - ez) * idy, dEy_z = (ey_k - ey) * idz;
+    const R dEx_z = (ex_k - ex) * idz, dEz_x = (ez_i - ez) * idx;
+    const R dEy_x = (ey_i - ey) * idx, dEx_y = (ex_j - ex) * idy;
+    const R cx = dEz_y - dEy_z, cy = dEx_z - dEz_x, cz = dEy_x - dEx_y;
+    R aHx, bHx, aHy, bHy, aHz, bHz;
+    if (a.h_uniform) {
+        aHx = a.lut[6 * a.nmat]; bHx = a.lut[7 * a.nmat];
+        aHy = a.lut[8 * a.nmat]; bHy = a.lut[9 * a.nmat];
+        aHz = a.lut[10 * a.nmat]; bHz = a.lut[11 * a.nmat];
+    } else {
+        const int m = mat_of<R>(a.matid, a.mid16, id);
+        aHx = a.lut[6 * a.nmat + m]; bHx = a.lut[7 * a.nmat + m];
+        aHy = a.lut[8 * a.nmat + m]; bHy = a.lut[9 * a.nmat + m];
+        aHz = a.lut[10 * a.nmat + m]; bHz = a.lut[11 * a.nmat + m];
+    }
+    R hx = aHx * ld<T, R>(a.Hx, id, a.fsc[3]) - bHx * cx;
+    R hy = aHy * ld<T, R>(a.Hy, id, a.fsc[4]) - bHy * cy;
+    R hz = aHz * ld<T, R>(a.Hz, id, a.fsc[5]) - bHz * cz;
+    if (a.cpml) {
+        const int np = a.np;
+        const bool px = in_pml(i, a.nx, np), py = in_pml(j, a.ny, np), pz = in_pml(k, a.nz, np);
+        if (px | py | pz) {
+            const R* CX = a.cpx; const R* CY = a.cpy; const R* CZ = a.cpz;
+            const int nx = a.nx, ny = a.ny, nz = a.nz;
+            if (py) hx -= bHx * cpml_term<T, R>(a.psi[6], psi_y(i, j, k, nx, ny, nz, np), dEz_y, CY[j], CY[3 * ny + j], CY[4 * ny + j], a.psc[6], a.ipsc[6]);
+            if (pz) hx -= bHx * (-cpml_term<T, R>(a.psi[7], psi_z(i, j, k, nx, ny, nz, np), dEy_z, CZ[k], CZ[3 * nz + k], CZ[4 * nz + k], a.psc[7], a.ipsc[7]));
+            if (pz) hy -= bHy * cpml_term<T, R>(a.psi[8], psi_z(i, j, k, nx, ny, nz, np), dEx_z, CZ[k], CZ[3 * nz + k], CZ[4 * nz + k], a.psc[8], a.ipsc[8]);
+            if (px) hy -= bHy * (-cpml_term<T, R>(a.psi[9], psi_x(i, j, k, nx, ny, nz, np), dEz_x, CX[i], CX[3 * nx + i], CX[4 * nx + i], a.psc[9], a.ipsc[9]));
+            if (px) hz -= bHz * cpml_term<T, R>(a.psi[10], psi_x(i, j, k, nx, ny, nz, np), dEy_x, CX[i], CX[3 * nx + i], CX[4 * nx + i], a.psc[10], a.ipsc[10]);
+            if (py) hz -= bHz * (-cpml_term<T, R>(a.psi[11], psi_y(i, j, k, nx, ny, nz, np), dEx_y, CY[j], CY[3 * ny + j], CY[4 * ny + j], a.psc[11], a.ipsc[11]));
+        }
+    }
+    st<T, R>(a.Hx, id, hx, a.ifsc[3]);
+    st<T, R>(a.Hy, id, hy, a.ifsc[4]);
+    st<T, R>(a.Hz, id, hz, a.ifsc[5]);
+}
+
+// Two-level polarization step (update_polarization_multi_region,
+// src/two_level_system.hpp:595-605).  Ndip = Ng0 = N0 on region cells.
+__device__ __forceinline__ double tls_polarize(const TLSCoef& c, double Nu, double Ng, double ez, double& Pz, double& Pp) {
+    const double Ng_frac = (Ng - Nu) / c.N0;
+    const double driving = c.N0 * Ng_frac * ez;
+    const double Pn = c.kapa * driving + c.pa2 * Pz + c.pa3 * Pp;
+    const double Po = Pz;
+    Pp = Po;
+    Pz = Pn;
+    return (Pn - Po) * c.inv_dt;
+}
+
+// Population step (update_populations_multi_region, src/two_level_system.hpp:633-662).
+__device__ __forceinline__ void tls_populate(const TLSCoef& c, double ez_new, double ez_old, double Pz, double Pp,
+                                             double& Nu, double& Ng) {
+    const double Nu_curr = Nu;
+    const double Ntotal = c.N0;
+    const double E_avg = 0.5 * (ez_new + ez_old);
+    const double delta_P = Pz - Pp;
+    const double stim = E_avg * delta_P * c.inv_dt * c.inv_hbar_omega;
+    const double dNu = -c.inv_tau * Nu_curr + stim;
+    const double dNg = +c.inv_tau * Nu_curr - stim;
+    Nu += c.dt * dNu;
+    Ng += c.dt * dNg;
+    Nu = (0.0 < Nu) ? Nu : 0.0;
+    Ng = (0.0 < Ng) ? Ng : 0.0;
+    if (c.clamp) {
+        const double total = Nu + Ng;
+        if (total > 1e-30) {
+            const double scale = Ntotal / total;
+            Nu *= scale;
+            Ng *= scale;
+        }
+        Nu = (Ntotal < Nu) ? Ntotal : Nu;
+        Ng = Ntotal - Nu;
+    }
+}
+
+// E update with J, the CPML-E correction, the TLS polarization (with Ez^n)
+// and the population update (with the final Ez^{n+1}) fused.  Threads cover
+// the whole grid: cells with i, j or k == 0 are never updated by the
+// reference's E pass (src/fdtd_stepper.hpp:73-75); if they are gain cells
+// their polarization and populations still advance with Ez_old = 0, as in
+// the reference where Ez_old is only written on updated cells.
+template <typename T, typename R>
+__global__ void __launch_bounds__(256) k_update_E(StepArgs<T, R> a) {
+    const int k = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int i = blockIdx.z;
+    if (k >= a.nz || j >= a.ny || i >= a.nx) return;
+    const long id = ((long)i * a.ny + j) * a.nz + k;
+    int reg = -1;
+    long g = 0;
+    if (a.gain) {
+        const int gi = i - a.g0x, gj = j - a.g0y, gk = k - a.g0z;
+        if ((unsigned)gi < (unsigned)a.gnx && (unsigned)gj < (unsigned)a.gny && (unsigned)gk < (unsigned)a.gnz) {
+            g = ((long)gi * a.gny + gj) * a.gnz + gk;
+            reg = a.greg[g];
+        }
+    }
+    const bool interior = (i > 0) & (j > 0) & (k > 0);
+    if (!interior) {
+        if (reg >= 0) {
+            const TLSCoef c = a.tls[reg];
+            double Nu = a.gNu[g], Ng = a.gNg[g], Pz = a.gPz[g], Pp = a.gPp[g];
+            const double ez = (double)ld<T, R>(a.Ez, id, a.fsc[2]);
+            (void)tls_polarize(c, Nu, Ng, ez, Pz, Pp);
+            tls_populate(c, ez, 0.0, Pz, Pp, Nu, Ng);
+            a.gNu[g] = Nu; a.gNg[g] = Ng; a.gPz[g] = Pz; a.gPp[g] = Pp;
+        }
+        return;
+    }
+    const long sI = a.sI, sJ = a.sJ;
+    const R hx = ld<T, R>(a.Hx, id, a.fsc[3]), hx_j = ld<T, R>(a.Hx, id - sJ, a.fsc[3]), hx_k = ld<T, R>(a.Hx, id - 1, a.fsc[3]);
+    const R hy = ld<T, R>(a.Hy, id, a.fsc[4]), hy_i = ld<T, R>(a.Hy, id - sI, a.fsc[4]), hy_k = ld<T, R>(a.Hy, id - 1, a.fsc[4]);
+    const R hz = ld<T, R>(a.Hz, id, a.fsc[5]), hz_i = ld<T, R>(a.Hz, id - sI, a.fsc[5]), hz_j = ld<T, R>(a.Hz, id - sJ, a.fsc[5]);
+    const R idx = a.idx[i], idy = a.idy[j], idz = a.idz[k];
+    const R dHz_y = (hz - hz_j) * idy, dHy_z = (hy - hy_k) * idz;
+    const R dHx_z = (hx - hx_k) * idz, dHz_x = (hz - hz_i) * idx;
+    const R dHy_x = (hy - hy_i) * idx, dHx_y = (hx - hx_j) * idy;
+    const R cx = dHz_y - dHy_z, cy = dHx_z - dHz_x, cz = dHy_x - dHx_y;
+    R jx = 0, jy = 0, jz = 0;
+    for (int s = 0; s < a.nsrc; ++s) {
+        const SourceBox b = a.src[s];
+        if (i >= b.i0 && i <= b.i1 && j >= b.j0 && j <= b.j1 && k >= b.k0 && k <= b.k1) {
+            const R v = (R)a.srcv[s];
+            if (b.comp == 0) jx += v; else if (b.comp == 1) jy += v; else jz += v;
+        }
+    }
+    const int m = mat_of<R>(a.matid, a.mid16, id);
+    const R bEx = a.lut[1 * a.nmat + m], bEy = a.lut[3 * a.nmat + m], bEz = a.lut[5 * a.nmat + m];
+    const R ex0 = ld<T, R>(a.Ex, id, a.fsc[0]), ey0 = ld<T, R>(a.Ey, id, a.fsc[1]), ez0 = ld<T, R>(a.Ez, id, a.fsc[2]);
+    double Nu = 0, Ng = 0, Pz = 0, Pp = 0;
+    R rz = cz - jz;
+    if (reg >= 0) {
+        Nu = a.gNu[g]; Ng = a.gNg[g]; Pz = a.gPz[g]; Pp = a.gPp[g];
+        const double dP = tls_polarize(a.tls[reg], Nu, Ng, (double)ez0, Pz, Pp);
+        rz = rz - (R)dP;
+    }
+    R ex, ey, ez;
+    if (a.lossless) {
+        ex = ex0 + bEx * (cx - jx);
+        ey = ey0 + bEy * (cy - jy);
+        ez = ez0 + bEz * rz;
+    } else {
+        ex = a.lut[0 * a.nmat + m] * ex0 + bEx * (cx - jx);
+        ey = a.lut[2 * a.nmat + m] * ey0 + bEy * (cy - jy);
+        ez = a.lut[4 * a.nmat + m] * ez0 + bEz * rz;
+    }
+    if (a.cpml) {
+        const int np = a.np;
+        const bool px = in_pml(i, a.nx, np), py = in_pml(j, a.ny, np), pz = in_pml(k, a.nz, np);
+        if (px | py | pz) {
+            const R* CX = a.cpx; const R* CY = a.cpy; const R* CZ = a.cpz;
+            const int nx = a.nx, ny = a.ny, nz = a.nz;
+            if (py) ex += bEx * cpml_term<T, R>(a.psi[0], psi_y(i, j, k, nx, ny, nz, np), dHz_y, CY[j], CY[1 * ny + j], CY[2 * ny + j], a.psc[0], a.ipsc[0]);
+            if (pz) ex += bEx * (-cpml_term<T, R>(a.psi[1], psi_z(i, j, k, nx, ny, nz, np), dHy_z, CZ[k], CZ[1 * nz + k], CZ[2 * nz + k], a.psc[1], a.ipsc[1]));
+            if (pz) ey += bEy * cpml_term<T, R>(a.psi[2], psi_z(i, j, k, nx, ny, nz, np), dHx_z, CZ[k], CZ[1 * nz + k], CZ[2 * nz + k], a.psc[2], a.ipsc[2]);
+            if (px) ey += bEy * (-cpml_term<T, R>(a.psi[3], psi_x(i, j, k, nx, ny, nz, np), dHz_x, CX[i], CX[1 * nx + i], CX[2 * nx + i], a.psc[3], a.ipsc[3]));
+            if (px) ez += bEz * cpml_term<T, R>(a.psi[4], psi_x(i, j, k, nx, ny, nz, np), dHy_x, CX[i], CX[1 * nx + i], CX[2 * nx + i], a.psc[4], a.ipsc[4]);
+            if (py) ez += bEz * (-cpml_term<T, R>(a.psi[5], psi_y(i, j, k, nx, ny, nz, np), dHx_y, CY[j], CY[1 * ny + j], CY[2 * ny + j], a.psc[5], a.ipsc[5]));
+        }
+    }
+    st<T, R>(a.Ex, id, ex, a.ifsc[0]);
+    st<T, R>(a.Ey, id, ey, a.ifsc[1]);
+    st<T, R>(a.Ez, id, ez, a.ifsc[2]);
+    if (reg >= 0) {
+        const TLSCoef& c = a.tls[reg];
+        // The reference stores Ez and reads it back for the population pass.
+        const double ez_new = stored_value<T, R>(ez, a.fsc[2], a.ifsc[2]);
+        tls_populate(c, ez_new, (double)ez0, Pz, Pp, Nu, Ng);
+        a.gNu[g] = Nu; a.gNg[g] = Ng; a.gPz[g] = Pz; a.gPp[g] = Pp;
+    }
+}
+
+// PECBoundary::apply_after_E (src/boundary.hpp:55-74): zero E on the six faces.
+template <typename T>
+__global__ void k_pec_faces(T* Ex, T* Ey, T* Ez, int nx, int ny, int nz) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long fxy = (long)nx * ny, fxz = (long)nx * nz, fyz = (long)ny * nz;
+    long id;
+    if (t < 2 * fyz) {
+        const long q = t % fyz;
+        const int i = t < fyz ? 0 : nx - 1;
+        id = ((long)i * ny + q / nz) * nz + q % nz;
+    } else if (t < 2 * fyz + 2 * fxz) {
+        const long u = t - 2 * fyz, q = u % fxz;
+        const int j = u < fxz ? 0 : ny - 1;
+        id = ((q / nz) * ny + j) * nz + q % nz;
+    } else if (t < 2 * fyz + 2 * fxz + 2 * fxy) {
+        const long u = t - 2 * fyz - 2 * fxz, q = u % fxy;
+        const int k = u < fxy ? 0 : nz - 1;
+        id = q * nz + k;
+    } else {
+        return;
+    }
+    Ex[id] = (T)0; Ey[id] = (T)0; Ez[id] = (T)0;
+}
+
+struct ProbeSlot {
+    long id;
+    int comp;
+    double* out;
+};
+struct ProbeArgs {
+    int n;
+    ProbeSlot p[16];
+};
+
+template <typename T, typename R>
+__device__ __forceinline__ double field_value(const StepArgs<T, R>& a, int comp, long id) {
+    const double ex = (double)ld<T, R>(a.Ex, id, a.fsc[0]), ey = (double)ld<T, R>(a.Ey, id, a.fsc[1]),
+                 ez = (double)ld<T, R>(a.Ez, id, a.fsc[2]);
+    const double hx = (double)ld<T, R>(a.Hx, id, a.fsc[3]), hy = (double)ld<T, R>(a.Hy, id, a.fsc[4]),
+                 hz = (double)ld<T, R>(a.Hz, id, a.fsc[5]);
+    switch (comp) {
+    case 0: return ex;
+    case 1: return ey;
+    case 2: return ez;
+    case 3: return hx;
+    case 4: return hy;
+    case 5: return hz;
+    case 6: return sqrt(ex * ex + ey * ey + ez * ez);
+    case 7: return sqrt(hx * hx + hy * hy + hz * hz);
+    case 8: return ey * hz - ez * hy;
+    case 9: return ez * hx - ex * hz;
+    case 10: return ex * hy - ey * hx;
+    case 11: {
+        const double sx = ey * hz - ez * hy, sy = ez * hx - ex * hz, sz = ex * hy - ey * hx;
+        return sqrt(sx * sx + sy * sy + sz * sz);
+    }
+    }
+    return 0.0;
+}
+
+// PointFieldDetector::record_after_E / get_field_value
+// (src/detectors/point_field_detector.hpp:111-124, src/detectors/idetector.hpp:79-109).
+template <typename T, typename R>
+__global__ void k_probes(StepArgs<T, R> a, ProbeArgs pa) {
+    const int t = threadIdx.x;
+    if (t >= pa.n) return;
+    *pa.p[t].out = field_value<T, R>(a, pa.p[t].comp, pa.p[t].id);
+}
+
+struct DftArgs {
+    int comp, i0, j0, k0, ny_b, nz_b;
+    long ncell;
+    int nf;
+    double wr[8], wi[8];
+    double* acc;
+};
+
+template <typename T, typename R>
+__global__ void k_dft(StepArgs<T, R> a, DftArgs d) {
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= d.ncell) return;
+    const int kk = (int)(c % d.nz_b);
+    const long r = c / d.nz_b;
+    const int jj = (int)(r % d.ny_b);
+    const int ii = (int)(r / d.ny_b);
+    const long id = ((long)(d.i0 + ii) * a.ny + (d.j0 + jj)) * a.nz + (d.k0 + kk);
+    const double F = field_value<T, R>(a, d.comp, id);
+    for (int f = 0; f < d.nf; ++f) {
+        double* p = d.acc + ((long)f * d.ncell + c) * 2;
+        p[0] += F * d.wr[f];
+        p[1] += F * d.wi[f];
+    }
+}
+
+// Stability reduction (StabilityMonitor::check_stability inputs,
+// src/stability_monitor.hpp:51-121, and calculate_conservation_error
+// :155-170).  max|.| is taken over the absolute-value bit patterns: for
+// non-negative IEEE values the integer order is the numeric order and every
+// NaN/Inf pattern is above every finite one, so one atomicMax yields both the
+// maximum and the NaN/Inf flag.
+struct MonOut {
+    unsigned long long maxE, maxH, maxP, nanN;
+    double pop_total, pop_init;
+};
+
+template <typename T, typename R>
+__device__ __forceinline__ unsigned long long absbits(const T* p, long i, R s) {
+    const double v = fabs((double)ld<T, R>(p, i, s));
+    return (unsigned long long)__double_as_longlong(v);
+}
+
+template <typename T, typename R>
+__global__ void __launch_bounds__(256) k_monitor(StepArgs<T, R> a, long N, long ngain, MonOut* out) {
+    unsigned long long mE = 0, mH = 0, mP = 0, nN = 0;
+    double pt = 0.0, pi = 0.0;
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long id = (long)blockIdx.x * blockDim.x + threadIdx.x; id < N; id += stride) {
+        unsigned long long v;
+        v = absbits<T, R>(a.Ex, id, a.fsc[0]); mE = v > mE ? v : mE;
+        v = absbits<T, R>(a.Ey, id, a.fsc[1]); mE = v > mE ? v : mE;
+        v = absbits<T, R>(a.Ez, id, a.fsc[2]); mE = v > mE ? v : mE;
+        v = absbits<T, R>(a.Hx, id, a.fsc[3]); mH = v > mH ? v : mH;
+        v = absbits<T, R>(a.Hy, id, a.fsc[4]); mH = v > mH ? v : mH;
+        v = absbits<T, R>(a.Hz, id, a.fsc[5]); mH = v > mH ? v : mH;
+    }
+    for (long g = (long)blockIdx.x * blockDim.x + threadIdx.x; g < ngain; g += stride) {
+        const int r = a.greg[g];
+        if (r < 0) continue;
+        const double pz = a.gPz[g], pp = a.gPp[g], nu = a.gNu[g], ng = a.gNg[g];
+        unsigned long long v = (unsigned long long)__double_as_longlong(fabs(pz));
+        mP = v > mP ? v : mP;
+        v = (unsigned long long)__double_as_longlong(fabs(pp));
+        // Pz_prev only contributes to the NaN test (dPz_dt is formed from Pz, Pz_prev)
+        if (v >= 0x7ff0000000000000ull) mP = v > mP ? v : mP;
+        v = (unsigned long long)__double_as_longlong(fabs(nu));
+        nN = v > nN ? v : nN;
+        v = (unsigned long long)__double_as_longlong(fabs(ng));
+        nN = v > nN ? v : nN;
+        if (a.tls[r].N0 > 1e10) {
+            pt += (nu + ng);
+            pi += a.tls[r].N0;
+        }
+    }
+    // wavefront reduction (64 lanes)
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned long long o;
+        o = __shfl_down(mE, off); mE = o > mE ? o : mE;
+        o = __shfl_down(mH, off); mH = o > mH ? o : mH;
+        o = __shfl_down(mP, off); mP = o > mP ? o : mP;
+        o = __shfl_down(nN, off); nN = o > nN ? o : nN;
+        pt += __shfl_down(pt, off);
+        pi += __shfl_down(pi, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(&out->maxE, mE);
+        atomicMax(&out->maxH, mH);
+        atomicMax(&out->maxP, mP);
+        atomicMax(&out->nanN, nN);
+        if (pt != 0.0) atomicAdd(&out->pop_total, pt);
+        if (pi != 0.0) atomicAdd(&out->pop_init, pi);
+    }
+}
+
+}  // namespace fdtdk

@@ -1,0 +1,176 @@
+/* fdtd_hip.h - C ABI of the MI355X (gfx950) per-timestep Yee update.
+ *
+ * Drop-in boundary.  One handle replaces, for one simulation, the calls the
+ * reference's serial step body makes on host vectors
+ * (3D_FDTD_version008 src/3D_FDTD_version008.cpp:269-349):
+ *
+ *   reference call (file:line)                                   replaced by
+ *   ------------------------------------------------------------ -----------------------
+ *   std::fill(J..)+ISource::inject_half_step  src/3D_FDTD_version008.cpp:270-275,
+ *                      src/sources/isource.hpp:28-31                fdtd_set_sources + fdtd_step values
+ *   update_polarization_multi_region      src/two_level_system.hpp:574-609   fdtd_step (fused, E kernel)
+ *   fdtd_update_H                         src/fdtd_stepper.hpp:8-47          fdtd_step (H kernel)
+ *   IBoundary::apply_after_H (CPML)       src/boundary.hpp:150-225           fdtd_step (fused, H kernel)
+ *   fdtd_update_E / _with_gain            src/fdtd_stepper.hpp:49-89,
+ *                                         src/two_level_system.hpp:299-342   fdtd_step (E kernel)
+ *   IBoundary::apply_after_E (CPML / PEC) src/boundary.hpp:227-300, 51-79    fdtd_step (fused / face kernel)
+ *   PointFieldDetector::record_after_E    src/detectors/point_field_detector.hpp:111-124
+ *                                                                            fdtd_add_probe + fdtd_read_probe
+ *   update_populations_multi_region       src/two_level_system.hpp:612-666   fdtd_step (fused, E kernel)
+ *   StabilityMonitor::check_stability     src/stability_monitor.hpp:42-144   fdtd_step monitor + fdtd_read_check
+ *
+ * Conventions: all host arrays are the reference's own layout (FP64, idx3
+ * order (i*NyT + j)*NzT + k, src/global_function.hpp:21-24, unpadded).  The
+ * library copies them to device memory it owns; the caller keeps ownership of
+ * its buffers.  Every entry point returns 0 on success or a negative FDTD_E*
+ * code; fdtd_last_error() returns the message of the last failure on the
+ * calling thread.  One handle binds one HIP stream and is not thread-safe;
+ * distinct handles may be driven from distinct host threads.
+ */
+#ifndef FDTD_HIP_H
+#define FDTD_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    FDTD_OK = 0,
+    FDTD_EINVAL = -1,     /* bad argument / unsupported configuration */
+    FDTD_EDEVICE = -2,    /* HIP runtime error or no device */
+    FDTD_ENOMEM = -3,     /* device allocation failed */
+    FDTD_EUNSTABLE = -4,  /* stability monitor verdict: simulation unstable */
+};
+
+/* Storage/compute precision of fields and CPML state.
+ * FP64: FP64 storage and arithmetic (parity path).
+ * FP32: FP32 storage and arithmetic; populations and polarization in FP64.
+ * FP16: FP16 storage with per-component power-of-two scales, FP32 arithmetic,
+ *       FP64 populations, polarization and DFT accumulators. */
+enum { FDTD_FP64 = 0, FDTD_FP32 = 1, FDTD_FP16 = 2 };
+
+typedef struct fdtd_sim* fdtd_handle;
+
+/* GridSpacing (src/global_function.hpp:92-105) and dt (src/params.hpp:265-266). */
+typedef struct {
+    int nx, ny, nz;                          /* NxT, NyT, NzT: field dimensions */
+    const double *inv_dx, *inv_dy, *inv_dz;  /* length >= nx, ny, nz */
+    double dt;
+} fdtd_grid_desc;
+
+/* MaterialGrids, src/global_function.hpp:70-90 (as filled by
+ * StructureScene::bake, src/structure_material.hpp:211-288): 12 arrays of
+ * nx*ny*nz values in idx3 order. */
+typedef struct {
+    const double *aEx, *bEx, *aEy, *bEy, *aEz, *bEz;
+    const double *aHx, *bHx, *aHy, *bHy, *aHz, *bHz;
+} fdtd_material_desc;
+
+/* Boundary, src/boundary.hpp:16-136.  type FDTD_BC_PEC (0) or FDTD_BC_CPML (1).
+ * For CPML the 1D profiles of CPMLBoundary (kx_/ky_/kz_, bE*_/cE*_,
+ * bH*_/cH*_, :87-91, filled by build_profiles_and_coeffs :303-373),
+ * lengths nx / ny / nz. */
+typedef struct {
+    int type;
+    int npml;
+    const double *kx, *ky, *kz;
+    const double *bEx, *cEx, *bEy, *cEy, *bEz, *cEz;
+    const double *bHx, *cHx, *bHy, *cHy, *bHz, *cHz;
+} fdtd_boundary_desc;
+
+/* Per-region TwoLevelParams after finalize / finalize_with_dt
+ * (src/two_level_system.hpp:17-110). */
+typedef struct {
+    double kapa, pa2, pa3, inv_dt;   /* kapa_coeff, pa2, pa3, inv_dt */
+    double tau, hbar, omega_a;       /* for inv_tau = 1/tau, inv_hbar_omega = 1/(hbar*omega_a) */
+    double N0;                       /* N0_total: Ndip = Ng0 = N0 on every cell of the region */
+    double dt;
+    int clamp;                       /* enable_population_clamp */
+} fdtd_tls_region;
+
+/* TwoLevelState + TLSRegionManager::cell_region_map
+ * (src/two_level_system.hpp:112-197, 488-571).  All arrays nx*ny*nz, idx3. */
+typedef struct {
+    int n_regions;
+    const fdtd_tls_region* regions;
+    const int* cell_region_map;           /* -1 = no TLS */
+    const double *Ndip, *Ng0;             /* validated against regions[].N0 */
+    const double *Nu, *Ng, *Pz, *Pz_prev; /* initial state */
+} fdtd_tls_desc;
+
+/* A current-source footprint: component (0 Jx, 1 Jy, 2 Jz) and an inclusive
+ * index box.  DipoleSource is a 1-cell box (src/sources/dipole_source.hpp:60-85),
+ * PlaneWaveSource a plane rectangle (src/sources/plane_wave_source.hpp:102-179).
+ * Per-step values are supplied to fdtd_step in source order. */
+typedef struct {
+    int component;
+    int i0, i1, j0, j1, k0, k1;
+} fdtd_source_desc;
+
+/* Stability reduction at one monitor step (StabilityMonitor inputs). */
+typedef struct {
+    long step;
+    double max_E, max_H, max_P;
+    int nan_E, nan_H, nan_P, nan_N;
+    double pop_total, pop_initial;  /* sums over Ndip > 1e10 cells */
+} fdtd_check_record;
+
+/* ---- lifecycle ---- */
+int fdtd_create(const fdtd_grid_desc* grid, const fdtd_material_desc* mat, const fdtd_boundary_desc* bc,
+                const fdtd_tls_desc* tls /* NULL = no gain */, int precision, fdtd_handle* out);
+int fdtd_destroy(fdtd_handle h);
+const char* fdtd_last_error(void);
+/* Device bytes held by the handle (admission control). */
+int fdtd_device_bytes(fdtd_handle h, size_t* bytes);
+
+/* ---- sources, probes, monitors ---- */
+int fdtd_set_sources(fdtd_handle h, int n, const fdtd_source_desc* src);
+/* Returns the probe id (>= 0) or a negative error.  component 0..11 as
+ * Detectors::FieldComponent (src/detectors/idetector.hpp:19-29).  Values are
+ * recorded after the E step whenever n % save_every == 0 (n = global step),
+ * up to `capacity` records. */
+int fdtd_add_probe(fdtd_handle h, int i, int j, int k, int component, int save_every, long capacity);
+/* Running DFT over a half-open index box (no reference counterpart).  Returns
+ * monitor id.  X(f) = sum_n F(t_n) exp(-i 2 pi f t_n) dt, t_n = (n+1) dt for E,
+ * (n+1/2) dt for H components. */
+int fdtd_add_dft(fdtd_handle h, int component, const int box[6], int n_freqs, const double* freqs);
+/* Stability monitor cadence (STABILITY_MONITOR_INTERVAL, default 10; 0 = off). */
+int fdtd_set_monitor(fdtd_handle h, int interval);
+
+/* ---- time stepping ----
+ * Advances nsteps steps starting at global step n0, in the reference's step
+ * order.  src_values[s * n_sources + q] is the J value source q contributes at
+ * step n0 + s (the host-side waveform value divided by the cell area for a
+ * dipole, or 2 E(t)/(eta0 d) * sign for a plane, exactly as the reference
+ * computes it).  Returns FDTD_EUNSTABLE if a monitor step in this call fails
+ * the reference's stability policy (NaN/Inf or more than 10 threshold
+ * warnings); fdtd_read_check gives the records. */
+int fdtd_step(fdtd_handle h, long n0, int nsteps, const double* src_values);
+int fdtd_synchronize(fdtd_handle h);
+
+/* ---- data movement (reference layout, FP64 on the host side) ----
+ * which: 0..5 = Ex,Ey,Ez,Hx,Hy,Hz; 6 = Nu, 7 = Ng, 8 = Pz, 9 = Pz_prev
+ * (TLS arrays are full-grid, zero outside gain cells). */
+int fdtd_download(fdtd_handle h, int which, double* out);
+int fdtd_upload(fdtd_handle h, int which, const double* in);
+/* Copies up to max records of probe p into out; *count receives the number
+ * recorded so far. */
+int fdtd_read_probe(fdtd_handle h, int probe, double* out, long max, long* count);
+/* Accumulators of DFT monitor m, layout [freq][cell][re, im], cell in box
+ * order (i, j, k with k fastest). */
+int fdtd_read_dft(fdtd_handle h, int m, double* out);
+/* Monitor records of the last fdtd_step call (up to max); *count = number. */
+int fdtd_read_check(fdtd_handle h, fdtd_check_record* out, int max, int* count);
+
+/* ---- introspection ---- */
+/* info[0..7] = nx, ny, nz, precision, n_materials (LUT entries), material id
+ * bytes per cell, gain cells, npml. */
+int fdtd_info(fdtd_handle h, long* info);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FDTD_HIP_H */

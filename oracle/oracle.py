@@ -1,0 +1,185 @@
+"""ctypes front-end for the CPU checkers.
+
+TEST INFRASTRUCTURE ONLY: the FP64 oracle (liboracle.so, a C restatement of
+the reference's serial step) and, where it was built, the compiled reference
+harness (_ref/libref.so).  Imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libref.so")
+
+_libs = {}
+
+
+def build(quiet: bool = True) -> None:
+    """Compile liboracle.so (and _ref/libref.so where the reference tree exists)."""
+    out = subprocess.run(["make", "-C", HERE, "-s"], capture_output=True, text=True)
+    if out.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + out.stdout + out.stderr)
+    if not quiet:
+        print(out.stdout, out.stderr)
+
+
+def available(impl: str = "oracle") -> bool:
+    return os.path.exists(ORACLE_SO if impl == "oracle" else REF_SO)
+
+
+def _lib(impl: str):
+    if impl in _libs:
+        return _libs[impl]
+    path = ORACLE_SO if impl == "oracle" else REF_SO
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} is not built (run make -C oracle)")
+    lib = C.CDLL(path)
+    p = "oracle_" if impl == "oracle" else "ref_"
+    vp = C.c_void_p
+    sig = {
+        "create": (vp, [vp]),
+        "destroy": (None, [vp]),
+        "step": (C.c_int, [vp, C.c_int]),
+        "info": (None, [vp, C.POINTER(C.c_long)]),
+        "dt": (C.c_double, [vp]),
+        "last_max_E": (C.c_double, [vp]),
+        "get_array": (C.c_long, [vp, C.c_char_p, vp]),
+        "get_region_map": (C.c_long, [vp, vp]),
+        "probe_records": (C.c_long, [vp, C.c_int]),
+        "probe_index": (C.c_int, [vp, C.c_int, C.POINTER(C.c_long)]),
+        "get_probe": (C.c_long, [vp, C.c_int, C.c_int, vp]),
+        "set_array": (C.c_int, [vp, C.c_char_p, vp]),
+    }
+    if impl == "oracle":
+        sig.update({
+            "get_region": (C.c_int, [vp, C.c_int, vp]),
+            "get_source": (C.c_int, [vp, C.c_int, C.c_long, vp]),
+            "get_dft": (C.c_long, [vp, C.c_int, vp]),
+            "time_steps": (C.c_double, [vp, C.c_int]),
+            "last_conservation": (C.c_double, [vp]),
+        })
+    fns = {}
+    for name, (res, args) in sig.items():
+        f = getattr(lib, p + name)
+        f.restype = res
+        f.argtypes = args
+        fns[name] = f
+    if impl == "oracle":
+        lib.oracle_threads.restype = C.c_int
+        fns["threads"] = lib.oracle_threads
+    _libs[impl] = fns
+    return fns
+
+
+class CpuSim:
+    """One CPU simulation (impl 'oracle' = the C restatement, 'ref' = the
+    compiled reference harness)."""
+
+    def __init__(self, spec, impl: str = "oracle"):
+        self.impl = impl
+        self.f = _lib(impl)
+        self._cspec = spec.to_c()
+        self.h = self.f["create"](C.addressof(self._cspec))
+        if not self.h:
+            raise RuntimeError("create failed")
+        info = self.info()
+        self.shape = (info[0], info[1], info[2])
+        self.npml = info[3]
+
+    def close(self):
+        if self.h:
+            self.f["destroy"](self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        out = (C.c_long * 14)()
+        self.f["info"](self.h, out)
+        return list(out)
+
+    @property
+    def dt(self) -> float:
+        return self.f["dt"](self.h)
+
+    @property
+    def steps_done(self) -> int:
+        return self.info()[4]
+
+    def step(self, n: int) -> int:
+        return self.f["step"](self.h, n)
+
+    def time_steps(self, n: int) -> float:
+        return self.f["time_steps"](self.h, n)
+
+    def array(self, name: str, shaped: bool = True) -> np.ndarray:
+        n = self.f["get_array"](self.h, name.encode(), None)
+        if n < 0:
+            raise KeyError(name)
+        out = np.empty(n, dtype=np.float64)
+        self.f["get_array"](self.h, name.encode(), out.ctypes.data_as(C.c_void_p))
+        if shaped and n == self.shape[0] * self.shape[1] * self.shape[2]:
+            return out.reshape(self.shape)
+        return out
+
+    def set_array(self, name: str, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+        if self.f["set_array"](self.h, name.encode(), a.ctypes.data_as(C.c_void_p)) != 0:
+            raise KeyError(name)
+
+    def region_map(self) -> np.ndarray:
+        n = self.f["get_region_map"](self.h, None)
+        out = np.empty(max(n, 0), dtype=np.int32)
+        if n > 0:
+            self.f["get_region_map"](self.h, out.ctypes.data_as(C.c_void_p))
+            return out.reshape(self.shape)
+        return out
+
+    def probe_index(self, p: int):
+        ijk = (C.c_long * 3)()
+        nc = self.f["probe_index"](self.h, p, ijk)
+        return tuple(ijk), nc
+
+    def probe(self, p: int, c: int = 0) -> np.ndarray:
+        n = self.f["probe_records"](self.h, p)
+        out = np.empty(max(n, 0), dtype=np.float64)
+        if n > 0:
+            self.f["get_probe"](self.h, p, c, out.ctypes.data_as(C.c_void_p))
+        return out
+
+    def region(self, r: int) -> np.ndarray:
+        out = np.empty(17, dtype=np.float64)
+        if self.f["get_region"](self.h, r, out.ctypes.data_as(C.c_void_p)) != 0:
+            raise IndexError(r)
+        return out
+
+    def source(self, q: int, n: int = 0) -> np.ndarray:
+        out = np.empty(19, dtype=np.float64)
+        if self.f["get_source"](self.h, q, n, out.ctypes.data_as(C.c_void_p)) != 0:
+            raise IndexError(q)
+        return out
+
+    def dft(self, m: int) -> np.ndarray:
+        n = self.f["get_dft"](self.h, m, None)
+        out = np.empty(n, dtype=np.float64)
+        self.f["get_dft"](self.h, m, out.ctypes.data_as(C.c_void_p))
+        return out
+
+    def stability(self):
+        info = self.info()
+        return {"stable": bool(info[8]), "fail_step": info[9], "warnings": info[10],
+                "max_E": self.f["last_max_E"](self.h)}
+
+
+def threads() -> int:
+    return _lib("oracle")["threads"]()

@@ -1,0 +1,353 @@
+// ref_driver.cpp - harness that runs the compiled reference (its own headers,
+// read in place from the reference tree via -I) behind the same C ABI as the
+// oracle.  TEST INFRASTRUCTURE ONLY: built by oracle/Makefile into
+// oracle/_ref/libref.so when the reference tree is present; used to write and
+// check golden vectors and to cross-check the oracle bitwise.
+//
+// It replaces only the reference's compile-time configuration and main():
+// the setup calls are the reference's (make_boundary, StructureScene::bake,
+// make_dipole_source, make_plane_wave_source, TLSRegionManager, TwoLevelParams,
+// StabilityMonitor) and the step order is the serial-mode body of
+// src/3D_FDTD_version008.cpp:269-349.  make_context (src/params.hpp:180-296)
+// is restated for the grid / dt part because it only supports the UNIFORM
+// and auto-mesh modes and writes files.
+#include <sstream>
+#include <iostream>
+#include <vector>
+#include <memory>
+#include <cstring>
+#include <streambuf>
+
+#include "global_function.hpp"
+#include "fdtd_stepper.hpp"
+#include "boundary.hpp"
+#include "structure_material.hpp"
+#include "two_level_system.hpp"
+#include "sources/dipole_source.hpp"
+#include "sources/plane_wave_source.hpp"
+#include "detectors/idetector.hpp"
+#include "stability_monitor.hpp"
+
+#include "../include/fdtd_scene.h"
+
+namespace {
+
+struct NullBuf : std::streambuf {
+    int overflow(int c) override { return c; }
+};
+
+struct Probe {
+    size_t i0, j0, k0;
+    std::vector<int> comps;
+    std::vector<double> rec;
+};
+
+}  // namespace
+
+struct ref_sim {
+    size_t NxT{}, NyT{}, NzT{}, N{}, npml{};
+    real dt{};
+    GridSpacing gs;
+    std::unique_ptr<IBoundary> bc;
+    StructureScene scene;
+    MaterialGrids mats;
+    std::vector<std::unique_ptr<Sources::ISource>> sources;
+    std::vector<real> Ex, Ey, Ez, Hx, Hy, Hz, Jx, Jy, Jz;
+    bool tls_on{false};
+    TwoLevelState tls_state;
+    TLSRegionManager tls_manager;
+    std::vector<Probe> probes;
+    size_t save_every{1};
+    size_t stab_interval{10};
+    StabilityMonitor mon;
+    long fail_step{-1};
+    double last_max_E{0};
+    size_t n{0};
+};
+
+extern "C" {
+
+ref_sim* ref_create(const fdtd_scene_spec* sp) {
+    NullBuf nb;
+    std::streambuf* old_out = std::cout.rdbuf(&nb);
+    std::streambuf* old_err = std::cerr.rdbuf(&nb);
+    auto* S = new ref_sim();
+    const size_t npml_spec = (size_t)sp->npml;
+    const size_t nxg = (size_t)sp->nx_core + 2 * npml_spec;
+    const size_t nyg = (size_t)sp->ny_core + 2 * npml_spec;
+    const size_t nzg = (size_t)sp->nz_core + 2 * npml_spec;
+    GridSpacing g;
+    if (sp->dx && sp->dy && sp->dz) {
+        g.dx.assign(sp->dx, sp->dx + nxg);
+        g.dy.assign(sp->dy, sp->dy + nyg);
+        g.dz.assign(sp->dz, sp->dz + nzg);
+    } else {
+        g.dx.assign(nxg, sp->dx_uniform);
+        g.dy.assign(nyg, sp->dy_uniform);
+        g.dz.assign(nzg, sp->dz_uniform);
+    }
+    build_cumulative_bounds(g.dx, g.x_bounds);
+    build_cumulative_bounds(g.dy, g.y_bounds);
+    build_cumulative_bounds(g.dz, g.z_bounds);
+    g.compute_inverses();
+    real dx_min = *std::min_element(g.dx.begin(), g.dx.end());
+    real dy_min = *std::min_element(g.dy.begin(), g.dy.end());
+    real dz_min = *std::min_element(g.dz.begin(), g.dz.end());
+    const real sum_inv2 = (1.0 / (dx_min * dx_min)) + (1.0 / (dy_min * dy_min)) + (1.0 / (dz_min * dz_min));
+    S->dt = sp->S / (PhysConst::C0 * std::sqrt(sum_inv2));
+
+    BoundaryParams bp;
+    bp.type = sp->boundary_type == FDTD_BC_PEC ? BcType::PEC : BcType::CPML_RC;
+    bp.cpml.npml = sp->npml;
+    bp.cpml.m = sp->cpml_m;
+    bp.cpml.Rerr = sp->cpml_rerr;
+    bp.cpml.alpha0 = sp->cpml_alpha0;
+    bp.cpml.kappa_max = sp->cpml_kappa_max;
+    bp.cpml.alpha_linear = sp->cpml_alpha_linear != 0;
+    S->bc = make_boundary(bp, sp->nx_core, sp->ny_core, sp->nz_core, g, S->dt, PhysConst::EPS0, PhysConst::MU0, PhysConst::C0);
+    S->NxT = S->bc->NxT(); S->NyT = S->bc->NyT(); S->NzT = S->bc->NzT();
+    S->npml = S->bc->npml();
+    S->N = S->NxT * S->NyT * S->NzT;
+    S->gs = g;
+    S->gs.npml = S->npml;
+
+    StructureScene& sc = S->scene;
+    sc.NxT = S->NxT; sc.NyT = S->NyT; sc.NzT = S->NzT;
+    sc.Nx_core = sp->nx_core; sc.Ny_core = sp->ny_core; sc.Nz_core = sp->nz_core;
+    sc.npml_cells = S->npml;
+    sc.grid_spacing = S->gs;
+    sc.bg = make_nk(sp->background_n);
+    const real pml_x = sc.grid_spacing.x_bounds[sc.npml_cells];
+    const real pml_y = sc.grid_spacing.y_bounds[sc.npml_cells];
+    const real pml_z = sc.grid_spacing.z_bounds[sc.npml_cells];
+    for (int q = 0; q < sp->n_structures; ++q) {
+        const fdtd_structure_def& s = sp->structures[q];
+        Material mat = make_nk(s.n);
+        StructureTLSConfig tls_cfg;
+        if (sp->tls_enabled && s.tls_material >= 0 && s.tls_material < sp->n_tls_materials) {
+            const fdtd_tls_material& m = sp->tls_materials[s.tls_material];
+            tls_cfg.enabled = true;
+            tls_cfg.lambda0 = m.lambda0; tls_cfg.gamma = m.gamma; tls_cfg.tau = m.tau;
+            tls_cfg.N0 = m.N0; tls_cfg.inversion_fraction = m.inversion_fraction;
+        }
+        if (s.type == FDTD_SHAPE_BOX)
+            sc.add_box(AABB{pml_x + s.params[0], pml_x + s.params[1], pml_y + s.params[2], pml_y + s.params[3],
+                            pml_z + s.params[4], pml_z + s.params[5]}, mat, tls_cfg);
+        else if (s.type == FDTD_SHAPE_SPHERE)
+            sc.add_sphere(pml_x + s.params[0], pml_y + s.params[1], pml_z + s.params[2], s.params[3], mat, tls_cfg);
+        else
+            sc.add_cylinder_z(pml_x + s.params[0], pml_y + s.params[1], s.params[2], pml_z + s.params[3],
+                              pml_z + s.params[4], mat, tls_cfg);
+    }
+    sc.bake(S->mats, S->dt, PhysConst::EPS0, PhysConst::MU0, sp->subpixel_samples > 0 ? sp->subpixel_samples : 8);
+    S->bc->bind_material_coeffs(&S->mats.bEx, &S->mats.bEy, &S->mats.bEz, &S->mats.bHx, &S->mats.bHy, &S->mats.bHz);
+
+    for (int d = 0; d < sp->n_dipoles; ++d) {
+        const fdtd_dipole_def& D = sp->dipoles[d];
+        Sources::SourceConfig cfg;
+        cfg.x = D.x; cfg.y = D.y; cfg.z = D.z;
+        cfg.amplitude = D.amplitude; cfg.frequency = D.frequency; cfg.wavelength = D.wavelength;
+        cfg.tau = D.tau; cfg.df_fwhm = D.df_fwhm; cfg.t0_factor = D.t0_factor;
+        cfg.waveform = static_cast<Sources::Waveform>(D.waveform);
+        cfg.polarization = static_cast<Sources::Polarization>(D.polarization);
+        auto src = Sources::make_dipole_source(cfg, S->gs, S->NyT, S->NzT, S->dt, sp->domain_x_min,
+                                               sp->domain_y_min, sp->domain_z_min);
+        if (src) S->sources.push_back(std::move(src));
+    }
+    for (int d = 0; d < sp->n_planes; ++d) {
+        const fdtd_plane_wave_def& D = sp->planes[d];
+        Sources::PlaneWaveConfig cfg;
+        cfg.injection_position = D.injection_position;
+        cfg.amplitude = D.amplitude; cfg.frequency = D.frequency; cfg.wavelength = D.wavelength;
+        cfg.tau = D.tau; cfg.df_fwhm = D.df_fwhm; cfg.t0_factor = D.t0_factor;
+        cfg.waveform = static_cast<Sources::Waveform>(D.waveform);
+        cfg.direction = static_cast<Sources::PlaneWaveDirection>(D.direction);
+        cfg.polarization = static_cast<Sources::Polarization>(D.polarization);
+        cfg.x_min = D.x_min; cfg.x_max = D.x_max; cfg.y_min = D.y_min; cfg.y_max = D.y_max;
+        cfg.z_min = D.z_min; cfg.z_max = D.z_max;
+        auto src = Sources::make_plane_wave_source(cfg, S->gs, S->NxT, S->NyT, S->NzT, S->npml, S->dt,
+                                                   sp->domain_x_min, sp->domain_y_min, sp->domain_z_min);
+        if (src) S->sources.push_back(std::move(src));
+    }
+
+    S->Ex.assign(S->N, 0); S->Ey.assign(S->N, 0); S->Ez.assign(S->N, 0);
+    S->Hx.assign(S->N, 0); S->Hy.assign(S->N, 0); S->Hz.assign(S->N, 0);
+    S->Jx.assign(S->N, 0); S->Jy.assign(S->N, 0); S->Jz.assign(S->N, 0);
+
+    // TLS block of main(), src/3D_FDTD_version008.cpp:100-172
+    S->tls_on = sp->tls_enabled && sc.has_any_tls();
+    if (S->tls_on) {
+        S->tls_state.allocate(S->NxT, S->NyT, S->NzT);
+        auto items = sc.get_tls_structures();
+        for (size_t idx = 0; idx < items.size(); ++idx) {
+            const auto* item = items[idx];
+            const auto& tc = item->tls_config;
+            AABB bb = item->shape->bounding_box();
+            TwoLevelParams rp;
+            rp.lambda0 = tc.lambda0; rp.gamma = tc.gamma; rp.tau = tc.tau; rp.N0_total = tc.N0;
+            rp.enable_population_clamp = sp->tls_clamp != 0;
+            S->tls_manager.add_region_from_structure(S->gs, bb.x0 - pml_x, bb.x1 - pml_x, bb.y0 - pml_y,
+                                                     bb.y1 - pml_y, bb.z0 - pml_z, bb.z1 - pml_z, rp,
+                                                     tc.inversion_fraction, item->shape.get(),
+                                                     "structure_" + std::to_string(idx));
+        }
+        S->tls_manager.initialize(S->tls_state, S->gs);
+        S->tls_manager.finalize_with_dt(S->dt);
+    }
+
+    // Probe placement of PointFieldDetector::initialize, point_field_detector.hpp:57-70
+    for (int p = 0; p < sp->n_probes; ++p) {
+        const fdtd_probe_def& D = sp->probes[p];
+        Probe pr;
+        pr.i0 = S->gs.physical_to_index_x(D.x - sp->domain_x_min);
+        pr.j0 = S->gs.physical_to_index_y(D.y - sp->domain_y_min);
+        pr.k0 = S->gs.physical_to_index_z(D.z - sp->domain_z_min);
+        pr.i0 = std::max(std::size_t(1), std::min(pr.i0, S->NxT - 2));
+        pr.j0 = std::max(std::size_t(1), std::min(pr.j0, S->NyT - 2));
+        pr.k0 = std::max(std::size_t(1), std::min(pr.k0, S->NzT - 2));
+        for (int c = 0; c < D.n_components; ++c) pr.comps.push_back(D.components[c]);
+        S->probes.push_back(pr);
+    }
+    S->save_every = sp->save_every > 0 ? (size_t)sp->save_every : 1;
+    S->stab_interval = sp->stability_interval > 0 ? (size_t)sp->stability_interval : 10;
+    std::cout.rdbuf(old_out);
+    std::cerr.rdbuf(old_err);
+    return S;
+}
+
+int ref_step(ref_sim* S, int nsteps) {
+    NullBuf nb;
+    std::streambuf* old_err = std::cerr.rdbuf(&nb);
+    const size_t NxT = S->NxT, NyT = S->NyT, NzT = S->NzT;
+    auto& g = S->gs;
+    auto& m = S->mats;
+    int rc = 0;
+    for (int s = 0; s < nsteps && rc == 0; ++s) {
+        const size_t n = S->n;
+        std::fill(S->Jx.begin(), S->Jx.end(), 0.0);
+        std::fill(S->Jy.begin(), S->Jy.end(), 0.0);
+        std::fill(S->Jz.begin(), S->Jz.end(), 0.0);
+        for (auto& src : S->sources) src->inject_half_step(n, S->dt, S->Jx, S->Jy, S->Jz);
+        if (S->tls_on) update_polarization_multi_region(NxT, NyT, NzT, S->tls_manager, S->Ez, S->tls_state);
+        fdtd_update_H(NxT, NyT, NzT, g.dx.data(), g.dy.data(), g.dz.data(), g.inv_dx.data(), g.inv_dy.data(),
+                      g.inv_dz.data(), m.aHx.data(), m.bHx.data(), m.aHy.data(), m.bHy.data(), m.aHz.data(),
+                      m.bHz.data(), S->Ex.data(), S->Ey.data(), S->Ez.data(), S->Hx.data(), S->Hy.data(), S->Hz.data());
+        S->bc->apply_after_H(S->Ex, S->Ey, S->Ez, S->Hx, S->Hy, S->Hz);
+        if (S->tls_on)
+            fdtd_update_E_with_gain(NxT, NyT, NzT, g.dx.data(), g.dy.data(), g.dz.data(), g.inv_dx.data(),
+                                    g.inv_dy.data(), g.inv_dz.data(), m.aEx.data(), m.bEx.data(), m.aEy.data(),
+                                    m.bEy.data(), m.aEz.data(), m.bEz.data(), S->Ex.data(), S->Ey.data(), S->Ez.data(),
+                                    S->Hx.data(), S->Hy.data(), S->Hz.data(), S->Jx.data(), S->Jy.data(), S->Jz.data(),
+                                    S->tls_state);
+        else
+            fdtd_update_E(NxT, NyT, NzT, g.dx.data(), g.dy.data(), g.dz.data(), g.inv_dx.data(), g.inv_dy.data(),
+                          g.inv_dz.data(), m.aEx.data(), m.bEx.data(), m.aEy.data(), m.bEy.data(), m.aEz.data(),
+                          m.bEz.data(), S->Ex.data(), S->Ey.data(), S->Ez.data(), S->Hx.data(), S->Hy.data(),
+                          S->Hz.data(), S->Jx.data(), S->Jy.data(), S->Jz.data());
+        S->bc->apply_after_E(S->Ex, S->Ey, S->Ez, S->Hx, S->Hy, S->Hz);
+        if (n % S->save_every == 0) {
+            for (auto& pr : S->probes) {
+                size_t id = idx3(pr.i0, pr.j0, pr.k0, NyT, NzT);
+                for (int c : pr.comps)
+                    pr.rec.push_back(Detectors::get_field_value(static_cast<Detectors::FieldComponent>(c), id, S->Ex,
+                                                                S->Ey, S->Ez, S->Hx, S->Hy, S->Hz));
+            }
+        }
+        if (S->tls_on) update_populations_multi_region(NxT, NyT, NzT, S->dt, S->tls_manager, S->Ez, S->tls_state);
+        S->n++;
+        if (n % S->stab_interval == 0) {
+            real cons = S->mon.calculate_conservation_error(S->tls_state);
+            bool ok = S->mon.check_stability(n, S->Ex, S->Ey, S->Ez, S->Hx, S->Hy, S->Hz, S->tls_state, cons);
+            S->last_max_E = S->mon.get_max_E(S->Ex, S->Ey, S->Ez);
+            if (!ok) { S->fail_step = (long)n; rc = 1; }
+        }
+    }
+    std::cerr.rdbuf(old_err);
+    return rc;
+}
+
+void ref_destroy(ref_sim* S) { delete S; }
+
+void ref_info(const ref_sim* S, long* out) {
+    out[0] = (long)S->NxT; out[1] = (long)S->NyT; out[2] = (long)S->NzT; out[3] = (long)S->npml;
+    out[4] = (long)S->n; out[5] = S->tls_on; out[6] = (long)S->tls_manager.num_regions(); out[7] = (long)S->sources.size();
+    out[8] = S->mon.simulation_stable; out[9] = S->fail_step; out[10] = (long)S->mon.instability_count;
+    out[11] = (long)S->gs.dx.size(); out[12] = (long)S->gs.dy.size(); out[13] = (long)S->gs.dz.size();
+}
+double ref_dt(const ref_sim* S) { return S->dt; }
+double ref_last_max_E(const ref_sim* S) { return S->last_max_E; }
+
+long ref_get_array(const ref_sim* S, const char* name, double* out) {
+    const std::string nm(name);
+    const std::vector<real>* v = nullptr;
+    auto* cp = dynamic_cast<const CPMLBoundary*>(S->bc.get());
+    const auto& m = S->mats;
+    const auto& t = S->tls_state;
+    static const char* fn[] = {"Ex", "Ey", "Ez", "Hx", "Hy", "Hz", "Jx", "Jy", "Jz"};
+    const std::vector<real>* fv[] = {&S->Ex, &S->Ey, &S->Ez, &S->Hx, &S->Hy, &S->Hz, &S->Jx, &S->Jy, &S->Jz};
+    for (int q = 0; q < 9; ++q) if (nm == fn[q]) v = fv[q];
+    static const char* mn[] = {"aEx", "bEx", "aEy", "bEy", "aEz", "bEz", "aHx", "bHx", "aHy", "bHy", "aHz", "bHz"};
+    const std::vector<real>* mv[] = {&m.aEx, &m.bEx, &m.aEy, &m.bEy, &m.aEz, &m.bEz, &m.aHx, &m.bHx, &m.aHy, &m.bHy, &m.aHz, &m.bHz};
+    for (int q = 0; q < 12; ++q) if (nm == mn[q]) v = mv[q];
+    if (S->tls_on) {
+        static const char* tn[] = {"Ng", "Nu", "Ng0", "Pz", "Pz_prev", "dPz_dt", "Ez_old", "Ndip"};
+        const std::vector<real>* tv[] = {&t.Ng, &t.Nu, &t.Ng0, &t.Pz, &t.Pz_prev, &t.dPz_dt, &t.Ez_old, &t.Ndip};
+        for (int q = 0; q < 8; ++q) if (nm == tn[q]) v = tv[q];
+    }
+    if (cp) {
+        static const char* pn[] = {"psi_Ex_y", "psi_Ex_z", "psi_Ey_z", "psi_Ey_x", "psi_Ez_x", "psi_Ez_y",
+                                   "psi_Hx_y", "psi_Hx_z", "psi_Hy_z", "psi_Hy_x", "psi_Hz_x", "psi_Hz_y",
+                                   "kx", "ky", "kz", "cpml_bEx", "cpml_cEx", "cpml_bEy", "cpml_cEy", "cpml_bEz",
+                                   "cpml_cEz", "cpml_bHx", "cpml_cHx", "cpml_bHy", "cpml_cHy", "cpml_bHz", "cpml_cHz"};
+        const std::vector<real>* pv[] = {&cp->psi_Ex_y_, &cp->psi_Ex_z_, &cp->psi_Ey_z_, &cp->psi_Ey_x_, &cp->psi_Ez_x_,
+                                         &cp->psi_Ez_y_, &cp->psi_Hx_y_, &cp->psi_Hx_z_, &cp->psi_Hy_z_, &cp->psi_Hy_x_,
+                                         &cp->psi_Hz_x_, &cp->psi_Hz_y_, &cp->kx_, &cp->ky_, &cp->kz_, &cp->bEx_,
+                                         &cp->cEx_, &cp->bEy_, &cp->cEy_, &cp->bEz_, &cp->cEz_, &cp->bHx_, &cp->cHx_,
+                                         &cp->bHy_, &cp->cHy_, &cp->bHz_, &cp->cHz_};
+        for (int q = 0; q < 27; ++q) if (nm == pn[q]) v = pv[q];
+    }
+    const auto& g = S->gs;
+    if (nm == "dx") v = &g.dx; if (nm == "dy") v = &g.dy; if (nm == "dz") v = &g.dz;
+    if (nm == "inv_dx") v = &g.inv_dx; if (nm == "inv_dy") v = &g.inv_dy; if (nm == "inv_dz") v = &g.inv_dz;
+    if (nm == "x_bounds") v = &g.x_bounds; if (nm == "y_bounds") v = &g.y_bounds; if (nm == "z_bounds") v = &g.z_bounds;
+    if (!v) return -1;
+    if (out) std::memcpy(out, v->data(), v->size() * sizeof(double));
+    return (long)v->size();
+}
+
+long ref_get_region_map(const ref_sim* S, int* out) {
+    if (!S->tls_on) return 0;
+    const auto& m = S->tls_manager.cell_region_map;
+    if (out) std::memcpy(out, m.data(), m.size() * sizeof(int));
+    return (long)m.size();
+}
+
+long ref_probe_records(const ref_sim* S, int p) {
+    if (p < 0 || p >= (int)S->probes.size()) return -1;
+    return (long)(S->probes[p].rec.size() / std::max<size_t>(1, S->probes[p].comps.size()));
+}
+int ref_probe_index(const ref_sim* S, int p, long* ijk) {
+    if (p < 0 || p >= (int)S->probes.size()) return -1;
+    ijk[0] = (long)S->probes[p].i0; ijk[1] = (long)S->probes[p].j0; ijk[2] = (long)S->probes[p].k0;
+    return (int)S->probes[p].comps.size();
+}
+long ref_get_probe(const ref_sim* S, int p, int c, double* out) {
+    if (p < 0 || p >= (int)S->probes.size()) return -1;
+    const auto& pr = S->probes[p];
+    const size_t nc = pr.comps.size();
+    if (c < 0 || (size_t)c >= nc) return -1;
+    const size_t nr = pr.rec.size() / nc;
+    for (size_t r = 0; r < nr; ++r) out[r] = pr.rec[r * nc + (size_t)c];
+    return (long)nr;
+}
+
+int ref_set_array(ref_sim* S, const char* name, const double* in) {
+    const std::string nm(name);
+    static const char* fn[] = {"Ex", "Ey", "Ez", "Hx", "Hy", "Hz"};
+    std::vector<real>* fv[] = {&S->Ex, &S->Ey, &S->Ez, &S->Hx, &S->Hy, &S->Hz};
+    for (int q = 0; q < 6; ++q)
+        if (nm == fn[q]) { std::memcpy(fv[q]->data(), in, S->N * sizeof(double)); return 0; }
+    return -1;
+}
+
+}  // extern "C"

@@ -1,0 +1,88 @@
+"""GPU parity: the HIP path through the C ABI against the FP64 CPU oracle on
+the same seeded inputs.
+
+Tolerances: FP64 device path <= 1e-12 relative L2 on every probe trace and
+final field (the reference's own FMA-contraction noise is ~1e-14 after 1000
+steps); FP32 storage/arithmetic <= 2e-4 relative L2 on fields and probes.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _run_pair(pkg, oracle_mod, sp, precision="fp64", steps=None):
+    steps = sp.n_steps if steps is None else steps
+    _say(f"[case] {sp.name} {precision} steps={steps} cells={sp.n_cells()}")
+    setup = pkg.Setup(sp)
+    sc = pkg.Scene(setup, precision)
+    sc.step(steps)
+    ref = oracle_mod.CpuSim(sp, "oracle")
+    ref.step(steps)
+    return setup, sc, ref
+
+
+def _compare(setup, sc, ref, tol, extra=()):
+    worst = {}
+    for name in FIELDS + tuple(extra):
+        a, b = sc.field(name), ref.array(name)
+        worst[name] = rel_l2(a, b)
+    for c in range(setup.n_channels):
+        worst[f"probe{c}"] = rel_l2(sc.probe(c), ref.probe(c))
+    _say("  " + " ".join(f"{k}={v:.2e}" for k, v in worst.items()))
+    bad = {k: v for k, v in worst.items() if not v <= tol}
+    assert not bad, f"exceeds {tol}: {bad}"
+
+
+def test_uniform_sphere_fp64_full(pkg, oracle_mod):
+    """The plain CPU case at full size: 64^3 core + 10-cell CPML, 1000 steps."""
+    sp = pkg.scene.uniform_sphere_case(core=64, npml=10, steps=1000)
+    setup, sc, ref = _run_pair(pkg, oracle_mod, sp, "fp64")
+    _compare(setup, sc, ref, 1e-12)
+    # probe record count = steps (save_every 1, n = 0 included)
+    assert len(sc.probe(0)) == sp.n_steps
+
+
+def test_psi_state_fp64(pkg, oracle_mod):
+    sp = pkg.scene.uniform_sphere_case(core=16, npml=6, steps=150)
+    setup, sc, ref = _run_pair(pkg, oracle_mod, sp, "fp64")
+    for name in pkg.runtime.PSI_NAMES:
+        assert rel_l2(sc.field(name), ref.array(name)) <= 1e-12, name
+
+
+def test_gain_slab_fp64(pkg, oracle_mod):
+    sp = pkg.scene.gain_slab_case(n_total=40, npml=6, steps=400, fine_len=100e-9, coarse=20e-9, fine=5e-9)
+    setup, sc, ref = _run_pair(pkg, oracle_mod, sp, "fp64")
+    _compare(setup, sc, ref, 1e-12, extra=("Nu", "Ng", "Pz", "Pz_prev"))
+
+
+def test_gain_cuboids_nonuniform_fp64(pkg, oracle_mod):
+    sp = pkg.scene.gain_slab_case(n_total=48, npml=6, steps=300, fine_len=100e-9, coarse=20e-9, fine=5e-9,
+                                  n_cuboids=6, seed=3)
+    setup, sc, ref = _run_pair(pkg, oracle_mod, sp, "fp64")
+    _compare(setup, sc, ref, 1e-12, extra=("Nu", "Ng", "Pz"))
+
+
+def test_fp32_uniform(pkg, oracle_mod):
+    sp = pkg.scene.uniform_sphere_case(core=32, npml=8, steps=400)
+    setup, sc, ref = _run_pair(pkg, oracle_mod, sp, "fp32")
+    _compare(setup, sc, ref, 2e-4)
+
+
+def test_fp32_gain(pkg, oracle_mod):
+    sp = pkg.scene.gain_slab_case(n_total=40, npml=6, steps=400, fine_len=100e-9, coarse=20e-9, fine=5e-9)
+    setup, sc, ref = _run_pair(pkg, oracle_mod, sp, "fp32")
+    _compare(setup, sc, ref, 2e-4, extra=("Pz",))
+    assert rel_l2(sc.field("Nu"), ref.array("Nu")) <= 1e-9
