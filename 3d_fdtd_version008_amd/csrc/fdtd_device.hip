@@ -115,9 +115,12 @@ struct fdtd_sim {
     double prev_max_E = 0.0;
     size_t inst_count = 0;
     bool stable = true;
-    // FP16 scales: fields (6) and psi (12); value = stored * scale
-    double fsc[6] = {1, 1, 1, 1, 1, 1};
-    double psc[12] = {1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1};
+    // FP16 scales (value = stored * scale), device-resident so kernels read
+    // the current ones: [0..5] fields, [6..17] psi, [18..35] inverses.
+    float* d_sc = nullptr;
+    float* d_factor = nullptr;
+    unsigned* d_maxbits = nullptr;
+    int fp16_interval = 10;
 
     ~fdtd_sim() {
         if (stream) hipStreamSynchronize(stream);
@@ -162,15 +165,8 @@ StepArgs<T, R> make_args(const fdtd_sim* s) {
     a.idx = (const R*)s->inv[0]; a.idy = (const R*)s->inv[1]; a.idz = (const R*)s->inv[2];
     a.cpml = s->bc_type == 1 && s->np > 0;
     a.cpx = (const R*)s->cp[0]; a.cpy = (const R*)s->cp[1]; a.cpz = (const R*)s->cp[2];
-    for (int q = 0; q < 12; ++q) {
-        a.psi[q] = (T*)s->psi[q];
-        a.psc[q] = (R)s->psc[q];
-        a.ipsc[q] = (R)(1.0 / s->psc[q]);
-    }
-    for (int q = 0; q < 6; ++q) {
-        a.fsc[q] = (R)s->fsc[q];
-        a.ifsc[q] = (R)(1.0 / s->fsc[q]);
-    }
+    for (int q = 0; q < 12; ++q) a.psi[q] = (T*)s->psi[q];
+    a.sc = s->d_sc;
     a.matid = s->matid;
     a.mid16 = s->mid16;
     a.lut = (const R*)s->lut;
@@ -351,13 +347,17 @@ int setup_gain(fdtd_sim* s, const fdtd_tls_desc* t) {
 }
 
 template <typename T, typename R>
-int launch_step(fdtd_sim* s, long n, int sidx) {
+int launch_step(fdtd_sim* s, long n, int sidx, hipEvent_t* ev) {
     StepArgs<T, R> a = make_args<T, R>(s);
     a.srcv = s->d_srcv + (long)sidx * s->nsrc;
     const dim3 blk(64, 4, 1);
     const dim3 grd((unsigned)((s->nz + 63) / 64), (unsigned)((s->ny + 3) / 4), (unsigned)s->nx);
+    if (ev) (void)hipEventRecord(ev[0], s->stream);
     hipLaunchKernelGGL((fdtdk::k_update_H<T, R>), grd, blk, 0, s->stream, a);
+    if (ev) (void)hipEventRecord(ev[1], s->stream);
+    if (ev) (void)hipEventRecord(ev[2], s->stream);
     hipLaunchKernelGGL((fdtdk::k_update_E<T, R>), grd, blk, 0, s->stream, a);
+    if (ev) (void)hipEventRecord(ev[3], s->stream);
     if (s->bc_type == 0) {
         const long faces = 2L * ((long)s->ny * s->nz + (long)s->nx * s->nz + (long)s->nx * s->ny);
         hipLaunchKernelGGL((fdtdk::k_pec_faces<T>), dim3((unsigned)((faces + 255) / 256)), dim3(256), 0, s->stream,
@@ -403,11 +403,11 @@ void launch_monitor(fdtd_sim* s, MonOut* slot) {
     hipLaunchKernelGGL((fdtdk::k_monitor<T, R>), dim3(1024), dim3(256), 0, s->stream, a, s->N, s->gain ? s->gbox : 0L, slot);
 }
 
-int dispatch_step(fdtd_sim* s, long n, int sidx) {
+int dispatch_step(fdtd_sim* s, long n, int sidx, hipEvent_t* ev) {
     switch (s->prec) {
-    case FDTD_FP64: return launch_step<double, double>(s, n, sidx);
-    case FDTD_FP32: return launch_step<float, float>(s, n, sidx);
-    default: return launch_step<uint16_t, float>(s, n, sidx);
+    case FDTD_FP64: return launch_step<double, double>(s, n, sidx, ev);
+    case FDTD_FP32: return launch_step<float, float>(s, n, sidx, ev);
+    default: return launch_step<uint16_t, float>(s, n, sidx, ev);
     }
 }
 void dispatch_monitor(fdtd_sim* s, MonOut* slot) {
@@ -416,6 +416,34 @@ void dispatch_monitor(fdtd_sim* s, MonOut* slot) {
     case FDTD_FP32: launch_monitor<float, float>(s, slot); break;
     default: launch_monitor<uint16_t, float>(s, slot); break;
     }
+}
+
+fdtdk::ArrSet fp16_arrays(const fdtd_sim* s) {
+    fdtdk::ArrSet a{};
+    for (int q = 0; q < 6; ++q) { a.p[q] = (const uint16_t*)s->F[q]; a.n[q] = s->N; }
+    for (int q = 0; q < 12; ++q) { a.p[6 + q] = (const uint16_t*)s->psi[q]; a.n[6 + q] = s->psi[q] ? s->psi_len[q] : 0; }
+    return a;
+}
+
+// Keeps every FP16 array's largest stored magnitude inside [2^6, 2^13] by
+// power-of-two rescaling (exact), entirely on the device.
+void fp16_maintain(fdtd_sim* s) {
+    const fdtdk::ArrSet a = fp16_arrays(s);
+    hipLaunchKernelGGL(fdtdk::k_fp16_maxima, dim3(512), dim3(256), 0, s->stream, a, s->d_maxbits);
+    hipLaunchKernelGGL(fdtdk::k_fp16_plan, dim3(1), dim3(64), 0, s->stream, s->d_sc, s->d_factor, s->d_maxbits);
+    hipLaunchKernelGGL(fdtdk::k_fp16_rescale, dim3(512), dim3(256), 0, s->stream, a, (const float*)s->d_factor);
+}
+
+int read_scales(fdtd_sim* s, float* out36) {
+    HIPCHK(hipMemcpyAsync(out36, s->d_sc, 36 * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+int write_scales(fdtd_sim* s, const float* in36) {
+    HIPCHK(hipMemcpyAsync(s->d_sc, in36, 36 * sizeof(float), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return 0;
 }
 
 double bits_to_double(unsigned long long b) {
@@ -558,6 +586,29 @@ int fdtd_create(const fdtd_grid_desc* g, const fdtd_material_desc* m, const fdtd
     if (!rc) rc = precision == FDTD_FP64 ? setup_coeffs<double>(s, g, m, bc) : setup_coeffs<float>(s, g, m, bc);
     if (!rc) rc = setup_gain(s, tls);
     if (!rc) rc = s->alloc(reinterpret_cast<void**>(&s->d_src), sizeof(SourceBox) * 64);
+    if (!rc) rc = s->alloc(reinterpret_cast<void**>(&s->d_sc), 36 * sizeof(float));
+    if (!rc) rc = s->alloc(reinterpret_cast<void**>(&s->d_factor), 18 * sizeof(float));
+    if (!rc) rc = s->alloc(reinterpret_cast<void**>(&s->d_maxbits), 18 * sizeof(unsigned));
+    if (!rc) {
+        // Initial FP16 scales: E ~ 1, H ~ E/eta0 (2^-8), psi_E ~ H/d, psi_H ~ E/d;
+        // they adapt from the first monitor step on.
+        float sc[36];
+        double imax = 0;
+        for (int i = 0; i < g->nx; ++i) imax = std::max(imax, g->inv_dx[i]);
+        for (int i = 0; i < g->ny; ++i) imax = std::max(imax, g->inv_dy[i]);
+        for (int i = 0; i < g->nz; ++i) imax = std::max(imax, g->inv_dz[i]);
+        int e;
+        std::frexp(imax, &e);
+        for (int q = 0; q < 18; ++q) sc[q] = 1.0f;
+        if (precision == FDTD_FP16) {
+            for (int q = 3; q < 6; ++q) sc[q] = std::ldexp(1.0f, -8);
+            for (int q = 0; q < 6; ++q) sc[6 + q] = std::ldexp(1.0f, e - 8);   // psi_E
+            for (int q = 6; q < 12; ++q) sc[6 + q] = std::ldexp(1.0f, e);      // psi_H
+        }
+        for (int q = 0; q < 18; ++q) sc[18 + q] = 1.0f / sc[q];
+        if (hipMemcpy(s->d_sc, sc, sizeof(sc), hipMemcpyHostToDevice) != hipSuccess) rc = fail(FDTD_EDEVICE, "scale init failed");
+        if (!rc && hipMemset(s->d_maxbits, 0, 18 * sizeof(unsigned)) != hipSuccess) rc = fail(FDTD_EDEVICE, "scale init failed");
+    }
     if (!rc && hipStreamSynchronize(s->stream) != hipSuccess) rc = fail(FDTD_EDEVICE, "initialisation failed");
     if (rc) {
         delete s;
@@ -626,13 +677,35 @@ int fdtd_add_dft(fdtd_handle h, int component, const int box[6], int n_freqs, co
     return (int)h->dfts.size() - 1;
 }
 
+int fdtd_set_fp16_scales(fdtd_handle h, const double* scales18) {
+    if (!h || !scales18) return fail(FDTD_EINVAL, "null argument");
+    if (h->prec != FDTD_FP16) return FDTD_OK;
+    float sc[36];
+    for (int q = 0; q < 18; ++q) {
+        int e;
+        const double fr = std::frexp(scales18[q], &e);
+        if (!(scales18[q] > 0) || fr != 0.5) return fail(FDTD_EINVAL, "FP16 scales must be positive powers of two");
+        sc[q] = (float)scales18[q];
+        sc[18 + q] = (float)(1.0 / scales18[q]);
+    }
+    return write_scales(h, sc);
+}
+
+int fdtd_get_fp16_scales(fdtd_handle h, double* scales18) {
+    if (!h || !scales18) return fail(FDTD_EINVAL, "null argument");
+    float sc[36];
+    if (int rc = read_scales(h, sc)) return rc;
+    for (int q = 0; q < 18; ++q) scales18[q] = sc[q];
+    return FDTD_OK;
+}
+
 int fdtd_set_monitor(fdtd_handle h, int interval) {
     if (!h || interval < 0) return fail(FDTD_EINVAL, "bad monitor interval");
     h->mon_interval = interval;
     return FDTD_OK;
 }
 
-int fdtd_step(fdtd_handle h, long n0, int nsteps, const double* src_values) {
+static int step_impl(fdtd_handle h, long n0, int nsteps, const double* src_values, double* ms) {
     if (!h || nsteps < 0 || n0 < 0) return fail(FDTD_EINVAL, "bad step arguments");
     if (nsteps == 0) return FDTD_OK;
     if (h->nsrc > 0) {
@@ -660,12 +733,35 @@ int fdtd_step(fdtd_handle h, long n0, int nsteps, const double* src_values) {
     }
     if (!mon_steps.empty()) HIPCHK(hipMemsetAsync(h->d_mon, 0, mon_steps.size() * sizeof(MonOut), h->stream));
     int slot = 0;
+    std::vector<hipEvent_t> ev;
+    if (ms) {
+        ev.resize((size_t)nsteps * 4 + 2);
+        for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+        HIPCHK(hipEventRecord(ev[0], h->stream));
+    }
     for (int s = 0; s < nsteps; ++s) {
         const long n = n0 + s;
-        dispatch_step(h, n, s);
+        dispatch_step(h, n, s, ms ? &ev[1 + (size_t)s * 4] : nullptr);
         if (h->mon_interval > 0 && n % h->mon_interval == 0) dispatch_monitor(h, h->d_mon + slot++);
+        if (h->prec == FDTD_FP16 && n % h->fp16_interval == 0) fp16_maintain(h);
     }
     HIPCHK(hipGetLastError());
+    if (ms) {
+        HIPCHK(hipEventRecord(ev.back(), h->stream));
+        HIPCHK(hipEventSynchronize(ev.back()));
+        double th = 0, te = 0;
+        for (int s = 0; s < nsteps; ++s) {
+            float a = 0, b = 0;
+            HIPCHK(hipEventElapsedTime(&a, ev[1 + (size_t)s * 4], ev[2 + (size_t)s * 4]));
+            HIPCHK(hipEventElapsedTime(&b, ev[3 + (size_t)s * 4], ev[4 + (size_t)s * 4]));
+            th += a;
+            te += b;
+        }
+        float tot = 0;
+        HIPCHK(hipEventElapsedTime(&tot, ev[0], ev.back()));
+        ms[0] = th; ms[1] = te; ms[2] = tot;
+        for (auto& e : ev) (void)hipEventDestroy(e);
+    }
     h->checks.clear();
     if (!mon_steps.empty()) {
         std::vector<MonOut> mo(mon_steps.size());
@@ -692,6 +788,15 @@ int fdtd_step(fdtd_handle h, long n0, int nsteps, const double* src_values) {
     return FDTD_OK;
 }
 
+int fdtd_step(fdtd_handle h, long n0, int nsteps, const double* src_values) {
+    return step_impl(h, n0, nsteps, src_values, nullptr);
+}
+
+int fdtd_step_timed(fdtd_handle h, long n0, int nsteps, const double* src_values, double* ms) {
+    if (!ms) return fail(FDTD_EINVAL, "null timing output");
+    return step_impl(h, n0, nsteps, src_values, ms);
+}
+
 int fdtd_synchronize(fdtd_handle h) {
     if (!h) return fail(FDTD_EINVAL, "null handle");
     HIPCHK(hipStreamSynchronize(h->stream));
@@ -701,7 +806,10 @@ int fdtd_synchronize(fdtd_handle h) {
 
 int fdtd_download(fdtd_handle h, int which, double* out) {
     if (!h || !out) return fail(FDTD_EINVAL, "null argument");
-    if (which >= 0 && which < 6) return download_any(h, h->F[which], (size_t)h->N, h->fsc[which], out);
+    float sc[36] = {};
+    if (h->prec == FDTD_FP16)
+        if (int rc = read_scales(h, sc)) return rc;
+    if (which >= 0 && which < 6) return download_any(h, h->F[which], (size_t)h->N, sc[which], out);
     if (which >= 6 && which < 10) {
         std::memset(out, 0, (size_t)h->N * sizeof(double));
         if (!h->gain) return FDTD_OK;
@@ -723,7 +831,7 @@ int fdtd_download(fdtd_handle h, int which, double* out) {
         std::memset(out, 0, (size_t)h->N * sizeof(double));
         if (!h->psi[q]) return FDTD_OK;
         std::vector<double> slab((size_t)h->psi_len[q]);
-        if (int rc = download_any(h, h->psi[q], slab.size(), h->psc[q], slab.data())) return rc;
+        if (int rc = download_any(h, h->psi[q], slab.size(), sc[6 + q], slab.data())) return rc;
         for (int i = 0; i < h->nx; ++i)
             for (int j = 0; j < h->ny; ++j)
                 for (int k = 0; k < h->nz; ++k) {
@@ -737,7 +845,23 @@ int fdtd_download(fdtd_handle h, int which, double* out) {
 
 int fdtd_upload(fdtd_handle h, int which, const double* in) {
     if (!h || !in) return fail(FDTD_EINVAL, "null argument");
-    if (which >= 0 && which < 6) return upload_any(h, h->F[which], in, (size_t)h->N, h->fsc[which]);
+    if (which >= 0 && which < 6) {
+        if (h->prec == FDTD_FP16) {  // choose the component's scale from the data
+            float sc[36];
+            if (int rc = read_scales(h, sc)) return rc;
+            double m = 0;
+            for (long i = 0; i < h->N; ++i) m = std::max(m, std::fabs(in[i]));
+            if (m > 0) {
+                int e;
+                std::frexp(m, &e);
+                sc[which] = std::ldexp(1.0f, e - 12);
+                sc[18 + which] = std::ldexp(1.0f, 12 - e);
+                if (int rc = write_scales(h, sc)) return rc;
+            }
+            return upload_any(h, h->F[which], in, (size_t)h->N, sc[which]);
+        }
+        return upload_any(h, h->F[which], in, (size_t)h->N, 1.0);
+    }
     if (which >= 6 && which < 10) {
         if (!h->gain) return FDTD_OK;
         double* dsts[4] = {h->gNu, h->gNg, h->gPz, h->gPp};

@@ -77,8 +77,9 @@ struct StepArgs {
     int cpml;
     // psi slabs: 0 Ex_y 1 Ex_z 2 Ey_z 3 Ey_x 4 Ez_x 5 Ez_y 6 Hx_y 7 Hx_z 8 Hy_z 9 Hy_x 10 Hz_x 11 Hz_y
     T* psi[12];
-    R psc[12], ipsc[12];
-    R fsc[6], ifsc[6];
+    // FP16 power-of-two scales (device memory, value = stored * scale):
+    // sc[0..5] fields, sc[6..17] psi, sc[18..35] their inverses.
+    const float* sc;
     // materials: lut[12][nmat] = aEx bEx aEy bEy aEz bEz aHx bHx aHy bHy aHz bHz
     const void* matid;
     int mid16;
@@ -145,9 +146,9 @@ __global__ void __launch_bounds__(256) k_update_H(StepArgs<T, R> a) {
     if (k >= a.nz - 1 || j >= a.ny - 1 || i >= a.nx - 1) return;
     const long id = ((long)i * a.ny + j) * a.nz + k;
     const long sI = a.sI, sJ = a.sJ;
-    const R ex = ld<T, R>(a.Ex, id, a.fsc[0]), ex_j = ld<T, R>(a.Ex, id + sJ, a.fsc[0]), ex_k = ld<T, R>(a.Ex, id + 1, a.fsc[0]);
-    const R ey = ld<T, R>(a.Ey, id, a.fsc[1]), ey_i = ld<T, R>(a.Ey, id + sI, a.fsc[1]), ey_k = ld<T, R>(a.Ey, id + 1, a.fsc[1]);
-    const R ez = ld<T, R>(a.Ez, id, a.fsc[2]), ez_i = ld<T, R>(a.Ez, id + sI, a.fsc[2]), ez_j = ld<T, R>(a.Ez, id + sJ, a.fsc[2]);
+    const R ex = ld<T, R>(a.Ex, id, a.sc[0]), ex_j = ld<T, R>(a.Ex, id + sJ, a.sc[0]), ex_k = ld<T, R>(a.Ex, id + 1, a.sc[0]);
+    const R ey = ld<T, R>(a.Ey, id, a.sc[1]), ey_i = ld<T, R>(a.Ey, id + sI, a.sc[1]), ey_k = ld<T, R>(a.Ey, id + 1, a.sc[1]);
+    const R ez = ld<T, R>(a.Ez, id, a.sc[2]), ez_i = ld<T, R>(a.Ez, id + sI, a.sc[2]), ez_j = ld<T, R>(a.Ez, id + sJ, a.sc[2]);
     const R idx = a.idx[i], idy = a.idy[j], idz = a.idz[k];
     const R dEz_y = (ez_j - ez) * idy, dEy_z = (ey_k - ey) * idz;
     const R dEx_z = (ex_k - ex) * idz, dEz_x = (ez_i - ez) * idx;
@@ -164,26 +165,26 @@ __global__ void __launch_bounds__(256) k_update_H(StepArgs<T, R> a) {
         aHy = a.lut[8 * a.nmat + m]; bHy = a.lut[9 * a.nmat + m];
         aHz = a.lut[10 * a.nmat + m]; bHz = a.lut[11 * a.nmat + m];
     }
-    R hx = aHx * ld<T, R>(a.Hx, id, a.fsc[3]) - bHx * cx;
-    R hy = aHy * ld<T, R>(a.Hy, id, a.fsc[4]) - bHy * cy;
-    R hz = aHz * ld<T, R>(a.Hz, id, a.fsc[5]) - bHz * cz;
+    R hx = aHx * ld<T, R>(a.Hx, id, a.sc[3]) - bHx * cx;
+    R hy = aHy * ld<T, R>(a.Hy, id, a.sc[4]) - bHy * cy;
+    R hz = aHz * ld<T, R>(a.Hz, id, a.sc[5]) - bHz * cz;
     if (a.cpml) {
         const int np = a.np;
         const bool px = in_pml(i, a.nx, np), py = in_pml(j, a.ny, np), pz = in_pml(k, a.nz, np);
         if (px | py | pz) {
             const R* CX = a.cpx; const R* CY = a.cpy; const R* CZ = a.cpz;
             const int nx = a.nx, ny = a.ny, nz = a.nz;
-            if (py) hx -= bHx * cpml_term<T, R>(a.psi[6], psi_y(i, j, k, nx, ny, nz, np), dEz_y, CY[j], CY[3 * ny + j], CY[4 * ny + j], a.psc[6], a.ipsc[6]);
-            if (pz) hx -= bHx * (-cpml_term<T, R>(a.psi[7], psi_z(i, j, k, nx, ny, nz, np), dEy_z, CZ[k], CZ[3 * nz + k], CZ[4 * nz + k], a.psc[7], a.ipsc[7]));
-            if (pz) hy -= bHy * cpml_term<T, R>(a.psi[8], psi_z(i, j, k, nx, ny, nz, np), dEx_z, CZ[k], CZ[3 * nz + k], CZ[4 * nz + k], a.psc[8], a.ipsc[8]);
-            if (px) hy -= bHy * (-cpml_term<T, R>(a.psi[9], psi_x(i, j, k, nx, ny, nz, np), dEz_x, CX[i], CX[3 * nx + i], CX[4 * nx + i], a.psc[9], a.ipsc[9]));
-            if (px) hz -= bHz * cpml_term<T, R>(a.psi[10], psi_x(i, j, k, nx, ny, nz, np), dEy_x, CX[i], CX[3 * nx + i], CX[4 * nx + i], a.psc[10], a.ipsc[10]);
-            if (py) hz -= bHz * (-cpml_term<T, R>(a.psi[11], psi_y(i, j, k, nx, ny, nz, np), dEx_y, CY[j], CY[3 * ny + j], CY[4 * ny + j], a.psc[11], a.ipsc[11]));
+            if (py) hx -= bHx * cpml_term<T, R>(a.psi[6], psi_y(i, j, k, nx, ny, nz, np), dEz_y, CY[j], CY[3 * ny + j], CY[4 * ny + j], a.sc[12], a.sc[30]);
+            if (pz) hx -= bHx * (-cpml_term<T, R>(a.psi[7], psi_z(i, j, k, nx, ny, nz, np), dEy_z, CZ[k], CZ[3 * nz + k], CZ[4 * nz + k], a.sc[13], a.sc[31]));
+            if (pz) hy -= bHy * cpml_term<T, R>(a.psi[8], psi_z(i, j, k, nx, ny, nz, np), dEx_z, CZ[k], CZ[3 * nz + k], CZ[4 * nz + k], a.sc[14], a.sc[32]);
+            if (px) hy -= bHy * (-cpml_term<T, R>(a.psi[9], psi_x(i, j, k, nx, ny, nz, np), dEz_x, CX[i], CX[3 * nx + i], CX[4 * nx + i], a.sc[15], a.sc[33]));
+            if (px) hz -= bHz * cpml_term<T, R>(a.psi[10], psi_x(i, j, k, nx, ny, nz, np), dEy_x, CX[i], CX[3 * nx + i], CX[4 * nx + i], a.sc[16], a.sc[34]);
+            if (py) hz -= bHz * (-cpml_term<T, R>(a.psi[11], psi_y(i, j, k, nx, ny, nz, np), dEx_y, CY[j], CY[3 * ny + j], CY[4 * ny + j], a.sc[17], a.sc[35]));
         }
     }
-    st<T, R>(a.Hx, id, hx, a.ifsc[3]);
-    st<T, R>(a.Hy, id, hy, a.ifsc[4]);
-    st<T, R>(a.Hz, id, hz, a.ifsc[5]);
+    st<T, R>(a.Hx, id, hx, a.sc[21]);
+    st<T, R>(a.Hy, id, hy, a.sc[22]);
+    st<T, R>(a.Hz, id, hz, a.sc[23]);
 }
 
 // Two-level polarization step (update_polarization_multi_region,
@@ -251,7 +252,7 @@ __global__ void __launch_bounds__(256) k_update_E(StepArgs<T, R> a) {
         if (reg >= 0) {
             const TLSCoef c = a.tls[reg];
             double Nu = a.gNu[g], Ng = a.gNg[g], Pz = a.gPz[g], Pp = a.gPp[g];
-            const double ez = (double)ld<T, R>(a.Ez, id, a.fsc[2]);
+            const double ez = (double)ld<T, R>(a.Ez, id, a.sc[2]);
             (void)tls_polarize(c, Nu, Ng, ez, Pz, Pp);
             tls_populate(c, ez, 0.0, Pz, Pp, Nu, Ng);
             a.gNu[g] = Nu; a.gNg[g] = Ng; a.gPz[g] = Pz; a.gPp[g] = Pp;
@@ -259,9 +260,9 @@ __global__ void __launch_bounds__(256) k_update_E(StepArgs<T, R> a) {
         return;
     }
     const long sI = a.sI, sJ = a.sJ;
-    const R hx = ld<T, R>(a.Hx, id, a.fsc[3]), hx_j = ld<T, R>(a.Hx, id - sJ, a.fsc[3]), hx_k = ld<T, R>(a.Hx, id - 1, a.fsc[3]);
-    const R hy = ld<T, R>(a.Hy, id, a.fsc[4]), hy_i = ld<T, R>(a.Hy, id - sI, a.fsc[4]), hy_k = ld<T, R>(a.Hy, id - 1, a.fsc[4]);
-    const R hz = ld<T, R>(a.Hz, id, a.fsc[5]), hz_i = ld<T, R>(a.Hz, id - sI, a.fsc[5]), hz_j = ld<T, R>(a.Hz, id - sJ, a.fsc[5]);
+    const R hx = ld<T, R>(a.Hx, id, a.sc[3]), hx_j = ld<T, R>(a.Hx, id - sJ, a.sc[3]), hx_k = ld<T, R>(a.Hx, id - 1, a.sc[3]);
+    const R hy = ld<T, R>(a.Hy, id, a.sc[4]), hy_i = ld<T, R>(a.Hy, id - sI, a.sc[4]), hy_k = ld<T, R>(a.Hy, id - 1, a.sc[4]);
+    const R hz = ld<T, R>(a.Hz, id, a.sc[5]), hz_i = ld<T, R>(a.Hz, id - sI, a.sc[5]), hz_j = ld<T, R>(a.Hz, id - sJ, a.sc[5]);
     const R idx = a.idx[i], idy = a.idy[j], idz = a.idz[k];
     const R dHz_y = (hz - hz_j) * idy, dHy_z = (hy - hy_k) * idz;
     const R dHx_z = (hx - hx_k) * idz, dHz_x = (hz - hz_i) * idx;
@@ -277,7 +278,7 @@ __global__ void __launch_bounds__(256) k_update_E(StepArgs<T, R> a) {
     }
     const int m = mat_of<R>(a.matid, a.mid16, id);
     const R bEx = a.lut[1 * a.nmat + m], bEy = a.lut[3 * a.nmat + m], bEz = a.lut[5 * a.nmat + m];
-    const R ex0 = ld<T, R>(a.Ex, id, a.fsc[0]), ey0 = ld<T, R>(a.Ey, id, a.fsc[1]), ez0 = ld<T, R>(a.Ez, id, a.fsc[2]);
+    const R ex0 = ld<T, R>(a.Ex, id, a.sc[0]), ey0 = ld<T, R>(a.Ey, id, a.sc[1]), ez0 = ld<T, R>(a.Ez, id, a.sc[2]);
     double Nu = 0, Ng = 0, Pz = 0, Pp = 0;
     R rz = cz - jz;
     if (reg >= 0) {
@@ -301,21 +302,21 @@ __global__ void __launch_bounds__(256) k_update_E(StepArgs<T, R> a) {
         if (px | py | pz) {
             const R* CX = a.cpx; const R* CY = a.cpy; const R* CZ = a.cpz;
             const int nx = a.nx, ny = a.ny, nz = a.nz;
-            if (py) ex += bEx * cpml_term<T, R>(a.psi[0], psi_y(i, j, k, nx, ny, nz, np), dHz_y, CY[j], CY[1 * ny + j], CY[2 * ny + j], a.psc[0], a.ipsc[0]);
-            if (pz) ex += bEx * (-cpml_term<T, R>(a.psi[1], psi_z(i, j, k, nx, ny, nz, np), dHy_z, CZ[k], CZ[1 * nz + k], CZ[2 * nz + k], a.psc[1], a.ipsc[1]));
-            if (pz) ey += bEy * cpml_term<T, R>(a.psi[2], psi_z(i, j, k, nx, ny, nz, np), dHx_z, CZ[k], CZ[1 * nz + k], CZ[2 * nz + k], a.psc[2], a.ipsc[2]);
-            if (px) ey += bEy * (-cpml_term<T, R>(a.psi[3], psi_x(i, j, k, nx, ny, nz, np), dHz_x, CX[i], CX[1 * nx + i], CX[2 * nx + i], a.psc[3], a.ipsc[3]));
-            if (px) ez += bEz * cpml_term<T, R>(a.psi[4], psi_x(i, j, k, nx, ny, nz, np), dHy_x, CX[i], CX[1 * nx + i], CX[2 * nx + i], a.psc[4], a.ipsc[4]);
-            if (py) ez += bEz * (-cpml_term<T, R>(a.psi[5], psi_y(i, j, k, nx, ny, nz, np), dHx_y, CY[j], CY[1 * ny + j], CY[2 * ny + j], a.psc[5], a.ipsc[5]));
+            if (py) ex += bEx * cpml_term<T, R>(a.psi[0], psi_y(i, j, k, nx, ny, nz, np), dHz_y, CY[j], CY[1 * ny + j], CY[2 * ny + j], a.sc[6], a.sc[24]);
+            if (pz) ex += bEx * (-cpml_term<T, R>(a.psi[1], psi_z(i, j, k, nx, ny, nz, np), dHy_z, CZ[k], CZ[1 * nz + k], CZ[2 * nz + k], a.sc[7], a.sc[25]));
+            if (pz) ey += bEy * cpml_term<T, R>(a.psi[2], psi_z(i, j, k, nx, ny, nz, np), dHx_z, CZ[k], CZ[1 * nz + k], CZ[2 * nz + k], a.sc[8], a.sc[26]);
+            if (px) ey += bEy * (-cpml_term<T, R>(a.psi[3], psi_x(i, j, k, nx, ny, nz, np), dHz_x, CX[i], CX[1 * nx + i], CX[2 * nx + i], a.sc[9], a.sc[27]));
+            if (px) ez += bEz * cpml_term<T, R>(a.psi[4], psi_x(i, j, k, nx, ny, nz, np), dHy_x, CX[i], CX[1 * nx + i], CX[2 * nx + i], a.sc[10], a.sc[28]);
+            if (py) ez += bEz * (-cpml_term<T, R>(a.psi[5], psi_y(i, j, k, nx, ny, nz, np), dHx_y, CY[j], CY[1 * ny + j], CY[2 * ny + j], a.sc[11], a.sc[29]));
         }
     }
-    st<T, R>(a.Ex, id, ex, a.ifsc[0]);
-    st<T, R>(a.Ey, id, ey, a.ifsc[1]);
-    st<T, R>(a.Ez, id, ez, a.ifsc[2]);
+    st<T, R>(a.Ex, id, ex, a.sc[18]);
+    st<T, R>(a.Ey, id, ey, a.sc[19]);
+    st<T, R>(a.Ez, id, ez, a.sc[20]);
     if (reg >= 0) {
         const TLSCoef& c = a.tls[reg];
         // The reference stores Ez and reads it back for the population pass.
-        const double ez_new = stored_value<T, R>(ez, a.fsc[2], a.ifsc[2]);
+        const double ez_new = stored_value<T, R>(ez, a.sc[2], a.sc[20]);
         tls_populate(c, ez_new, (double)ez0, Pz, Pp, Nu, Ng);
         a.gNu[g] = Nu; a.gNg[g] = Ng; a.gPz[g] = Pz; a.gPp[g] = Pp;
     }
@@ -357,10 +358,10 @@ struct ProbeArgs {
 
 template <typename T, typename R>
 __device__ __forceinline__ double field_value(const StepArgs<T, R>& a, int comp, long id) {
-    const double ex = (double)ld<T, R>(a.Ex, id, a.fsc[0]), ey = (double)ld<T, R>(a.Ey, id, a.fsc[1]),
-                 ez = (double)ld<T, R>(a.Ez, id, a.fsc[2]);
-    const double hx = (double)ld<T, R>(a.Hx, id, a.fsc[3]), hy = (double)ld<T, R>(a.Hy, id, a.fsc[4]),
-                 hz = (double)ld<T, R>(a.Hz, id, a.fsc[5]);
+    const double ex = (double)ld<T, R>(a.Ex, id, a.sc[0]), ey = (double)ld<T, R>(a.Ey, id, a.sc[1]),
+                 ez = (double)ld<T, R>(a.Ez, id, a.sc[2]);
+    const double hx = (double)ld<T, R>(a.Hx, id, a.sc[3]), hy = (double)ld<T, R>(a.Hy, id, a.sc[4]),
+                 hz = (double)ld<T, R>(a.Hz, id, a.sc[5]);
     switch (comp) {
     case 0: return ex;
     case 1: return ey;
@@ -439,12 +440,12 @@ __global__ void __launch_bounds__(256) k_monitor(StepArgs<T, R> a, long N, long 
     const long stride = (long)gridDim.x * blockDim.x;
     for (long id = (long)blockIdx.x * blockDim.x + threadIdx.x; id < N; id += stride) {
         unsigned long long v;
-        v = absbits<T, R>(a.Ex, id, a.fsc[0]); mE = v > mE ? v : mE;
-        v = absbits<T, R>(a.Ey, id, a.fsc[1]); mE = v > mE ? v : mE;
-        v = absbits<T, R>(a.Ez, id, a.fsc[2]); mE = v > mE ? v : mE;
-        v = absbits<T, R>(a.Hx, id, a.fsc[3]); mH = v > mH ? v : mH;
-        v = absbits<T, R>(a.Hy, id, a.fsc[4]); mH = v > mH ? v : mH;
-        v = absbits<T, R>(a.Hz, id, a.fsc[5]); mH = v > mH ? v : mH;
+        v = absbits<T, R>(a.Ex, id, a.sc[0]); mE = v > mE ? v : mE;
+        v = absbits<T, R>(a.Ey, id, a.sc[1]); mE = v > mE ? v : mE;
+        v = absbits<T, R>(a.Ez, id, a.sc[2]); mE = v > mE ? v : mE;
+        v = absbits<T, R>(a.Hx, id, a.sc[3]); mH = v > mH ? v : mH;
+        v = absbits<T, R>(a.Hy, id, a.sc[4]); mH = v > mH ? v : mH;
+        v = absbits<T, R>(a.Hz, id, a.sc[5]); mH = v > mH ? v : mH;
     }
     for (long g = (long)blockIdx.x * blockDim.x + threadIdx.x; g < ngain; g += stride) {
         const int r = a.greg[g];
@@ -481,6 +482,61 @@ __global__ void __launch_bounds__(256) k_monitor(StepArgs<T, R> a, long N, long 
         atomicMax(&out->nanN, nN);
         if (pt != 0.0) atomicAdd(&out->pop_total, pt);
         if (pi != 0.0) atomicAdd(&out->pop_init, pi);
+    }
+}
+
+// ---- FP16 storage: adaptive per-component power-of-two scales ----
+// Maxima of |stored value| per array (6 fields, 12 psi slabs), as float bits
+// (non-negative floats order like their bit patterns).
+struct ArrSet {
+    const uint16_t* p[18];
+    long n[18];
+};
+__global__ void __launch_bounds__(256) k_fp16_maxima(ArrSet s, unsigned* maxbits) {
+    for (int c = 0; c < 18; ++c) {
+        if (!s.p[c]) continue;
+        unsigned m = 0;
+        const long stride = (long)gridDim.x * blockDim.x;
+        for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < s.n[c]; i += stride) {
+            const float v = fabsf(__half2float(reinterpret_cast<const __half*>(s.p[c])[i]));
+            const unsigned b = __float_as_uint(v);
+            m = b > m ? b : m;
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            const unsigned o = __shfl_down(m, off);
+            m = o > m ? o : m;
+        }
+        if ((threadIdx.x & 63) == 0 && m) atomicMax(&maxbits[c], m);
+    }
+}
+// Decide new scales: keep the largest stored magnitude near 2^12 (16x
+// headroom below the FP16 maximum, normal range down to 2^-26 of it).  A
+// scale moves only when the maximum leaves [2^6, 2^13]; the stored values are
+// then multiplied by an exact power of two.
+__global__ void k_fp16_plan(float* sc, float* factor, unsigned* maxbits) {
+    const int c = threadIdx.x;
+    if (c >= 18) return;
+    const float m = __uint_as_float(maxbits[c]);
+    maxbits[c] = 0u;
+    float f = 1.0f;
+    if (m > 8192.0f || (m > 0.0f && m < 64.0f)) {
+        int e;
+        frexpf(m, &e);             // m = fr * 2^e, fr in [0.5, 1)
+        const int shift = e - 12;  // new stored max in [2^11, 2^12)
+        f = ldexpf(1.0f, -shift);
+        sc[c] = ldexpf(sc[c], shift);
+        sc[18 + c] = ldexpf(sc[18 + c], -shift);
+    }
+    factor[c] = f;
+}
+__global__ void __launch_bounds__(256) k_fp16_rescale(ArrSet s, const float* factor) {
+    for (int c = 0; c < 18; ++c) {
+        const float f = factor[c];
+        if (f == 1.0f || !s.p[c]) continue;
+        __half* a = reinterpret_cast<__half*>(const_cast<uint16_t*>(s.p[c]));
+        const long stride = (long)gridDim.x * blockDim.x;
+        for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < s.n[c]; i += stride)
+            a[i] = __float2half_rn(__half2float(a[i]) * f);
     }
 }
 

@@ -738,6 +738,16 @@ int fdtd_scene_step(fdtd_scene* sc, int nsteps) {
     return rc;
 }
 
+int fdtd_scene_step_timed(fdtd_scene* sc, int nsteps, double* ms) {
+    if (!sc || nsteps < 0 || !ms) return FDTD_EINVAL;
+    const size_t ns = sc->setup->src.size();
+    sc->vals.resize((size_t)nsteps * std::max<size_t>(ns, 1));
+    if (ns) fdtd_setup_source_values(sc->setup, sc->n, nsteps, sc->vals.data());
+    const int rc = fdtd_step_timed(sc->h, sc->n, nsteps, ns ? sc->vals.data() : nullptr, ms);
+    if (rc == FDTD_OK || rc == FDTD_EUNSTABLE) sc->n += nsteps;
+    return rc;
+}
+
 long fdtd_scene_steps_done(const fdtd_scene* sc) { return sc ? sc->n : -1; }
 
 fdtd_sim* fdtd_scene_handle(fdtd_scene* sc) { return sc ? sc->h : nullptr; }

@@ -29,10 +29,11 @@ for _q, _n in enumerate(PSI_NAMES):
 EXPORTS = [
     "fdtd_create", "fdtd_destroy", "fdtd_last_error", "fdtd_device_bytes", "fdtd_set_sources", "fdtd_add_probe",
     "fdtd_add_dft", "fdtd_set_monitor", "fdtd_step", "fdtd_synchronize", "fdtd_download", "fdtd_upload",
-    "fdtd_read_probe", "fdtd_read_dft", "fdtd_read_check", "fdtd_info",
+    "fdtd_read_probe", "fdtd_read_dft", "fdtd_read_check", "fdtd_info", "fdtd_step_timed",
     "fdtd_setup_build", "fdtd_setup_free", "fdtd_setup_info", "fdtd_setup_dt", "fdtd_setup_array",
     "fdtd_setup_source_values", "fdtd_setup_probe", "fdtd_scene_create", "fdtd_scene_destroy", "fdtd_scene_step",
-    "fdtd_scene_steps_done", "fdtd_scene_handle",
+    "fdtd_scene_steps_done", "fdtd_scene_handle", "fdtd_scene_step_timed", "fdtd_set_fp16_scales",
+    "fdtd_get_fp16_scales",
 ]
 
 
@@ -90,6 +91,10 @@ def load() -> C.CDLL:
         "fdtd_scene_create": (i, [vp, i, l, C.POINTER(vp)]),
         "fdtd_scene_destroy": (i, [vp]),
         "fdtd_scene_step": (i, [vp, i]),
+        "fdtd_scene_step_timed": (i, [vp, i, vp]),
+        "fdtd_step_timed": (i, [vp, l, i, vp, vp]),
+        "fdtd_set_fp16_scales": (i, [vp, vp]),
+        "fdtd_get_fp16_scales": (i, [vp, vp]),
         "fdtd_scene_steps_done": (l, [vp]),
         "fdtd_scene_handle": (vp, [vp]),
     }
@@ -203,6 +208,12 @@ class Scene:
         if check:
             _check(rc)
         return rc
+
+    def step_timed(self, nsteps: int):
+        """Steps and returns (H-kernel ms, E-kernel ms, call ms) from HIP events."""
+        ms = (C.c_double * 3)()
+        _check(self.lib.fdtd_scene_step_timed(self.sc, nsteps, ms))
+        return tuple(ms)
 
     def synchronize(self):
         _check(self.lib.fdtd_synchronize(self.h))

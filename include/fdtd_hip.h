@@ -137,6 +137,14 @@ int fdtd_add_probe(fdtd_handle h, int i, int j, int k, int component, int save_e
  * monitor id.  X(f) = sum_n F(t_n) exp(-i 2 pi f t_n) dt, t_n = (n+1) dt for E,
  * (n+1/2) dt for H components. */
 int fdtd_add_dft(fdtd_handle h, int component, const int box[6], int n_freqs, const double* freqs);
+/* FP16 storage scales (value = stored * scale), 6 fields Ex..Hz then the 12
+ * psi arrays (Ex_y Ex_z Ey_z Ey_x Ez_x Ez_y Hx_y Hx_z Hy_z Hy_x Hz_x Hz_y);
+ * powers of two.  They start from an estimate (E 1, H 2^-8, psi from the
+ * finest spacing) and adapt on the device every 10 steps so the largest
+ * stored magnitude stays within [2^6, 2^13].  No-ops for FP64/FP32 handles.
+ * No counterpart in the reference (FP64 only). */
+int fdtd_set_fp16_scales(fdtd_handle h, const double* scales18);
+int fdtd_get_fp16_scales(fdtd_handle h, double* scales18);
 /* Stability monitor cadence (STABILITY_MONITOR_INTERVAL, default 10; 0 = off). */
 int fdtd_set_monitor(fdtd_handle h, int interval);
 
@@ -150,6 +158,12 @@ int fdtd_set_monitor(fdtd_handle h, int interval);
  * warnings); fdtd_read_check gives the records. */
 int fdtd_step(fdtd_handle h, long n0, int nsteps, const double* src_values);
 int fdtd_synchronize(fdtd_handle h);
+/* fdtd_step that also times, with HIP events on the handle's stream, every
+ * H-kernel and E-kernel launch: ms[0] = sum of H kernel times, ms[1] = sum of
+ * E kernel times, ms[2] = elapsed time of the whole call (all in ms).
+ * Synchronizes.  No counterpart in the reference (its timing is the
+ * per-10-step wall clock of src/3D_FDTD_version008.cpp:351-353). */
+int fdtd_step_timed(fdtd_handle h, long n0, int nsteps, const double* src_values, double* ms);
 
 /* ---- data movement (reference layout, FP64 on the host side) ----
  * which: 0..5 = Ex,Ey,Ez,Hx,Hy,Hz; 6 = Nu, 7 = Ng, 8 = Pz, 9 = Pz_prev

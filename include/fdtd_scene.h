@@ -152,6 +152,8 @@ int  fdtd_scene_create(const fdtd_setup* setup, int precision, long probe_capaci
 int  fdtd_scene_destroy(fdtd_scene* sc);
 /* Advances nsteps steps (source values computed on the host per step). */
 int  fdtd_scene_step(fdtd_scene* sc, int nsteps);
+/* fdtd_scene_step through fdtd_step_timed (per-kernel HIP-event times, ms[3]). */
+int  fdtd_scene_step_timed(fdtd_scene* sc, int nsteps, double* ms);
 long fdtd_scene_steps_done(const fdtd_scene* sc);
 /* Device handle of the scene (fdtd_hip.h entry points). */
 struct fdtd_sim* fdtd_scene_handle(fdtd_scene* sc);

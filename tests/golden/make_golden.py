@@ -1,0 +1,54 @@
+"""Writes the golden fixtures tests/golden/<case>.npz from the compiled
+reference (oracle/_ref/libref.so: the reference's own headers built by
+oracle/Makefile, driven in the reference's serial step order).
+
+Each fixture holds, after the case's step count: the six fields, the TLS
+state (when the case has gain), every probe trace, dt, the grid shape, and the
+stability verdict.  Run from the repository root:
+    make -C oracle && python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, HERE)
+
+import oracle  # noqa: E402
+import cases  # noqa: E402
+
+
+def fixture(name, make):
+    sp = make()
+    sim = oracle.CpuSim(sp, "ref")
+    sim.step(sp.n_steps)
+    out = {"dt": np.array(sim.dt), "shape": np.array(sim.shape), "steps": np.array(sp.n_steps)}
+    for f in cases.FIELDS:
+        out[f] = sim.array(f)
+    if sim.info()[5]:
+        for f in cases.TLS_FIELDS:
+            out[f] = sim.array(f)
+    c = 0
+    for p in range(len(sp.probes)):
+        (ijk, nc) = sim.probe_index(p)
+        for q in range(nc):
+            out[f"probe{c}"] = sim.probe(p, q)
+            out[f"probe{c}_ijk"] = np.array(ijk)
+            c += 1
+    st = sim.stability()
+    out["stable"] = np.array(int(st["stable"]))
+    out["max_E"] = np.array(st["max_E"])
+    path = os.path.join(HERE, f"{name}.npz")
+    np.savez_compressed(path, **out)
+    print(f"{name}: {os.path.getsize(path) / 1024:.0f} KiB, shape {sim.shape}")
+
+
+if __name__ == "__main__":
+    if not oracle.available("ref"):
+        sys.exit("oracle/_ref/libref.so is not built (needs the reference tree; run make -C oracle)")
+    for name, make in cases.CASES.items():
+        fixture(name, make)

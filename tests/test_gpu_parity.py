@@ -86,3 +86,51 @@ def test_fp32_gain(pkg, oracle_mod):
     setup, sc, ref = _run_pair(pkg, oracle_mod, sp, "fp32")
     _compare(setup, sc, ref, 2e-4, extra=("Pz",))
     assert rel_l2(sc.field("Nu"), ref.array("Nu")) <= 1e-9
+
+
+GOLDEN = __import__("os").path.join(__import__("os").path.dirname(__file__), "golden")
+
+
+@pytest.mark.parametrize("name", ["uniform_sphere", "gain_slab", "gain_clamp_dipole", "plane_waves", "pec_cavity",
+                                  "graded_cuboids"])
+def test_fp64_matches_reference_golden(pkg, name):
+    """The device FP64 path against the compiled reference's own outputs."""
+    sys.path.insert(0, GOLDEN)
+    import cases
+
+    sp = cases.CASES[name]()
+    g = np.load(__import__("os").path.join(GOLDEN, f"{name}.npz"))
+    _say(f"[golden] {name} steps={int(g['steps'])}")
+    setup = pkg.Setup(sp)
+    sc = pkg.Scene(setup, "fp64")
+    sc.step(int(g["steps"]))
+    worst = {}
+    for f in cases.FIELDS:
+        worst[f] = rel_l2(sc.field(f), g[f])
+    if "Nu" in g.files:
+        for f in cases.TLS_FIELDS:
+            worst[f] = rel_l2(sc.field(f), g[f])
+    for c in range(setup.n_channels):
+        worst[f"probe{c}"] = rel_l2(sc.probe(c), g[f"probe{c}"])
+    bitwise = all(np.array_equal(sc.field(f), g[f]) for f in cases.FIELDS)
+    _say("  bitwise fields: %s; " % bitwise + " ".join(f"{k}={v:.1e}" for k, v in worst.items()))
+    bad = {k: v for k, v in worst.items() if not v <= 1e-12}
+    assert not bad, bad
+
+
+def test_fp16_storage_adaptive_scales(pkg, oracle_mod):
+    """FP16 storage with device-adapted power-of-two scales stays finite and
+    close to the FP64 oracle on a short run (the accuracy of the precision form
+    at full length is measured by bench/precision_report)."""
+    sp = pkg.scene.gain_slab_case(n_total=40, npml=6, steps=300, fine_len=100e-9, coarse=20e-9, fine=5e-9)
+    setup, sc, ref = _run_pair(pkg, oracle_mod, sp, "fp16")
+    lib = pkg.load()
+    import ctypes as C
+    scales = (C.c_double * 18)()
+    assert lib.fdtd_get_fp16_scales(sc.h, scales) == 0
+    assert all(s > 0 and math.frexp(s)[0] == 0.5 for s in scales)
+    errs = {c: rel_l2(sc.probe(c), ref.probe(c)) for c in range(setup.n_channels)}
+    _say("  fp16 probe rel L2: " + " ".join(f"{c}={v:.2e}" for c, v in errs.items()) + f" scales={list(scales)[:6]}")
+    assert max(errs.values()) < 2e-2
+    for f in FIELDS:
+        assert np.isfinite(sc.field(f)).all()

@@ -1,0 +1,90 @@
+"""The product's host setup (lib3dfdtd_hip.so, fdtd_setup_*: the C++ restatement
+of make_context / CPML profiles / bake / TLS / sources / probes) against the
+oracle's, bitwise.  CPU only: no device is touched."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "golden"))
+import cases  # noqa: E402
+
+SETUP_ARRAYS = ["aEx", "bEx", "aEy", "bEy", "aEz", "bEz", "aHx", "bHx", "aHy", "bHy", "aHz", "bHz",
+                "dx", "dy", "dz", "inv_dx", "inv_dy", "inv_dz", "x_bounds", "y_bounds", "z_bounds",
+                "kx", "ky", "kz", "cpml_bEx", "cpml_cEx", "cpml_bEy", "cpml_cEy", "cpml_bEz", "cpml_cEz",
+                "cpml_bHx", "cpml_cHx", "cpml_bHy", "cpml_cHy", "cpml_bHz", "cpml_cHz",
+                "Ndip", "Ng0", "Nu", "Ng"]
+
+
+def _specs(pkg):
+    out = [(n, f()) for n, f in cases.CASES.items()]
+    out.append(("uniform_sphere_64", pkg.scene.uniform_sphere_case(core=64, npml=10, steps=10)))
+    out.append(("gain_slab_64", pkg.scene.gain_slab_case(n_total=64, npml=8, steps=10, fine=5e-9, coarse=20e-9,
+                                                          fine_len=100e-9, n_cuboids=8, seed=5)))
+    return out
+
+
+@pytest.fixture(scope="module")
+def pairs(pkg, oracle_mod):
+    return [(n, sp, pkg.Setup(sp), oracle_mod.CpuSim(sp, "oracle")) for n, sp in _specs(pkg)]
+
+
+def test_grid_and_dt(pairs):
+    for name, sp, s, o in pairs:
+        assert s.shape == o.shape, name
+        assert s.npml == o.npml, name
+        assert s.dt == o.dt, name
+
+
+def test_setup_arrays_bitwise(pairs):
+    for name, sp, s, o in pairs:
+        for a in SETUP_ARRAYS:
+            if a in ("Ndip", "Ng0", "Nu", "Ng") and not s.tls_on:
+                continue
+            assert np.array_equal(s.array(a), o.array(a)), (name, a)
+        if s.tls_on:
+            assert np.array_equal(s.array("region_map").astype(np.int32), o.region_map()), name
+
+
+def test_tls_coefficients(pairs):
+    for name, sp, s, o in pairs:
+        if not s.tls_on:
+            continue
+        c = s.array("tls_coeffs").reshape(-1, 13)
+        for r in range(s.n_regions):
+            ref = o.region(r)  # lambda0 omega_a gamma tau mu_z N0 hbar kapa pa2 pa3 inv_dt i0 i1 j0 j1 k0 k1
+            assert c[r, 0] == ref[7] and c[r, 1] == ref[8] and c[r, 2] == ref[9] and c[r, 3] == ref[10], name
+            assert c[r, 4] == ref[1] and c[r, 5] == ref[4] and c[r, 6] == ref[5], name
+            assert tuple(c[r, 7:13]) == tuple(ref[11:17]), name
+
+
+def test_source_values_bitwise(pairs):
+    for name, sp, s, o in pairs:
+        vals = s.source_values(0, 60)
+        for q in range(s.n_sources):
+            ref = np.array([o.source(q, n)[14] for n in range(60)])
+            assert np.array_equal(vals[:, q], ref), (name, q)
+        # continuation from a later step gives the same values
+        later = s.source_values(40, 20)
+        assert np.array_equal(later, vals[40:60]), name
+
+
+def test_probe_placement(pairs):
+    for name, sp, s, o in pairs:
+        c = 0
+        for p in range(len(sp.probes)):
+            ijk, nc = o.probe_index(p)
+            for q in range(nc):
+                sijk, comp = s.probe(c)
+                assert sijk == ijk and comp == sp.probes[p].components[q], (name, c)
+                c += 1
+        assert c == s.n_channels
+
+
+def test_invalid_spec_rejected(pkg):
+    sp = pkg.scene.uniform_sphere_case(core=8, npml=6, steps=1)
+    bad = sp.copy(nx_core=0)
+    with pytest.raises(pkg.FdtdError):
+        pkg.Setup(bad)
