@@ -82,8 +82,16 @@ struct fdtd_sim {
     size_t rsz = 8;     // bytes per compute value
     std::vector<void*> allocs;
     size_t bytes = 0;
-    void* F[6] = {};
-    void* psi[12] = {};
+    // Fields and psi_H are double-buffered: the fused step reads buffer `cur`
+    // and writes buffer cur^1; the two-kernel step updates buffer `cur` in
+    // place.  psi_E lives in buffer 0 only (owner-only, in place).
+    void* F2[2][6] = {};
+    void* psi2[2][12] = {};
+    void* F[6] = {};    // = F2[cur]
+    void* psi[12] = {}; // = psi2[cur] (psi_E always psi2[0])
+    int cur = 0;
+    int mode = 1;       // 0: H and E kernels, 1: fused step kernel
+    int tj = 4, ichunk = 32;
     long psi_len[12] = {};
     void* inv[3] = {};
     void* cp[3] = {};
@@ -162,10 +170,12 @@ StepArgs<T, R> make_args(const fdtd_sim* s) {
     a.sJ = s->nz;
     a.Ex = (T*)s->F[0]; a.Ey = (T*)s->F[1]; a.Ez = (T*)s->F[2];
     a.Hx = (T*)s->F[3]; a.Hy = (T*)s->F[4]; a.Hz = (T*)s->F[5];
+    a.Exw = a.Ex; a.Eyw = a.Ey; a.Ezw = a.Ez; a.Hxw = a.Hx; a.Hyw = a.Hy; a.Hzw = a.Hz;
+    a.pec = s->bc_type == 0;
     a.idx = (const R*)s->inv[0]; a.idy = (const R*)s->inv[1]; a.idz = (const R*)s->inv[2];
     a.cpml = s->bc_type == 1 && s->np > 0;
     a.cpx = (const R*)s->cp[0]; a.cpy = (const R*)s->cp[1]; a.cpz = (const R*)s->cp[2];
-    for (int q = 0; q < 12; ++q) a.psi[q] = (T*)s->psi[q];
+    for (int q = 0; q < 12; ++q) a.psi[q] = a.psiw[q] = (T*)s->psi[q];
     a.sc = s->d_sc;
     a.matid = s->matid;
     a.mid16 = s->mid16;
@@ -346,22 +356,54 @@ int setup_gain(fdtd_sim* s, const fdtd_tls_desc* t) {
     return s->upload(&s->d_tls, s->h_tls.data(), s->h_tls.size());
 }
 
+template <typename T, typename R, int TJ>
+void launch_fused(fdtd_sim* s, StepArgs<T, R>& a) {
+    const int nkt = (s->nz + 63) / 64;
+    const int njt = (s->ny + TJ - 1) / TJ;
+    const int nch = (s->nx + s->ichunk - 1) / s->ichunk;
+    const int nwg = nkt * njt * nch;
+    hipLaunchKernelGGL((fdtdk::k_step_fused<T, R, TJ>), dim3((unsigned)nwg), dim3(64 * (TJ + 2)), 0, s->stream, a,
+                       s->ichunk, nkt, njt, nwg);
+}
+
+void set_cur(fdtd_sim* s, int c) {
+    s->cur = c;
+    for (int q = 0; q < 6; ++q) s->F[q] = s->F2[c][q];
+    for (int q = 0; q < 12; ++q) s->psi[q] = (q < 6) ? s->psi2[0][q] : s->psi2[c][q];
+}
+
 template <typename T, typename R>
 int launch_step(fdtd_sim* s, long n, int sidx, hipEvent_t* ev) {
     StepArgs<T, R> a = make_args<T, R>(s);
     a.srcv = s->d_srcv + (long)sidx * s->nsrc;
-    const dim3 blk(64, 4, 1);
-    const dim3 grd((unsigned)((s->nz + 63) / 64), (unsigned)((s->ny + 3) / 4), (unsigned)s->nx);
-    if (ev) (void)hipEventRecord(ev[0], s->stream);
-    hipLaunchKernelGGL((fdtdk::k_update_H<T, R>), grd, blk, 0, s->stream, a);
-    if (ev) (void)hipEventRecord(ev[1], s->stream);
-    if (ev) (void)hipEventRecord(ev[2], s->stream);
-    hipLaunchKernelGGL((fdtdk::k_update_E<T, R>), grd, blk, 0, s->stream, a);
-    if (ev) (void)hipEventRecord(ev[3], s->stream);
-    if (s->bc_type == 0) {
-        const long faces = 2L * ((long)s->ny * s->nz + (long)s->nx * s->nz + (long)s->nx * s->ny);
-        hipLaunchKernelGGL((fdtdk::k_pec_faces<T>), dim3((unsigned)((faces + 255) / 256)), dim3(256), 0, s->stream,
-                           a.Ex, a.Ey, a.Ez, s->nx, s->ny, s->nz);
+    if (s->mode == 1) {
+        const int o = s->cur ^ 1;
+        a.Exw = (T*)s->F2[o][0]; a.Eyw = (T*)s->F2[o][1]; a.Ezw = (T*)s->F2[o][2];
+        a.Hxw = (T*)s->F2[o][3]; a.Hyw = (T*)s->F2[o][4]; a.Hzw = (T*)s->F2[o][5];
+        for (int q = 6; q < 12; ++q) a.psiw[q] = (T*)s->psi2[o][q];
+        if (ev) (void)hipEventRecord(ev[0], s->stream);
+        switch (s->tj) {
+        case 2: launch_fused<T, R, 2>(s, a); break;
+        case 6: launch_fused<T, R, 6>(s, a); break;
+        case 8: launch_fused<T, R, 8>(s, a); break;
+        case 14: launch_fused<T, R, 14>(s, a); break;
+        default: launch_fused<T, R, 4>(s, a); break;
+        }
+        if (ev) (void)hipEventRecord(ev[1], s->stream);
+        if (ev) (void)hipEventRecord(ev[2], s->stream);
+        if (ev) (void)hipEventRecord(ev[3], s->stream);
+        set_cur(s, o);
+        a = make_args<T, R>(s);
+        a.srcv = s->d_srcv + (long)sidx * s->nsrc;
+    } else {
+        const dim3 blk(64, 4, 1);
+        const dim3 grd((unsigned)((s->nz + 63) / 64), (unsigned)((s->ny + 3) / 4), (unsigned)s->nx);
+        if (ev) (void)hipEventRecord(ev[0], s->stream);
+        hipLaunchKernelGGL((fdtdk::k_update_H<T, R>), grd, blk, 0, s->stream, a);
+        if (ev) (void)hipEventRecord(ev[1], s->stream);
+        if (ev) (void)hipEventRecord(ev[2], s->stream);
+        hipLaunchKernelGGL((fdtdk::k_update_E<T, R>), grd, blk, 0, s->stream, a);
+        if (ev) (void)hipEventRecord(ev[3], s->stream);
     }
     // probes: PointFieldDetector records when n % save_every == 0
     fdtdk::ProbeArgs pa{};
@@ -400,7 +442,8 @@ int launch_step(fdtd_sim* s, long n, int sidx, hipEvent_t* ev) {
 template <typename T, typename R>
 void launch_monitor(fdtd_sim* s, MonOut* slot) {
     StepArgs<T, R> a = make_args<T, R>(s);
-    hipLaunchKernelGGL((fdtdk::k_monitor<T, R>), dim3(1024), dim3(256), 0, s->stream, a, s->N, s->gain ? s->gbox : 0L, slot);
+    const long nb = std::min<long>(std::max<long>((s->N + 1023) / 1024, 1), 16384);
+    hipLaunchKernelGGL((fdtdk::k_monitor<T, R>), dim3((unsigned)nb), dim3(256), 0, s->stream, a, s->N, s->gain ? s->gbox : 0L, slot);
 }
 
 int dispatch_step(fdtd_sim* s, long n, int sidx, hipEvent_t* ev) {
@@ -432,6 +475,13 @@ void fp16_maintain(fdtd_sim* s) {
     hipLaunchKernelGGL(fdtdk::k_fp16_maxima, dim3(512), dim3(256), 0, s->stream, a, s->d_maxbits);
     hipLaunchKernelGGL(fdtdk::k_fp16_plan, dim3(1), dim3(64), 0, s->stream, s->d_sc, s->d_factor, s->d_maxbits);
     hipLaunchKernelGGL(fdtdk::k_fp16_rescale, dim3(512), dim3(256), 0, s->stream, a, (const float*)s->d_factor);
+    fdtdk::ArrSet o{};  // the other buffer holds the never-updated cells too
+    for (int q = 0; q < 6; ++q) { o.p[q] = (const uint16_t*)s->F2[s->cur ^ 1][q]; o.n[q] = s->N; }
+    for (int q = 6; q < 12; ++q) {
+        o.p[6 + q] = (const uint16_t*)s->psi2[s->cur ^ 1][q];
+        o.n[6 + q] = o.p[6 + q] ? s->psi_len[q] : 0;
+    }
+    hipLaunchKernelGGL(fdtdk::k_fp16_rescale, dim3(512), dim3(256), 0, s->stream, o, (const float*)s->d_factor);
 }
 
 int read_scales(fdtd_sim* s, float* out36) {
@@ -569,19 +619,24 @@ int fdtd_create(const fdtd_grid_desc* g, const fdtd_material_desc* m, const fdtd
         delete s;
         return fail(FDTD_EDEVICE, "hipStreamCreate failed");
     }
-    for (int q = 0; q < 6 && !rc; ++q) rc = s->alloc(&s->F[q], (size_t)s->N * s->esz);
+    s->mode = 1;
+    for (int b = 0; b < 2; ++b)
+        for (int q = 0; q < 6 && !rc; ++q) rc = s->alloc(&s->F2[b][q], (size_t)s->N * s->esz);
     if (!rc && s->bc_type == 1) {
         const long lx = 2L * s->np * s->ny * s->nz, ly = (long)s->nx * 2 * s->np * s->nz, lz = (long)s->nx * s->ny * 2 * s->np;
         static const int axis[12] = {1, 2, 2, 0, 0, 1, 1, 2, 2, 0, 0, 1};
         for (int q = 0; q < 12 && !rc; ++q) {
             s->psi_len[q] = axis[q] == 0 ? lx : (axis[q] == 1 ? ly : lz);
-            rc = s->alloc(&s->psi[q], (size_t)s->psi_len[q] * s->esz);
+            for (int b = 0; b < (q < 6 ? 1 : 2) && !rc; ++b) rc = s->alloc(&s->psi2[b][q], (size_t)s->psi_len[q] * s->esz);
         }
     }
     if (!rc) {
-        for (int q = 0; q < 6; ++q) hipMemsetAsync(s->F[q], 0, (size_t)s->N * s->esz, s->stream);
-        for (int q = 0; q < 12; ++q)
-            if (s->psi[q]) hipMemsetAsync(s->psi[q], 0, (size_t)s->psi_len[q] * s->esz, s->stream);
+        for (int b = 0; b < 2; ++b) {
+            for (int q = 0; q < 6; ++q) hipMemsetAsync(s->F2[b][q], 0, (size_t)s->N * s->esz, s->stream);
+            for (int q = 0; q < 12; ++q)
+                if (s->psi2[b][q]) hipMemsetAsync(s->psi2[b][q], 0, (size_t)s->psi_len[q] * s->esz, s->stream);
+        }
+        set_cur(s, 0);
     }
     if (!rc) rc = precision == FDTD_FP64 ? setup_coeffs<double>(s, g, m, bc) : setup_coeffs<float>(s, g, m, bc);
     if (!rc) rc = setup_gain(s, tls);
@@ -696,6 +751,18 @@ int fdtd_get_fp16_scales(fdtd_handle h, double* scales18) {
     float sc[36];
     if (int rc = read_scales(h, sc)) return rc;
     for (int q = 0; q < 18; ++q) scales18[q] = sc[q];
+    return FDTD_OK;
+}
+
+int fdtd_set_kernel_mode(fdtd_handle h, int mode, int rows_per_tile, int planes_per_chunk) {
+    if (!h || mode < 0 || mode > 1) return fail(FDTD_EINVAL, "mode must be 0 (two kernels) or 1 (fused)");
+    if (rows_per_tile != 0 && rows_per_tile != 2 && rows_per_tile != 4 && rows_per_tile != 6 && rows_per_tile != 8 &&
+        rows_per_tile != 14)
+        return fail(FDTD_EINVAL, "rows_per_tile must be 2, 4, 6, 8 or 14");
+    if (planes_per_chunk < 0) return fail(FDTD_EINVAL, "planes_per_chunk must be >= 0");
+    h->mode = mode;
+    if (rows_per_tile) h->tj = rows_per_tile;
+    if (planes_per_chunk) h->ichunk = planes_per_chunk;
     return FDTD_OK;
 }
 
@@ -854,13 +921,15 @@ int fdtd_upload(fdtd_handle h, int which, const double* in) {
             if (m > 0) {
                 int e;
                 std::frexp(m, &e);
-                sc[which] = std::ldexp(1.0f, e - 12);
-                sc[18 + which] = std::ldexp(1.0f, 12 - e);
+                sc[which] = std::ldexp(1.0f, e - 8);
+                sc[18 + which] = std::ldexp(1.0f, 8 - e);
                 if (int rc = write_scales(h, sc)) return rc;
             }
-            return upload_any(h, h->F[which], in, (size_t)h->N, sc[which]);
+            int rc = upload_any(h, h->F2[0][which], in, (size_t)h->N, sc[which]);
+            return rc ? rc : upload_any(h, h->F2[1][which], in, (size_t)h->N, sc[which]);
         }
-        return upload_any(h, h->F[which], in, (size_t)h->N, 1.0);
+        int rc = upload_any(h, h->F2[0][which], in, (size_t)h->N, 1.0);
+        return rc ? rc : upload_any(h, h->F2[1][which], in, (size_t)h->N, 1.0);
     }
     if (which >= 6 && which < 10) {
         if (!h->gain) return FDTD_OK;

@@ -16,8 +16,9 @@
 // corrections per cell), fdtd_update_E_with_gain and the two TLS passes
 // (src/two_level_system.hpp:299-342, 574-666).  Fusing the passes into one
 // thread per cell only removes memory round trips; each value goes through
-// the same operations in the same order (HIP contracts a*b+c into FMA, which
-// moves FP64 results by ~1e-16 relative per operation).
+// the same operations in the same order.  The library is compiled with
+// -ffp-contract=off, so every FP64 operation is individually rounded as on
+// the host.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <hip/hip_fp16.h>
@@ -70,13 +71,16 @@ struct StepArgs {
     int nx, ny, nz, np;
     long sI;
     int sJ;
-    T *Ex, *Ey, *Ez, *Hx, *Hy, *Hz;
+    T *Ex, *Ey, *Ez, *Hx, *Hy, *Hz;        // read buffers (E^n, H^{n-1/2})
+    T *Exw, *Eyw, *Ezw, *Hxw, *Hyw, *Hzw;  // write buffers (the same arrays when updating in place)
+    int pec;
     const R *idx, *idy, *idz;
     // CPML 1D profiles, per axis [5][n]: kappa, bE, cE, bH, cH
     const R *cpx, *cpy, *cpz;
     int cpml;
     // psi slabs: 0 Ex_y 1 Ex_z 2 Ey_z 3 Ey_x 4 Ez_x 5 Ez_y 6 Hx_y 7 Hx_z 8 Hy_z 9 Hy_x 10 Hz_x 11 Hz_y
     T* psi[12];
+    T* psiw[12];  // psi_H write buffers (6..11); psi_E (0..5) is updated in place
     // FP16 power-of-two scales (device memory, value = stored * scale):
     // sc[0..5] fields, sc[6..17] psi, sc[18..35] their inverses.
     const float* sc;
@@ -124,27 +128,34 @@ __device__ __forceinline__ int mat_of(const void* m, int mid16, long id) {
     return mid16 ? (int)reinterpret_cast<const uint16_t*>(m)[id] : (int)reinterpret_cast<const uint8_t*>(m)[id];
 }
 
-// One CPML correction: psi = b*psi + c*d; field -= / += coef * (+-(corr - d)).
-// Returns the correction term (corr - d) with corr = d/kappa + psi_new.
+// One CPML correction (src/boundary.hpp:182-223, 257-298):
+//   psi = b*psi + c*d;  corr = d/kappa + psi;  returns (corr - d).
+// psi is read from `pr` and, when `wr` is non-null, stored to `pw`.
 template <typename T, typename R>
-__device__ __forceinline__ R cpml_term(T* psi, long pi, R d, R kap, R b, R c, R sc, R isc) {
+__device__ __forceinline__ R cpml_term(const T* pr, T* pw, bool wr, long pi, R d, R kap, R b, R c, R sc, R isc) {
     R corr = d / kap;
-    R p = b * ld<T, R>(psi, pi, sc) + c * d;
-    st<T, R>(psi, pi, p, isc);
+    R p = b * ld<T, R>(pr, pi, sc) + c * d;
+    if (wr) st<T, R>(pw, pi, p, isc);
     corr += p;
     return corr - d;
 }
 
-// H update with the CPML-H correction fused (fdtd_update_H, src/fdtd_stepper.hpp:8-47
-// + CPMLBoundary::apply_after_H, src/boundary.hpp:150-225).  Threads cover
-// i < NxT-1, j < NyT-1, k < NzT-1; blockDim = (64, BY).
+template <typename R>
+struct H3 {
+    R x, y, z;
+};
+
+// H^{n+1/2} of one cell: fdtd_update_H (src/fdtd_stepper.hpp:31-46) followed
+// by the six CPMLBoundary::apply_after_H corrections (src/boundary.hpp:172-224)
+// in the reference's order.  Reads E^n, H^{n-1/2} and psi_H from the read
+// buffers; stores psi_H to the write buffers when `own` (halo recomputation
+// by a neighbouring tile must not store).  Cells the reference never updates
+// (i = NxT-1, j = NyT-1 or k = NzT-1) return their stored value.
 template <typename T, typename R>
-__global__ void __launch_bounds__(256) k_update_H(StepArgs<T, R> a) {
-    const int k = blockIdx.x * 64 + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int i = blockIdx.z;
-    if (k >= a.nz - 1 || j >= a.ny - 1 || i >= a.nx - 1) return;
+__device__ __forceinline__ H3<R> h_new(const StepArgs<T, R>& a, int i, int j, int k, bool own) {
     const long id = ((long)i * a.ny + j) * a.nz + k;
+    if (i >= a.nx - 1 || j >= a.ny - 1 || k >= a.nz - 1)
+        return H3<R>{ld<T, R>(a.Hx, id, a.sc[3]), ld<T, R>(a.Hy, id, a.sc[4]), ld<T, R>(a.Hz, id, a.sc[5])};
     const long sI = a.sI, sJ = a.sJ;
     const R ex = ld<T, R>(a.Ex, id, a.sc[0]), ex_j = ld<T, R>(a.Ex, id + sJ, a.sc[0]), ex_k = ld<T, R>(a.Ex, id + 1, a.sc[0]);
     const R ey = ld<T, R>(a.Ey, id, a.sc[1]), ey_i = ld<T, R>(a.Ey, id + sI, a.sc[1]), ey_k = ld<T, R>(a.Ey, id + 1, a.sc[1]);
@@ -174,17 +185,33 @@ __global__ void __launch_bounds__(256) k_update_H(StepArgs<T, R> a) {
         if (px | py | pz) {
             const R* CX = a.cpx; const R* CY = a.cpy; const R* CZ = a.cpz;
             const int nx = a.nx, ny = a.ny, nz = a.nz;
-            if (py) hx -= bHx * cpml_term<T, R>(a.psi[6], psi_y(i, j, k, nx, ny, nz, np), dEz_y, CY[j], CY[3 * ny + j], CY[4 * ny + j], a.sc[12], a.sc[30]);
-            if (pz) hx -= bHx * (-cpml_term<T, R>(a.psi[7], psi_z(i, j, k, nx, ny, nz, np), dEy_z, CZ[k], CZ[3 * nz + k], CZ[4 * nz + k], a.sc[13], a.sc[31]));
-            if (pz) hy -= bHy * cpml_term<T, R>(a.psi[8], psi_z(i, j, k, nx, ny, nz, np), dEx_z, CZ[k], CZ[3 * nz + k], CZ[4 * nz + k], a.sc[14], a.sc[32]);
-            if (px) hy -= bHy * (-cpml_term<T, R>(a.psi[9], psi_x(i, j, k, nx, ny, nz, np), dEz_x, CX[i], CX[3 * nx + i], CX[4 * nx + i], a.sc[15], a.sc[33]));
-            if (px) hz -= bHz * cpml_term<T, R>(a.psi[10], psi_x(i, j, k, nx, ny, nz, np), dEy_x, CX[i], CX[3 * nx + i], CX[4 * nx + i], a.sc[16], a.sc[34]);
-            if (py) hz -= bHz * (-cpml_term<T, R>(a.psi[11], psi_y(i, j, k, nx, ny, nz, np), dEx_y, CY[j], CY[3 * ny + j], CY[4 * ny + j], a.sc[17], a.sc[35]));
+            if (py) hx -= bHx * cpml_term<T, R>(a.psi[6], a.psiw[6], own, psi_y(i, j, k, nx, ny, nz, np), dEz_y, CY[j], CY[3 * ny + j], CY[4 * ny + j], a.sc[12], a.sc[30]);
+            if (pz) hx -= bHx * (-cpml_term<T, R>(a.psi[7], a.psiw[7], own, psi_z(i, j, k, nx, ny, nz, np), dEy_z, CZ[k], CZ[3 * nz + k], CZ[4 * nz + k], a.sc[13], a.sc[31]));
+            if (pz) hy -= bHy * cpml_term<T, R>(a.psi[8], a.psiw[8], own, psi_z(i, j, k, nx, ny, nz, np), dEx_z, CZ[k], CZ[3 * nz + k], CZ[4 * nz + k], a.sc[14], a.sc[32]);
+            if (px) hy -= bHy * (-cpml_term<T, R>(a.psi[9], a.psiw[9], own, psi_x(i, j, k, nx, ny, nz, np), dEz_x, CX[i], CX[3 * nx + i], CX[4 * nx + i], a.sc[15], a.sc[33]));
+            if (px) hz -= bHz * cpml_term<T, R>(a.psi[10], a.psiw[10], own, psi_x(i, j, k, nx, ny, nz, np), dEy_x, CX[i], CX[3 * nx + i], CX[4 * nx + i], a.sc[16], a.sc[34]);
+            if (py) hz -= bHz * (-cpml_term<T, R>(a.psi[11], a.psiw[11], own, psi_y(i, j, k, nx, ny, nz, np), dEx_y, CY[j], CY[3 * ny + j], CY[4 * ny + j], a.sc[17], a.sc[35]));
         }
     }
-    st<T, R>(a.Hx, id, hx, a.sc[21]);
-    st<T, R>(a.Hy, id, hy, a.sc[22]);
-    st<T, R>(a.Hz, id, hz, a.sc[23]);
+    if (own) {
+        st<T, R>(a.Hxw, id, hx, a.sc[21]);
+        st<T, R>(a.Hyw, id, hy, a.sc[22]);
+        st<T, R>(a.Hzw, id, hz, a.sc[23]);
+    }
+    // the value a following read of the stored H returns
+    return H3<R>{(R)stored_value<T, R>(hx, a.sc[3], a.sc[21]), (R)stored_value<T, R>(hy, a.sc[4], a.sc[22]),
+                 (R)stored_value<T, R>(hz, a.sc[5], a.sc[23])};
+}
+
+// H update kernel of the two-kernel step.  Threads cover the grid with
+// blockDim = (64, BY); each thread owns one cell.
+template <typename T, typename R>
+__global__ void __launch_bounds__(256) k_update_H(StepArgs<T, R> a) {
+    const int k = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int i = blockIdx.z;
+    if (k >= a.nz - 1 || j >= a.ny - 1 || i >= a.nx - 1) return;
+    (void)h_new<T, R>(a, i, j, k, true);
 }
 
 // Two-level polarization step (update_polarization_multi_region,
@@ -225,18 +252,21 @@ __device__ __forceinline__ void tls_populate(const TLSCoef& c, double ez_new, do
     }
 }
 
-// E update with J, the CPML-E correction, the TLS polarization (with Ez^n)
-// and the population update (with the final Ez^{n+1}) fused.  Threads cover
-// the whole grid: cells with i, j or k == 0 are never updated by the
-// reference's E pass (src/fdtd_stepper.hpp:73-75); if they are gain cells
-// their polarization and populations still advance with Ez_old = 0, as in
-// the reference where Ez_old is only written on updated cells.
+// E^{n+1} of one cell with J, the CPML-E correction, the TLS polarization
+// (with Ez^n) and the population update (with the final Ez^{n+1}) fused:
+// fdtd_update_E(_with_gain) (src/fdtd_stepper.hpp:73-88,
+// src/two_level_system.hpp:320-341), CPMLBoundary::apply_after_E
+// (src/boundary.hpp:248-299), update_polarization/populations_multi_region
+// (src/two_level_system.hpp:585-608, 623-665).  The caller passes H^{n+1/2} at
+// the cell and at its -x, -y, -z neighbours.  Cells with i, j or k == 0 are
+// never updated by the reference's E pass; if they are gain cells their
+// polarization and populations still advance with Ez_old = 0, as in the
+// reference where Ez_old is only written on updated cells.  With a PEC
+// boundary the six faces are written as zero (PECBoundary::apply_after_E,
+// src/boundary.hpp:55-74).
 template <typename T, typename R>
-__global__ void __launch_bounds__(256) k_update_E(StepArgs<T, R> a) {
-    const int k = blockIdx.x * 64 + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int i = blockIdx.z;
-    if (k >= a.nz || j >= a.ny || i >= a.nx) return;
+__device__ __forceinline__ void e_new(const StepArgs<T, R>& a, int i, int j, int k, R hx, R hy, R hz, R hx_j, R hz_j,
+                                      R hx_k, R hy_k, R hy_i, R hz_i) {
     const long id = ((long)i * a.ny + j) * a.nz + k;
     int reg = -1;
     long g = 0;
@@ -248,6 +278,7 @@ __global__ void __launch_bounds__(256) k_update_E(StepArgs<T, R> a) {
         }
     }
     const bool interior = (i > 0) & (j > 0) & (k > 0);
+    const bool pec_face = a.pec && (!interior || i == a.nx - 1 || j == a.ny - 1 || k == a.nz - 1);
     if (!interior) {
         if (reg >= 0) {
             const TLSCoef c = a.tls[reg];
@@ -257,12 +288,13 @@ __global__ void __launch_bounds__(256) k_update_E(StepArgs<T, R> a) {
             tls_populate(c, ez, 0.0, Pz, Pp, Nu, Ng);
             a.gNu[g] = Nu; a.gNg[g] = Ng; a.gPz[g] = Pz; a.gPp[g] = Pp;
         }
+        if (pec_face) {
+            st<T, R>(a.Exw, id, (R)0, a.sc[18]);
+            st<T, R>(a.Eyw, id, (R)0, a.sc[19]);
+            st<T, R>(a.Ezw, id, (R)0, a.sc[20]);
+        }
         return;
     }
-    const long sI = a.sI, sJ = a.sJ;
-    const R hx = ld<T, R>(a.Hx, id, a.sc[3]), hx_j = ld<T, R>(a.Hx, id - sJ, a.sc[3]), hx_k = ld<T, R>(a.Hx, id - 1, a.sc[3]);
-    const R hy = ld<T, R>(a.Hy, id, a.sc[4]), hy_i = ld<T, R>(a.Hy, id - sI, a.sc[4]), hy_k = ld<T, R>(a.Hy, id - 1, a.sc[4]);
-    const R hz = ld<T, R>(a.Hz, id, a.sc[5]), hz_i = ld<T, R>(a.Hz, id - sI, a.sc[5]), hz_j = ld<T, R>(a.Hz, id - sJ, a.sc[5]);
     const R idx = a.idx[i], idy = a.idy[j], idz = a.idz[k];
     const R dHz_y = (hz - hz_j) * idy, dHy_z = (hy - hy_k) * idz;
     const R dHx_z = (hx - hx_k) * idz, dHz_x = (hz - hz_i) * idx;
@@ -302,23 +334,121 @@ __global__ void __launch_bounds__(256) k_update_E(StepArgs<T, R> a) {
         if (px | py | pz) {
             const R* CX = a.cpx; const R* CY = a.cpy; const R* CZ = a.cpz;
             const int nx = a.nx, ny = a.ny, nz = a.nz;
-            if (py) ex += bEx * cpml_term<T, R>(a.psi[0], psi_y(i, j, k, nx, ny, nz, np), dHz_y, CY[j], CY[1 * ny + j], CY[2 * ny + j], a.sc[6], a.sc[24]);
-            if (pz) ex += bEx * (-cpml_term<T, R>(a.psi[1], psi_z(i, j, k, nx, ny, nz, np), dHy_z, CZ[k], CZ[1 * nz + k], CZ[2 * nz + k], a.sc[7], a.sc[25]));
-            if (pz) ey += bEy * cpml_term<T, R>(a.psi[2], psi_z(i, j, k, nx, ny, nz, np), dHx_z, CZ[k], CZ[1 * nz + k], CZ[2 * nz + k], a.sc[8], a.sc[26]);
-            if (px) ey += bEy * (-cpml_term<T, R>(a.psi[3], psi_x(i, j, k, nx, ny, nz, np), dHz_x, CX[i], CX[1 * nx + i], CX[2 * nx + i], a.sc[9], a.sc[27]));
-            if (px) ez += bEz * cpml_term<T, R>(a.psi[4], psi_x(i, j, k, nx, ny, nz, np), dHy_x, CX[i], CX[1 * nx + i], CX[2 * nx + i], a.sc[10], a.sc[28]);
-            if (py) ez += bEz * (-cpml_term<T, R>(a.psi[5], psi_y(i, j, k, nx, ny, nz, np), dHx_y, CY[j], CY[1 * ny + j], CY[2 * ny + j], a.sc[11], a.sc[29]));
+            if (py) ex += bEx * cpml_term<T, R>(a.psi[0], a.psi[0], true, psi_y(i, j, k, nx, ny, nz, np), dHz_y, CY[j], CY[1 * ny + j], CY[2 * ny + j], a.sc[6], a.sc[24]);
+            if (pz) ex += bEx * (-cpml_term<T, R>(a.psi[1], a.psi[1], true, psi_z(i, j, k, nx, ny, nz, np), dHy_z, CZ[k], CZ[1 * nz + k], CZ[2 * nz + k], a.sc[7], a.sc[25]));
+            if (pz) ey += bEy * cpml_term<T, R>(a.psi[2], a.psi[2], true, psi_z(i, j, k, nx, ny, nz, np), dHx_z, CZ[k], CZ[1 * nz + k], CZ[2 * nz + k], a.sc[8], a.sc[26]);
+            if (px) ey += bEy * (-cpml_term<T, R>(a.psi[3], a.psi[3], true, psi_x(i, j, k, nx, ny, nz, np), dHz_x, CX[i], CX[1 * nx + i], CX[2 * nx + i], a.sc[9], a.sc[27]));
+            if (px) ez += bEz * cpml_term<T, R>(a.psi[4], a.psi[4], true, psi_x(i, j, k, nx, ny, nz, np), dHy_x, CX[i], CX[1 * nx + i], CX[2 * nx + i], a.sc[10], a.sc[28]);
+            if (py) ez += bEz * (-cpml_term<T, R>(a.psi[5], a.psi[5], true, psi_y(i, j, k, nx, ny, nz, np), dHx_y, CY[j], CY[1 * ny + j], CY[2 * ny + j], a.sc[11], a.sc[29]));
         }
     }
-    st<T, R>(a.Ex, id, ex, a.sc[18]);
-    st<T, R>(a.Ey, id, ey, a.sc[19]);
-    st<T, R>(a.Ez, id, ez, a.sc[20]);
+    if (pec_face) { ex = 0; ey = 0; ez = 0; }
+    st<T, R>(a.Exw, id, ex, a.sc[18]);
+    st<T, R>(a.Eyw, id, ey, a.sc[19]);
+    st<T, R>(a.Ezw, id, ez, a.sc[20]);
     if (reg >= 0) {
         const TLSCoef& c = a.tls[reg];
         // The reference stores Ez and reads it back for the population pass.
         const double ez_new = stored_value<T, R>(ez, a.sc[2], a.sc[20]);
         tls_populate(c, ez_new, (double)ez0, Pz, Pp, Nu, Ng);
         a.gNu[g] = Nu; a.gNg[g] = Ng; a.gPz[g] = Pz; a.gPp[g] = Pp;
+    }
+}
+
+// E update kernel of the two-kernel step: H^{n+1/2} neighbours from memory.
+template <typename T, typename R>
+__global__ void __launch_bounds__(256) k_update_E(StepArgs<T, R> a) {
+    const int k = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int i = blockIdx.z;
+    if (k >= a.nz || j >= a.ny || i >= a.nx) return;
+    const long id = ((long)i * a.ny + j) * a.nz + k;
+    const bool interior = (i > 0) & (j > 0) & (k > 0);
+    R hx = 0, hy = 0, hz = 0, hx_j = 0, hz_j = 0, hx_k = 0, hy_k = 0, hy_i = 0, hz_i = 0;
+    if (interior) {
+        const long sI = a.sI, sJ = a.sJ;
+        hx = ld<T, R>(a.Hx, id, a.sc[3]); hx_j = ld<T, R>(a.Hx, id - sJ, a.sc[3]); hx_k = ld<T, R>(a.Hx, id - 1, a.sc[3]);
+        hy = ld<T, R>(a.Hy, id, a.sc[4]); hy_i = ld<T, R>(a.Hy, id - sI, a.sc[4]); hy_k = ld<T, R>(a.Hy, id - 1, a.sc[4]);
+        hz = ld<T, R>(a.Hz, id, a.sc[5]); hz_i = ld<T, R>(a.Hz, id - sI, a.sc[5]); hz_j = ld<T, R>(a.Hz, id - sJ, a.sc[5]);
+    }
+    e_new<T, R>(a, i, j, k, hx, hy, hz, hx_j, hz_j, hx_k, hy_k, hy_i, hz_i);
+}
+
+// Fused step kernel: one launch advances H and E of the whole grid.
+// A work-group owns a tile of 64 k-values (one per lane) by TJ rows in j and
+// marches along i over a chunk of planes.  It has TJ+2 wavefronts: wavefront
+// 1..TJ own one row each; wavefront 0 recomputes H of the row below the tile
+// (j0-1) and wavefront TJ+1 recomputes H of the column left of the tile
+// (k0-1) for the TJ rows.  Per plane every wavefront computes its H at once
+// (all loads of a plane in flight together), the halo values go through LDS,
+// one barrier, then the owning wavefronts update E with H(i-1) carried in
+// registers from the previous plane and H(k-1) from the neighbouring lane.
+// Reads come from buffer A (E^n, H^{n-1/2}, psi_H) and writes go to buffer B,
+// so no tile ever reads a neighbour's half-updated state; psi_E and the gain
+// state are touched only by their owning thread and stay in place.  A chunk
+// starting at i0 > 0 first recomputes H(i0-1) of its cells.  Work-group ids
+// are remapped so that each XCD (ids are dealt round-robin over 8) receives a
+// contiguous range of tiles and keeps the halo rows in its own L2.
+template <typename T, typename R, int TJ>
+__global__ void __launch_bounds__(64 * (TJ + 2)) k_step_fused(StepArgs<T, R> a, int ichunk, int nkt, int njt, int nwg) {
+    __shared__ R lj[2][TJ][2][64];  // H(x, z) of rows j0-1 .. j0+TJ-2
+    __shared__ R lk[2][TJ][2];      // H(x, y) at k0-1 of rows j0 .. j0+TJ-1
+    const int b = blockIdx.x;
+    const int bb = (nwg % 8 == 0) ? (b % 8) * (nwg / 8) + b / 8 : b;
+    const int kt = bb % nkt;
+    const int rest = bb / nkt;
+    const int jt = rest % njt;
+    const int c = rest / njt;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int k0 = kt * 64, j0 = jt * TJ;
+    const int i0 = c * ichunk;
+    const int i1 = min(i0 + ichunk, a.nx);
+    int j, k;
+    bool act, own;
+    if (w <= TJ) {
+        j = j0 + w - 1;
+        k = k0 + lane;
+        act = (j >= 0) & (j < a.ny) & (k < a.nz);
+        own = w >= 1;
+    } else {
+        j = j0 + lane;
+        k = k0 - 1;
+        act = (lane < TJ) & (j < a.ny) & (k >= 0);
+        own = false;
+    }
+    const bool erow = own && act;
+    R hyi = 0, hzi = 0;
+    if (erow && i0 > 0) {
+        const H3<R> h = h_new<T, R>(a, i0 - 1, j, k, false);
+        hyi = h.y;
+        hzi = h.z;
+    }
+    int par = 0;
+    for (int i = i0; i < i1; ++i) {
+        H3<R> h{0, 0, 0};
+        if (act) h = h_new<T, R>(a, i, j, k, own);
+        if (w < TJ) {
+            lj[par][w][0][lane] = h.x;
+            lj[par][w][1][lane] = h.z;
+        }
+        if (w == TJ + 1 && lane < TJ) {
+            lk[par][lane][0] = h.x;
+            lk[par][lane][1] = h.y;
+        }
+        R hxk = __shfl_up(h.x, 1), hyk = __shfl_up(h.y, 1);
+        __syncthreads();
+        if (erow) {
+            const int r = w - 1;
+            if (lane == 0) {
+                hxk = lk[par][r][0];
+                hyk = lk[par][r][1];
+            }
+            const R hxj = lj[par][r][0][lane], hzj = lj[par][r][1][lane];
+            e_new<T, R>(a, i, j, k, h.x, h.y, h.z, hxj, hzj, hxk, hyk, hyi, hzi);
+            hyi = h.y;
+            hzi = h.z;
+        }
+        par ^= 1;
     }
 }
 
@@ -509,20 +639,21 @@ __global__ void __launch_bounds__(256) k_fp16_maxima(ArrSet s, unsigned* maxbits
         if ((threadIdx.x & 63) == 0 && m) atomicMax(&maxbits[c], m);
     }
 }
-// Decide new scales: keep the largest stored magnitude near 2^12 (16x
-// headroom below the FP16 maximum, normal range down to 2^-26 of it).  A
-// scale moves only when the maximum leaves [2^6, 2^13]; the stored values are
-// then multiplied by an exact power of two.
+// Decide new scales: keep the largest stored magnitude near 2^8 (2^8 x
+// headroom below the FP16 maximum for growth between two checks; values down
+// to 2^-22 of the maximum stay normal).  A scale moves only when the maximum
+// leaves [2^2, 2^11]; the stored values are then multiplied by an exact
+// power of two.
 __global__ void k_fp16_plan(float* sc, float* factor, unsigned* maxbits) {
     const int c = threadIdx.x;
     if (c >= 18) return;
     const float m = __uint_as_float(maxbits[c]);
     maxbits[c] = 0u;
     float f = 1.0f;
-    if (m > 8192.0f || (m > 0.0f && m < 64.0f)) {
+    if (m > 2048.0f || (m > 0.0f && m < 4.0f)) {
         int e;
-        frexpf(m, &e);             // m = fr * 2^e, fr in [0.5, 1)
-        const int shift = e - 12;  // new stored max in [2^11, 2^12)
+        frexpf(m, &e);            // m = fr * 2^e, fr in [0.5, 1)
+        const int shift = e - 8;  // new stored max in [2^7, 2^8)
         f = ldexpf(1.0f, -shift);
         sc[c] = ldexpf(sc[c], shift);
         sc[18 + c] = ldexpf(sc[18 + c], -shift);

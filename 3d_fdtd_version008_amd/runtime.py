@@ -33,7 +33,7 @@ EXPORTS = [
     "fdtd_setup_build", "fdtd_setup_free", "fdtd_setup_info", "fdtd_setup_dt", "fdtd_setup_array",
     "fdtd_setup_source_values", "fdtd_setup_probe", "fdtd_scene_create", "fdtd_scene_destroy", "fdtd_scene_step",
     "fdtd_scene_steps_done", "fdtd_scene_handle", "fdtd_scene_step_timed", "fdtd_set_fp16_scales",
-    "fdtd_get_fp16_scales",
+    "fdtd_get_fp16_scales", "fdtd_set_kernel_mode",
 ]
 
 
@@ -94,6 +94,7 @@ def load() -> C.CDLL:
         "fdtd_scene_step_timed": (i, [vp, i, vp]),
         "fdtd_step_timed": (i, [vp, l, i, vp, vp]),
         "fdtd_set_fp16_scales": (i, [vp, vp]),
+        "fdtd_set_kernel_mode": (i, [vp, i, i, i]),
         "fdtd_get_fp16_scales": (i, [vp, vp]),
         "fdtd_scene_steps_done": (l, [vp]),
         "fdtd_scene_handle": (vp, [vp]),
@@ -214,6 +215,10 @@ class Scene:
         ms = (C.c_double * 3)()
         _check(self.lib.fdtd_scene_step_timed(self.sc, nsteps, ms))
         return tuple(ms)
+
+    def set_kernel_mode(self, mode: int, rows_per_tile: int = 0, planes_per_chunk: int = 0):
+        """0: separate H/E kernels; 1: fused step kernel (tile rows, x-chunk)."""
+        _check(self.lib.fdtd_set_kernel_mode(self.h, mode, rows_per_tile, planes_per_chunk))
 
     def synchronize(self):
         _check(self.lib.fdtd_synchronize(self.h))

@@ -382,6 +382,7 @@ def gain_slab_case(n_total: int = 256, npml: int = 12, steps: int = 5000, latera
     probes = [Probe(Lx / 2, Ly / 2, z_mid, (0, 2)), Probe(Lx / 2 + 3 * lateral, Ly / 2, z_mid + 50e-9, (0, 2, 4)),
               Probe(Lx / 2, Ly / 2, src_z - 100e-9, (0,)), Probe(Lx / 3, Ly / 4, slab_z0 - 60e-9, (0, 1))]
     kmon = int(np.searchsorted(zb, pml_z + src_z - 150e-9)) - 1
+    kmon = min(max(kmon, npml), nzt - npml - 1)
     dfts = [DFTMonitor(0, (npml, nxt - npml, npml, nyt - npml, kmon, kmon + 1),
                        (C0 / 0.95e-6, C0 / 1.0e-6, C0 / 1.05e-6))]
     return SceneSpec(

@@ -28,13 +28,15 @@ METRIC = "Gcell-updates/s & % HBM roofline; p50/p99 latency of batched solves"
 
 
 def algorithmic_bytes(spec, setup, prec_bytes: int, mat_bytes: int):
-    """Algorithmic HBM bytes per step of the H kernel and of the E kernel.
+    """Algorithmic HBM bytes per step: of the H kernel, of the E kernel, and
+    of the whole step (the fused kernel's unit).
 
-    Per cell: H kernel reads E, H (6 values) and writes H (3): 9 s; E kernel
-    reads H, E (6), writes E (3) and reads one material id: 9 s + m.  Each
-    CPML slab cell adds, per active axis, 2 psi values read+written (4 s) in
-    each kernel.  Each gain cell adds Pz, Pz_prev, Nu, Ng (FP64) read and
-    written plus a region byte: 65 B, in the E kernel."""
+    Step (per cell-update): read E and H once, write E and H once: 12 s, plus
+    the material id m; each CPML slab cell carries, per active axis, 4 psi
+    values (2 for H, 2 for E) read and written: 8 s; each gain cell carries
+    Pz, Pz_prev, Nu, Ng (FP64) read and written plus its region byte: 65 B.
+    Separate kernels: H reads E, H (6 s) and writes H (3 s); E reads H, E,
+    writes E: 9 s each, with the psi and gain terms split between them."""
     nx, ny, nz = setup.shape
     N = nx * ny * nz
     np_ = setup.npml
@@ -45,7 +47,8 @@ def algorithmic_bytes(spec, setup, prec_bytes: int, mat_bytes: int):
     gain_cells = int((setup.array("region_map").ravel() >= 0).sum()) if setup.tls_on else 0
     h_bytes = 9 * s * N + 4 * s * slab
     e_bytes = (9 * s + mat_bytes) * N + 4 * s * slab + 65 * gain_cells
-    return h_bytes, e_bytes, gain_cells
+    step_bytes = (12 * s + mat_bytes) * N + 8 * s * slab + 65 * gain_cells
+    return h_bytes, e_bytes, step_bytes, gain_cells
 
 
 def cpu_baseline(spec, steps: int):
@@ -74,6 +77,9 @@ def main():
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mode", type=int, default=1, help="0: separate H/E kernels, 1: fused step kernel")
+    ap.add_argument("--tj", type=int, default=0, help="fused tile rows (2,4,6,8,14); 0 = library default")
+    ap.add_argument("--ichunk", type=int, default=0, help="fused x-planes per chunk; 0 = library default")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported; this build measures one GPU (N=1).",
@@ -87,6 +93,7 @@ def main():
     t_setup = time.time() - t0
     sc = pkg.Scene(setup, args.precision, probe_capacity=args.steps + args.warmup + 16)
     info = sc.info()
+    sc.set_kernel_mode(args.mode, args.tj, args.ichunk)
     sc.step(args.warmup)
     sc.synchronize()
     t1 = time.perf_counter()
@@ -96,13 +103,18 @@ def main():
     N = setup.shape[0] * setup.shape[1] * setup.shape[2]
     value = N * args.steps / wall / 1e9
     prec_bytes = {"fp64": 8, "fp32": 4, "fp16": 2}[args.precision]
-    h_bytes, e_bytes, gain_cells = algorithmic_bytes(spec, setup, prec_bytes, info["mat_id_bytes"])
+    h_bytes, e_bytes, step_bytes, gain_cells = algorithmic_bytes(spec, setup, prec_bytes, info["mat_id_bytes"])
     t_h = ms_h / args.steps / 1e3
     t_e = ms_e / args.steps / 1e3
-    if t_e >= t_h:
+    if args.mode == 1:
+        kname, kbytes, kt = "k_step_fused", step_bytes, t_h
+        kernels = {"k_step_fused": round(ms_h / args.steps, 4)}
+    elif t_e >= t_h:
         kname, kbytes, kt = "k_update_E", e_bytes, t_e
     else:
         kname, kbytes, kt = "k_update_H", h_bytes, t_h
+    if args.mode != 1:
+        kernels = {"k_update_H": round(ms_h / args.steps, 4), "k_update_E": round(ms_e / args.steps, 4)}
     achieved = kbytes / kt / 1e9
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "pmc_traffic.json")
@@ -127,12 +139,15 @@ def main():
         "data": "synthetic",
         "config": {"workload": spec.name, "grid": list(setup.shape), "cells": N, "npml": setup.npml,
                    "precision": args.precision, "gain_cells": gain_cells, "materials": info["n_materials"],
-                   "mat_id_bytes": info["mat_id_bytes"], "setup_s": round(t_setup, 2)},
+                   "mat_id_bytes": info["mat_id_bytes"], "setup_s": round(t_setup, 2),
+                   "kernel_mode": "fused" if args.mode == 1 else "h_e_kernels", "tile_rows": args.tj,
+                   "planes_per_chunk": args.ichunk},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes_per_launch": kbytes, "avg_launch_us": round(kt * 1e6, 2)},
-        "kernels_ms_per_step": {"k_update_H": round(ms_h / args.steps, 4), "k_update_E": round(ms_e / args.steps, 4)},
-        "step_bytes_fraction": round((h_bytes + e_bytes) / (wall / args.steps) / 1e9 / HBM_PEAK_GBS, 4),
+        "kernels_ms_per_step": kernels,
+        "step_alg_bytes_per_cell": round(step_bytes / N, 2),
+        "step_roofline_frac": round(step_bytes / (wall / args.steps) / 1e9 / HBM_PEAK_GBS, 4),
     }
     sc.close()
     if not args.no_cpu:

@@ -86,7 +86,10 @@ typedef struct {
     size_t ncell;
 } dft_t;
 
-typedef struct oracle_sim {
+typedef struct oracle_sim oracle_sim;
+void oracle_destroy(oracle_sim* S);
+
+struct oracle_sim {
     size_t NxT, NyT, NzT, N, npml;
     size_t nx_grid, ny_grid, nz_grid; /* spacing-array lengths */
     double dt;
@@ -123,7 +126,7 @@ typedef struct oracle_sim {
     int st_stable;
     long st_fail_step;
     size_t n;          /* steps done */
-} oracle_sim;
+};
 
 static inline size_t idx3(size_t i, size_t j, size_t k, size_t Ny, size_t Nz) {
     return (i * Ny + j) * Nz + k; /* src/global_function.hpp:21-24 */
@@ -1049,6 +1052,15 @@ oracle_sim* oracle_create(const fdtd_scene_spec* sp) {
     }
     S->save_every = sp->save_every > 0 ? (size_t)sp->save_every : 1;
 
+    for (int m = 0; m < sp->n_dfts; ++m) { /* reject monitors outside the grid */
+        const fdtd_dft_def* D = &sp->dfts[m];
+        if (D->i0 < 0 || D->i1 > (int)NxT || D->i0 >= D->i1 || D->j0 < 0 || D->j1 > (int)NyT || D->j0 >= D->j1 ||
+            D->k0 < 0 || D->k1 > (int)NzT || D->k0 >= D->k1 || D->n_freqs < 1 || D->n_freqs > 8 ||
+            D->component < 0 || D->component > 5) {
+            oracle_destroy(S);
+            return NULL;
+        }
+    }
     S->ndft = sp->n_dfts;
     S->dfts = (dft_t*)calloc((size_t)(S->ndft ? S->ndft : 1), sizeof(dft_t));
     for (int m = 0; m < S->ndft; ++m) {

@@ -23,48 +23,85 @@ def _say(msg):
     sys.stdout.flush()
 
 
-def _run_pair(pkg, oracle_mod, sp, precision="fp64", steps=None):
+_ORACLE_CACHE = {}
+
+
+def _run_pair(pkg, oracle_mod, sp, precision="fp64", steps=None, mode=None):
     steps = sp.n_steps if steps is None else steps
-    _say(f"[case] {sp.name} {precision} steps={steps} cells={sp.n_cells()}")
+    _say(f"[case] {sp.name} {precision} mode={mode} steps={steps} cells={sp.n_cells()}")
     setup = pkg.Setup(sp)
     sc = pkg.Scene(setup, precision)
+    if mode is not None:
+        sc.set_kernel_mode(*mode)
     sc.step(steps)
-    ref = oracle_mod.CpuSim(sp, "oracle")
-    ref.step(steps)
+    key = (sp.name, steps)
+    ref = _ORACLE_CACHE.get(key)
+    if ref is None:
+        ref = oracle_mod.CpuSim(sp, "oracle")
+        ref.step(steps)
+        _ORACLE_CACHE.clear()
+        _ORACLE_CACHE[key] = ref
     return setup, sc, ref
 
 
-def _compare(setup, sc, ref, tol, extra=()):
+def _channels(sp):
+    """Probe channel c of the device path -> (probe, component index) of the oracle."""
+    out = []
+    for p, pr in enumerate(sp.probes):
+        for q in range(len(pr.components)):
+            out.append((p, q))
+    return out
+
+
+def _compare(setup, sc, ref, tol, extra=(), vector=False):
+    """Relative L2 per field (vector=True: of the stacked E and H vector fields,
+    so a component that is pure round-off noise by symmetry is judged against
+    the field it belongs to) and per probe channel."""
     worst = {}
-    for name in FIELDS + tuple(extra):
+    if vector:
+        for grp, names in (("E", FIELDS[:3]), ("H", FIELDS[3:])):
+            a = np.stack([sc.field(n) for n in names])
+            b = np.stack([ref.array(n) for n in names])
+            worst[grp] = rel_l2(a, b)
+        names = tuple(extra)
+    else:
+        names = FIELDS + tuple(extra)
+    for name in names:
         a, b = sc.field(name), ref.array(name)
         worst[name] = rel_l2(a, b)
-    for c in range(setup.n_channels):
-        worst[f"probe{c}"] = rel_l2(sc.probe(c), ref.probe(c))
+    for c, (p, q) in enumerate(_channels(setup.spec)):
+        worst[f"probe{c}"] = rel_l2(sc.probe(c), ref.probe(p, q))
     _say("  " + " ".join(f"{k}={v:.2e}" for k, v in worst.items()))
     bad = {k: v for k, v in worst.items() if not v <= tol}
     assert not bad, f"exceeds {tol}: {bad}"
 
 
-def test_uniform_sphere_fp64_full(pkg, oracle_mod):
-    """The plain CPU case at full size: 64^3 core + 10-cell CPML, 1000 steps."""
+MODES = [(0, 0, 0), (1, 2, 16), (1, 4, 32), (1, 14, 7)]
+
+
+@pytest.mark.parametrize("mode", MODES, ids=lambda m: f"mode{m[0]}_tj{m[1]}_ic{m[2]}")
+def test_uniform_sphere_fp64_full(pkg, oracle_mod, mode):
+    """The plain CPU case at full size: 64^3 core + 10-cell CPML, 1000 steps,
+    for the two-kernel step and the fused step at several tile shapes."""
     sp = pkg.scene.uniform_sphere_case(core=64, npml=10, steps=1000)
-    setup, sc, ref = _run_pair(pkg, oracle_mod, sp, "fp64")
+    setup, sc, ref = _run_pair(pkg, oracle_mod, sp, "fp64", mode=mode)
     _compare(setup, sc, ref, 1e-12)
     # probe record count = steps (save_every 1, n = 0 included)
     assert len(sc.probe(0)) == sp.n_steps
 
 
-def test_psi_state_fp64(pkg, oracle_mod):
+@pytest.mark.parametrize("mode", MODES[:2], ids=lambda m: f"mode{m[0]}")
+def test_psi_state_fp64(pkg, oracle_mod, mode):
     sp = pkg.scene.uniform_sphere_case(core=16, npml=6, steps=150)
-    setup, sc, ref = _run_pair(pkg, oracle_mod, sp, "fp64")
+    setup, sc, ref = _run_pair(pkg, oracle_mod, sp, "fp64", mode=mode)
     for name in pkg.runtime.PSI_NAMES:
         assert rel_l2(sc.field(name), ref.array(name)) <= 1e-12, name
 
 
-def test_gain_slab_fp64(pkg, oracle_mod):
+@pytest.mark.parametrize("mode", MODES, ids=lambda m: f"mode{m[0]}_tj{m[1]}_ic{m[2]}")
+def test_gain_slab_fp64(pkg, oracle_mod, mode):
     sp = pkg.scene.gain_slab_case(n_total=40, npml=6, steps=400, fine_len=100e-9, coarse=20e-9, fine=5e-9)
-    setup, sc, ref = _run_pair(pkg, oracle_mod, sp, "fp64")
+    setup, sc, ref = _run_pair(pkg, oracle_mod, sp, "fp64", mode=mode)
     _compare(setup, sc, ref, 1e-12, extra=("Nu", "Ng", "Pz", "Pz_prev"))
 
 
@@ -78,22 +115,23 @@ def test_gain_cuboids_nonuniform_fp64(pkg, oracle_mod):
 def test_fp32_uniform(pkg, oracle_mod):
     sp = pkg.scene.uniform_sphere_case(core=32, npml=8, steps=400)
     setup, sc, ref = _run_pair(pkg, oracle_mod, sp, "fp32")
-    _compare(setup, sc, ref, 2e-4)
+    _compare(setup, sc, ref, 2e-4, vector=True)
 
 
 def test_fp32_gain(pkg, oracle_mod):
     sp = pkg.scene.gain_slab_case(n_total=40, npml=6, steps=400, fine_len=100e-9, coarse=20e-9, fine=5e-9)
     setup, sc, ref = _run_pair(pkg, oracle_mod, sp, "fp32")
-    _compare(setup, sc, ref, 2e-4, extra=("Pz",))
+    _compare(setup, sc, ref, 2e-4, extra=("Pz",), vector=True)
     assert rel_l2(sc.field("Nu"), ref.array("Nu")) <= 1e-9
 
 
 GOLDEN = __import__("os").path.join(__import__("os").path.dirname(__file__), "golden")
 
 
+@pytest.mark.parametrize("mode", [(0, 0, 0), (1, 2, 8)], ids=lambda m: f"mode{m[0]}")
 @pytest.mark.parametrize("name", ["uniform_sphere", "gain_slab", "gain_clamp_dipole", "plane_waves", "pec_cavity",
                                   "graded_cuboids"])
-def test_fp64_matches_reference_golden(pkg, name):
+def test_fp64_matches_reference_golden(pkg, name, mode):
     """The device FP64 path against the compiled reference's own outputs."""
     sys.path.insert(0, GOLDEN)
     import cases
@@ -103,6 +141,7 @@ def test_fp64_matches_reference_golden(pkg, name):
     _say(f"[golden] {name} steps={int(g['steps'])}")
     setup = pkg.Setup(sp)
     sc = pkg.Scene(setup, "fp64")
+    sc.set_kernel_mode(*mode)
     sc.step(int(g["steps"]))
     worst = {}
     for f in cases.FIELDS:
@@ -129,7 +168,7 @@ def test_fp16_storage_adaptive_scales(pkg, oracle_mod):
     scales = (C.c_double * 18)()
     assert lib.fdtd_get_fp16_scales(sc.h, scales) == 0
     assert all(s > 0 and math.frexp(s)[0] == 0.5 for s in scales)
-    errs = {c: rel_l2(sc.probe(c), ref.probe(c)) for c in range(setup.n_channels)}
+    errs = {c: rel_l2(sc.probe(c), ref.probe(p, q)) for c, (p, q) in enumerate(_channels(sp))}
     _say("  fp16 probe rel L2: " + " ".join(f"{c}={v:.2e}" for c, v in errs.items()) + f" scales={list(scales)[:6]}")
     assert max(errs.values()) < 2e-2
     for f in FIELDS:
