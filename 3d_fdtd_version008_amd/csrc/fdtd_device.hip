@@ -442,7 +442,7 @@ int launch_step(fdtd_sim* s, long n, int sidx, hipEvent_t* ev) {
 template <typename T, typename R>
 void launch_monitor(fdtd_sim* s, MonOut* slot) {
     StepArgs<T, R> a = make_args<T, R>(s);
-    const long nb = std::min<long>(std::max<long>((s->N + 1023) / 1024, 1), 16384);
+    const long nb = std::min<long>(std::max<long>((s->N + 2047) / 2048, 1), 2048);
     hipLaunchKernelGGL((fdtdk::k_monitor<T, R>), dim3((unsigned)nb), dim3(256), 0, s->stream, a, s->N, s->gain ? s->gbox : 0L, slot);
 }
 
@@ -641,13 +641,13 @@ int fdtd_create(const fdtd_grid_desc* g, const fdtd_material_desc* m, const fdtd
     if (!rc) rc = precision == FDTD_FP64 ? setup_coeffs<double>(s, g, m, bc) : setup_coeffs<float>(s, g, m, bc);
     if (!rc) rc = setup_gain(s, tls);
     if (!rc) rc = s->alloc(reinterpret_cast<void**>(&s->d_src), sizeof(SourceBox) * 64);
-    if (!rc) rc = s->alloc(reinterpret_cast<void**>(&s->d_sc), 36 * sizeof(float));
+    if (!rc) rc = s->alloc(reinterpret_cast<void**>(&s->d_sc), 54 * sizeof(float));
     if (!rc) rc = s->alloc(reinterpret_cast<void**>(&s->d_factor), 18 * sizeof(float));
     if (!rc) rc = s->alloc(reinterpret_cast<void**>(&s->d_maxbits), 18 * sizeof(unsigned));
     if (!rc) {
         // Initial FP16 scales: E ~ 1, H ~ E/eta0 (2^-8), psi_E ~ H/d, psi_H ~ E/d;
         // they adapt from the first monitor step on.
-        float sc[36];
+        float sc[54];
         double imax = 0;
         for (int i = 0; i < g->nx; ++i) imax = std::max(imax, g->inv_dx[i]);
         for (int i = 0; i < g->ny; ++i) imax = std::max(imax, g->inv_dy[i]);
@@ -661,6 +661,7 @@ int fdtd_create(const fdtd_grid_desc* g, const fdtd_material_desc* m, const fdtd
             for (int q = 6; q < 12; ++q) sc[6 + q] = std::ldexp(1.0f, e);      // psi_H
         }
         for (int q = 0; q < 18; ++q) sc[18 + q] = 1.0f / sc[q];
+        for (int q = 0; q < 18; ++q) sc[36 + q] = std::ldexp(sc[q], -6);  // floors
         if (hipMemcpy(s->d_sc, sc, sizeof(sc), hipMemcpyHostToDevice) != hipSuccess) rc = fail(FDTD_EDEVICE, "scale init failed");
         if (!rc && hipMemset(s->d_maxbits, 0, 18 * sizeof(unsigned)) != hipSuccess) rc = fail(FDTD_EDEVICE, "scale init failed");
     }
@@ -735,15 +736,18 @@ int fdtd_add_dft(fdtd_handle h, int component, const int box[6], int n_freqs, co
 int fdtd_set_fp16_scales(fdtd_handle h, const double* scales18) {
     if (!h || !scales18) return fail(FDTD_EINVAL, "null argument");
     if (h->prec != FDTD_FP16) return FDTD_OK;
-    float sc[36];
+    float sc[54];
     for (int q = 0; q < 18; ++q) {
         int e;
         const double fr = std::frexp(scales18[q], &e);
         if (!(scales18[q] > 0) || fr != 0.5) return fail(FDTD_EINVAL, "FP16 scales must be positive powers of two");
         sc[q] = (float)scales18[q];
         sc[18 + q] = (float)(1.0 / scales18[q]);
+        sc[36 + q] = std::ldexp(sc[q], -6);
     }
-    return write_scales(h, sc);
+    HIPCHK(hipMemcpyAsync(h->d_sc, sc, sizeof(sc), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return FDTD_OK;
 }
 
 int fdtd_get_fp16_scales(fdtd_handle h, double* scales18) {

@@ -40,7 +40,11 @@ __device__ __forceinline__ R ld(const T* p, long i, R scale) {
 template <typename T, typename R>
 __device__ __forceinline__ void st(T* p, long i, R v, R inv_scale) {
     if constexpr (sizeof(T) == 2) {
-        reinterpret_cast<__half*>(p)[i] = __float2half_rn((float)(v * inv_scale));
+        // saturate at the largest finite FP16 value: an overflow between two
+        // scale checks is clipped (and the next check raises the scale)
+        // instead of turning into Inf/NaN
+        const float x = fminf(fmaxf((float)(v * inv_scale), -65504.0f), 65504.0f);
+        reinterpret_cast<__half*>(p)[i] = __float2half_rn(x);
     } else {
         (void)inv_scale;
         p[i] = (T)v;
@@ -50,7 +54,8 @@ __device__ __forceinline__ void st(T* p, long i, R v, R inv_scale) {
 template <typename T, typename R>
 __device__ __forceinline__ double stored_value(R v, R scale, R inv_scale) {
     if constexpr (sizeof(T) == 2) {
-        return (double)(__half2float(__float2half_rn((float)(v * inv_scale))) * scale);
+        const float x = fminf(fmaxf((float)(v * inv_scale), -65504.0f), 65504.0f);
+        return (double)(__half2float(__float2half_rn(x)) * scale);
     } else {
         (void)scale; (void)inv_scale;
         return (double)(T)v;
@@ -128,14 +133,14 @@ __device__ __forceinline__ int mat_of(const void* m, int mid16, long id) {
     return mid16 ? (int)reinterpret_cast<const uint16_t*>(m)[id] : (int)reinterpret_cast<const uint8_t*>(m)[id];
 }
 
-// One CPML correction (src/boundary.hpp:182-223, 257-298):
-//   psi = b*psi + c*d;  corr = d/kappa + psi;  returns (corr - d).
-// psi is read from `pr` and, when `wr` is non-null, stored to `pw`.
-template <typename T, typename R>
-__device__ __forceinline__ R cpml_term(const T* pr, T* pw, bool wr, long pi, R d, R kap, R b, R c, R sc, R isc) {
+// One CPML correction (src/boundary.hpp:182-223, 257-298) on an already
+// loaded psi:  psi = b*psi + c*d;  corr = d/kappa + psi;  returns (corr - d)
+// and the new psi in `pnew`.
+template <typename R>
+__device__ __forceinline__ R cpml_term(R psi_old, R d, R kap, R b, R c, R& pnew) {
     R corr = d / kap;
-    R p = b * ld<T, R>(pr, pi, sc) + c * d;
-    if (wr) st<T, R>(pw, pi, p, isc);
+    const R p = b * psi_old + c * d;
+    pnew = p;
     corr += p;
     return corr - d;
 }
@@ -145,52 +150,107 @@ struct H3 {
     R x, y, z;
 };
 
+// Inputs of one cell's H update, gathered by h_load so that a caller can
+// issue them ahead of use (the fused kernel loads plane i+1 while it
+// computes plane i).
+template <typename R>
+struct HIn {
+    R ex, ex_j, ex_k, ey, ey_i, ey_k, ez, ez_i, ez_j, hx, hy, hz;
+    R p[6];  // psi_H: Hx_y, Hx_z, Hy_z, Hy_x, Hz_x, Hz_y (where in a slab)
+    int m;   // material id (when H coefficients vary)
+    bool upd;
+};
+
+template <typename T, typename R>
+__device__ __forceinline__ HIn<R> h_load(const StepArgs<T, R>& a, int i, int j, int k) {
+    HIn<R> v;
+    const long id = ((long)i * a.ny + j) * a.nz + k;
+    v.hx = ld<T, R>(a.Hx, id, a.sc[3]);
+    v.hy = ld<T, R>(a.Hy, id, a.sc[4]);
+    v.hz = ld<T, R>(a.Hz, id, a.sc[5]);
+    v.upd = (i < a.nx - 1) & (j < a.ny - 1) & (k < a.nz - 1);
+    v.m = 0;
+    for (int q = 0; q < 6; ++q) v.p[q] = 0;
+    v.ex = v.ex_j = v.ex_k = v.ey = v.ey_i = v.ey_k = v.ez = v.ez_i = v.ez_j = 0;
+    if (!v.upd) return v;
+    const long sI = a.sI, sJ = a.sJ;
+    v.ex = ld<T, R>(a.Ex, id, a.sc[0]); v.ex_j = ld<T, R>(a.Ex, id + sJ, a.sc[0]); v.ex_k = ld<T, R>(a.Ex, id + 1, a.sc[0]);
+    v.ey = ld<T, R>(a.Ey, id, a.sc[1]); v.ey_i = ld<T, R>(a.Ey, id + sI, a.sc[1]); v.ey_k = ld<T, R>(a.Ey, id + 1, a.sc[1]);
+    v.ez = ld<T, R>(a.Ez, id, a.sc[2]); v.ez_i = ld<T, R>(a.Ez, id + sI, a.sc[2]); v.ez_j = ld<T, R>(a.Ez, id + sJ, a.sc[2]);
+    if (!a.h_uniform) v.m = mat_of<R>(a.matid, a.mid16, id);
+    if (a.cpml) {
+        const int np = a.np, nx = a.nx, ny = a.ny, nz = a.nz;
+        const bool px = in_pml(i, nx, np), py = in_pml(j, ny, np), pz = in_pml(k, nz, np);
+        if (py) {
+            const long q = psi_y(i, j, k, nx, ny, nz, np);
+            v.p[0] = ld<T, R>(a.psi[6], q, a.sc[12]);
+            v.p[5] = ld<T, R>(a.psi[11], q, a.sc[17]);
+        }
+        if (pz) {
+            const long q = psi_z(i, j, k, nx, ny, nz, np);
+            v.p[1] = ld<T, R>(a.psi[7], q, a.sc[13]);
+            v.p[2] = ld<T, R>(a.psi[8], q, a.sc[14]);
+        }
+        if (px) {
+            const long q = psi_x(i, j, k, nx, ny, nz, np);
+            v.p[3] = ld<T, R>(a.psi[9], q, a.sc[15]);
+            v.p[4] = ld<T, R>(a.psi[10], q, a.sc[16]);
+        }
+    }
+    return v;
+}
+
 // H^{n+1/2} of one cell: fdtd_update_H (src/fdtd_stepper.hpp:31-46) followed
 // by the six CPMLBoundary::apply_after_H corrections (src/boundary.hpp:172-224)
-// in the reference's order.  Reads E^n, H^{n-1/2} and psi_H from the read
-// buffers; stores psi_H to the write buffers when `own` (halo recomputation
-// by a neighbouring tile must not store).  Cells the reference never updates
-// (i = NxT-1, j = NyT-1 or k = NzT-1) return their stored value.
+// in the reference's order.  Stores H and psi_H to the write buffers when
+// `own` (halo recomputation by a neighbouring tile must not store).  Cells the
+// reference never updates (i = NxT-1, j = NyT-1 or k = NzT-1) return their
+// stored value.  Returns the values a later read of the stored H gives.
 template <typename T, typename R>
-__device__ __forceinline__ H3<R> h_new(const StepArgs<T, R>& a, int i, int j, int k, bool own) {
+__device__ __forceinline__ H3<R> h_compute(const StepArgs<T, R>& a, int i, int j, int k, const HIn<R>& v, bool own) {
+    if (!v.upd) return H3<R>{v.hx, v.hy, v.hz};
     const long id = ((long)i * a.ny + j) * a.nz + k;
-    if (i >= a.nx - 1 || j >= a.ny - 1 || k >= a.nz - 1)
-        return H3<R>{ld<T, R>(a.Hx, id, a.sc[3]), ld<T, R>(a.Hy, id, a.sc[4]), ld<T, R>(a.Hz, id, a.sc[5])};
-    const long sI = a.sI, sJ = a.sJ;
-    const R ex = ld<T, R>(a.Ex, id, a.sc[0]), ex_j = ld<T, R>(a.Ex, id + sJ, a.sc[0]), ex_k = ld<T, R>(a.Ex, id + 1, a.sc[0]);
-    const R ey = ld<T, R>(a.Ey, id, a.sc[1]), ey_i = ld<T, R>(a.Ey, id + sI, a.sc[1]), ey_k = ld<T, R>(a.Ey, id + 1, a.sc[1]);
-    const R ez = ld<T, R>(a.Ez, id, a.sc[2]), ez_i = ld<T, R>(a.Ez, id + sI, a.sc[2]), ez_j = ld<T, R>(a.Ez, id + sJ, a.sc[2]);
     const R idx = a.idx[i], idy = a.idy[j], idz = a.idz[k];
-    const R dEz_y = (ez_j - ez) * idy, dEy_z = (ey_k - ey) * idz;
-    const R dEx_z = (ex_k - ex) * idz, dEz_x = (ez_i - ez) * idx;
-    const R dEy_x = (ey_i - ey) * idx, dEx_y = (ex_j - ex) * idy;
+    const R dEz_y = (v.ez_j - v.ez) * idy, dEy_z = (v.ey_k - v.ey) * idz;
+    const R dEx_z = (v.ex_k - v.ex) * idz, dEz_x = (v.ez_i - v.ez) * idx;
+    const R dEy_x = (v.ey_i - v.ey) * idx, dEx_y = (v.ex_j - v.ex) * idy;
     const R cx = dEz_y - dEy_z, cy = dEx_z - dEz_x, cz = dEy_x - dEx_y;
-    R aHx, bHx, aHy, bHy, aHz, bHz;
-    if (a.h_uniform) {
-        aHx = a.lut[6 * a.nmat]; bHx = a.lut[7 * a.nmat];
-        aHy = a.lut[8 * a.nmat]; bHy = a.lut[9 * a.nmat];
-        aHz = a.lut[10 * a.nmat]; bHz = a.lut[11 * a.nmat];
-    } else {
-        const int m = mat_of<R>(a.matid, a.mid16, id);
-        aHx = a.lut[6 * a.nmat + m]; bHx = a.lut[7 * a.nmat + m];
-        aHy = a.lut[8 * a.nmat + m]; bHy = a.lut[9 * a.nmat + m];
-        aHz = a.lut[10 * a.nmat + m]; bHz = a.lut[11 * a.nmat + m];
-    }
-    R hx = aHx * ld<T, R>(a.Hx, id, a.sc[3]) - bHx * cx;
-    R hy = aHy * ld<T, R>(a.Hy, id, a.sc[4]) - bHy * cy;
-    R hz = aHz * ld<T, R>(a.Hz, id, a.sc[5]) - bHz * cz;
+    const int m = v.m, nm = a.nmat;
+    const R aHx = a.lut[6 * nm + m], bHx = a.lut[7 * nm + m];
+    const R aHy = a.lut[8 * nm + m], bHy = a.lut[9 * nm + m];
+    const R aHz = a.lut[10 * nm + m], bHz = a.lut[11 * nm + m];
+    R hx = aHx * v.hx - bHx * cx;
+    R hy = aHy * v.hy - bHy * cy;
+    R hz = aHz * v.hz - bHz * cz;
     if (a.cpml) {
-        const int np = a.np;
-        const bool px = in_pml(i, a.nx, np), py = in_pml(j, a.ny, np), pz = in_pml(k, a.nz, np);
+        const int np = a.np, nx = a.nx, ny = a.ny, nz = a.nz;
+        const bool px = in_pml(i, nx, np), py = in_pml(j, ny, np), pz = in_pml(k, nz, np);
         if (px | py | pz) {
             const R* CX = a.cpx; const R* CY = a.cpy; const R* CZ = a.cpz;
-            const int nx = a.nx, ny = a.ny, nz = a.nz;
-            if (py) hx -= bHx * cpml_term<T, R>(a.psi[6], a.psiw[6], own, psi_y(i, j, k, nx, ny, nz, np), dEz_y, CY[j], CY[3 * ny + j], CY[4 * ny + j], a.sc[12], a.sc[30]);
-            if (pz) hx -= bHx * (-cpml_term<T, R>(a.psi[7], a.psiw[7], own, psi_z(i, j, k, nx, ny, nz, np), dEy_z, CZ[k], CZ[3 * nz + k], CZ[4 * nz + k], a.sc[13], a.sc[31]));
-            if (pz) hy -= bHy * cpml_term<T, R>(a.psi[8], a.psiw[8], own, psi_z(i, j, k, nx, ny, nz, np), dEx_z, CZ[k], CZ[3 * nz + k], CZ[4 * nz + k], a.sc[14], a.sc[32]);
-            if (px) hy -= bHy * (-cpml_term<T, R>(a.psi[9], a.psiw[9], own, psi_x(i, j, k, nx, ny, nz, np), dEz_x, CX[i], CX[3 * nx + i], CX[4 * nx + i], a.sc[15], a.sc[33]));
-            if (px) hz -= bHz * cpml_term<T, R>(a.psi[10], a.psiw[10], own, psi_x(i, j, k, nx, ny, nz, np), dEy_x, CX[i], CX[3 * nx + i], CX[4 * nx + i], a.sc[16], a.sc[34]);
-            if (py) hz -= bHz * (-cpml_term<T, R>(a.psi[11], a.psiw[11], own, psi_y(i, j, k, nx, ny, nz, np), dEx_y, CY[j], CY[3 * ny + j], CY[4 * ny + j], a.sc[17], a.sc[35]));
+            R pn[6];
+            if (py) hx -= bHx * cpml_term<R>(v.p[0], dEz_y, CY[j], CY[3 * ny + j], CY[4 * ny + j], pn[0]);
+            if (pz) hx -= bHx * (-cpml_term<R>(v.p[1], dEy_z, CZ[k], CZ[3 * nz + k], CZ[4 * nz + k], pn[1]));
+            if (pz) hy -= bHy * cpml_term<R>(v.p[2], dEx_z, CZ[k], CZ[3 * nz + k], CZ[4 * nz + k], pn[2]);
+            if (px) hy -= bHy * (-cpml_term<R>(v.p[3], dEz_x, CX[i], CX[3 * nx + i], CX[4 * nx + i], pn[3]));
+            if (px) hz -= bHz * cpml_term<R>(v.p[4], dEy_x, CX[i], CX[3 * nx + i], CX[4 * nx + i], pn[4]);
+            if (py) hz -= bHz * (-cpml_term<R>(v.p[5], dEx_y, CY[j], CY[3 * ny + j], CY[4 * ny + j], pn[5]));
+            if (own) {
+                if (py) {
+                    const long q = psi_y(i, j, k, nx, ny, nz, np);
+                    st<T, R>(a.psiw[6], q, pn[0], a.sc[30]);
+                    st<T, R>(a.psiw[11], q, pn[5], a.sc[35]);
+                }
+                if (pz) {
+                    const long q = psi_z(i, j, k, nx, ny, nz, np);
+                    st<T, R>(a.psiw[7], q, pn[1], a.sc[31]);
+                    st<T, R>(a.psiw[8], q, pn[2], a.sc[32]);
+                }
+                if (px) {
+                    const long q = psi_x(i, j, k, nx, ny, nz, np);
+                    st<T, R>(a.psiw[9], q, pn[3], a.sc[33]);
+                    st<T, R>(a.psiw[10], q, pn[4], a.sc[34]);
+                }
+            }
         }
     }
     if (own) {
@@ -198,9 +258,13 @@ __device__ __forceinline__ H3<R> h_new(const StepArgs<T, R>& a, int i, int j, in
         st<T, R>(a.Hyw, id, hy, a.sc[22]);
         st<T, R>(a.Hzw, id, hz, a.sc[23]);
     }
-    // the value a following read of the stored H returns
     return H3<R>{(R)stored_value<T, R>(hx, a.sc[3], a.sc[21]), (R)stored_value<T, R>(hy, a.sc[4], a.sc[22]),
                  (R)stored_value<T, R>(hz, a.sc[5], a.sc[23])};
+}
+
+template <typename T, typename R>
+__device__ __forceinline__ H3<R> h_new(const StepArgs<T, R>& a, int i, int j, int k, bool own) {
+    return h_compute<T, R>(a, i, j, k, h_load<T, R>(a, i, j, k), own);
 }
 
 // H update kernel of the two-kernel step.  Threads cover the grid with
@@ -252,6 +316,64 @@ __device__ __forceinline__ void tls_populate(const TLSCoef& c, double ez_new, do
     }
 }
 
+// Inputs of one cell's E update that do not depend on H^{n+1/2}.
+template <typename R>
+struct EIn {
+    R ex0, ey0, ez0;
+    R p[6];  // psi_E: Ex_y, Ex_z, Ey_z, Ey_x, Ez_x, Ez_y (where in a slab)
+    int m, reg;
+    long g;
+    double Nu, Ng, Pz, Pp;
+};
+
+template <typename T, typename R>
+__device__ __forceinline__ EIn<R> e_load(const StepArgs<T, R>& a, int i, int j, int k) {
+    EIn<R> v;
+    const long id = ((long)i * a.ny + j) * a.nz + k;
+    v.reg = -1;
+    v.g = 0;
+    v.Nu = v.Ng = v.Pz = v.Pp = 0;
+    if (a.gain) {
+        const int gi = i - a.g0x, gj = j - a.g0y, gk = k - a.g0z;
+        if ((unsigned)gi < (unsigned)a.gnx && (unsigned)gj < (unsigned)a.gny && (unsigned)gk < (unsigned)a.gnz) {
+            v.g = ((long)gi * a.gny + gj) * a.gnz + gk;
+            v.reg = a.greg[v.g];
+            if (v.reg >= 0) {
+                v.Nu = a.gNu[v.g]; v.Ng = a.gNg[v.g]; v.Pz = a.gPz[v.g]; v.Pp = a.gPp[v.g];
+            }
+        }
+    }
+    for (int q = 0; q < 6; ++q) v.p[q] = 0;
+    v.ex0 = v.ey0 = 0;
+    v.ez0 = ld<T, R>(a.Ez, id, a.sc[2]);
+    v.m = 0;
+    const bool interior = (i > 0) & (j > 0) & (k > 0);
+    if (!interior) return v;
+    v.ex0 = ld<T, R>(a.Ex, id, a.sc[0]);
+    v.ey0 = ld<T, R>(a.Ey, id, a.sc[1]);
+    v.m = mat_of<R>(a.matid, a.mid16, id);
+    if (a.cpml) {
+        const int np = a.np, nx = a.nx, ny = a.ny, nz = a.nz;
+        const bool px = in_pml(i, nx, np), py = in_pml(j, ny, np), pz = in_pml(k, nz, np);
+        if (py) {
+            const long q = psi_y(i, j, k, nx, ny, nz, np);
+            v.p[0] = ld<T, R>(a.psi[0], q, a.sc[6]);
+            v.p[5] = ld<T, R>(a.psi[5], q, a.sc[11]);
+        }
+        if (pz) {
+            const long q = psi_z(i, j, k, nx, ny, nz, np);
+            v.p[1] = ld<T, R>(a.psi[1], q, a.sc[7]);
+            v.p[2] = ld<T, R>(a.psi[2], q, a.sc[8]);
+        }
+        if (px) {
+            const long q = psi_x(i, j, k, nx, ny, nz, np);
+            v.p[3] = ld<T, R>(a.psi[3], q, a.sc[9]);
+            v.p[4] = ld<T, R>(a.psi[4], q, a.sc[10]);
+        }
+    }
+    return v;
+}
+
 // E^{n+1} of one cell with J, the CPML-E correction, the TLS polarization
 // (with Ez^n) and the population update (with the final Ez^{n+1}) fused:
 // fdtd_update_E(_with_gain) (src/fdtd_stepper.hpp:73-88,
@@ -265,25 +387,18 @@ __device__ __forceinline__ void tls_populate(const TLSCoef& c, double ez_new, do
 // boundary the six faces are written as zero (PECBoundary::apply_after_E,
 // src/boundary.hpp:55-74).
 template <typename T, typename R>
-__device__ __forceinline__ void e_new(const StepArgs<T, R>& a, int i, int j, int k, R hx, R hy, R hz, R hx_j, R hz_j,
-                                      R hx_k, R hy_k, R hy_i, R hz_i) {
+__device__ __forceinline__ void e_compute(const StepArgs<T, R>& a, int i, int j, int k, const EIn<R>& v, R hx, R hy,
+                                          R hz, R hx_j, R hz_j, R hx_k, R hy_k, R hy_i, R hz_i) {
     const long id = ((long)i * a.ny + j) * a.nz + k;
-    int reg = -1;
-    long g = 0;
-    if (a.gain) {
-        const int gi = i - a.g0x, gj = j - a.g0y, gk = k - a.g0z;
-        if ((unsigned)gi < (unsigned)a.gnx && (unsigned)gj < (unsigned)a.gny && (unsigned)gk < (unsigned)a.gnz) {
-            g = ((long)gi * a.gny + gj) * a.gnz + gk;
-            reg = a.greg[g];
-        }
-    }
+    const int reg = v.reg;
+    const long g = v.g;
     const bool interior = (i > 0) & (j > 0) & (k > 0);
     const bool pec_face = a.pec && (!interior || i == a.nx - 1 || j == a.ny - 1 || k == a.nz - 1);
     if (!interior) {
         if (reg >= 0) {
             const TLSCoef c = a.tls[reg];
-            double Nu = a.gNu[g], Ng = a.gNg[g], Pz = a.gPz[g], Pp = a.gPp[g];
-            const double ez = (double)ld<T, R>(a.Ez, id, a.sc[2]);
+            double Nu = v.Nu, Ng = v.Ng, Pz = v.Pz, Pp = v.Pp;
+            const double ez = (double)v.ez0;
             (void)tls_polarize(c, Nu, Ng, ez, Pz, Pp);
             tls_populate(c, ez, 0.0, Pz, Pp, Nu, Ng);
             a.gNu[g] = Nu; a.gNg[g] = Ng; a.gPz[g] = Pz; a.gPp[g] = Pp;
@@ -304,42 +419,55 @@ __device__ __forceinline__ void e_new(const StepArgs<T, R>& a, int i, int j, int
     for (int s = 0; s < a.nsrc; ++s) {
         const SourceBox b = a.src[s];
         if (i >= b.i0 && i <= b.i1 && j >= b.j0 && j <= b.j1 && k >= b.k0 && k <= b.k1) {
-            const R v = (R)a.srcv[s];
-            if (b.comp == 0) jx += v; else if (b.comp == 1) jy += v; else jz += v;
+            const R val = (R)a.srcv[s];
+            if (b.comp == 0) jx += val; else if (b.comp == 1) jy += val; else jz += val;
         }
     }
-    const int m = mat_of<R>(a.matid, a.mid16, id);
-    const R bEx = a.lut[1 * a.nmat + m], bEy = a.lut[3 * a.nmat + m], bEz = a.lut[5 * a.nmat + m];
-    const R ex0 = ld<T, R>(a.Ex, id, a.sc[0]), ey0 = ld<T, R>(a.Ey, id, a.sc[1]), ez0 = ld<T, R>(a.Ez, id, a.sc[2]);
-    double Nu = 0, Ng = 0, Pz = 0, Pp = 0;
+    const int m = v.m, nm = a.nmat;
+    const R bEx = a.lut[1 * nm + m], bEy = a.lut[3 * nm + m], bEz = a.lut[5 * nm + m];
+    double Nu = v.Nu, Ng = v.Ng, Pz = v.Pz, Pp = v.Pp;
     R rz = cz - jz;
     if (reg >= 0) {
-        Nu = a.gNu[g]; Ng = a.gNg[g]; Pz = a.gPz[g]; Pp = a.gPp[g];
-        const double dP = tls_polarize(a.tls[reg], Nu, Ng, (double)ez0, Pz, Pp);
+        const double dP = tls_polarize(a.tls[reg], Nu, Ng, (double)v.ez0, Pz, Pp);
         rz = rz - (R)dP;
     }
     R ex, ey, ez;
     if (a.lossless) {
-        ex = ex0 + bEx * (cx - jx);
-        ey = ey0 + bEy * (cy - jy);
-        ez = ez0 + bEz * rz;
+        ex = v.ex0 + bEx * (cx - jx);
+        ey = v.ey0 + bEy * (cy - jy);
+        ez = v.ez0 + bEz * rz;
     } else {
-        ex = a.lut[0 * a.nmat + m] * ex0 + bEx * (cx - jx);
-        ey = a.lut[2 * a.nmat + m] * ey0 + bEy * (cy - jy);
-        ez = a.lut[4 * a.nmat + m] * ez0 + bEz * rz;
+        ex = a.lut[0 * nm + m] * v.ex0 + bEx * (cx - jx);
+        ey = a.lut[2 * nm + m] * v.ey0 + bEy * (cy - jy);
+        ez = a.lut[4 * nm + m] * v.ez0 + bEz * rz;
     }
     if (a.cpml) {
-        const int np = a.np;
-        const bool px = in_pml(i, a.nx, np), py = in_pml(j, a.ny, np), pz = in_pml(k, a.nz, np);
+        const int np = a.np, nx = a.nx, ny = a.ny, nz = a.nz;
+        const bool px = in_pml(i, nx, np), py = in_pml(j, ny, np), pz = in_pml(k, nz, np);
         if (px | py | pz) {
             const R* CX = a.cpx; const R* CY = a.cpy; const R* CZ = a.cpz;
-            const int nx = a.nx, ny = a.ny, nz = a.nz;
-            if (py) ex += bEx * cpml_term<T, R>(a.psi[0], a.psi[0], true, psi_y(i, j, k, nx, ny, nz, np), dHz_y, CY[j], CY[1 * ny + j], CY[2 * ny + j], a.sc[6], a.sc[24]);
-            if (pz) ex += bEx * (-cpml_term<T, R>(a.psi[1], a.psi[1], true, psi_z(i, j, k, nx, ny, nz, np), dHy_z, CZ[k], CZ[1 * nz + k], CZ[2 * nz + k], a.sc[7], a.sc[25]));
-            if (pz) ey += bEy * cpml_term<T, R>(a.psi[2], a.psi[2], true, psi_z(i, j, k, nx, ny, nz, np), dHx_z, CZ[k], CZ[1 * nz + k], CZ[2 * nz + k], a.sc[8], a.sc[26]);
-            if (px) ey += bEy * (-cpml_term<T, R>(a.psi[3], a.psi[3], true, psi_x(i, j, k, nx, ny, nz, np), dHz_x, CX[i], CX[1 * nx + i], CX[2 * nx + i], a.sc[9], a.sc[27]));
-            if (px) ez += bEz * cpml_term<T, R>(a.psi[4], a.psi[4], true, psi_x(i, j, k, nx, ny, nz, np), dHy_x, CX[i], CX[1 * nx + i], CX[2 * nx + i], a.sc[10], a.sc[28]);
-            if (py) ez += bEz * (-cpml_term<T, R>(a.psi[5], a.psi[5], true, psi_y(i, j, k, nx, ny, nz, np), dHx_y, CY[j], CY[1 * ny + j], CY[2 * ny + j], a.sc[11], a.sc[29]));
+            R pn[6];
+            if (py) ex += bEx * cpml_term<R>(v.p[0], dHz_y, CY[j], CY[1 * ny + j], CY[2 * ny + j], pn[0]);
+            if (pz) ex += bEx * (-cpml_term<R>(v.p[1], dHy_z, CZ[k], CZ[1 * nz + k], CZ[2 * nz + k], pn[1]));
+            if (pz) ey += bEy * cpml_term<R>(v.p[2], dHx_z, CZ[k], CZ[1 * nz + k], CZ[2 * nz + k], pn[2]);
+            if (px) ey += bEy * (-cpml_term<R>(v.p[3], dHz_x, CX[i], CX[1 * nx + i], CX[2 * nx + i], pn[3]));
+            if (px) ez += bEz * cpml_term<R>(v.p[4], dHy_x, CX[i], CX[1 * nx + i], CX[2 * nx + i], pn[4]);
+            if (py) ez += bEz * (-cpml_term<R>(v.p[5], dHx_y, CY[j], CY[1 * ny + j], CY[2 * ny + j], pn[5]));
+            if (py) {
+                const long q = psi_y(i, j, k, nx, ny, nz, np);
+                st<T, R>(a.psi[0], q, pn[0], a.sc[24]);
+                st<T, R>(a.psi[5], q, pn[5], a.sc[29]);
+            }
+            if (pz) {
+                const long q = psi_z(i, j, k, nx, ny, nz, np);
+                st<T, R>(a.psi[1], q, pn[1], a.sc[25]);
+                st<T, R>(a.psi[2], q, pn[2], a.sc[26]);
+            }
+            if (px) {
+                const long q = psi_x(i, j, k, nx, ny, nz, np);
+                st<T, R>(a.psi[3], q, pn[3], a.sc[27]);
+                st<T, R>(a.psi[4], q, pn[4], a.sc[28]);
+            }
         }
     }
     if (pec_face) { ex = 0; ey = 0; ez = 0; }
@@ -350,7 +478,7 @@ __device__ __forceinline__ void e_new(const StepArgs<T, R>& a, int i, int j, int
         const TLSCoef& c = a.tls[reg];
         // The reference stores Ez and reads it back for the population pass.
         const double ez_new = stored_value<T, R>(ez, a.sc[2], a.sc[20]);
-        tls_populate(c, ez_new, (double)ez0, Pz, Pp, Nu, Ng);
+        tls_populate(c, ez_new, (double)v.ez0, Pz, Pp, Nu, Ng);
         a.gNu[g] = Nu; a.gNg[g] = Ng; a.gPz[g] = Pz; a.gPp[g] = Pp;
     }
 }
@@ -371,7 +499,8 @@ __global__ void __launch_bounds__(256) k_update_E(StepArgs<T, R> a) {
         hy = ld<T, R>(a.Hy, id, a.sc[4]); hy_i = ld<T, R>(a.Hy, id - sI, a.sc[4]); hy_k = ld<T, R>(a.Hy, id - 1, a.sc[4]);
         hz = ld<T, R>(a.Hz, id, a.sc[5]); hz_i = ld<T, R>(a.Hz, id - sI, a.sc[5]); hz_j = ld<T, R>(a.Hz, id - sJ, a.sc[5]);
     }
-    e_new<T, R>(a, i, j, k, hx, hy, hz, hx_j, hz_j, hx_k, hy_k, hy_i, hz_i);
+    const EIn<R> v = e_load<T, R>(a, i, j, k);
+    e_compute<T, R>(a, i, j, k, v, hx, hy, hz, hx_j, hz_j, hx_k, hy_k, hy_i, hz_i);
 }
 
 // Fused step kernel: one launch advances H and E of the whole grid.
@@ -379,16 +508,17 @@ __global__ void __launch_bounds__(256) k_update_E(StepArgs<T, R> a) {
 // marches along i over a chunk of planes.  It has TJ+2 wavefronts: wavefront
 // 1..TJ own one row each; wavefront 0 recomputes H of the row below the tile
 // (j0-1) and wavefront TJ+1 recomputes H of the column left of the tile
-// (k0-1) for the TJ rows.  Per plane every wavefront computes its H at once
-// (all loads of a plane in flight together), the halo values go through LDS,
-// one barrier, then the owning wavefronts update E with H(i-1) carried in
-// registers from the previous plane and H(k-1) from the neighbouring lane.
-// Reads come from buffer A (E^n, H^{n-1/2}, psi_H) and writes go to buffer B,
-// so no tile ever reads a neighbour's half-updated state; psi_E and the gain
-// state are touched only by their owning thread and stay in place.  A chunk
-// starting at i0 > 0 first recomputes H(i0-1) of its cells.  Work-group ids
-// are remapped so that each XCD (ids are dealt round-robin over 8) receives a
-// contiguous range of tiles and keeps the halo rows in its own L2.
+// (k0-1) for the TJ rows.  The loop is software-pipelined: the H and E
+// inputs of plane i+1 are loaded while plane i is computed.  Per plane every
+// wavefront computes its H, the halo values go through LDS, one barrier, then
+// the owning wavefronts update E with H(i-1) carried in registers from the
+// previous plane and H(k-1) from the neighbouring lane.  Reads come from
+// buffer A (E^n, H^{n-1/2}, psi_H), writes go to buffer B, so no tile ever
+// reads a neighbour's half-updated state; psi_E and the gain state are
+// touched only by their owning thread and stay in place.  A chunk starting at
+// i0 > 0 first recomputes H(i0-1) of its cells.  Work-group ids are remapped
+// so that each XCD (ids are dealt round-robin over 8) receives a contiguous
+// range of tiles and keeps the halo rows in its own L2.
 template <typename T, typename R, int TJ>
 __global__ void __launch_bounds__(64 * (TJ + 2)) k_step_fused(StepArgs<T, R> a, int ichunk, int nkt, int njt, int nwg) {
     __shared__ R lj[2][TJ][2][64];  // H(x, z) of rows j0-1 .. j0+TJ-2
@@ -423,10 +553,19 @@ __global__ void __launch_bounds__(64 * (TJ + 2)) k_step_fused(StepArgs<T, R> a, 
         hyi = h.y;
         hzi = h.z;
     }
+    HIn<R> hin;
+    EIn<R> ein;
+    if (act) hin = h_load<T, R>(a, i0, j, k);
+    if (erow) ein = e_load<T, R>(a, i0, j, k);
     int par = 0;
     for (int i = i0; i < i1; ++i) {
+        HIn<R> hnext;
+        EIn<R> enext;
+        const bool more = i + 1 < i1;
+        if (act && more) hnext = h_load<T, R>(a, i + 1, j, k);
+        if (erow && more) enext = e_load<T, R>(a, i + 1, j, k);
         H3<R> h{0, 0, 0};
-        if (act) h = h_new<T, R>(a, i, j, k, own);
+        if (act) h = h_compute<T, R>(a, i, j, k, hin, own);
         if (w < TJ) {
             lj[par][w][0][lane] = h.x;
             lj[par][w][1][lane] = h.z;
@@ -444,11 +583,15 @@ __global__ void __launch_bounds__(64 * (TJ + 2)) k_step_fused(StepArgs<T, R> a, 
                 hyk = lk[par][r][1];
             }
             const R hxj = lj[par][r][0][lane], hzj = lj[par][r][1][lane];
-            e_new<T, R>(a, i, j, k, h.x, h.y, h.z, hxj, hzj, hxk, hyk, hyi, hzi);
+            e_compute<T, R>(a, i, j, k, ein, h.x, h.y, h.z, hxj, hzj, hxk, hyk, hyi, hzi);
             hyi = h.y;
             hzi = h.z;
         }
         par ^= 1;
+        if (more) {
+            hin = hnext;
+            ein = enext;
+        }
     }
 }
 
@@ -595,7 +738,8 @@ __global__ void __launch_bounds__(256) k_monitor(StepArgs<T, R> a, long N, long 
             pi += a.tls[r].N0;
         }
     }
-    // wavefront reduction (64 lanes)
+    // wavefront reduction (64 lanes), then across the block's wavefronts in
+    // LDS: one set of atomics per block.
     for (int off = 32; off > 0; off >>= 1) {
         unsigned long long o;
         o = __shfl_down(mE, off); mE = o > mE ? o : mE;
@@ -605,11 +749,28 @@ __global__ void __launch_bounds__(256) k_monitor(StepArgs<T, R> a, long N, long 
         pt += __shfl_down(pt, off);
         pi += __shfl_down(pi, off);
     }
+    __shared__ unsigned long long su[4][4];
+    __shared__ double sd[2][4];
+    const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
-        atomicMax(&out->maxE, mE);
-        atomicMax(&out->maxH, mH);
-        atomicMax(&out->maxP, mP);
-        atomicMax(&out->nanN, nN);
+        su[0][w] = mE; su[1][w] = mH; su[2][w] = mP; su[3][w] = nN;
+        sd[0][w] = pt; sd[1][w] = pi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = (int)(blockDim.x >> 6);
+        for (int q = 1; q < nw; ++q) {
+            mE = su[0][q] > mE ? su[0][q] : mE;
+            mH = su[1][q] > mH ? su[1][q] : mH;
+            mP = su[2][q] > mP ? su[2][q] : mP;
+            nN = su[3][q] > nN ? su[3][q] : nN;
+            pt += sd[0][q];
+            pi += sd[1][q];
+        }
+        if (mE) atomicMax(&out->maxE, mE);
+        if (mH) atomicMax(&out->maxH, mH);
+        if (mP) atomicMax(&out->maxP, mP);
+        if (nN) atomicMax(&out->nanN, nN);
         if (pt != 0.0) atomicAdd(&out->pop_total, pt);
         if (pi != 0.0) atomicAdd(&out->pop_init, pi);
     }
@@ -644,6 +805,9 @@ __global__ void __launch_bounds__(256) k_fp16_maxima(ArrSet s, unsigned* maxbits
 // to 2^-22 of the maximum stay normal).  A scale moves only when the maximum
 // leaves [2^2, 2^11]; the stored values are then multiplied by an exact
 // power of two.
+// sc[36 + c] is the floor of scale c: a scale never drops below it, so a
+// field that is still growing from zero keeps the scale its physical
+// magnitude calls for instead of one fitted to its first tiny values.
 __global__ void k_fp16_plan(float* sc, float* factor, unsigned* maxbits) {
     const int c = threadIdx.x;
     if (c >= 18) return;
@@ -652,11 +816,14 @@ __global__ void k_fp16_plan(float* sc, float* factor, unsigned* maxbits) {
     float f = 1.0f;
     if (m > 2048.0f || (m > 0.0f && m < 4.0f)) {
         int e;
-        frexpf(m, &e);            // m = fr * 2^e, fr in [0.5, 1)
-        const int shift = e - 8;  // new stored max in [2^7, 2^8)
-        f = ldexpf(1.0f, -shift);
-        sc[c] = ldexpf(sc[c], shift);
-        sc[18 + c] = ldexpf(sc[18 + c], -shift);
+        frexpf(m, &e);      // m = fr * 2^e, fr in [0.5, 1)
+        int shift = e - 8;  // new stored max in [2^7, 2^8)
+        while (shift < 0 && ldexpf(sc[c], shift) < sc[36 + c]) ++shift;
+        if (shift != 0) {
+            f = ldexpf(1.0f, -shift);
+            sc[c] = ldexpf(sc[c], shift);
+            sc[18 + c] = ldexpf(sc[18 + c], -shift);
+        }
     }
     factor[c] = f;
 }

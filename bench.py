@@ -51,6 +51,23 @@ def algorithmic_bytes(spec, setup, prec_bytes: int, mat_bytes: int):
     return h_bytes, e_bytes, step_bytes, gain_cells
 
 
+def copy_calibration(nbytes: int) -> None:
+    """One plain device-to-device copy of nbytes (reads nbytes, writes nbytes),
+    so a counter run can relate FETCH_SIZE / WRITE_SIZE to known traffic."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    hip.hipMemset(a, 1, C.c_size_t(nbytes))
+    hip.hipDeviceSynchronize()
+    assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
 def cpu_baseline(spec, steps: int):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle  # checker / CPU baseline only
@@ -80,6 +97,9 @@ def main():
     ap.add_argument("--mode", type=int, default=1, help="0: separate H/E kernels, 1: fused step kernel")
     ap.add_argument("--tj", type=int, default=0, help="fused tile rows (2,4,6,8,14); 0 = library default")
     ap.add_argument("--ichunk", type=int, default=0, help="fused x-planes per chunk; 0 = library default")
+    ap.add_argument("--copy-calib", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (a known-size kernel for "
+                         "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported; this build measures one GPU (N=1).",
@@ -87,6 +107,8 @@ def main():
         sys.exit(2)
 
     pkg = importlib.import_module("3d_fdtd_version008_amd")
+    if args.copy_calib:
+        copy_calibration(args.copy_calib << 20)
     spec = pkg.scene.gain_slab_case(n_total=args.size, npml=12, steps=5000)
     t0 = time.time()
     setup = pkg.Setup(spec)

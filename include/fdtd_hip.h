@@ -140,9 +140,11 @@ int fdtd_add_dft(fdtd_handle h, int component, const int box[6], int n_freqs, co
 /* FP16 storage scales (value = stored * scale), 6 fields Ex..Hz then the 12
  * psi arrays (Ex_y Ex_z Ey_z Ey_x Ez_x Ez_y Hx_y Hx_z Hy_z Hy_x Hz_x Hz_y);
  * powers of two.  They start from an estimate (E 1, H 2^-8, psi from the
- * finest spacing) and adapt on the device every 10 steps so the largest
- * stored magnitude stays within [2^6, 2^13].  No-ops for FP64/FP32 handles.
- * No counterpart in the reference (FP64 only). */
+ * finest spacing; fdtd_set_fp16_scales replaces it) and adapt on the device
+ * every 10 steps so the largest stored magnitude stays within [2^2, 2^11],
+ * never dropping below 2^-6 of the value last set.  Stores saturate at the
+ * FP16 maximum.  No-ops for FP64/FP32 handles.  No counterpart in the
+ * reference (FP64 only). */
 int fdtd_set_fp16_scales(fdtd_handle h, const double* scales18);
 int fdtd_get_fp16_scales(fdtd_handle h, double* scales18);
 /* Kernel schedule: mode 0 = separate H and E kernels (in place), mode 1 =
