@@ -91,7 +91,7 @@ struct fdtd_sim {
     void* psi[12] = {}; // = psi2[cur] (psi_E always psi2[0])
     int cur = 0;
     int mode = 1;       // 0: H and E kernels, 1: fused step kernel
-    int tj = 4, ichunk = 32;
+    int ichunk = 8;     // x-planes per work-group of the fused kernel
     long psi_len[12] = {};
     void* inv[3] = {};
     void* cp[3] = {};
@@ -172,7 +172,6 @@ StepArgs<T, R> make_args(const fdtd_sim* s) {
     a.Hx = (T*)s->F[3]; a.Hy = (T*)s->F[4]; a.Hz = (T*)s->F[5];
     a.Exw = a.Ex; a.Eyw = a.Ey; a.Ezw = a.Ez; a.Hxw = a.Hx; a.Hyw = a.Hy; a.Hzw = a.Hz;
     a.pec = s->bc_type == 0;
-    a.idx = (const R*)s->inv[0]; a.idy = (const R*)s->inv[1]; a.idz = (const R*)s->inv[2];
     a.cpml = s->bc_type == 1 && s->np > 0;
     a.cpx = (const R*)s->cp[0]; a.cpy = (const R*)s->cp[1]; a.cpz = (const R*)s->cp[2];
     for (int q = 0; q < 12; ++q) a.psi[q] = a.psiw[q] = (T*)s->psi[q];
@@ -192,24 +191,19 @@ StepArgs<T, R> make_args(const fdtd_sim* s) {
     a.greg = s->greg;
     a.gPz = s->gPz; a.gPp = s->gPp; a.gNu = s->gNu; a.gNg = s->gNg;
     a.tls = s->d_tls;
+    a.ntls = (int)s->h_tls.size();
     return a;
 }
 
 template <typename R>
 int setup_coeffs(fdtd_sim* s, const fdtd_grid_desc* g, const fdtd_material_desc* m, const fdtd_boundary_desc* bc) {
-    // spacing inverses
+    // per-axis tables [7][n]: inverse spacing, kappa, bE, cE, bH, cH, 1/kappa
     const int dims[3] = {s->nx, s->ny, s->nz};
     const double* inv[3] = {g->inv_dx, g->inv_dy, g->inv_dz};
     for (int a = 0; a < 3; ++a) {
-        auto v = to_r<R>(inv[a], (size_t)dims[a]);
-        R* d;
-        if (int rc = s->upload(&d, v.data(), v.size())) return rc;
-        s->inv[a] = d;
-    }
-    // CPML profiles [5][n]: kappa, bE, cE, bH, cH
-    for (int a = 0; a < 3; ++a) {
         const int n = dims[a];
-        std::vector<R> v((size_t)5 * n);
+        std::vector<R> v((size_t)7 * n);
+        for (int i = 0; i < n; ++i) v[(size_t)5 * n + i] = 0;
         for (int i = 0; i < n; ++i) {
             if (s->bc_type == 1) {
                 const double* K[3] = {bc->kx, bc->ky, bc->kz};
@@ -217,14 +211,18 @@ int setup_coeffs(fdtd_sim* s, const fdtd_grid_desc* g, const fdtd_material_desc*
                 const double* CE[3] = {bc->cEx, bc->cEy, bc->cEz};
                 const double* BH[3] = {bc->bHx, bc->bHy, bc->bHz};
                 const double* CH[3] = {bc->cHx, bc->cHy, bc->cHz};
-                v[i] = (R)K[a][i];
-                v[(size_t)n + i] = (R)BE[a][i];
-                v[(size_t)2 * n + i] = (R)CE[a][i];
-                v[(size_t)3 * n + i] = (R)BH[a][i];
-                v[(size_t)4 * n + i] = (R)CH[a][i];
+                v[(size_t)n + i] = (R)K[a][i];
+                v[(size_t)2 * n + i] = (R)BE[a][i];
+                v[(size_t)3 * n + i] = (R)CE[a][i];
+                v[(size_t)4 * n + i] = (R)BH[a][i];
+                v[(size_t)5 * n + i] = (R)CH[a][i];
             } else {
-                v[i] = 1; v[(size_t)n + i] = 1; v[(size_t)2 * n + i] = 0; v[(size_t)3 * n + i] = 1; v[(size_t)4 * n + i] = 0;
+                v[(size_t)n + i] = 1; v[(size_t)2 * n + i] = 1; v[(size_t)3 * n + i] = 0;
+                v[(size_t)4 * n + i] = 1; v[(size_t)5 * n + i] = 0;
             }
+            v[i] = (R)inv[a][i];
+            const double kap = s->bc_type == 1 ? (a == 0 ? bc->kx : a == 1 ? bc->ky : bc->kz)[i] : 1.0;
+            v[(size_t)6 * n + i] = (R)(1.0 / kap);
         }
         R* d;
         if (int rc = s->upload(&d, v.data(), v.size())) return rc;
@@ -350,20 +348,40 @@ int setup_gain(fdtd_sim* s, const fdtd_tls_desc* t) {
         c.kapa = R.kapa; c.pa2 = R.pa2; c.pa3 = R.pa3; c.inv_dt = R.inv_dt;
         c.inv_tau = 1.0 / R.tau;                        // src/two_level_system.hpp:637
         c.inv_hbar_omega = 1.0 / (R.hbar * R.omega_a);  // :638
-        c.N0 = R.N0; c.dt = R.dt; c.clamp = R.clamp;
+        c.N0 = R.N0; c.inv_N0 = 1.0 / R.N0; c.dt = R.dt; c.clamp = R.clamp;
         s->h_tls[(size_t)r] = c;
     }
     return s->upload(&s->d_tls, s->h_tls.data(), s->h_tls.size());
 }
 
-template <typename T, typename R, int TJ>
-void launch_fused(fdtd_sim* s, StepArgs<T, R>& a) {
-    const int nkt = (s->nz + 63) / 64;
-    const int njt = (s->ny + TJ - 1) / TJ;
-    const int nch = (s->nx + s->ichunk - 1) / s->ichunk;
-    const int nwg = nkt * njt * nch;
-    hipLaunchKernelGGL((fdtdk::k_step_fused<T, R, TJ>), dim3((unsigned)nwg), dim3(64 * (TJ + 2)), 0, s->stream, a,
-                       s->ichunk, nkt, njt, nwg);
+template <typename T, typename R>
+void launch_rows(fdtd_sim* s, StepArgs<T, R>& a) {
+    const int njt = (s->ny + fdtdk::ROW_WAVES - 1) / fdtdk::ROW_WAVES;
+    const int nic = (s->nx + s->ichunk - 1) / s->ichunk;
+    const int nwg = njt * nic;
+    const int nseg = (s->nz + 63) / 64;
+    const unsigned lds = (unsigned)fdtdk::row_lds_bytes<T, R>(s->nz, nseg);
+    hipLaunchKernelGGL((fdtdk::k_step_rows<T, R>), dim3((unsigned)nwg), dim3(64 * fdtdk::ROW_WAVES), lds, s->stream, a,
+                       s->ichunk, njt, nwg, nseg);
+}
+template <typename T, typename R>
+bool rows_fit(const fdtd_sim* s) {
+    return fdtdk::row_lds_bytes<T, R>(s->nz, (s->nz + 63) / 64) <= 48 * 1024 &&
+           (unsigned long)s->N * sizeof(T) < (1ul << 32);
+}
+// The row kernel keeps the z tables and its carried state in LDS and
+// addresses fields by 32-bit offsets.
+// It also stages its small tables (sources, TLS regions) in LDS and needs
+// cell-invariant H coefficients; other grids use the two-kernel schedule.
+bool rows_ok(const fdtd_sim* s) {
+    if (!s->h_uniform || s->nsrc > fdtdk::STAGE_MAX_SRC || (int)s->h_tls.size() > fdtdk::STAGE_MAX_TLS ||
+        s->ichunk > fdtdk::ROW_MAX_CHUNK || s->nz > fdtdk::ROW_MAX_NZ)
+        return false;
+    switch (s->prec) {
+    case FDTD_FP64: return rows_fit<double, double>(s);
+    case FDTD_FP32: return rows_fit<float, float>(s);
+    default: return rows_fit<uint16_t, float>(s);
+    }
 }
 
 void set_cur(fdtd_sim* s, int c) {
@@ -376,19 +394,13 @@ template <typename T, typename R>
 int launch_step(fdtd_sim* s, long n, int sidx, hipEvent_t* ev) {
     StepArgs<T, R> a = make_args<T, R>(s);
     a.srcv = s->d_srcv + (long)sidx * s->nsrc;
-    if (s->mode == 1) {
+    if (s->mode == 1 && rows_ok(s)) {
         const int o = s->cur ^ 1;
         a.Exw = (T*)s->F2[o][0]; a.Eyw = (T*)s->F2[o][1]; a.Ezw = (T*)s->F2[o][2];
         a.Hxw = (T*)s->F2[o][3]; a.Hyw = (T*)s->F2[o][4]; a.Hzw = (T*)s->F2[o][5];
         for (int q = 6; q < 12; ++q) a.psiw[q] = (T*)s->psi2[o][q];
         if (ev) (void)hipEventRecord(ev[0], s->stream);
-        switch (s->tj) {
-        case 2: launch_fused<T, R, 2>(s, a); break;
-        case 6: launch_fused<T, R, 6>(s, a); break;
-        case 8: launch_fused<T, R, 8>(s, a); break;
-        case 14: launch_fused<T, R, 14>(s, a); break;
-        default: launch_fused<T, R, 4>(s, a); break;
-        }
+        launch_rows<T, R>(s, a);
         if (ev) (void)hipEventRecord(ev[1], s->stream);
         if (ev) (void)hipEventRecord(ev[2], s->stream);
         if (ev) (void)hipEventRecord(ev[3], s->stream);
@@ -758,14 +770,11 @@ int fdtd_get_fp16_scales(fdtd_handle h, double* scales18) {
     return FDTD_OK;
 }
 
-int fdtd_set_kernel_mode(fdtd_handle h, int mode, int rows_per_tile, int planes_per_chunk) {
-    if (!h || mode < 0 || mode > 1) return fail(FDTD_EINVAL, "mode must be 0 (two kernels) or 1 (fused)");
-    if (rows_per_tile != 0 && rows_per_tile != 2 && rows_per_tile != 4 && rows_per_tile != 6 && rows_per_tile != 8 &&
-        rows_per_tile != 14)
-        return fail(FDTD_EINVAL, "rows_per_tile must be 2, 4, 6, 8 or 14");
-    if (planes_per_chunk < 0) return fail(FDTD_EINVAL, "planes_per_chunk must be >= 0");
+int fdtd_set_kernel_mode(fdtd_handle h, int mode, int planes_per_chunk) {
+    if (!h || mode < 0 || mode > 1) return fail(FDTD_EINVAL, "mode must be 0 (H and E kernels) or 1 (fused step kernel)");
+    if (planes_per_chunk < 0 || planes_per_chunk > fdtdk::ROW_MAX_CHUNK)
+        return fail(FDTD_EINVAL, "planes_per_chunk must be in [0, 128]");
     h->mode = mode;
-    if (rows_per_tile) h->tj = rows_per_tile;
     if (planes_per_chunk) h->ichunk = planes_per_chunk;
     return FDTD_OK;
 }

@@ -67,7 +67,7 @@ struct SourceBox {
 };
 
 struct TLSCoef {
-    double kapa, pa2, pa3, inv_dt, inv_tau, inv_hbar_omega, N0, dt;
+    double kapa, pa2, pa3, inv_dt, inv_tau, inv_hbar_omega, N0, inv_N0, dt;
     int clamp;
 };
 
@@ -79,8 +79,8 @@ struct StepArgs {
     T *Ex, *Ey, *Ez, *Hx, *Hy, *Hz;        // read buffers (E^n, H^{n-1/2})
     T *Exw, *Eyw, *Ezw, *Hxw, *Hyw, *Hzw;  // write buffers (the same arrays when updating in place)
     int pec;
-    const R *idx, *idy, *idz;
-    // CPML 1D profiles, per axis [5][n]: kappa, bE, cE, bH, cH
+    // Per-axis tables [7][n]: inverse spacing, then the CPML 1D profiles
+    // kappa, bE, cE, bH, cH (kappa 1, b 1, c 0 outside the slabs), 1/kappa.
     const R *cpx, *cpy, *cpz;
     int cpml;
     // psi slabs: 0 Ex_y 1 Ex_z 2 Ey_z 3 Ey_x 4 Ez_x 5 Ez_y 6 Hx_y 7 Hx_z 8 Hy_z 9 Hy_x 10 Hz_x 11 Hz_y
@@ -106,6 +106,7 @@ struct StepArgs {
     const int8_t* greg;
     double *gPz, *gPp, *gNu, *gNg;
     const TLSCoef* tls;
+    int ntls;
 };
 
 __device__ __forceinline__ bool in_pml(int i, int n, int np) {
@@ -133,12 +134,92 @@ __device__ __forceinline__ int mat_of(const void* m, int mid16, long id) {
     return mid16 ? (int)reinterpret_cast<const uint16_t*>(m)[id] : (int)reinterpret_cast<const uint8_t*>(m)[id];
 }
 
+// Where a kernel reads the per-axis tables from: global memory, or the
+// work-group's LDS copy of the rows it touches (row r of axis entry n at
+// p[r * stride + n - off]).
+template <typename R>
+struct Axes {
+    const R* x;
+    int sx, ox;
+    const R* y;
+    int sy, oy;
+    const R* z;
+    int sz, oz;
+    __device__ __forceinline__ R X(int r, int n) const { return x[r * sx + n - ox]; }
+    __device__ __forceinline__ R Y(int r, int n) const { return y[r * sy + n - oy]; }
+    __device__ __forceinline__ R Z(int r, int n) const { return z[r * sz + n - oz]; }
+};
+template <typename T, typename R>
+__device__ __forceinline__ Axes<R> global_axes(const StepArgs<T, R>& a) {
+    return Axes<R>{a.cpx, a.nx, 0, a.cpy, a.ny, 0, a.cpz, a.nz, 0};
+}
+
+// Small read-only tables the compute phase reads: axis tables, the
+// cell-invariant H coefficients, source footprints and values, TLS
+// coefficients.  The fused kernel points them at LDS copies so that the
+// compute phase issues no vector-memory loads (the in-order load counter
+// would otherwise make it wait for every prefetch in flight).
+template <typename R>
+struct Tabs {
+    Axes<R> ax;
+    const R* hco;           // aHx bHx aHy bHy aHz bHz when cell-invariant
+    const SourceBox* src;
+    const double* srcv;
+    const TLSCoef* tls;
+    // When set, the psi and gain arrays are reached through `far`, an LDS
+    // table of byte offsets from `far_base` read at each use (psi 0..11,
+    // psi_H write 12..17, gain Nu Ng Pz Pp 18..21, region ids 22), so that
+    // their addresses do not occupy scalar registers for the whole kernel.
+    bool use_far;
+    char* far_base;
+    const long* far;
+};
+template <typename T, typename R>
+__device__ __forceinline__ Tabs<R> global_tabs(const StepArgs<T, R>& a) {
+    return Tabs<R>{global_axes(a), a.lut + 6 * a.nmat, a.src, a.srcv, a.tls, false, nullptr, nullptr};
+}
+constexpr int FAR_SLOTS = 23;
+// Offsets from a kernel-argument pointer keep the global address space
+// known to the compiler (global, not flat, loads and stores).
+// The slot index goes through a register-only asm (no instruction) so that
+// the compiler cannot hoist the table read out of the step loop.
+template <typename P, typename R>
+__device__ __forceinline__ P far_ptr(const Tabs<R>& tb, int q) {
+    int slot = q;
+    asm volatile("" : "+s"(slot));
+    return reinterpret_cast<P>(tb.far_base + tb.far[slot]);
+}
+// psi array q as read (0..11; psi_E is updated in place, so also its write array)
+template <typename T, typename R>
+__device__ __forceinline__ T* psi_rd(const StepArgs<T, R>& a, const Tabs<R>& tb, int q) {
+    return tb.use_far ? far_ptr<T*>(tb, q) : a.psi[q];
+}
+// psi_H write array (q = 6..11)
+template <typename T, typename R>
+__device__ __forceinline__ T* psi_wr(const StepArgs<T, R>& a, const Tabs<R>& tb, int q) {
+    return tb.use_far ? far_ptr<T*>(tb, q + 6) : a.psiw[q];
+}
+// gain state array: 0 Nu, 1 Ng, 2 Pz, 3 Pz_prev
+template <typename T, typename R>
+__device__ __forceinline__ double* gain_arr(const StepArgs<T, R>& a, const Tabs<R>& tb, int q) {
+    if (tb.use_far) return far_ptr<double*>(tb, 18 + q);
+    return q == 0 ? a.gNu : q == 1 ? a.gNg : q == 2 ? a.gPz : a.gPp;
+}
+
 // One CPML correction (src/boundary.hpp:182-223, 257-298) on an already
 // loaded psi:  psi = b*psi + c*d;  corr = d/kappa + psi;  returns (corr - d)
-// and the new psi in `pnew`.
+// and the new psi in `pnew`.  FP64 divides as the reference does; the FP32
+// and FP16 forms multiply by 1/kappa (rounded from the FP64 value).
 template <typename R>
-__device__ __forceinline__ R cpml_term(R psi_old, R d, R kap, R b, R c, R& pnew) {
-    R corr = d / kap;
+__device__ __forceinline__ R cpml_term(R psi_old, R d, R kap, R ikap, R b, R c, R& pnew) {
+    R corr;
+    if constexpr (sizeof(R) == 8) {
+        (void)ikap;
+        corr = d / kap;
+    } else {
+        (void)kap;
+        corr = d * ikap;
+    }
     const R p = b * psi_old + c * d;
     pnew = p;
     corr += p;
@@ -157,7 +238,7 @@ template <typename R>
 struct HIn {
     R ex, ex_j, ex_k, ey, ey_i, ey_k, ez, ez_i, ez_j, hx, hy, hz;
     R p[6];  // psi_H: Hx_y, Hx_z, Hy_z, Hy_x, Hz_x, Hz_y (where in a slab)
-    int m;   // material id (when H coefficients vary)
+    R cH[6]; // aHx bHx aHy bHy aHz bHz (when H coefficients vary per cell)
     bool upd;
 };
 
@@ -169,15 +250,18 @@ __device__ __forceinline__ HIn<R> h_load(const StepArgs<T, R>& a, int i, int j, 
     v.hy = ld<T, R>(a.Hy, id, a.sc[4]);
     v.hz = ld<T, R>(a.Hz, id, a.sc[5]);
     v.upd = (i < a.nx - 1) & (j < a.ny - 1) & (k < a.nz - 1);
-    v.m = 0;
     for (int q = 0; q < 6; ++q) v.p[q] = 0;
+    for (int q = 0; q < 6; ++q) v.cH[q] = 0;
     v.ex = v.ex_j = v.ex_k = v.ey = v.ey_i = v.ey_k = v.ez = v.ez_i = v.ez_j = 0;
     if (!v.upd) return v;
     const long sI = a.sI, sJ = a.sJ;
     v.ex = ld<T, R>(a.Ex, id, a.sc[0]); v.ex_j = ld<T, R>(a.Ex, id + sJ, a.sc[0]); v.ex_k = ld<T, R>(a.Ex, id + 1, a.sc[0]);
     v.ey = ld<T, R>(a.Ey, id, a.sc[1]); v.ey_i = ld<T, R>(a.Ey, id + sI, a.sc[1]); v.ey_k = ld<T, R>(a.Ey, id + 1, a.sc[1]);
     v.ez = ld<T, R>(a.Ez, id, a.sc[2]); v.ez_i = ld<T, R>(a.Ez, id + sI, a.sc[2]); v.ez_j = ld<T, R>(a.Ez, id + sJ, a.sc[2]);
-    if (!a.h_uniform) v.m = mat_of<R>(a.matid, a.mid16, id);
+    if (!a.h_uniform) {
+        const int m = mat_of<R>(a.matid, a.mid16, id);
+        for (int q = 0; q < 6; ++q) v.cH[q] = a.lut[(6 + q) * a.nmat + m];
+    }
     if (a.cpml) {
         const int np = a.np, nx = a.nx, ny = a.ny, nz = a.nz;
         const bool px = in_pml(i, nx, np), py = in_pml(j, ny, np), pz = in_pml(k, nz, np);
@@ -207,18 +291,24 @@ __device__ __forceinline__ HIn<R> h_load(const StepArgs<T, R>& a, int i, int j, 
 // reference never updates (i = NxT-1, j = NyT-1 or k = NzT-1) return their
 // stored value.  Returns the values a later read of the stored H gives.
 template <typename T, typename R>
-__device__ __forceinline__ H3<R> h_compute(const StepArgs<T, R>& a, int i, int j, int k, const HIn<R>& v, bool own) {
+__device__ __forceinline__ H3<R> h_compute(const StepArgs<T, R>& a, const Tabs<R>& tb, int i, int j, int k,
+                                           const HIn<R>& v, bool own) {
+    const Axes<R>& ax = tb.ax;
     if (!v.upd) return H3<R>{v.hx, v.hy, v.hz};
     const long id = ((long)i * a.ny + j) * a.nz + k;
-    const R idx = a.idx[i], idy = a.idy[j], idz = a.idz[k];
+    const R idx = ax.X(0, i), idy = ax.Y(0, j), idz = ax.Z(0, k);
     const R dEz_y = (v.ez_j - v.ez) * idy, dEy_z = (v.ey_k - v.ey) * idz;
     const R dEx_z = (v.ex_k - v.ex) * idz, dEz_x = (v.ez_i - v.ez) * idx;
     const R dEy_x = (v.ey_i - v.ey) * idx, dEx_y = (v.ex_j - v.ex) * idy;
     const R cx = dEz_y - dEy_z, cy = dEx_z - dEz_x, cz = dEy_x - dEx_y;
-    const int m = v.m, nm = a.nmat;
-    const R aHx = a.lut[6 * nm + m], bHx = a.lut[7 * nm + m];
-    const R aHy = a.lut[8 * nm + m], bHy = a.lut[9 * nm + m];
-    const R aHz = a.lut[10 * nm + m], bHz = a.lut[11 * nm + m];
+    R aHx, bHx, aHy, bHy, aHz, bHz;
+    if (a.h_uniform) {  // cell-invariant
+        const int nm = tb.hco == a.lut + 6 * a.nmat ? a.nmat : 1;
+        aHx = tb.hco[0]; bHx = tb.hco[nm]; aHy = tb.hco[2 * nm];
+        bHy = tb.hco[3 * nm]; aHz = tb.hco[4 * nm]; bHz = tb.hco[5 * nm];
+    } else {
+        aHx = v.cH[0]; bHx = v.cH[1]; aHy = v.cH[2]; bHy = v.cH[3]; aHz = v.cH[4]; bHz = v.cH[5];
+    }
     R hx = aHx * v.hx - bHx * cx;
     R hy = aHy * v.hy - bHy * cy;
     R hz = aHz * v.hz - bHz * cz;
@@ -226,29 +316,28 @@ __device__ __forceinline__ H3<R> h_compute(const StepArgs<T, R>& a, int i, int j
         const int np = a.np, nx = a.nx, ny = a.ny, nz = a.nz;
         const bool px = in_pml(i, nx, np), py = in_pml(j, ny, np), pz = in_pml(k, nz, np);
         if (px | py | pz) {
-            const R* CX = a.cpx; const R* CY = a.cpy; const R* CZ = a.cpz;
             R pn[6];
-            if (py) hx -= bHx * cpml_term<R>(v.p[0], dEz_y, CY[j], CY[3 * ny + j], CY[4 * ny + j], pn[0]);
-            if (pz) hx -= bHx * (-cpml_term<R>(v.p[1], dEy_z, CZ[k], CZ[3 * nz + k], CZ[4 * nz + k], pn[1]));
-            if (pz) hy -= bHy * cpml_term<R>(v.p[2], dEx_z, CZ[k], CZ[3 * nz + k], CZ[4 * nz + k], pn[2]);
-            if (px) hy -= bHy * (-cpml_term<R>(v.p[3], dEz_x, CX[i], CX[3 * nx + i], CX[4 * nx + i], pn[3]));
-            if (px) hz -= bHz * cpml_term<R>(v.p[4], dEy_x, CX[i], CX[3 * nx + i], CX[4 * nx + i], pn[4]);
-            if (py) hz -= bHz * (-cpml_term<R>(v.p[5], dEx_y, CY[j], CY[3 * ny + j], CY[4 * ny + j], pn[5]));
+            if (py) hx -= bHx * cpml_term<R>(v.p[0], dEz_y, ax.Y(1, j), ax.Y(6, j), ax.Y(4, j), ax.Y(5, j), pn[0]);
+            if (pz) hx -= bHx * (-cpml_term<R>(v.p[1], dEy_z, ax.Z(1, k), ax.Z(6, k), ax.Z(4, k), ax.Z(5, k), pn[1]));
+            if (pz) hy -= bHy * cpml_term<R>(v.p[2], dEx_z, ax.Z(1, k), ax.Z(6, k), ax.Z(4, k), ax.Z(5, k), pn[2]);
+            if (px) hy -= bHy * (-cpml_term<R>(v.p[3], dEz_x, ax.X(1, i), ax.X(6, i), ax.X(4, i), ax.X(5, i), pn[3]));
+            if (px) hz -= bHz * cpml_term<R>(v.p[4], dEy_x, ax.X(1, i), ax.X(6, i), ax.X(4, i), ax.X(5, i), pn[4]);
+            if (py) hz -= bHz * (-cpml_term<R>(v.p[5], dEx_y, ax.Y(1, j), ax.Y(6, j), ax.Y(4, j), ax.Y(5, j), pn[5]));
             if (own) {
                 if (py) {
                     const long q = psi_y(i, j, k, nx, ny, nz, np);
-                    st<T, R>(a.psiw[6], q, pn[0], a.sc[30]);
-                    st<T, R>(a.psiw[11], q, pn[5], a.sc[35]);
+                    st<T, R>(psi_wr(a, tb, 6), q, pn[0], a.sc[30]);
+                    st<T, R>(psi_wr(a, tb, 11), q, pn[5], a.sc[35]);
                 }
                 if (pz) {
                     const long q = psi_z(i, j, k, nx, ny, nz, np);
-                    st<T, R>(a.psiw[7], q, pn[1], a.sc[31]);
-                    st<T, R>(a.psiw[8], q, pn[2], a.sc[32]);
+                    st<T, R>(psi_wr(a, tb, 7), q, pn[1], a.sc[31]);
+                    st<T, R>(psi_wr(a, tb, 8), q, pn[2], a.sc[32]);
                 }
                 if (px) {
                     const long q = psi_x(i, j, k, nx, ny, nz, np);
-                    st<T, R>(a.psiw[9], q, pn[3], a.sc[33]);
-                    st<T, R>(a.psiw[10], q, pn[4], a.sc[34]);
+                    st<T, R>(psi_wr(a, tb, 9), q, pn[3], a.sc[33]);
+                    st<T, R>(psi_wr(a, tb, 10), q, pn[4], a.sc[34]);
                 }
             }
         }
@@ -263,8 +352,8 @@ __device__ __forceinline__ H3<R> h_compute(const StepArgs<T, R>& a, int i, int j
 }
 
 template <typename T, typename R>
-__device__ __forceinline__ H3<R> h_new(const StepArgs<T, R>& a, int i, int j, int k, bool own) {
-    return h_compute<T, R>(a, i, j, k, h_load<T, R>(a, i, j, k), own);
+__device__ __forceinline__ H3<R> h_new(const StepArgs<T, R>& a, const Tabs<R>& tb, int i, int j, int k, bool own) {
+    return h_compute<T, R>(a, tb, i, j, k, h_load<T, R>(a, i, j, k), own);
 }
 
 // H update kernel of the two-kernel step.  Threads cover the grid with
@@ -275,13 +364,16 @@ __global__ void __launch_bounds__(256) k_update_H(StepArgs<T, R> a) {
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
     const int i = blockIdx.z;
     if (k >= a.nz - 1 || j >= a.ny - 1 || i >= a.nx - 1) return;
-    (void)h_new<T, R>(a, i, j, k, true);
+    (void)h_new<T, R>(a, global_tabs(a), i, j, k, true);
 }
 
 // Two-level polarization step (update_polarization_multi_region,
-// src/two_level_system.hpp:595-605).  Ndip = Ng0 = N0 on region cells.
+// src/two_level_system.hpp:595-605).  Ndip = Ng0 = N0 on region cells.  The
+// FP64 form divides by N0 as the reference does; the FP32/FP16 forms (whose
+// populations are still FP64) multiply by 1/N0.
+template <bool EXACT>
 __device__ __forceinline__ double tls_polarize(const TLSCoef& c, double Nu, double Ng, double ez, double& Pz, double& Pp) {
-    const double Ng_frac = (Ng - Nu) / c.N0;
+    const double Ng_frac = EXACT ? (Ng - Nu) / c.N0 : (Ng - Nu) * c.inv_N0;
     const double driving = c.N0 * Ng_frac * ez;
     const double Pn = c.kapa * driving + c.pa2 * Pz + c.pa3 * Pp;
     const double Po = Pz;
@@ -321,13 +413,25 @@ template <typename R>
 struct EIn {
     R ex0, ey0, ez0;
     R p[6];  // psi_E: Ex_y, Ex_z, Ey_z, Ey_x, Ez_x, Ez_y (where in a slab)
-    int m, reg;
+    R cE[6]; // aEx bEx aEy bEy aEz bEz of the cell's material
+    int reg;
     long g;
     double Nu, Ng, Pz, Pp;
 };
 
+// Material id of an E cell (the LUT row its coefficients come from).
 template <typename T, typename R>
-__device__ __forceinline__ EIn<R> e_load(const StepArgs<T, R>& a, int i, int j, int k) {
+__device__ __forceinline__ int e_mat(const StepArgs<T, R>& a, int i, int j, int k) {
+    if (!((i > 0) & (j > 0) & (k > 0))) return 0;
+    return mat_of<R>(a.matid, a.mid16, ((long)i * a.ny + j) * a.nz + k);
+}
+
+// Gathers the E inputs of a cell whose material id `m` is already known (the
+// fused kernel loads ids one plane earlier, so this gather never waits on a
+// load issued in the same pass).  Gain state is loaded for every cell of the
+// gain box (zero outside the regions), so no load depends on the region id.
+template <typename T, typename R>
+__device__ __forceinline__ EIn<R> e_load(const StepArgs<T, R>& a, int i, int j, int k, int m) {
     EIn<R> v;
     const long id = ((long)i * a.ny + j) * a.nz + k;
     v.reg = -1;
@@ -338,20 +442,27 @@ __device__ __forceinline__ EIn<R> e_load(const StepArgs<T, R>& a, int i, int j, 
         if ((unsigned)gi < (unsigned)a.gnx && (unsigned)gj < (unsigned)a.gny && (unsigned)gk < (unsigned)a.gnz) {
             v.g = ((long)gi * a.gny + gj) * a.gnz + gk;
             v.reg = a.greg[v.g];
-            if (v.reg >= 0) {
-                v.Nu = a.gNu[v.g]; v.Ng = a.gNg[v.g]; v.Pz = a.gPz[v.g]; v.Pp = a.gPp[v.g];
-            }
+            v.Nu = a.gNu[v.g]; v.Ng = a.gNg[v.g]; v.Pz = a.gPz[v.g]; v.Pp = a.gPp[v.g];
         }
     }
     for (int q = 0; q < 6; ++q) v.p[q] = 0;
     v.ex0 = v.ey0 = 0;
     v.ez0 = ld<T, R>(a.Ez, id, a.sc[2]);
-    v.m = 0;
+    for (int q = 0; q < 6; ++q) v.cE[q] = 0;
     const bool interior = (i > 0) & (j > 0) & (k > 0);
     if (!interior) return v;
     v.ex0 = ld<T, R>(a.Ex, id, a.sc[0]);
     v.ey0 = ld<T, R>(a.Ey, id, a.sc[1]);
-    v.m = mat_of<R>(a.matid, a.mid16, id);
+    {
+        v.cE[1] = a.lut[1 * a.nmat + m];
+        v.cE[3] = a.lut[3 * a.nmat + m];
+        v.cE[5] = a.lut[5 * a.nmat + m];
+        if (!a.lossless) {
+            v.cE[0] = a.lut[0 * a.nmat + m];
+            v.cE[2] = a.lut[2 * a.nmat + m];
+            v.cE[4] = a.lut[4 * a.nmat + m];
+        }
+    }
     if (a.cpml) {
         const int np = a.np, nx = a.nx, ny = a.ny, nz = a.nz;
         const bool px = in_pml(i, nx, np), py = in_pml(j, ny, np), pz = in_pml(k, nz, np);
@@ -387,8 +498,11 @@ __device__ __forceinline__ EIn<R> e_load(const StepArgs<T, R>& a, int i, int j, 
 // boundary the six faces are written as zero (PECBoundary::apply_after_E,
 // src/boundary.hpp:55-74).
 template <typename T, typename R>
-__device__ __forceinline__ void e_compute(const StepArgs<T, R>& a, int i, int j, int k, const EIn<R>& v, R hx, R hy,
-                                          R hz, R hx_j, R hz_j, R hx_k, R hy_k, R hy_i, R hz_i) {
+__device__ __forceinline__ void e_compute(const StepArgs<T, R>& a, const Tabs<R>& tb, int i, int j, int k,
+                                          const EIn<R>& v, R hx, R hy, R hz, R hx_j, R hz_j, R hx_k, R hy_k, R hy_i,
+                                          R hz_i) {
+    const Axes<R>& ax = tb.ax;
+    const TLSCoef* tls = tb.tls;
     const long id = ((long)i * a.ny + j) * a.nz + k;
     const int reg = v.reg;
     const long g = v.g;
@@ -396,12 +510,12 @@ __device__ __forceinline__ void e_compute(const StepArgs<T, R>& a, int i, int j,
     const bool pec_face = a.pec && (!interior || i == a.nx - 1 || j == a.ny - 1 || k == a.nz - 1);
     if (!interior) {
         if (reg >= 0) {
-            const TLSCoef c = a.tls[reg];
+            const TLSCoef c = tls[reg];
             double Nu = v.Nu, Ng = v.Ng, Pz = v.Pz, Pp = v.Pp;
             const double ez = (double)v.ez0;
-            (void)tls_polarize(c, Nu, Ng, ez, Pz, Pp);
+            (void)tls_polarize<sizeof(R) == 8>(c, Nu, Ng, ez, Pz, Pp);
             tls_populate(c, ez, 0.0, Pz, Pp, Nu, Ng);
-            a.gNu[g] = Nu; a.gNg[g] = Ng; a.gPz[g] = Pz; a.gPp[g] = Pp;
+            gain_arr(a, tb, 0)[g] = Nu; gain_arr(a, tb, 1)[g] = Ng; gain_arr(a, tb, 2)[g] = Pz; gain_arr(a, tb, 3)[g] = Pp;
         }
         if (pec_face) {
             st<T, R>(a.Exw, id, (R)0, a.sc[18]);
@@ -410,25 +524,24 @@ __device__ __forceinline__ void e_compute(const StepArgs<T, R>& a, int i, int j,
         }
         return;
     }
-    const R idx = a.idx[i], idy = a.idy[j], idz = a.idz[k];
+    const R idx = ax.X(0, i), idy = ax.Y(0, j), idz = ax.Z(0, k);
     const R dHz_y = (hz - hz_j) * idy, dHy_z = (hy - hy_k) * idz;
     const R dHx_z = (hx - hx_k) * idz, dHz_x = (hz - hz_i) * idx;
     const R dHy_x = (hy - hy_i) * idx, dHx_y = (hx - hx_j) * idy;
     const R cx = dHz_y - dHy_z, cy = dHx_z - dHz_x, cz = dHy_x - dHx_y;
     R jx = 0, jy = 0, jz = 0;
     for (int s = 0; s < a.nsrc; ++s) {
-        const SourceBox b = a.src[s];
+        const SourceBox b = tb.src[s];
         if (i >= b.i0 && i <= b.i1 && j >= b.j0 && j <= b.j1 && k >= b.k0 && k <= b.k1) {
-            const R val = (R)a.srcv[s];
+            const R val = (R)tb.srcv[s];
             if (b.comp == 0) jx += val; else if (b.comp == 1) jy += val; else jz += val;
         }
     }
-    const int m = v.m, nm = a.nmat;
-    const R bEx = a.lut[1 * nm + m], bEy = a.lut[3 * nm + m], bEz = a.lut[5 * nm + m];
+    const R bEx = v.cE[1], bEy = v.cE[3], bEz = v.cE[5];
     double Nu = v.Nu, Ng = v.Ng, Pz = v.Pz, Pp = v.Pp;
     R rz = cz - jz;
     if (reg >= 0) {
-        const double dP = tls_polarize(a.tls[reg], Nu, Ng, (double)v.ez0, Pz, Pp);
+        const double dP = tls_polarize<sizeof(R) == 8>(tls[reg], Nu, Ng, (double)v.ez0, Pz, Pp);
         rz = rz - (R)dP;
     }
     R ex, ey, ez;
@@ -437,36 +550,35 @@ __device__ __forceinline__ void e_compute(const StepArgs<T, R>& a, int i, int j,
         ey = v.ey0 + bEy * (cy - jy);
         ez = v.ez0 + bEz * rz;
     } else {
-        ex = a.lut[0 * nm + m] * v.ex0 + bEx * (cx - jx);
-        ey = a.lut[2 * nm + m] * v.ey0 + bEy * (cy - jy);
-        ez = a.lut[4 * nm + m] * v.ez0 + bEz * rz;
+        ex = v.cE[0] * v.ex0 + bEx * (cx - jx);
+        ey = v.cE[2] * v.ey0 + bEy * (cy - jy);
+        ez = v.cE[4] * v.ez0 + bEz * rz;
     }
     if (a.cpml) {
         const int np = a.np, nx = a.nx, ny = a.ny, nz = a.nz;
         const bool px = in_pml(i, nx, np), py = in_pml(j, ny, np), pz = in_pml(k, nz, np);
         if (px | py | pz) {
-            const R* CX = a.cpx; const R* CY = a.cpy; const R* CZ = a.cpz;
             R pn[6];
-            if (py) ex += bEx * cpml_term<R>(v.p[0], dHz_y, CY[j], CY[1 * ny + j], CY[2 * ny + j], pn[0]);
-            if (pz) ex += bEx * (-cpml_term<R>(v.p[1], dHy_z, CZ[k], CZ[1 * nz + k], CZ[2 * nz + k], pn[1]));
-            if (pz) ey += bEy * cpml_term<R>(v.p[2], dHx_z, CZ[k], CZ[1 * nz + k], CZ[2 * nz + k], pn[2]);
-            if (px) ey += bEy * (-cpml_term<R>(v.p[3], dHz_x, CX[i], CX[1 * nx + i], CX[2 * nx + i], pn[3]));
-            if (px) ez += bEz * cpml_term<R>(v.p[4], dHy_x, CX[i], CX[1 * nx + i], CX[2 * nx + i], pn[4]);
-            if (py) ez += bEz * (-cpml_term<R>(v.p[5], dHx_y, CY[j], CY[1 * ny + j], CY[2 * ny + j], pn[5]));
+            if (py) ex += bEx * cpml_term<R>(v.p[0], dHz_y, ax.Y(1, j), ax.Y(6, j), ax.Y(2, j), ax.Y(3, j), pn[0]);
+            if (pz) ex += bEx * (-cpml_term<R>(v.p[1], dHy_z, ax.Z(1, k), ax.Z(6, k), ax.Z(2, k), ax.Z(3, k), pn[1]));
+            if (pz) ey += bEy * cpml_term<R>(v.p[2], dHx_z, ax.Z(1, k), ax.Z(6, k), ax.Z(2, k), ax.Z(3, k), pn[2]);
+            if (px) ey += bEy * (-cpml_term<R>(v.p[3], dHz_x, ax.X(1, i), ax.X(6, i), ax.X(2, i), ax.X(3, i), pn[3]));
+            if (px) ez += bEz * cpml_term<R>(v.p[4], dHy_x, ax.X(1, i), ax.X(6, i), ax.X(2, i), ax.X(3, i), pn[4]);
+            if (py) ez += bEz * (-cpml_term<R>(v.p[5], dHx_y, ax.Y(1, j), ax.Y(6, j), ax.Y(2, j), ax.Y(3, j), pn[5]));
             if (py) {
                 const long q = psi_y(i, j, k, nx, ny, nz, np);
-                st<T, R>(a.psi[0], q, pn[0], a.sc[24]);
-                st<T, R>(a.psi[5], q, pn[5], a.sc[29]);
+                st<T, R>(psi_rd(a, tb, 0), q, pn[0], a.sc[24]);
+                st<T, R>(psi_rd(a, tb, 5), q, pn[5], a.sc[29]);
             }
             if (pz) {
                 const long q = psi_z(i, j, k, nx, ny, nz, np);
-                st<T, R>(a.psi[1], q, pn[1], a.sc[25]);
-                st<T, R>(a.psi[2], q, pn[2], a.sc[26]);
+                st<T, R>(psi_rd(a, tb, 1), q, pn[1], a.sc[25]);
+                st<T, R>(psi_rd(a, tb, 2), q, pn[2], a.sc[26]);
             }
             if (px) {
                 const long q = psi_x(i, j, k, nx, ny, nz, np);
-                st<T, R>(a.psi[3], q, pn[3], a.sc[27]);
-                st<T, R>(a.psi[4], q, pn[4], a.sc[28]);
+                st<T, R>(psi_rd(a, tb, 3), q, pn[3], a.sc[27]);
+                st<T, R>(psi_rd(a, tb, 4), q, pn[4], a.sc[28]);
             }
         }
     }
@@ -475,11 +587,11 @@ __device__ __forceinline__ void e_compute(const StepArgs<T, R>& a, int i, int j,
     st<T, R>(a.Eyw, id, ey, a.sc[19]);
     st<T, R>(a.Ezw, id, ez, a.sc[20]);
     if (reg >= 0) {
-        const TLSCoef& c = a.tls[reg];
+        const TLSCoef& c = tls[reg];
         // The reference stores Ez and reads it back for the population pass.
         const double ez_new = stored_value<T, R>(ez, a.sc[2], a.sc[20]);
         tls_populate(c, ez_new, (double)v.ez0, Pz, Pp, Nu, Ng);
-        a.gNu[g] = Nu; a.gNg[g] = Ng; a.gPz[g] = Pz; a.gPp[g] = Pp;
+        gain_arr(a, tb, 0)[g] = Nu; gain_arr(a, tb, 1)[g] = Ng; gain_arr(a, tb, 2)[g] = Pz; gain_arr(a, tb, 3)[g] = Pp;
     }
 }
 
@@ -499,98 +611,322 @@ __global__ void __launch_bounds__(256) k_update_E(StepArgs<T, R> a) {
         hy = ld<T, R>(a.Hy, id, a.sc[4]); hy_i = ld<T, R>(a.Hy, id - sI, a.sc[4]); hy_k = ld<T, R>(a.Hy, id - 1, a.sc[4]);
         hz = ld<T, R>(a.Hz, id, a.sc[5]); hz_i = ld<T, R>(a.Hz, id - sI, a.sc[5]); hz_j = ld<T, R>(a.Hz, id - sJ, a.sc[5]);
     }
-    const EIn<R> v = e_load<T, R>(a, i, j, k);
-    e_compute<T, R>(a, i, j, k, v, hx, hy, hz, hx_j, hz_j, hx_k, hy_k, hy_i, hz_i);
+    const EIn<R> v = e_load<T, R>(a, i, j, k, e_mat(a, i, j, k));
+    e_compute<T, R>(a, global_tabs(a), i, j, k, v, hx, hy, hz, hx_j, hz_j, hx_k, hy_k, hy_i, hz_i);
 }
 
-// Fused step kernel: one launch advances H and E of the whole grid.
-// A work-group owns a tile of 64 k-values (one per lane) by TJ rows in j and
-// marches along i over a chunk of planes.  It has TJ+2 wavefronts: wavefront
-// 1..TJ own one row each; wavefront 0 recomputes H of the row below the tile
-// (j0-1) and wavefront TJ+1 recomputes H of the column left of the tile
-// (k0-1) for the TJ rows.  The loop is software-pipelined: the H and E
-// inputs of plane i+1 are loaded while plane i is computed.  Per plane every
-// wavefront computes its H, the halo values go through LDS, one barrier, then
-// the owning wavefronts update E with H(i-1) carried in registers from the
-// previous plane and H(k-1) from the neighbouring lane.  Reads come from
-// buffer A (E^n, H^{n-1/2}, psi_H), writes go to buffer B, so no tile ever
-// reads a neighbour's half-updated state; psi_E and the gain state are
-// touched only by their owning thread and stay in place.  A chunk starting at
-// i0 > 0 first recomputes H(i0-1) of its cells.  Work-group ids are remapped
-// so that each XCD (ids are dealt round-robin over 8) receives a contiguous
-// range of tiles and keeps the halo rows in its own L2.
-template <typename T, typename R, int TJ>
-__global__ void __launch_bounds__(64 * (TJ + 2)) k_step_fused(StepArgs<T, R> a, int ichunk, int nkt, int njt, int nwg) {
-    __shared__ R lj[2][TJ][2][64];  // H(x, z) of rows j0-1 .. j0+TJ-2
-    __shared__ R lk[2][TJ][2];      // H(x, y) at k0-1 of rows j0 .. j0+TJ-1
+constexpr int STAGE_MAX_TLS = 16;  // TLS regions staged in LDS by the row kernel
+constexpr int STAGE_MAX_SRC = 16;  // sources staged in LDS by the row kernel
+
+// Row-marching step kernel: one launch advances H and E of the whole grid,
+// with no barrier and no data exchange between wavefronts.  A wavefront owns
+// one j-row over a chunk of x-planes [i0, i1) and, per plane, walks the row's
+// NCH 64-cell k-segments.  Per (plane, segment) item it
+//   - computes H^{n+1/2} of its own cells and stores it,
+//   - recomputes Hx, Hz of the row below (j-1), which its E update needs,
+//     without storing them (the owner of that row stores the same values),
+//   - takes H(k-1) from the neighbouring lane (lane 0: the previous segment's
+//     lane 63), H(i-1) from registers carried from the previous plane,
+//   - updates E^{n+1} with J, CPML and gain.
+// E^n(i+1) loaded for the H update is carried to the next plane as E^n(i).
+// Latency is hidden by occupancy (many independent wavefronts per CU), not by
+// prefetching into registers: with the in-order load counter, a prefetch
+// that is issued under a branch forces the compiler to wait for it as well.
+// The compute phase reads only registers and LDS.  Reads come from buffer
+// A (E^n, H^{n-1/2}, psi_H), H/E/psi_H writes go to buffer B; psi_E and the
+// gain state are touched only by their owner and stay in place.
+constexpr int ROW_WAVES = 8;           // wavefronts (rows) per work-group
+constexpr int ROW_MAX_CHUNK = 128;     // planes per chunk
+constexpr int ROW_MAX_NZ = 1024;       // segments per row <= 16
+
+// Loads by 32-bit element offset (the byte offset fits 32 bits: checked on
+// the host), so that an address is one VGPR added to the array base.
+template <typename T, typename R>
+__device__ __forceinline__ R ldu(const T* p, unsigned i, R scale) {
+    return ld<T, R>(reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + i * (unsigned)sizeof(T)), 0, scale);
+}
+__device__ __forceinline__ int matu(const void* m, int mid16, unsigned i) {
+    const char* b = reinterpret_cast<const char*>(m);
+    return mid16 ? (int)*reinterpret_cast<const uint16_t*>(b + 2u * i) : (int)*reinterpret_cast<const uint8_t*>(b + i);
+}
+
+template <typename R>
+struct RowIn {
+    R ex, hx, hy, hz, eyn, ezn, exk, eyk, exj, ezj;  // own row: E, H at i; Ey, Ez at i+1; Ex, Ey at k+1; Ex, Ez at j+1
+    R gex, gez, ghx, ghz, geyn, geyk;                // row j-1: Ex, Ez, Hx, Hz at i; Ey at i+1, at k+1
+    R ph[6];  // psi_H of the own cell (Hx_y, Hx_z, Hy_z, Hy_x, Hz_x, Hz_y)
+    R qh[4];  // psi_H of the cell at j-1 (Hx_y, Hx_z, Hz_x, Hz_y)
+    R pe[6];  // psi_E of the own cell
+    R cE[6];
+    int reg;  // TLS region of the cell (-1: none)
+    double Nu, Ng, Pz, Pp;
+};
+
+// Material id of a row-kernel item, loaded one item ahead.  Every lane reads
+// a valid cell (its value is discarded where the cell is not an E cell).
+template <typename T, typename R>
+__device__ __forceinline__ int row_mat(const StepArgs<T, R>& a, int i, int j, int k) {
+    const unsigned id = ((unsigned)min(i, a.nx - 1) * a.ny + j) * a.nz + (k < a.nz ? k : 0);
+    return matu(a.matid, a.mid16, id);
+}
+
+// Gathers the inputs of one item in one round trip: the coefficient gathers
+// use the material id loaded one item earlier (`m`), then the id of the next
+// item is issued (`mnext`), then the field, psi and gain loads.  The field
+// loads are unconditional (a lane that does not need a value reads its own
+// cell instead and discards it).
+template <typename T, typename R>
+__device__ __forceinline__ void row_load(const StepArgs<T, R>& a, const Tabs<R>& tb, int i, int j, int k, int m, int in,
+                                         int kn, int& mnext, RowIn<R>& v) {
+    const int nx = a.nx, ny = a.ny, nz = a.nz, np = a.np;
+    const unsigned sI = (unsigned)a.sI, sJ = (unsigned)a.sJ;
+    const bool kin = k < nz;
+    const unsigned id = ((unsigned)i * ny + j) * nz + (kin ? k : 0);  // a valid cell for every lane
+    const bool hup = kin & (i < nx - 1) & (j < ny - 1) & (k < nz - 1);
+    const bool halo = kin & (j > 0);
+    const bool gup = halo & (i < nx - 1) & (k < nz - 1);
+    const bool nxt = kin & (i + 1 < nx);
+    const bool ein = kin & (i > 0) & (j > 0) & (k > 0);
+    const int mm = ein ? m : 0;
+    for (int q = 0; q < 6; ++q) v.cE[q] = a.lut[q * a.nmat + mm];
+    mnext = row_mat(a, in, j, kn);
+    const R ex = ldu<T, R>(a.Ex, id, a.sc[0]);
+    const R hx = ldu<T, R>(a.Hx, id, a.sc[3]);
+    const R hy = ldu<T, R>(a.Hy, id, a.sc[4]);
+    const R hz = ldu<T, R>(a.Hz, id, a.sc[5]);
+    const R eyn = ldu<T, R>(a.Ey, nxt ? id + sI : id, a.sc[1]);
+    const R ezn = ldu<T, R>(a.Ez, nxt ? id + sI : id, a.sc[2]);
+    const R exk = ldu<T, R>(a.Ex, hup ? id + 1 : id, a.sc[0]);
+    const R eyk = ldu<T, R>(a.Ey, hup ? id + 1 : id, a.sc[1]);
+    const R exj = ldu<T, R>(a.Ex, hup ? id + sJ : id, a.sc[0]);
+    const R ezj = ldu<T, R>(a.Ez, hup ? id + sJ : id, a.sc[2]);
+    const R ghx = ldu<T, R>(a.Hx, halo ? id - sJ : id, a.sc[3]);
+    const R ghz = ldu<T, R>(a.Hz, halo ? id - sJ : id, a.sc[5]);
+    const R geyn = ldu<T, R>(a.Ey, (halo & nxt) ? id - sJ + sI : id, a.sc[1]);
+    const R gex = ldu<T, R>(a.Ex, gup ? id - sJ : id, a.sc[0]);
+    const R gez = ldu<T, R>(a.Ez, gup ? id - sJ : id, a.sc[2]);
+    const R geyk = ldu<T, R>(a.Ey, gup ? id - sJ + 1 : id, a.sc[1]);
+    // Values a lane does not need are left as loaded: the compute reads a
+    // value only where its condition holds (a select here would wait for
+    // each load as it is issued).
+    v.ex = ex; v.hx = hx; v.hy = hy; v.hz = hz; v.eyn = eyn; v.ezn = ezn;
+    v.exk = exk; v.eyk = eyk; v.exj = exj; v.ezj = ezj;
+    v.ghx = ghx; v.ghz = ghz; v.geyn = geyn; v.gex = gex; v.gez = gez; v.geyk = geyk;
+    for (int q = 0; q < 6; ++q) v.ph[q] = v.pe[q] = 0;
+    for (int q = 0; q < 4; ++q) v.qh[q] = 0;
+    if (a.cpml) {
+        const bool px = in_pml(i, nx, np), py = in_pml(j, ny, np), pz = in_pml(k, nz, np);
+        const bool pyg = (j > 0) && in_pml(j - 1, ny, np);
+        if (hup) {
+            if (py) {
+                const unsigned q = (unsigned)psi_y(i, j, k, nx, ny, nz, np);
+                v.ph[0] = ldu<T, R>(psi_rd(a, tb, 6), q, a.sc[12]);
+                v.ph[5] = ldu<T, R>(psi_rd(a, tb, 11), q, a.sc[17]);
+            }
+            if (pz) {
+                const unsigned q = (unsigned)psi_z(i, j, k, nx, ny, nz, np);
+                v.ph[1] = ldu<T, R>(psi_rd(a, tb, 7), q, a.sc[13]);
+                v.ph[2] = ldu<T, R>(psi_rd(a, tb, 8), q, a.sc[14]);
+            }
+            if (px) {
+                const unsigned q = (unsigned)psi_x(i, j, k, nx, ny, nz, np);
+                v.ph[3] = ldu<T, R>(psi_rd(a, tb, 9), q, a.sc[15]);
+                v.ph[4] = ldu<T, R>(psi_rd(a, tb, 10), q, a.sc[16]);
+            }
+        }
+        if (gup) {
+            if (pyg) {
+                const unsigned q = (unsigned)psi_y(i, j - 1, k, nx, ny, nz, np);
+                v.qh[0] = ldu<T, R>(psi_rd(a, tb, 6), q, a.sc[12]);
+                v.qh[3] = ldu<T, R>(psi_rd(a, tb, 11), q, a.sc[17]);
+            }
+            if (pz) v.qh[1] = ldu<T, R>(psi_rd(a, tb, 7), (unsigned)psi_z(i, j - 1, k, nx, ny, nz, np), a.sc[13]);
+            if (px) v.qh[2] = ldu<T, R>(psi_rd(a, tb, 10), (unsigned)psi_x(i, j - 1, k, nx, ny, nz, np), a.sc[16]);
+        }
+        if (ein) {
+            if (py) {
+                const unsigned q = (unsigned)psi_y(i, j, k, nx, ny, nz, np);
+                v.pe[0] = ldu<T, R>(psi_rd(a, tb, 0), q, a.sc[6]);
+                v.pe[5] = ldu<T, R>(psi_rd(a, tb, 5), q, a.sc[11]);
+            }
+            if (pz) {
+                const unsigned q = (unsigned)psi_z(i, j, k, nx, ny, nz, np);
+                v.pe[1] = ldu<T, R>(psi_rd(a, tb, 1), q, a.sc[7]);
+                v.pe[2] = ldu<T, R>(psi_rd(a, tb, 2), q, a.sc[8]);
+            }
+            if (px) {
+                const unsigned q = (unsigned)psi_x(i, j, k, nx, ny, nz, np);
+                v.pe[3] = ldu<T, R>(psi_rd(a, tb, 3), q, a.sc[9]);
+                v.pe[4] = ldu<T, R>(psi_rd(a, tb, 4), q, a.sc[10]);
+            }
+        }
+    }
+    v.reg = -1;
+    v.Nu = v.Ng = v.Pz = v.Pp = 0;
+    if (a.gain && kin) {
+        const int gi = i - a.g0x, gj = j - a.g0y, gk = k - a.g0z;
+        if ((unsigned)gi < (unsigned)a.gnx && (unsigned)gj < (unsigned)a.gny && (unsigned)gk < (unsigned)a.gnz) {
+            const unsigned g = ((unsigned)gi * a.gny + gj) * a.gnz + gk;
+            v.reg = far_ptr<const int8_t*>(tb, 22)[g];
+            v.Nu = gain_arr(a, tb, 0)[g]; v.Ng = gain_arr(a, tb, 1)[g];
+            v.Pz = gain_arr(a, tb, 2)[g]; v.Pp = gain_arr(a, tb, 3)[g];
+        }
+    }
+}
+
+// Dynamic LDS of the row kernel: the z axis tables [7][nz] and, per
+// wavefront, the state carried from plane to plane [5][nseg][64] in the
+// storage format (exact: every carried value is a stored value).
+template <typename T, typename R>
+__host__ __device__ constexpr unsigned long row_lds_bytes(int nz, int nseg) {
+    return (unsigned long)7 * nz * sizeof(R) + (unsigned long)ROW_WAVES * 5 * nseg * 64 * sizeof(T);
+}
+
+template <typename T, typename R>
+__global__ void __launch_bounds__(64 * ROW_WAVES) k_step_rows(StepArgs<T, R> a, int ichunk, int njt, int nwg, int nseg) {
+    extern __shared__ __align__(16) unsigned char row_lds[];
+    __shared__ R ty[7][ROW_WAVES + 1];
+    __shared__ R tx[7][ROW_MAX_CHUNK + 1];
+    __shared__ R th[6];
+    __shared__ TLSCoef tl[STAGE_MAX_TLS];
+    __shared__ SourceBox tsb[STAGE_MAX_SRC];
+    __shared__ double tsv[STAGE_MAX_SRC];
+    __shared__ float tsc[54];
+    __shared__ long tfar[FAR_SLOTS];
+    R* tz = reinterpret_cast<R*>(row_lds);
     const int b = blockIdx.x;
     const int bb = (nwg % 8 == 0) ? (b % 8) * (nwg / 8) + b / 8 : b;
-    const int kt = bb % nkt;
-    const int rest = bb / nkt;
-    const int jt = rest % njt;
-    const int c = rest / njt;
+    const int jt = bb % njt;
+    const int ic = bb / njt;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int k0 = kt * 64, j0 = jt * TJ;
-    const int i0 = c * ichunk;
+    const int j0 = jt * ROW_WAVES;
+    const int j = j0 + w;
+    const int i0 = ic * ichunk;
     const int i1 = min(i0 + ichunk, a.nx);
-    int j, k;
-    bool act, own;
-    if (w <= TJ) {
-        j = j0 + w - 1;
-        k = k0 + lane;
-        act = (j >= 0) & (j < a.ny) & (k < a.nz);
-        own = w >= 1;
-    } else {
-        j = j0 + lane;
-        k = k0 - 1;
-        act = (lane < TJ) & (j < a.ny) & (k >= 0);
-        own = false;
+    for (int t = threadIdx.x; t < 7 * a.nz; t += blockDim.x) tz[t] = a.cpz[t];
+    for (int t = threadIdx.x; t < 7 * (ROW_WAVES + 1); t += blockDim.x) {
+        const int r = t / (ROW_WAVES + 1), n = j0 - 1 + t % (ROW_WAVES + 1);
+        ty[r][t % (ROW_WAVES + 1)] = (n >= 0 && n < a.ny) ? a.cpy[r * a.ny + n] : (R)0;
     }
-    const bool erow = own && act;
-    R hyi = 0, hzi = 0;
-    if (erow && i0 > 0) {
-        const H3<R> h = h_new<T, R>(a, i0 - 1, j, k, false);
-        hyi = h.y;
-        hzi = h.z;
+    for (int t = threadIdx.x; t < 7 * (ROW_MAX_CHUNK + 1); t += blockDim.x) {
+        const int r = t / (ROW_MAX_CHUNK + 1), n = i0 - 1 + t % (ROW_MAX_CHUNK + 1);
+        tx[r][t % (ROW_MAX_CHUNK + 1)] = (n >= 0 && n < a.nx) ? a.cpx[r * a.nx + n] : (R)0;
     }
-    HIn<R> hin;
-    EIn<R> ein;
-    if (act) hin = h_load<T, R>(a, i0, j, k);
-    if (erow) ein = e_load<T, R>(a, i0, j, k);
-    int par = 0;
-    for (int i = i0; i < i1; ++i) {
-        HIn<R> hnext;
-        EIn<R> enext;
-        const bool more = i + 1 < i1;
-        if (act && more) hnext = h_load<T, R>(a, i + 1, j, k);
-        if (erow && more) enext = e_load<T, R>(a, i + 1, j, k);
-        H3<R> h{0, 0, 0};
-        if (act) h = h_compute<T, R>(a, i, j, k, hin, own);
-        if (w < TJ) {
-            lj[par][w][0][lane] = h.x;
-            lj[par][w][1][lane] = h.z;
-        }
-        if (w == TJ + 1 && lane < TJ) {
-            lk[par][lane][0] = h.x;
-            lk[par][lane][1] = h.y;
-        }
-        R hxk = __shfl_up(h.x, 1), hyk = __shfl_up(h.y, 1);
-        __syncthreads();
-        if (erow) {
-            const int r = w - 1;
-            if (lane == 0) {
-                hxk = lk[par][r][0];
-                hyk = lk[par][r][1];
+    if (threadIdx.x < 6) th[threadIdx.x] = a.lut[(6 + threadIdx.x) * a.nmat];
+    if (a.gain && threadIdx.x < a.ntls) tl[threadIdx.x] = a.tls[threadIdx.x];
+    if (threadIdx.x < a.nsrc) {
+        tsb[threadIdx.x] = a.src[threadIdx.x];
+        tsv[threadIdx.x] = a.srcv[threadIdx.x];
+    }
+    if (sizeof(T) == 2 && threadIdx.x < 54) tsc[threadIdx.x] = a.sc[threadIdx.x];
+    if (threadIdx.x < FAR_SLOTS) {
+        const int q = threadIdx.x;
+        const char* p = q < 12 ? (const char*)a.psi[q] : q < 18 ? (const char*)a.psiw[q - 6]
+                      : q == 18 ? (const char*)a.gNu : q == 19 ? (const char*)a.gNg : q == 20 ? (const char*)a.gPz
+                      : q == 21 ? (const char*)a.gPp : (const char*)a.greg;
+        tfar[q] = p ? (long)(p - (const char*)a.lut) : 0;
+    }
+    __syncthreads();
+    if (j >= a.ny) return;  // no barrier follows
+    StepArgs<T, R> s = a;
+    if (sizeof(T) == 2) s.sc = tsc;
+    const Tabs<R> tb{Axes<R>{&tx[0][0], ROW_MAX_CHUNK + 1, i0 - 1, &ty[0][0], ROW_WAVES + 1, j0 - 1, tz, s.nz, 0},
+                     &th[0], &tsb[0], &tsv[0], &tl[0], true, (char*)a.lut, &tfar[0]};
+    // this wavefront's carried state: E^n(i) of the own row (Ey, Ez) and of
+    // row j-1 (Ey), H^{n+1/2}(i-1) of the own row (Hy, Hz)
+    T* cw = reinterpret_cast<T*>(row_lds + 7 * s.nz * sizeof(R)) + (long)w * 5 * nseg * 64 + lane;
+    auto carry = [&](int q, int c) -> T& { return cw[(q * nseg + c) * 64]; };
+    auto put = [&](int q, int c, R v, int f) {
+        if constexpr (sizeof(T) == 2) carry(q, c) = (T)__half_as_ushort(__float2half_rn((float)(v * s.sc[18 + f])));
+        else carry(q, c) = (T)v;
+    };
+    auto get = [&](int q, int c, int f) -> R { return ld<T, R>(&carry(q, c), 0, s.sc[f]); };
+#pragma unroll 1
+    for (int c = 0; c < nseg; ++c) {
+        const int k = c * 64 + lane;
+        R ey = 0, ez = 0, eyg = 0, hyp = 0, hzp = 0;
+        if (k < s.nz) {
+            const long id = ((long)i0 * s.ny + j) * s.nz + k;
+            ey = ld<T, R>(s.Ey, id, s.sc[1]);
+            ez = ld<T, R>(s.Ez, id, s.sc[2]);
+            if (j > 0) eyg = ld<T, R>(s.Ey, id - s.sJ, s.sc[1]);
+            if (i0 > 0) {
+                const H3<R> h = h_new<T, R>(s, tb, i0 - 1, j, k, false);
+                hyp = h.y;
+                hzp = h.z;
             }
-            const R hxj = lj[par][r][0][lane], hzj = lj[par][r][1][lane];
-            e_compute<T, R>(a, i, j, k, ein, h.x, h.y, h.z, hxj, hzj, hxk, hyk, hyi, hzi);
-            hyi = h.y;
-            hzi = h.z;
         }
-        par ^= 1;
-        if (more) {
-            hin = hnext;
-            ein = enext;
+        put(0, c, ey, 1);
+        put(1, c, ez, 2);
+        put(2, c, eyg, 1);
+        put(3, c, hyp, 4);
+        put(4, c, hzp, 5);
+    }
+    int mcur = row_mat(s, i0, j, lane);
+    asm volatile("" : "+v"(mcur));  // arrived before the loop (see below)
+    for (int i = i0; i < i1; ++i) {
+        R xprev = 0, yprev = 0;  // H of the previous segment's last lane
+#pragma unroll 1
+        for (int c = 0; c < nseg; ++c) {
+            const int k = c * 64 + lane;
+            const bool last = c + 1 == nseg;
+            RowIn<R> cur;
+            int mnext;
+            row_load<T, R>(s, tb, i, j, k, mcur, last ? i + 1 : i, last ? lane : k + 64, mnext, cur);
+            // The id was issued before this item's loads, so it has arrived by
+            // the time they have.  Taking it here (a register-only asm, no
+            // instruction) ends its load's tracking, so that the gathers of the
+            // next item do not wait for this item's stores.
+            asm volatile("" : "+v"(mnext));
+            mcur = mnext;
+            const R eyc = get(0, c, 1), ezc = get(1, c, 2), eyg = get(2, c, 1);
+            const R hyp = get(3, c, 4), hzp = get(4, c, 5);
+            const bool kin = k < s.nz;
+            const bool hup = kin & (i < s.nx - 1) & (j < s.ny - 1) & (k < s.nz - 1);
+            // H of the own cell
+            HIn<R> hv;
+            hv.ex = cur.ex; hv.ex_j = cur.exj; hv.ex_k = cur.exk;
+            hv.ey = eyc; hv.ey_i = cur.eyn; hv.ey_k = cur.eyk;
+            hv.ez = ezc; hv.ez_i = cur.ezn; hv.ez_j = cur.ezj;
+            hv.hx = cur.hx; hv.hy = cur.hy; hv.hz = cur.hz;
+            for (int q = 0; q < 6; ++q) { hv.p[q] = cur.ph[q]; hv.cH[q] = 0; }
+            hv.upd = hup;
+            const H3<R> h = h_compute<T, R>(s, tb, i, j, k, hv, kin);
+            // Hx, Hz of the cell at j-1
+            R gx = 0, gz = 0;
+            if (j > 0) {
+                HIn<R> gv;
+                gv.ex = cur.gex; gv.ex_j = cur.ex; gv.ex_k = 0;
+                gv.ey = eyg; gv.ey_i = cur.geyn; gv.ey_k = cur.geyk;
+                gv.ez = cur.gez; gv.ez_i = 0; gv.ez_j = ezc;
+                gv.hx = cur.ghx; gv.hy = 0; gv.hz = cur.ghz;
+                gv.p[0] = cur.qh[0]; gv.p[1] = cur.qh[1]; gv.p[2] = 0;
+                gv.p[3] = 0; gv.p[4] = cur.qh[2]; gv.p[5] = cur.qh[3];
+                for (int q = 0; q < 6; ++q) gv.cH[q] = 0;
+                gv.upd = kin & (i < s.nx - 1) & (k < s.nz - 1);
+                const H3<R> g = h_compute<T, R>(s, tb, i, j - 1, k, gv, false);
+                gx = g.x;
+                gz = g.z;
+            }
+            // H(k-1): the neighbouring lane, lane 0 from the previous segment
+            R hxk = __shfl_up(h.x, 1), hyk = __shfl_up(h.y, 1);
+            if (lane == 0) { hxk = xprev; hyk = yprev; }
+            xprev = __shfl(h.x, 63);
+            yprev = __shfl(h.y, 63);
+            if (kin) {
+                EIn<R> ev;
+                ev.ex0 = cur.ex; ev.ey0 = eyc; ev.ez0 = ezc;
+                for (int q = 0; q < 6; ++q) { ev.p[q] = cur.pe[q]; ev.cE[q] = cur.cE[q]; }
+                ev.reg = cur.reg;
+                ev.g = ev.reg >= 0 ? ((long)(i - s.g0x) * s.gny + (j - s.g0y)) * s.gnz + (k - s.g0z) : 0;
+                ev.Nu = cur.Nu; ev.Ng = cur.Ng; ev.Pz = cur.Pz; ev.Pp = cur.Pp;
+                e_compute<T, R>(s, tb, i, j, k, ev, h.x, h.y, h.z, gx, gz, hxk, hyk, hyp, hzp);
+            }
+            put(0, c, cur.eyn, 1);
+            put(1, c, cur.ezn, 2);
+            put(2, c, cur.geyn, 1);
+            put(3, c, h.y, 4);
+            put(4, c, h.z, 5);
         }
     }
 }

@@ -94,7 +94,7 @@ def load() -> C.CDLL:
         "fdtd_scene_step_timed": (i, [vp, i, vp]),
         "fdtd_step_timed": (i, [vp, l, i, vp, vp]),
         "fdtd_set_fp16_scales": (i, [vp, vp]),
-        "fdtd_set_kernel_mode": (i, [vp, i, i, i]),
+        "fdtd_set_kernel_mode": (i, [vp, i, i]),
         "fdtd_get_fp16_scales": (i, [vp, vp]),
         "fdtd_scene_steps_done": (l, [vp]),
         "fdtd_scene_handle": (vp, [vp]),
@@ -216,9 +216,9 @@ class Scene:
         _check(self.lib.fdtd_scene_step_timed(self.sc, nsteps, ms))
         return tuple(ms)
 
-    def set_kernel_mode(self, mode: int, rows_per_tile: int = 0, planes_per_chunk: int = 0):
-        """0: separate H/E kernels; 1: fused step kernel (tile rows, x-chunk)."""
-        _check(self.lib.fdtd_set_kernel_mode(self.h, mode, rows_per_tile, planes_per_chunk))
+    def set_kernel_mode(self, mode: int, planes_per_chunk: int = 0):
+        """0: separate H/E kernels; 1: fused step kernel (x-planes per work-group)."""
+        _check(self.lib.fdtd_set_kernel_mode(self.h, mode, planes_per_chunk))
 
     def synchronize(self):
         _check(self.lib.fdtd_synchronize(self.h))

@@ -95,8 +95,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mode", type=int, default=1, help="0: separate H/E kernels, 1: fused step kernel")
-    ap.add_argument("--tj", type=int, default=0, help="fused tile rows (2,4,6,8,14); 0 = library default")
-    ap.add_argument("--ichunk", type=int, default=0, help="fused x-planes per chunk; 0 = library default")
+    ap.add_argument("--ichunk", type=int, default=0, help="fused kernel x-planes per work-group; 0 = library default")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (a known-size kernel for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -115,7 +114,7 @@ def main():
     t_setup = time.time() - t0
     sc = pkg.Scene(setup, args.precision, probe_capacity=args.steps + args.warmup + 16)
     info = sc.info()
-    sc.set_kernel_mode(args.mode, args.tj, args.ichunk)
+    sc.set_kernel_mode(args.mode, args.ichunk)
     sc.step(args.warmup)
     sc.synchronize()
     t1 = time.perf_counter()
@@ -129,13 +128,13 @@ def main():
     t_h = ms_h / args.steps / 1e3
     t_e = ms_e / args.steps / 1e3
     if args.mode == 1:
-        kname, kbytes, kt = "k_step_fused", step_bytes, t_h
-        kernels = {"k_step_fused": round(ms_h / args.steps, 4)}
+        kname, kbytes, kt = "k_step_rows", step_bytes, t_h
+        kernels = {kname: round(ms_h / args.steps, 4)}
     elif t_e >= t_h:
         kname, kbytes, kt = "k_update_E", e_bytes, t_e
     else:
         kname, kbytes, kt = "k_update_H", h_bytes, t_h
-    if args.mode != 1:
+    if args.mode == 0:
         kernels = {"k_update_H": round(ms_h / args.steps, 4), "k_update_E": round(ms_e / args.steps, 4)}
     achieved = kbytes / kt / 1e9
     traffic = None
@@ -162,7 +161,7 @@ def main():
         "config": {"workload": spec.name, "grid": list(setup.shape), "cells": N, "npml": setup.npml,
                    "precision": args.precision, "gain_cells": gain_cells, "materials": info["n_materials"],
                    "mat_id_bytes": info["mat_id_bytes"], "setup_s": round(t_setup, 2),
-                   "kernel_mode": "fused" if args.mode == 1 else "h_e_kernels", "tile_rows": args.tj,
+                   "kernel_mode": ["h_e_kernels", "fused_rows"][args.mode],
                    "planes_per_chunk": args.ichunk},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,

@@ -76,10 +76,10 @@ def _compare(setup, sc, ref, tol, extra=(), vector=False):
     assert not bad, f"exceeds {tol}: {bad}"
 
 
-MODES = [(0, 0, 0), (1, 2, 16), (1, 4, 32), (1, 14, 7)]
+MODES = [(0, 0), (1, 16), (1, 7), (1, 128)]
 
 
-@pytest.mark.parametrize("mode", MODES, ids=lambda m: f"mode{m[0]}_tj{m[1]}_ic{m[2]}")
+@pytest.mark.parametrize("mode", MODES, ids=lambda m: f"mode{m[0]}_ic{m[1]}")
 def test_uniform_sphere_fp64_full(pkg, oracle_mod, mode):
     """The plain CPU case at full size: 64^3 core + 10-cell CPML, 1000 steps,
     for the two-kernel step and the fused step at several tile shapes."""
@@ -98,7 +98,7 @@ def test_psi_state_fp64(pkg, oracle_mod, mode):
         assert rel_l2(sc.field(name), ref.array(name)) <= 1e-12, name
 
 
-@pytest.mark.parametrize("mode", MODES, ids=lambda m: f"mode{m[0]}_tj{m[1]}_ic{m[2]}")
+@pytest.mark.parametrize("mode", MODES, ids=lambda m: f"mode{m[0]}_ic{m[1]}")
 def test_gain_slab_fp64(pkg, oracle_mod, mode):
     sp = pkg.scene.gain_slab_case(n_total=40, npml=6, steps=400, fine_len=100e-9, coarse=20e-9, fine=5e-9)
     setup, sc, ref = _run_pair(pkg, oracle_mod, sp, "fp64", mode=mode)
@@ -112,23 +112,46 @@ def test_gain_cuboids_nonuniform_fp64(pkg, oracle_mod):
     _compare(setup, sc, ref, 1e-12, extra=("Nu", "Ng", "Pz"))
 
 
-def test_fp32_uniform(pkg, oracle_mod):
+@pytest.mark.parametrize("mode", MODES[:2], ids=lambda m: f"mode{m[0]}")
+def test_fp32_uniform(pkg, oracle_mod, mode):
     sp = pkg.scene.uniform_sphere_case(core=32, npml=8, steps=400)
-    setup, sc, ref = _run_pair(pkg, oracle_mod, sp, "fp32")
+    setup, sc, ref = _run_pair(pkg, oracle_mod, sp, "fp32", mode=mode)
     _compare(setup, sc, ref, 2e-4, vector=True)
 
 
-def test_fp32_gain(pkg, oracle_mod):
+@pytest.mark.parametrize("mode", MODES[:2], ids=lambda m: f"mode{m[0]}")
+def test_fp32_gain(pkg, oracle_mod, mode):
     sp = pkg.scene.gain_slab_case(n_total=40, npml=6, steps=400, fine_len=100e-9, coarse=20e-9, fine=5e-9)
-    setup, sc, ref = _run_pair(pkg, oracle_mod, sp, "fp32")
+    setup, sc, ref = _run_pair(pkg, oracle_mod, sp, "fp32", mode=mode)
     _compare(setup, sc, ref, 2e-4, extra=("Pz",), vector=True)
     assert rel_l2(sc.field("Nu"), ref.array("Nu")) <= 1e-9
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp16"])
+def test_step_kernels_agree_bitwise(pkg, precision):
+    """The two-kernel step and the fused step kernel run the same operations
+    per value, so their results are identical in every storage format."""
+    sp = pkg.scene.gain_slab_case(n_total=44, npml=6, steps=120, fine_len=100e-9, coarse=20e-9, fine=5e-9,
+                                  n_cuboids=4, seed=5)
+    setup = pkg.Setup(sp)
+    runs = {}
+    for mode in [(0, 0), (1, 16), (1, 5)]:
+        _say(f"[case] {sp.name} {precision} mode={mode}")
+        sc = pkg.Scene(setup, precision)
+        sc.set_kernel_mode(*mode)
+        sc.step(sp.n_steps)
+        runs[mode] = {n: sc.field(n) for n in FIELDS + ("Nu", "Pz") + tuple(pkg.runtime.PSI_NAMES)}
+        sc.close()
+    base = runs[(0, 0)]
+    for mode, got in runs.items():
+        diff = [n for n in base if not np.array_equal(base[n], got[n])]
+        assert not diff, f"mode {mode} differs in {diff}"
 
 
 GOLDEN = __import__("os").path.join(__import__("os").path.dirname(__file__), "golden")
 
 
-@pytest.mark.parametrize("mode", [(0, 0, 0), (1, 2, 8)], ids=lambda m: f"mode{m[0]}")
+@pytest.mark.parametrize("mode", [(0, 0), (1, 8)], ids=lambda m: f"mode{m[0]}")
 @pytest.mark.parametrize("name", ["uniform_sphere", "gain_slab", "gain_clamp_dipole", "plane_waves", "pec_cavity",
                                   "graded_cuboids"])
 def test_fp64_matches_reference_golden(pkg, name, mode):
@@ -168,8 +191,17 @@ def test_fp16_storage_adaptive_scales(pkg, oracle_mod):
     scales = (C.c_double * 18)()
     assert lib.fdtd_get_fp16_scales(sc.h, scales) == 0
     assert all(s > 0 and math.frexp(s)[0] == 0.5 for s in scales)
-    errs = {c: rel_l2(sc.probe(c), ref.probe(p, q)) for c, (p, q) in enumerate(_channels(sp))}
+    # Each probe is judged as the vector it records: a component that is zero
+    # by symmetry at the probe point carries only rounding noise on its own.
+    errs = {}
+    for p, pr in enumerate(sp.probes):
+        chans = [c for c, (pp, _) in enumerate(_channels(sp)) if pp == p]
+        a = np.stack([sc.probe(c) for c in chans])
+        b = np.stack([ref.probe(p, q) for q in range(len(pr.components))])
+        errs[p] = rel_l2(a, b)
     _say("  fp16 probe rel L2: " + " ".join(f"{c}={v:.2e}" for c, v in errs.items()) + f" scales={list(scales)[:6]}")
-    assert max(errs.values()) < 2e-2
+    # FP16 keeps 11 significant bits: each store rounds by up to 2^-12, and
+    # 300 steps of such rounding leave about 1e-3 relative error
+    assert max(errs.values()) < 3e-3
     for f in FIELDS:
         assert np.isfinite(sc.field(f)).all()
