@@ -361,7 +361,8 @@ __device__ __forceinline__ H3<R> h_new(const StepArgs<T, R>& a, const Tabs<R>& t
 template <typename T, typename R>
 __global__ void __launch_bounds__(256) k_update_H(StepArgs<T, R> a) {
     const int k = blockIdx.x * 64 + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    // blockDim.x is one wavefront, so the row is uniform across it
+    const int j = blockIdx.y * blockDim.y + __builtin_amdgcn_readfirstlane(threadIdx.y);
     const int i = blockIdx.z;
     if (k >= a.nz - 1 || j >= a.ny - 1 || i >= a.nx - 1) return;
     (void)h_new<T, R>(a, global_tabs(a), i, j, k, true);
@@ -531,10 +532,13 @@ __device__ __forceinline__ void e_compute(const StepArgs<T, R>& a, const Tabs<R>
     const R cx = dHz_y - dHy_z, cy = dHx_z - dHz_x, cz = dHy_x - dHx_y;
     R jx = 0, jy = 0, jz = 0;
     for (int s = 0; s < a.nsrc; ++s) {
+        // i and j are uniform across the wavefront: test them (and read the
+        // value) only for boxes the row crosses
         const SourceBox b = tb.src[s];
-        if (i >= b.i0 && i <= b.i1 && j >= b.j0 && j <= b.j1 && k >= b.k0 && k <= b.k1) {
+        if (i >= b.i0 && i <= b.i1 && j >= b.j0 && j <= b.j1) {
             const R val = (R)tb.srcv[s];
-            if (b.comp == 0) jx += val; else if (b.comp == 1) jy += val; else jz += val;
+            const R add = (k >= b.k0 && k <= b.k1) ? val : (R)0;
+            if (b.comp == 0) jx += add; else if (b.comp == 1) jy += add; else jz += add;
         }
     }
     const R bEx = v.cE[1], bEy = v.cE[3], bEz = v.cE[5];
@@ -599,7 +603,8 @@ __device__ __forceinline__ void e_compute(const StepArgs<T, R>& a, const Tabs<R>
 template <typename T, typename R>
 __global__ void __launch_bounds__(256) k_update_E(StepArgs<T, R> a) {
     const int k = blockIdx.x * 64 + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    // blockDim.x is one wavefront, so the row is uniform across it
+    const int j = blockIdx.y * blockDim.y + __builtin_amdgcn_readfirstlane(threadIdx.y);
     const int i = blockIdx.z;
     if (k >= a.nz || j >= a.ny || i >= a.nx) return;
     const long id = ((long)i * a.ny + j) * a.nz + k;
@@ -798,7 +803,7 @@ __global__ void __launch_bounds__(64 * ROW_WAVES) k_step_rows(StepArgs<T, R> a, 
     const int bb = (nwg % 8 == 0) ? (b % 8) * (nwg / 8) + b / 8 : b;
     const int jt = bb % njt;
     const int ic = bb / njt;
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int j0 = jt * ROW_WAVES;
     const int j = j0 + w;
     const int i0 = ic * ichunk;
