@@ -13,4 +13,4 @@ importlib.import_module("3d_fdtd_version008_amd").
 """
 from . import scene  # noqa: F401
 from . import runtime  # noqa: F401
-from .runtime import Setup, Scene, FdtdError, load, library_path  # noqa: F401
+from .runtime import Batch, Setup, Scene, FdtdError, load, library_path  # noqa: F401

@@ -129,6 +129,7 @@ struct fdtd_sim {
     float* d_factor = nullptr;
     unsigned* d_maxbits = nullptr;
     int fp16_interval = 10;
+    struct fdtd_batch* batch = nullptr;  // the batch this domain belongs to
 
     ~fdtd_sim() {
         if (stream) hipStreamSynchronize(stream);
@@ -482,18 +483,18 @@ fdtdk::ArrSet fp16_arrays(const fdtd_sim* s) {
 
 // Keeps every FP16 array's largest stored magnitude inside [2^6, 2^13] by
 // power-of-two rescaling (exact), entirely on the device.
-void fp16_maintain(fdtd_sim* s) {
+void fp16_maintain(fdtd_sim* s, hipStream_t st) {
     const fdtdk::ArrSet a = fp16_arrays(s);
-    hipLaunchKernelGGL(fdtdk::k_fp16_maxima, dim3(512), dim3(256), 0, s->stream, a, s->d_maxbits);
-    hipLaunchKernelGGL(fdtdk::k_fp16_plan, dim3(1), dim3(64), 0, s->stream, s->d_sc, s->d_factor, s->d_maxbits);
-    hipLaunchKernelGGL(fdtdk::k_fp16_rescale, dim3(512), dim3(256), 0, s->stream, a, (const float*)s->d_factor);
+    hipLaunchKernelGGL(fdtdk::k_fp16_maxima, dim3(512), dim3(256), 0, st, a, s->d_maxbits);
+    hipLaunchKernelGGL(fdtdk::k_fp16_plan, dim3(1), dim3(64), 0, st, s->d_sc, s->d_factor, s->d_maxbits);
+    hipLaunchKernelGGL(fdtdk::k_fp16_rescale, dim3(512), dim3(256), 0, st, a, (const float*)s->d_factor);
     fdtdk::ArrSet o{};  // the other buffer holds the never-updated cells too
     for (int q = 0; q < 6; ++q) { o.p[q] = (const uint16_t*)s->F2[s->cur ^ 1][q]; o.n[q] = s->N; }
     for (int q = 6; q < 12; ++q) {
         o.p[6 + q] = (const uint16_t*)s->psi2[s->cur ^ 1][q];
         o.n[6 + q] = o.p[6 + q] ? s->psi_len[q] : 0;
     }
-    hipLaunchKernelGGL(fdtdk::k_fp16_rescale, dim3(512), dim3(256), 0, s->stream, o, (const float*)s->d_factor);
+    hipLaunchKernelGGL(fdtdk::k_fp16_rescale, dim3(512), dim3(256), 0, st, o, (const float*)s->d_factor);
 }
 
 int read_scales(fdtd_sim* s, float* out36) {
@@ -687,6 +688,7 @@ int fdtd_create(const fdtd_grid_desc* g, const fdtd_material_desc* m, const fdtd
 }
 
 int fdtd_destroy(fdtd_handle h) {
+    if (h && h->batch) fdtd_batch_remove(h->batch, h);
     delete h;
     return FDTD_OK;
 }
@@ -787,6 +789,7 @@ int fdtd_set_monitor(fdtd_handle h, int interval) {
 
 static int step_impl(fdtd_handle h, long n0, int nsteps, const double* src_values, double* ms) {
     if (!h || nsteps < 0 || n0 < 0) return fail(FDTD_EINVAL, "bad step arguments");
+    if (h->batch) return fail(FDTD_EINVAL, "domain is a batch member: step it through its batch");
     if (nsteps == 0) return FDTD_OK;
     if (h->nsrc > 0) {
         if (!src_values) return fail(FDTD_EINVAL, "src_values required");
@@ -823,7 +826,7 @@ static int step_impl(fdtd_handle h, long n0, int nsteps, const double* src_value
         const long n = n0 + s;
         dispatch_step(h, n, s, ms ? &ev[1 + (size_t)s * 4] : nullptr);
         if (h->mon_interval > 0 && n % h->mon_interval == 0) dispatch_monitor(h, h->d_mon + slot++);
-        if (h->prec == FDTD_FP16 && n % h->fp16_interval == 0) fp16_maintain(h);
+        if (h->prec == FDTD_FP16 && n % h->fp16_interval == 0) fp16_maintain(h, h->stream);
     }
     HIPCHK(hipGetLastError());
     if (ms) {
@@ -995,6 +998,371 @@ int fdtd_info(fdtd_handle h, long* info) {
     info[0] = h->nx; info[1] = h->ny; info[2] = h->nz; info[3] = h->prec;
     info[4] = h->nmat; info[5] = h->mid16 ? 2 : 1; info[6] = h->ngain; info[7] = h->np;
     return FDTD_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batches (include/fdtd_hip.h, fdtd_batch_*).
+
+struct fdtd_batch {
+    int prec = 0;
+    hipStream_t stream = nullptr;
+    std::vector<fdtd_sim*> doms;
+    int rows = 4, wg_per_cu = 4, chunk = 4;
+    int n_cu = 256;
+    bool use_graph = false;
+    // device tables (grown, never shrunk)
+    std::vector<void*> allocs;
+    void* d_args = nullptr;
+    size_t args_cap = 0;
+    int* d_cum = nullptr;
+    size_t cum_cap = 0;
+    fdtdk::BatchCtl* d_ctl = nullptr;
+    fdtdk::BatchDom* d_dom = nullptr;
+    size_t dom_cap = 0;
+    double* d_srcv = nullptr;
+    size_t srcv_cap = 0;
+    fdtdk::BatchProbe* d_probes = nullptr;
+    size_t probes_cap = 0;
+    fdtdk::BatchDft* d_dfts = nullptr;
+    size_t dfts_cap = 0;
+    MonOut* d_mon = nullptr;
+    size_t mon_cap = 0;
+    // host copies
+    std::vector<char> h_args;
+    std::vector<int> h_cum;
+    std::vector<fdtdk::BatchDom> h_dom;
+    std::vector<fdtdk::BatchProbe> h_probes;
+    std::vector<fdtdk::BatchDft> h_dfts;
+    std::vector<double> h_srcv;
+    std::vector<long> srcv_off;
+    int slots = 0;
+    // captured step block
+    hipGraphExec_t gexec = nullptr;
+    int g_nsteps = -1;
+    unsigned long g_version = 0;
+    unsigned long version = 1;  // bumped when membership, tuning or device tables change
+    std::vector<int> g_phase;
+
+    ~fdtd_batch() {
+        if (stream) hipStreamSynchronize(stream);
+        if (gexec) hipGraphExecDestroy(gexec);
+        for (void* p : allocs) hipFree(p);
+        if (stream) hipStreamDestroy(stream);
+    }
+    // (re)allocates a device table of at least `need` bytes
+    int ensure(void** p, size_t* cap, size_t need) {
+        if (need <= *cap && *p) return 0;
+        size_t n = std::max<size_t>(need, 256);
+        n = std::max(n, *cap * 2);
+        void* q;
+        hipError_t e = hipMalloc(&q, n);
+        if (e != hipSuccess) return fail(FDTD_ENOMEM, "batch table allocation failed");
+        allocs.push_back(q);
+        *p = q;
+        *cap = n;
+        ++version;
+        return 0;
+    }
+};
+
+namespace {
+
+template <typename T, typename R>
+int batch_tables(fdtd_batch* b, int nsteps, const long* n0) {
+    const int nd = (int)b->doms.size();
+    b->h_args.resize(sizeof(StepArgs<T, R>) * nd);
+    StepArgs<T, R>* args = reinterpret_cast<StepArgs<T, R>*>(b->h_args.data());
+    // source values: domain d at srcv_off[d], [nsteps][nsrc]
+    b->srcv_off.assign(nd, 0);
+    size_t nsv = 0;
+    for (int d = 0; d < nd; ++d) {
+        b->srcv_off[d] = (long)nsv;
+        nsv += (size_t)nsteps * b->doms[d]->nsrc;
+    }
+    if (int rc = b->ensure((void**)&b->d_srcv, &b->srcv_cap, std::max<size_t>(nsv, 1) * sizeof(double))) return rc;
+    b->h_cum.assign(nd + 1, 0);
+    b->h_dom.assign(nd, fdtdk::BatchDom{});
+    b->h_probes.clear();
+    b->h_dfts.clear();
+    int min_iv = 0;
+    for (int d = 0; d < nd; ++d) {
+        fdtd_sim* s = b->doms[d];
+        args[d] = make_args<T, R>(s);
+        args[d].srcv = b->d_srcv + b->srcv_off[d];
+        const long tiles = (long)s->nx * ((s->ny + b->rows - 1) / b->rows) * ((s->nz + 63) / 64);
+        if ((long)b->h_cum[d] + tiles > 0x7fffffffL) return fail(FDTD_EINVAL, "batch too large (tile count)");
+        b->h_cum[d + 1] = b->h_cum[d] + (int)tiles;
+        fdtdk::BatchDom& m = b->h_dom[d];
+        m.n0 = n0[d];
+        m.N = s->N;
+        m.ngain = s->gain ? s->gbox : 0;
+        m.interval = s->mon_interval;
+        m.mon_first = m.interval > 0 ? (n0[d] + m.interval - 1) / m.interval : 0;
+        if (m.interval > 0) min_iv = min_iv ? std::min(min_iv, m.interval) : m.interval;
+        for (const Probe& p : s->probes)
+            b->h_probes.push_back(fdtdk::BatchProbe{d, p.comp, p.id, (long)p.save_every, p.cap, p.d_buf});
+        for (const Dft& q : s->dfts) {
+            fdtdk::BatchDft m2{};
+            m2.dom = d;
+            m2.d.comp = q.comp;
+            m2.d.i0 = q.box[0]; m2.d.j0 = q.box[2]; m2.d.k0 = q.box[4];
+            m2.d.ny_b = q.box[3] - q.box[2];
+            m2.d.nz_b = q.box[5] - q.box[4];
+            m2.d.ncell = q.ncell;
+            m2.d.nf = q.nf;
+            m2.d.acc = q.d_acc;
+            m2.dt = s->dt;
+            for (int f = 0; f < q.nf; ++f) m2.omega[f] = 2.0 * 3.14159265358979323846 * q.freqs[f];
+            b->h_dfts.push_back(m2);
+        }
+    }
+    b->slots = min_iv > 0 ? nsteps / min_iv + 2 : 0;
+    int rc;
+    if ((rc = b->ensure(&b->d_args, &b->args_cap, b->h_args.size()))) return rc;
+    if ((rc = b->ensure((void**)&b->d_cum, &b->cum_cap, b->h_cum.size() * sizeof(int)))) return rc;
+    if ((rc = b->ensure((void**)&b->d_dom, &b->dom_cap, b->h_dom.size() * sizeof(fdtdk::BatchDom)))) return rc;
+    if ((rc = b->ensure((void**)&b->d_probes, &b->probes_cap, b->h_probes.size() * sizeof(fdtdk::BatchProbe)))) return rc;
+    if ((rc = b->ensure((void**)&b->d_dfts, &b->dfts_cap, b->h_dfts.size() * sizeof(fdtdk::BatchDft)))) return rc;
+    if ((rc = b->ensure((void**)&b->d_mon, &b->mon_cap, (size_t)nd * std::max(b->slots, 1) * sizeof(MonOut)))) return rc;
+    if (!b->d_ctl) {
+        size_t cap = 0;
+        if ((rc = b->ensure((void**)&b->d_ctl, &cap, sizeof(fdtdk::BatchCtl)))) return rc;
+    }
+    return 0;
+}
+
+// The launches of one batch step (also what a graph captures).
+template <typename T, typename R>
+void batch_launch_step(fdtd_batch* b, int s, const long* n0) {
+    const int nd = (int)b->doms.size();
+    const StepArgs<T, R>* args = reinterpret_cast<const StepArgs<T, R>*>(b->d_args);
+    hipLaunchKernelGGL(fdtdk::k_batch_begin, dim3(1), dim3(64), 0, b->stream, b->d_ctl);
+    const dim3 blk(64, (unsigned)b->rows, 1);
+    const int total = b->h_cum[nd];
+    const int nwg = std::max(1, std::min(b->n_cu * b->wg_per_cu * (4 / b->rows), (total + b->chunk - 1) / b->chunk));
+    hipLaunchKernelGGL((fdtdk::k_batch_step<T, R, false>), dim3((unsigned)nwg), blk, 0, b->stream, args, b->d_cum, nd,
+                       b->chunk, b->d_ctl);
+    hipLaunchKernelGGL((fdtdk::k_batch_step<T, R, true>), dim3((unsigned)nwg), blk, 0, b->stream, args, b->d_cum, nd,
+                       b->chunk, b->d_ctl);
+    if (!b->h_probes.empty())
+        hipLaunchKernelGGL((fdtdk::k_batch_probes<T, R>), dim3((unsigned)((b->h_probes.size() + 63) / 64)), dim3(64), 0,
+                           b->stream, args, b->d_dom, b->d_probes, (int)b->h_probes.size(), b->d_ctl);
+    if (!b->h_dfts.empty()) {
+        long mx = 1;
+        for (const auto& m : b->h_dfts) mx = std::max(mx, m.d.ncell);
+        hipLaunchKernelGGL((fdtdk::k_batch_dft<T, R>), dim3((unsigned)std::min<long>((mx + 255) / 256, 256),
+                           (unsigned)b->h_dfts.size()), dim3(256), 0, b->stream, args, b->d_dom, b->d_dfts, b->d_ctl);
+    }
+    bool mon = false;
+    for (int d = 0; d < nd && !mon; ++d) {
+        const int iv = b->doms[d]->mon_interval;
+        mon = iv > 0 && (n0[d] + s) % iv == 0;
+    }
+    if (mon && b->slots > 0)
+        hipLaunchKernelGGL((fdtdk::k_batch_monitor<T, R>), dim3(64, (unsigned)nd), dim3(256), 0, b->stream, args,
+                           b->d_dom, b->d_mon, b->slots, b->d_ctl);
+    if (sizeof(T) == 2)
+        for (int d = 0; d < nd; ++d)
+            if ((n0[d] + s) % b->doms[d]->fp16_interval == 0) fp16_maintain(b->doms[d], b->stream);
+}
+
+template <typename T, typename R>
+int batch_step(fdtd_batch* b, int nsteps, const long* n0, const double* const* srcv, double* ms) {
+    const int nd = (int)b->doms.size();
+    if (int rc = batch_tables<T, R>(b, nsteps, n0)) return rc;
+    for (int d = 0; d < nd; ++d) {
+        fdtd_sim* s = b->doms[d];
+        if (s->nsrc > 0 && (!srcv || !srcv[d])) return fail(FDTD_EINVAL, "source values missing for a batch member");
+    }
+    // uploads for this call (outside any graph)
+    b->h_srcv.resize(std::max<size_t>(1, (size_t)b->srcv_off.back() + (size_t)nsteps * b->doms.back()->nsrc));
+    for (int d = 0; d < nd; ++d)
+        if (b->doms[d]->nsrc > 0)
+            std::memcpy(b->h_srcv.data() + b->srcv_off[d], srcv[d], (size_t)nsteps * b->doms[d]->nsrc * sizeof(double));
+    HIPCHK(hipMemcpyAsync(b->d_srcv, b->h_srcv.data(), b->h_srcv.size() * sizeof(double), hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipMemcpyAsync(b->d_args, b->h_args.data(), b->h_args.size(), hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipMemcpyAsync(b->d_cum, b->h_cum.data(), b->h_cum.size() * sizeof(int), hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipMemcpyAsync(b->d_dom, b->h_dom.data(), b->h_dom.size() * sizeof(fdtdk::BatchDom), hipMemcpyHostToDevice,
+                          b->stream));
+    if (!b->h_probes.empty())
+        HIPCHK(hipMemcpyAsync(b->d_probes, b->h_probes.data(), b->h_probes.size() * sizeof(fdtdk::BatchProbe),
+                              hipMemcpyHostToDevice, b->stream));
+    if (!b->h_dfts.empty())
+        HIPCHK(hipMemcpyAsync(b->d_dfts, b->h_dfts.data(), b->h_dfts.size() * sizeof(fdtdk::BatchDft),
+                              hipMemcpyHostToDevice, b->stream));
+    if (b->slots > 0)
+        HIPCHK(hipMemsetAsync(b->d_mon, 0, (size_t)nd * b->slots * sizeof(MonOut), b->stream));
+    hipLaunchKernelGGL(fdtdk::k_batch_reset, dim3(1), dim3(64), 0, b->stream, b->d_ctl);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (ms) {
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipEventRecord(e0, b->stream));
+    }
+    // The monitor and FP16 rescaling run on each member's own cadence: a
+    // captured block is valid for the same step phases only.
+    std::vector<int> phase;
+    for (int d = 0; d < nd; ++d) {
+        const fdtd_sim* m = b->doms[d];
+        phase.push_back(m->mon_interval > 0 ? (int)(n0[d] % m->mon_interval) : -1);
+        if (b->prec == FDTD_FP16) phase.push_back((int)(n0[d] % m->fp16_interval));
+    }
+    if (b->use_graph) {
+        if (!b->gexec || b->g_nsteps != nsteps || b->g_version != b->version || b->g_phase != phase) {
+            if (b->gexec) {
+                (void)hipGraphExecDestroy(b->gexec);
+                b->gexec = nullptr;
+            }
+            hipGraph_t g;
+            HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            for (int s = 0; s < nsteps; ++s) batch_launch_step<T, R>(b, s, n0);
+            HIPCHK(hipStreamEndCapture(b->stream, &g));
+            const hipError_t e = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) return fail(FDTD_EDEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+            b->g_nsteps = nsteps;
+            b->g_version = b->version;
+            b->g_phase = phase;
+        }
+        HIPCHK(hipGraphLaunch(b->gexec, b->stream));
+    } else {
+        for (int s = 0; s < nsteps; ++s) batch_launch_step<T, R>(b, s, n0);
+    }
+    HIPCHK(hipGetLastError());
+    if (ms) {
+        HIPCHK(hipEventRecord(e1, b->stream));
+        HIPCHK(hipEventSynchronize(e1));
+        float t = 0;
+        HIPCHK(hipEventElapsedTime(&t, e0, e1));
+        *ms = t;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+    // per-member bookkeeping: probe counts, monitor records and policy
+    std::vector<MonOut> mo((size_t)nd * std::max(b->slots, 1));
+    if (b->slots > 0)
+        HIPCHK(hipMemcpyAsync(mo.data(), b->d_mon, (size_t)nd * b->slots * sizeof(MonOut), hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    bool ok = true;
+    for (int d = 0; d < nd; ++d) {
+        fdtd_sim* s = b->doms[d];
+        const long n_end = n0[d] + nsteps;
+        for (Probe& p : s->probes) p.count = std::min(p.cap, (n_end - 1) / p.save_every + 1);
+        s->checks.clear();
+        const fdtdk::BatchDom& m = b->h_dom[d];
+        if (m.interval <= 0) continue;
+        for (long n = m.mon_first * m.interval; n < n_end; n += m.interval) {
+            const long slot = n / m.interval - m.mon_first;
+            if (slot >= b->slots) break;
+            const MonOut& q = mo[(size_t)d * b->slots + slot];
+            fdtd_check_record r{};
+            const unsigned long long inf = 0x7ff0000000000000ull;
+            r.step = n;
+            r.nan_E = q.maxE >= inf; r.nan_H = q.maxH >= inf;
+            r.nan_P = q.maxP >= inf; r.nan_N = q.nanN >= inf;
+            r.max_E = bits_to_double(q.maxE);
+            r.max_H = bits_to_double(q.maxH);
+            r.max_P = bits_to_double(q.maxP);
+            r.pop_total = q.pop_total;
+            r.pop_initial = q.pop_init;
+            s->checks.push_back(r);
+            if (!apply_policy(s, r)) ok = false;
+        }
+    }
+    return ok ? FDTD_OK : fail(FDTD_EUNSTABLE, "a batch member is unstable");
+}
+
+}  // namespace
+
+extern "C" {
+
+int fdtd_batch_create(int precision, fdtd_batch_handle* out) {
+    if (!out || precision < FDTD_FP64 || precision > FDTD_FP16) return fail(FDTD_EINVAL, "bad batch arguments");
+    *out = nullptr;
+    auto* b = new fdtd_batch();
+    b->prec = precision;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&b->n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete b;
+        return fail(FDTD_EDEVICE, "no HIP device for the batch");
+    }
+    *out = b;
+    return FDTD_OK;
+}
+
+int fdtd_batch_destroy(fdtd_batch_handle b) {
+    if (!b) return FDTD_OK;
+    for (fdtd_sim* s : b->doms) s->batch = nullptr;
+    delete b;
+    return FDTD_OK;
+}
+
+int fdtd_batch_add(fdtd_batch_handle b, fdtd_handle h) {
+    if (!b || !h) return fail(FDTD_EINVAL, "null argument");
+    if (h->batch) return fail(FDTD_EINVAL, "domain already belongs to a batch");
+    if (h->prec != b->prec) return fail(FDTD_EINVAL, "domain precision differs from the batch's");
+    // the batch steps in place in whichever buffer holds the current state
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->batch = b;
+    b->doms.push_back(h);
+    ++b->version;
+    return (int)b->doms.size() - 1;
+}
+
+int fdtd_batch_remove(fdtd_batch_handle b, fdtd_handle h) {
+    if (!b || !h) return fail(FDTD_EINVAL, "null argument");
+    auto it = std::find(b->doms.begin(), b->doms.end(), h);
+    if (it == b->doms.end()) return fail(FDTD_EINVAL, "domain is not a member of this batch");
+    HIPCHK(hipStreamSynchronize(b->stream));
+    b->doms.erase(it);
+    h->batch = nullptr;
+    ++b->version;
+    return FDTD_OK;
+}
+
+int fdtd_batch_count(fdtd_batch_handle b) { return b ? (int)b->doms.size() : fail(FDTD_EINVAL, "null batch"); }
+
+int fdtd_batch_set_tuning(fdtd_batch_handle b, int rows, int wg_per_cu, int chunk) {
+    if (!b || (rows != 1 && rows != 2 && rows != 4) || wg_per_cu < 1 || wg_per_cu > 16 || chunk < 1 || chunk > 64)
+        return fail(FDTD_EINVAL, "tuning out of range");
+    b->rows = rows;
+    b->wg_per_cu = wg_per_cu;
+    b->chunk = chunk;
+    ++b->version;
+    return FDTD_OK;
+}
+
+int fdtd_batch_get_tuning(fdtd_batch_handle b, int* rows, int* wg_per_cu, int* chunk) {
+    if (!b || !rows || !wg_per_cu || !chunk) return fail(FDTD_EINVAL, "null argument");
+    *rows = b->rows;
+    *wg_per_cu = b->wg_per_cu;
+    *chunk = b->chunk;
+    return FDTD_OK;
+}
+
+int fdtd_batch_set_graph(fdtd_batch_handle b, int on) {
+    if (!b) return fail(FDTD_EINVAL, "null batch");
+    b->use_graph = on != 0;
+    return FDTD_OK;
+}
+
+int fdtd_batch_step(fdtd_batch_handle b, int nsteps, const long* n0, const double* const* src_values, double* ms) {
+    if (!b || nsteps < 0 || (!b->doms.empty() && !n0)) return fail(FDTD_EINVAL, "bad batch step arguments");
+    if (nsteps == 0 || b->doms.empty()) {
+        if (ms) *ms = 0;
+        return FDTD_OK;
+    }
+    for (size_t d = 0; d < b->doms.size(); ++d)
+        if (n0[d] < 0) return fail(FDTD_EINVAL, "negative step index");
+    switch (b->prec) {
+    case FDTD_FP64: return batch_step<double, double>(b, nsteps, n0, src_values, ms);
+    case FDTD_FP32: return batch_step<float, float>(b, nsteps, n0, src_values, ms);
+    default: return batch_step<uint16_t, float>(b, nsteps, n0, src_values, ms);
+    }
 }
 
 }  // extern "C"

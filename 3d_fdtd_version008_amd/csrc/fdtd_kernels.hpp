@@ -1014,9 +1014,7 @@ struct DftArgs {
 };
 
 template <typename T, typename R>
-__global__ void k_dft(StepArgs<T, R> a, DftArgs d) {
-    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= d.ncell) return;
+__device__ __forceinline__ void dft_cell(const StepArgs<T, R>& a, const DftArgs& d, long c) {
     const int kk = (int)(c % d.nz_b);
     const long r = c / d.nz_b;
     const int jj = (int)(r % d.ny_b);
@@ -1028,6 +1026,12 @@ __global__ void k_dft(StepArgs<T, R> a, DftArgs d) {
         p[0] += F * d.wr[f];
         p[1] += F * d.wi[f];
     }
+}
+
+template <typename T, typename R>
+__global__ void k_dft(StepArgs<T, R> a, DftArgs d) {
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < d.ncell) dft_cell<T, R>(a, d, c);
 }
 
 // Stability reduction (StabilityMonitor::check_stability inputs,
@@ -1048,11 +1052,12 @@ __device__ __forceinline__ unsigned long long absbits(const T* p, long i, R s) {
 }
 
 template <typename T, typename R>
-__global__ void __launch_bounds__(256) k_monitor(StepArgs<T, R> a, long N, long ngain, MonOut* out) {
+__device__ __forceinline__ void monitor_body(const StepArgs<T, R>& a, long N, long ngain, MonOut* out, int bid,
+                                             int nblk) {
     unsigned long long mE = 0, mH = 0, mP = 0, nN = 0;
     double pt = 0.0, pi = 0.0;
-    const long stride = (long)gridDim.x * blockDim.x;
-    for (long id = (long)blockIdx.x * blockDim.x + threadIdx.x; id < N; id += stride) {
+    const long stride = (long)nblk * blockDim.x;
+    for (long id = (long)bid * blockDim.x + threadIdx.x; id < N; id += stride) {
         unsigned long long v;
         v = absbits<T, R>(a.Ex, id, a.sc[0]); mE = v > mE ? v : mE;
         v = absbits<T, R>(a.Ey, id, a.sc[1]); mE = v > mE ? v : mE;
@@ -1061,7 +1066,7 @@ __global__ void __launch_bounds__(256) k_monitor(StepArgs<T, R> a, long N, long 
         v = absbits<T, R>(a.Hy, id, a.sc[4]); mH = v > mH ? v : mH;
         v = absbits<T, R>(a.Hz, id, a.sc[5]); mH = v > mH ? v : mH;
     }
-    for (long g = (long)blockIdx.x * blockDim.x + threadIdx.x; g < ngain; g += stride) {
+    for (long g = (long)bid * blockDim.x + threadIdx.x; g < ngain; g += stride) {
         const int r = a.greg[g];
         if (r < 0) continue;
         const double pz = a.gPz[g], pp = a.gPp[g], nu = a.gNu[g], ng = a.gNg[g];
@@ -1115,6 +1120,152 @@ __global__ void __launch_bounds__(256) k_monitor(StepArgs<T, R> a, long N, long 
         if (pt != 0.0) atomicAdd(&out->pop_total, pt);
         if (pi != 0.0) atomicAdd(&out->pop_init, pi);
     }
+}
+
+template <typename T, typename R>
+__global__ void __launch_bounds__(256) k_monitor(StepArgs<T, R> a, long N, long ngain, MonOut* out) {
+    monitor_body<T, R>(a, N, ngain, out, blockIdx.x, gridDim.x);
+}
+
+// ---- Multi-domain (batch) step ----
+// Many independent domains of one precision advance together: per step one
+// persistent H launch and one persistent E launch cover every domain, their
+// work-groups taking `chunk` tiles (64 k-values by blockDim.y rows of one
+// x-plane) at a time from a device counter, and one launch each records the
+// probes, accumulates the DFT monitors and reduces the stability monitor of
+// every domain.  Step-dependent inputs (source values, probe slots, DFT
+// phases, monitor slots) are indexed by a device-side step counter, so a block
+// of steps can be captured once in a HIP graph and replayed.  Each cell goes
+// through the same operations as in the single-domain step, so a domain's
+// results do not depend on the batch it runs in.
+struct BatchCtl {
+    int ctr_h, ctr_e;  // work counters of the persistent H and E launches
+    int lstep;         // step index within the current block
+    int pad;
+};
+struct BatchDom {
+    long n0;         // global step index of the block's first step
+    long N, ngain;   // cells, gain-box cells (monitor)
+    long mon_first;  // first monitor slot of the block: ceil(n0 / interval)
+    int interval;    // monitor cadence (0: off)
+    int pad;
+};
+struct BatchProbe {
+    int dom, comp;
+    long id;
+    long save_every, cap;
+    double* out;
+};
+struct BatchDft {
+    int dom;
+    DftArgs d;       // wr/wi unused: phases are computed from the step index
+    double dt;
+    double omega[8];
+};
+
+__global__ void k_batch_reset(BatchCtl* c) {
+    if (threadIdx.x == 0) { c->ctr_h = 0; c->ctr_e = 0; c->lstep = -1; }
+}
+__global__ void k_batch_begin(BatchCtl* c) {
+    if (threadIdx.x == 0) { c->ctr_h = 0; c->ctr_e = 0; c->lstep += 1; }
+}
+
+// Domain of tile t: cum[d] <= t < cum[d+1].
+__device__ __forceinline__ int batch_domain(const int* __restrict__ cum, int ndom, int t) {
+    int lo = 0, hi = ndom;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (cum[mid] <= t) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+template <typename T, typename R, bool EPASS>
+__global__ void __launch_bounds__(256) k_batch_step(const StepArgs<T, R>* __restrict__ doms, const int* __restrict__ cum,
+                                                    int ndom, int chunk, BatchCtl* ctl) {
+    __shared__ int first_s;
+    const int tj = blockDim.y;
+    const int ty = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    const int total = cum[ndom];
+    const int lstep = ctl->lstep;
+    for (;;) {
+        if (threadIdx.x == 0 && threadIdx.y == 0) first_s = atomicAdd(EPASS ? &ctl->ctr_e : &ctl->ctr_h, chunk);
+        __syncthreads();
+        const int first = first_s;
+        __syncthreads();
+        if (first >= total) return;
+        const int last = min(first + chunk, total);
+        int d = batch_domain(cum, ndom, first);
+        for (int t = first; t < last; ++t) {
+            while (t >= cum[d + 1]) ++d;
+            StepArgs<T, R> a = doms[d];
+            const int lt = t - cum[d];
+            const int nkt = (a.nz + 63) >> 6, njt = (a.ny + tj - 1) / tj;
+            const int kt = lt % nkt, r = lt / nkt, jt = r % njt, i = r / njt;
+            const int k = kt * 64 + threadIdx.x, j = jt * tj + ty;
+            if (!EPASS) {
+                if (k < a.nz - 1 && j < a.ny - 1 && i < a.nx - 1) (void)h_new<T, R>(a, global_tabs(a), i, j, k, true);
+            } else if (k < a.nz && j < a.ny) {
+                a.srcv += (long)lstep * a.nsrc;
+                const long id = ((long)i * a.ny + j) * a.nz + k;
+                const bool interior = (i > 0) & (j > 0) & (k > 0);
+                R hx = 0, hy = 0, hz = 0, hx_j = 0, hz_j = 0, hx_k = 0, hy_k = 0, hy_i = 0, hz_i = 0;
+                if (interior) {
+                    const long sI = a.sI, sJ = a.sJ;
+                    hx = ld<T, R>(a.Hx, id, a.sc[3]); hx_j = ld<T, R>(a.Hx, id - sJ, a.sc[3]);
+                    hx_k = ld<T, R>(a.Hx, id - 1, a.sc[3]);
+                    hy = ld<T, R>(a.Hy, id, a.sc[4]); hy_i = ld<T, R>(a.Hy, id - sI, a.sc[4]);
+                    hy_k = ld<T, R>(a.Hy, id - 1, a.sc[4]);
+                    hz = ld<T, R>(a.Hz, id, a.sc[5]); hz_i = ld<T, R>(a.Hz, id - sI, a.sc[5]);
+                    hz_j = ld<T, R>(a.Hz, id - sJ, a.sc[5]);
+                }
+                const EIn<R> v = e_load<T, R>(a, i, j, k, e_mat(a, i, j, k));
+                e_compute<T, R>(a, global_tabs(a), i, j, k, v, hx, hy, hz, hx_j, hz_j, hx_k, hy_k, hy_i, hz_i);
+            }
+        }
+    }
+}
+
+template <typename T, typename R>
+__global__ void k_batch_probes(const StepArgs<T, R>* __restrict__ doms, const BatchDom* __restrict__ bd,
+                               const BatchProbe* __restrict__ pr, int npr, const BatchCtl* ctl) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npr) return;
+    const BatchProbe b = pr[p];
+    const long n = bd[b.dom].n0 + ctl->lstep;
+    if (n % b.save_every != 0) return;
+    const long slot = n / b.save_every;
+    if (slot < b.cap) b.out[slot] = field_value<T, R>(doms[b.dom], b.comp, b.id);
+}
+
+template <typename T, typename R>
+__global__ void k_batch_dft(const StepArgs<T, R>* __restrict__ doms, const BatchDom* __restrict__ bd,
+                            const BatchDft* __restrict__ dd, const BatchCtl* ctl) {
+    const BatchDft& m = dd[blockIdx.y];
+    DftArgs d = m.d;
+    const long n = bd[m.dom].n0 + ctl->lstep;
+    const double t = (d.comp < 3) ? ((double)n + 1.0) * m.dt : ((double)n + 0.5) * m.dt;
+    for (int f = 0; f < d.nf; ++f) {
+        d.wr[f] = cos(m.omega[f] * t) * m.dt;
+        d.wi[f] = -sin(m.omega[f] * t) * m.dt;
+    }
+    const StepArgs<T, R> a = doms[m.dom];
+    for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < d.ncell; c += (long)gridDim.x * blockDim.x)
+        dft_cell<T, R>(a, d, c);
+}
+
+template <typename T, typename R>
+__global__ void __launch_bounds__(256) k_batch_monitor(const StepArgs<T, R>* __restrict__ doms,
+                                                       const BatchDom* __restrict__ bd, MonOut* out, int slots_per_dom,
+                                                       const BatchCtl* ctl) {
+    const int d = blockIdx.y;
+    const BatchDom m = bd[d];
+    if (m.interval <= 0) return;
+    const long n = m.n0 + ctl->lstep;
+    if (n % m.interval != 0) return;
+    const long slot = n / m.interval - m.mon_first;
+    if (slot < 0 || slot >= slots_per_dom) return;
+    monitor_body<T, R>(doms[d], m.N, m.ngain, out + (long)d * slots_per_dom + slot, blockIdx.x, gridDim.x);
 }
 
 // ---- FP16 storage: adaptive per-component power-of-two scales ----

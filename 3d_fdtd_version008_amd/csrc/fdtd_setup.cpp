@@ -14,6 +14,10 @@
 // skips shapes whose (slightly widened) bounding box excludes a sample, which
 // cannot change which shape last contains it.
 #include <algorithm>
+#include <array>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -629,6 +633,7 @@ struct fdtd_scene {
     fdtd_handle h = nullptr;
     long n = 0;
     std::vector<double> vals;
+    struct fdtd_scene_batch* batch = nullptr;  // the scene batch it belongs to
 };
 
 extern "C" {
@@ -754,6 +759,7 @@ int fdtd_scene_create(const fdtd_setup* S, int precision, long probe_capacity, f
 
 int fdtd_scene_destroy(fdtd_scene* sc) {
     if (!sc) return FDTD_OK;
+    if (sc->batch) fdtd_scene_batch_remove(sc->batch, sc);
     fdtd_destroy(sc->h);
     delete sc;
     return FDTD_OK;
@@ -782,5 +788,143 @@ int fdtd_scene_step_timed(fdtd_scene* sc, int nsteps, double* ms) {
 long fdtd_scene_steps_done(const fdtd_scene* sc) { return sc ? sc->n : -1; }
 
 fdtd_sim* fdtd_scene_handle(fdtd_scene* sc) { return sc ? sc->h : nullptr; }
+
+}  // extern "C"
+
+struct fdtd_scene_batch {
+    fdtd_batch_handle b = nullptr;
+    int prec = 0;
+    std::vector<fdtd_scene*> scenes;
+    std::vector<std::vector<double>> vals;
+};
+
+namespace {
+struct TuneKey {
+    int prec, count, log2cells;
+    bool operator<(const TuneKey& o) const {
+        return std::tie(prec, count, log2cells) < std::tie(o.prec, o.count, o.log2cells);
+    }
+};
+std::map<TuneKey, std::array<int, 3>>& tune_cache() {
+    static std::map<TuneKey, std::array<int, 3>> m;
+    return m;
+}
+std::mutex& tune_mutex() {
+    static std::mutex m;
+    return m;
+}
+}  // namespace
+
+extern "C" {
+
+int fdtd_scene_batch_create(int precision, fdtd_scene_batch** out) {
+    if (!out) return FDTD_EINVAL;
+    *out = nullptr;
+    fdtd_batch_handle b = nullptr;
+    if (int rc = fdtd_batch_create(precision, &b)) return rc;
+    auto* sb = new fdtd_scene_batch();
+    sb->b = b;
+    sb->prec = precision;
+    *out = sb;
+    return FDTD_OK;
+}
+
+int fdtd_scene_batch_destroy(fdtd_scene_batch* sb) {
+    if (!sb) return FDTD_OK;
+    for (fdtd_scene* sc : sb->scenes) sc->batch = nullptr;
+    fdtd_batch_destroy(sb->b);
+    delete sb;
+    return FDTD_OK;
+}
+
+int fdtd_scene_batch_add(fdtd_scene_batch* sb, fdtd_scene* sc) {
+    if (!sb || !sc) return FDTD_EINVAL;
+    const int rc = fdtd_batch_add(sb->b, sc->h);
+    if (rc < 0) return rc;
+    sb->scenes.push_back(sc);
+    sc->batch = sb;
+    return rc;
+}
+
+int fdtd_scene_batch_remove(fdtd_scene_batch* sb, fdtd_scene* sc) {
+    if (!sb || !sc) return FDTD_EINVAL;
+    auto it = std::find(sb->scenes.begin(), sb->scenes.end(), sc);
+    if (it == sb->scenes.end()) return FDTD_EINVAL;
+    const int rc = fdtd_batch_remove(sb->b, sc->h);
+    if (rc) return rc;
+    sb->scenes.erase(it);
+    sc->batch = nullptr;
+    return FDTD_OK;
+}
+
+int fdtd_scene_batch_count(const fdtd_scene_batch* sb) { return sb ? (int)sb->scenes.size() : FDTD_EINVAL; }
+
+int fdtd_scene_batch_step(fdtd_scene_batch* sb, int nsteps, double* ms) {
+    if (!sb || nsteps < 0) return FDTD_EINVAL;
+    const size_t nd = sb->scenes.size();
+    std::vector<long> n0(nd);
+    std::vector<const double*> sv(nd, nullptr);
+    sb->vals.resize(nd);
+    for (size_t d = 0; d < nd; ++d) {
+        fdtd_scene* sc = sb->scenes[d];
+        n0[d] = sc->n;
+        const size_t ns = sc->setup->src.size();
+        if (ns) {
+            sb->vals[d].resize((size_t)nsteps * ns);
+            fdtd_setup_source_values(sc->setup, sc->n, nsteps, sb->vals[d].data());
+            sv[d] = sb->vals[d].data();
+        }
+    }
+    const int rc = fdtd_batch_step(sb->b, nsteps, n0.data(), sv.data(), ms);
+    if (rc == FDTD_OK || rc == FDTD_EUNSTABLE)
+        for (fdtd_scene* sc : sb->scenes) sc->n += nsteps;
+    return rc;
+}
+
+int fdtd_scene_batch_autotune(fdtd_scene_batch* sb, int trial_steps, int* rows, int* wg_per_cu, int* chunk) {
+    if (!sb || trial_steps < 1 || !rows || !wg_per_cu || !chunk) return FDTD_EINVAL;
+    long cells = 0;
+    for (fdtd_scene* sc : sb->scenes) cells += (long)sc->setup->NxT * sc->setup->NyT * sc->setup->NzT;
+    int l2 = 0;
+    while ((1L << l2) < cells) ++l2;
+    const TuneKey key{sb->prec, (int)sb->scenes.size(), l2};
+    {
+        std::lock_guard<std::mutex> g(tune_mutex());
+        auto it = tune_cache().find(key);
+        if (it != tune_cache().end()) {
+            *rows = it->second[0]; *wg_per_cu = it->second[1]; *chunk = it->second[2];
+            const int rc = fdtd_batch_set_tuning(sb->b, *rows, *wg_per_cu, *chunk);
+            return rc ? rc : 0;
+        }
+    }
+    int taken = 0;
+    double ms = 0;
+    // warm-up (table uploads, first-touch costs) under the current tiling
+    int rc = fdtd_scene_batch_step(sb, trial_steps, &ms);
+    if (rc) return rc;
+    taken += trial_steps;
+    double best = 1e300;
+    std::array<int, 3> win{4, 4, 4};
+    for (int r : {1, 2, 4})
+        for (int w : {2, 4, 8})
+            for (int c : {1, 4, 16}) {
+                if ((rc = fdtd_batch_set_tuning(sb->b, r, w, c))) return rc;
+                if ((rc = fdtd_scene_batch_step(sb, trial_steps, &ms))) return rc;
+                taken += trial_steps;
+                if (ms < best) {
+                    best = ms;
+                    win = {r, w, c};
+                }
+            }
+    if ((rc = fdtd_batch_set_tuning(sb->b, win[0], win[1], win[2]))) return rc;
+    {
+        std::lock_guard<std::mutex> g(tune_mutex());
+        tune_cache()[key] = win;
+    }
+    *rows = win[0]; *wg_per_cu = win[1]; *chunk = win[2];
+    return taken;
+}
+
+fdtd_batch_handle fdtd_scene_batch_handle(fdtd_scene_batch* sb) { return sb ? sb->b : nullptr; }
 
 }  // extern "C"

@@ -34,6 +34,10 @@ EXPORTS = [
     "fdtd_setup_source_values", "fdtd_setup_probe", "fdtd_scene_create", "fdtd_scene_destroy", "fdtd_scene_step",
     "fdtd_scene_steps_done", "fdtd_scene_handle", "fdtd_scene_step_timed", "fdtd_set_fp16_scales",
     "fdtd_get_fp16_scales", "fdtd_set_kernel_mode",
+    "fdtd_batch_create", "fdtd_batch_destroy", "fdtd_batch_add", "fdtd_batch_remove", "fdtd_batch_count",
+    "fdtd_batch_set_tuning", "fdtd_batch_get_tuning", "fdtd_batch_set_graph", "fdtd_batch_step",
+    "fdtd_scene_batch_create", "fdtd_scene_batch_destroy", "fdtd_scene_batch_add", "fdtd_scene_batch_remove",
+    "fdtd_scene_batch_count", "fdtd_scene_batch_step", "fdtd_scene_batch_autotune", "fdtd_scene_batch_handle",
 ]
 
 
@@ -98,6 +102,17 @@ def load() -> C.CDLL:
         "fdtd_get_fp16_scales": (i, [vp, vp]),
         "fdtd_scene_steps_done": (l, [vp]),
         "fdtd_scene_handle": (vp, [vp]),
+        "fdtd_batch_set_tuning": (i, [vp, i, i, i]),
+        "fdtd_batch_get_tuning": (i, [vp, C.POINTER(i), C.POINTER(i), C.POINTER(i)]),
+        "fdtd_batch_set_graph": (i, [vp, i]),
+        "fdtd_scene_batch_create": (i, [i, C.POINTER(vp)]),
+        "fdtd_scene_batch_destroy": (i, [vp]),
+        "fdtd_scene_batch_add": (i, [vp, vp]),
+        "fdtd_scene_batch_remove": (i, [vp, vp]),
+        "fdtd_scene_batch_count": (i, [vp]),
+        "fdtd_scene_batch_step": (i, [vp, i, C.POINTER(d)]),
+        "fdtd_scene_batch_autotune": (i, [vp, i, C.POINTER(i), C.POINTER(i), C.POINTER(i)]),
+        "fdtd_scene_batch_handle": (vp, [vp]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(lib, name)
@@ -264,3 +279,65 @@ class Scene:
         b = C.c_size_t()
         _check(self.lib.fdtd_device_bytes(self.h, C.byref(b)))
         return b.value
+
+
+class Batch:
+    """Scenes of one precision stepped together: one persistent H launch and
+    one persistent E launch per step for all members (fdtd_scene_batch_*)."""
+
+    def __init__(self, precision="fp32"):
+        self.lib = load()
+        self.precision = PRECISION[precision] if isinstance(precision, str) else int(precision)
+        h = C.c_void_p()
+        _check(self.lib.fdtd_scene_batch_create(self.precision, C.byref(h)))
+        self.sb = h
+        self.b = self.lib.fdtd_scene_batch_handle(self.sb)
+        self.members = []
+
+    def add(self, scene: "Scene") -> int:
+        slot = _check(self.lib.fdtd_scene_batch_add(self.sb, scene.sc))
+        self.members.append(scene)
+        return slot
+
+    def remove(self, scene: "Scene"):
+        _check(self.lib.fdtd_scene_batch_remove(self.sb, scene.sc))
+        self.members.remove(scene)
+
+    def __len__(self):
+        return len(self.members)
+
+    def step(self, nsteps: int) -> float:
+        """Advances every member nsteps steps; returns the device milliseconds."""
+        ms = C.c_double()
+        _check(self.lib.fdtd_scene_batch_step(self.sb, nsteps, C.byref(ms)))
+        return ms.value
+
+    def set_graph(self, on: bool = True):
+        _check(self.lib.fdtd_batch_set_graph(self.b, int(on)))
+
+    def set_tuning(self, rows: int, wg_per_cu: int, chunk: int):
+        _check(self.lib.fdtd_batch_set_tuning(self.b, rows, wg_per_cu, chunk))
+
+    def tuning(self):
+        r, w, c = C.c_int(), C.c_int(), C.c_int()
+        _check(self.lib.fdtd_batch_get_tuning(self.b, C.byref(r), C.byref(w), C.byref(c)))
+        return r.value, w.value, c.value
+
+    def autotune(self, trial_steps: int = 5):
+        """Times every candidate tiling on real steps and keeps the fastest;
+        returns ((rows, wg_per_cu, chunk), steps taken)."""
+        r, w, c = C.c_int(), C.c_int(), C.c_int()
+        taken = _check(self.lib.fdtd_scene_batch_autotune(self.sb, trial_steps, C.byref(r), C.byref(w), C.byref(c)))
+        return (r.value, w.value, c.value), taken
+
+    def close(self):
+        if getattr(self, "sb", None):
+            self.lib.fdtd_scene_batch_destroy(self.sb)
+            self.sb = None
+            self.members = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

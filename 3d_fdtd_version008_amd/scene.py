@@ -399,3 +399,55 @@ def precision_case(steps: int = 4000, shape=(512, 512, 384), n_cuboids: int = 64
     sp = gain_slab_case(npml=12, steps=steps, shape=shape, n_cuboids=n_cuboids, seed=seed)
     sp.name = f"precision_{shape[0]}x{shape[1]}x{shape[2]}"
     return sp
+
+
+def batch_case(index: int, seed: int = 1, steps: int = 2000, lo: int = 32, hi: int = 160) -> SceneSpec:
+    """One domain of the batching workload (seeded by (seed, index)): each
+    total axis log-uniform in [lo, hi] = [32, 160] cells, CPML 8..12 cells, a uniform mesh
+    (spacing U[10, 40] nm) or a z axis graded 10 -> 40 nm, 1..8 dielectric
+    blocks (eps_r U[1, 12]), an Ez point dipole (lambda U[0.8, 1.6] um), and in
+    a quarter of the domains a two-level gain slab with a DFT plane above it.
+    One Ez probe per domain."""
+    rng = np.random.default_rng([seed, index])
+    dims = [int(round(math.exp(rng.uniform(math.log(lo), math.log(hi))))) for _ in range(3)]
+    npml = int(rng.integers(8, 13))
+    graded = bool(rng.random() < 0.5)
+    if graded:
+        lat = 20e-9
+        dx, dy = np.full(dims[0], lat), np.full(dims[1], lat)
+        dz = graded_axis(dims[2], 10e-9, 40e-9, max(4, dims[2] // 5) * 10e-9)
+    else:
+        d = float(rng.uniform(10e-9, 40e-9))
+        dx, dy, dz = np.full(dims[0], d), np.full(dims[1], d), np.full(dims[2], d)
+    bounds = [np.concatenate([[0.0], np.cumsum(a)]) for a in (dx, dy, dz)]
+    core_lo = [b[npml] for b in bounds]
+    core_len = [b[n - npml] - b[npml] for b, n in zip(bounds, dims)]
+    structures = []
+    for _ in range(int(rng.integers(1, 9))):
+        lo = [float(rng.uniform(0, 0.7)) * L for L in core_len]
+        hi = [lo[a] + float(rng.uniform(0.1, 0.3)) * core_len[a] for a in range(3)]
+        eps = float(rng.uniform(1.0, 12.0))
+        structures.append(Structure(SHAPE_BOX, math.sqrt(eps), (lo[0], hi[0], lo[1], hi[1], lo[2], hi[2])))
+    tls, dfts = [], []
+    gain = bool(rng.random() < 0.25)
+    zmid = 0.5 * core_len[2]
+    if gain:
+        big = 1.0
+        th = 0.2 * core_len[2]
+        structures.append(Structure(SHAPE_BOX, 1.5, (-big, big, -big, big, zmid - th / 2, zmid + th / 2),
+                                    tls_material=0))
+        tls = [TLSMaterial(lambda0=1.0e-6, gamma=2 * math.pi * 20e12, tau=1e-9, N0=1e24, inversion_fraction=0.3)]
+        kmon = int(np.searchsorted(bounds[2], core_lo[2] + zmid + 0.3 * core_len[2])) - 1
+        kmon = min(max(kmon, npml), dims[2] - npml - 1)
+        dfts = [DFTMonitor(2, (npml, dims[0] - npml, npml, dims[1] - npml, kmon, kmon + 1),
+                           (C0 / 0.95e-6, C0 / 1.0e-6, C0 / 1.05e-6))]
+    src = [0.5 * core_len[0], 0.5 * core_len[1], zmid + 0.25 * core_len[2]]
+    lam = float(rng.uniform(0.8e-6, 1.6e-6))
+    dip = Dipole(src[0], src[1], src[2], amplitude=1e-7, wavelength=lam, tau=3e-15, t0_factor=3.0,
+                 waveform=WF_GAUSS_MOD_SINE, polarization=POL["Ez"])
+    probes = [Probe(0.5 * core_len[0], 0.4 * core_len[1], zmid, (2,))]
+    return SceneSpec(
+        nx_core=dims[0] - 2 * npml, ny_core=dims[1] - 2 * npml, nz_core=dims[2] - 2 * npml, npml=npml,
+        dx_uniform=float(dx[0]), dy_uniform=float(dy[0]), dz_uniform=float(dz.min()), dx=dx, dy=dy, dz=dz,
+        S=0.99, structures=structures, tls_materials=tls, dipoles=[dip], probes=probes, dfts=dfts,
+        save_every=1, n_steps=steps, name=f"batch_{seed}_{index}_{dims[0]}x{dims[1]}x{dims[2]}")

@@ -194,6 +194,35 @@ int fdtd_read_check(fdtd_handle h, fdtd_check_record* out, int max, int* count);
  * bytes per cell, gain cells, npml. */
 int fdtd_info(fdtd_handle h, long* info);
 
+/* ---- batches: many independent domains of one precision stepped together ----
+ * No reference counterpart (the reference runs one domain per process).  A
+ * batch advances every member domain with one persistent H launch and one
+ * persistent E launch per step (work-groups take tiles of all domains from a
+ * device counter) plus one launch each for probes, DFT monitors and the
+ * stability monitor of all domains.  Results of a domain are identical
+ * whatever batch it runs in.  A handle belongs to at most one batch and must
+ * not be stepped on its own while it is a member. */
+typedef struct fdtd_batch* fdtd_batch_handle;
+int fdtd_batch_create(int precision, fdtd_batch_handle* out);
+int fdtd_batch_destroy(fdtd_batch_handle b);
+/* Adds a domain; returns its slot (>= 0) or an error. */
+int fdtd_batch_add(fdtd_batch_handle b, fdtd_handle h);
+int fdtd_batch_remove(fdtd_batch_handle b, fdtd_handle h);
+int fdtd_batch_count(fdtd_batch_handle b);
+/* Tile rows per work-group (1, 2 or 4), persistent work-groups per CU
+ * (1..16) and tiles taken per counter increment (1..64). */
+int fdtd_batch_set_tuning(fdtd_batch_handle b, int rows, int wg_per_cu, int chunk);
+int fdtd_batch_get_tuning(fdtd_batch_handle b, int* rows, int* wg_per_cu, int* chunk);
+/* on != 0: capture the launches of an fdtd_batch_step call in a HIP graph and
+ * replay it for later calls with the same step count and membership. */
+int fdtd_batch_set_graph(fdtd_batch_handle b, int on);
+/* Advances every member nsteps steps.  n0[d] is member d's global step index
+ * (slot order), src_values[d] its source values [nsteps][n_sources] as for
+ * fdtd_step (may be NULL for a domain without sources).  ms (may be NULL)
+ * receives the device time of the call.  Returns FDTD_EUNSTABLE if any
+ * member fails the stability policy (fdtd_read_check on its handle). */
+int fdtd_batch_step(fdtd_batch_handle b, int nsteps, const long* n0, const double* const* src_values, double* ms);
+
 #ifdef __cplusplus
 }
 #endif

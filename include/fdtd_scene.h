@@ -158,6 +158,26 @@ long fdtd_scene_steps_done(const fdtd_scene* sc);
 /* Device handle of the scene (fdtd_hip.h entry points). */
 struct fdtd_sim* fdtd_scene_handle(fdtd_scene* sc);
 
+/* ---- batches of scenes (fdtd_batch_* with host-computed source values) ----
+ * Member scenes keep their own step counters; a batch step advances every
+ * member by nsteps.  Destroying the batch leaves the scenes alive. */
+typedef struct fdtd_scene_batch fdtd_scene_batch;
+int  fdtd_scene_batch_create(int precision, fdtd_scene_batch** out);
+int  fdtd_scene_batch_destroy(fdtd_scene_batch* sb);
+int  fdtd_scene_batch_add(fdtd_scene_batch* sb, fdtd_scene* sc);
+int  fdtd_scene_batch_remove(fdtd_scene_batch* sb, fdtd_scene* sc);
+int  fdtd_scene_batch_count(const fdtd_scene_batch* sb);
+/* ms (may be NULL) receives the device time of the call. */
+int  fdtd_scene_batch_step(fdtd_scene_batch* sb, int nsteps, double* ms);
+/* Autotuner: runs trial_steps real steps under each candidate tiling (rows
+ * 1, 2, 4 x work-groups per CU 2, 4, 8 x tiles per counter increment 1, 4,
+ * 16), keeps the fastest and returns it.  A winner is remembered per
+ * (precision, member count, total cells rounded to a power of two) for the
+ * life of the process; a later call with the same key applies it without
+ * trials.  Returns the number of steps taken. */
+int  fdtd_scene_batch_autotune(fdtd_scene_batch* sb, int trial_steps, int* rows, int* wg_per_cu, int* chunk);
+struct fdtd_batch* fdtd_scene_batch_handle(fdtd_scene_batch* sb);
+
 #ifdef __cplusplus
 }
 #endif
