@@ -1070,7 +1070,7 @@ struct fdtd_batch {
 namespace {
 
 template <typename T, typename R>
-int batch_tables(fdtd_batch* b, int nsteps, const long* n0) {
+int batch_tables(fdtd_batch* b, int nsteps, const long* n0, const int* msteps) {
     const int nd = (int)b->doms.size();
     b->h_args.resize(sizeof(StepArgs<T, R>) * nd);
     StepArgs<T, R>* args = reinterpret_cast<StepArgs<T, R>*>(b->h_args.data());
@@ -1096,6 +1096,7 @@ int batch_tables(fdtd_batch* b, int nsteps, const long* n0) {
         b->h_cum[d + 1] = b->h_cum[d] + (int)tiles;
         fdtdk::BatchDom& m = b->h_dom[d];
         m.n0 = n0[d];
+        m.steps = msteps[d];
         m.N = s->N;
         m.ngain = s->gain ? s->gbox : 0;
         m.interval = s->mon_interval;
@@ -1135,17 +1136,17 @@ int batch_tables(fdtd_batch* b, int nsteps, const long* n0) {
 
 // The launches of one batch step (also what a graph captures).
 template <typename T, typename R>
-void batch_launch_step(fdtd_batch* b, int s, const long* n0) {
+void batch_launch_step(fdtd_batch* b, int s, const long* n0, const int* msteps) {
     const int nd = (int)b->doms.size();
     const StepArgs<T, R>* args = reinterpret_cast<const StepArgs<T, R>*>(b->d_args);
     hipLaunchKernelGGL(fdtdk::k_batch_begin, dim3(1), dim3(64), 0, b->stream, b->d_ctl);
     const dim3 blk(64, (unsigned)b->rows, 1);
     const int total = b->h_cum[nd];
     const int nwg = std::max(1, std::min(b->n_cu * b->wg_per_cu * (4 / b->rows), (total + b->chunk - 1) / b->chunk));
-    hipLaunchKernelGGL((fdtdk::k_batch_step<T, R, false>), dim3((unsigned)nwg), blk, 0, b->stream, args, b->d_cum, nd,
-                       b->chunk, b->d_ctl);
-    hipLaunchKernelGGL((fdtdk::k_batch_step<T, R, true>), dim3((unsigned)nwg), blk, 0, b->stream, args, b->d_cum, nd,
-                       b->chunk, b->d_ctl);
+    hipLaunchKernelGGL((fdtdk::k_batch_step<T, R, false>), dim3((unsigned)nwg), blk, 0, b->stream, args, b->d_cum,
+                       b->d_dom, nd, b->chunk, b->d_ctl);
+    hipLaunchKernelGGL((fdtdk::k_batch_step<T, R, true>), dim3((unsigned)nwg), blk, 0, b->stream, args, b->d_cum,
+                       b->d_dom, nd, b->chunk, b->d_ctl);
     if (!b->h_probes.empty())
         hipLaunchKernelGGL((fdtdk::k_batch_probes<T, R>), dim3((unsigned)((b->h_probes.size() + 63) / 64)), dim3(64), 0,
                            b->stream, args, b->d_dom, b->d_probes, (int)b->h_probes.size(), b->d_ctl);
@@ -1158,20 +1159,20 @@ void batch_launch_step(fdtd_batch* b, int s, const long* n0) {
     bool mon = false;
     for (int d = 0; d < nd && !mon; ++d) {
         const int iv = b->doms[d]->mon_interval;
-        mon = iv > 0 && (n0[d] + s) % iv == 0;
+        mon = iv > 0 && s < msteps[d] && (n0[d] + s) % iv == 0;
     }
     if (mon && b->slots > 0)
         hipLaunchKernelGGL((fdtdk::k_batch_monitor<T, R>), dim3(64, (unsigned)nd), dim3(256), 0, b->stream, args,
                            b->d_dom, b->d_mon, b->slots, b->d_ctl);
     if (sizeof(T) == 2)
         for (int d = 0; d < nd; ++d)
-            if ((n0[d] + s) % b->doms[d]->fp16_interval == 0) fp16_maintain(b->doms[d], b->stream);
+            if (s < msteps[d] && (n0[d] + s) % b->doms[d]->fp16_interval == 0) fp16_maintain(b->doms[d], b->stream);
 }
 
 template <typename T, typename R>
-int batch_step(fdtd_batch* b, int nsteps, const long* n0, const double* const* srcv, double* ms) {
+int batch_step(fdtd_batch* b, int nsteps, const long* n0, const int* msteps, const double* const* srcv, double* ms) {
     const int nd = (int)b->doms.size();
-    if (int rc = batch_tables<T, R>(b, nsteps, n0)) return rc;
+    if (int rc = batch_tables<T, R>(b, nsteps, n0, msteps)) return rc;
     for (int d = 0; d < nd; ++d) {
         fdtd_sim* s = b->doms[d];
         if (s->nsrc > 0 && (!srcv || !srcv[d])) return fail(FDTD_EINVAL, "source values missing for a batch member");
@@ -1180,7 +1181,7 @@ int batch_step(fdtd_batch* b, int nsteps, const long* n0, const double* const* s
     b->h_srcv.resize(std::max<size_t>(1, (size_t)b->srcv_off.back() + (size_t)nsteps * b->doms.back()->nsrc));
     for (int d = 0; d < nd; ++d)
         if (b->doms[d]->nsrc > 0)
-            std::memcpy(b->h_srcv.data() + b->srcv_off[d], srcv[d], (size_t)nsteps * b->doms[d]->nsrc * sizeof(double));
+            std::memcpy(b->h_srcv.data() + b->srcv_off[d], srcv[d], (size_t)msteps[d] * b->doms[d]->nsrc * sizeof(double));
     HIPCHK(hipMemcpyAsync(b->d_srcv, b->h_srcv.data(), b->h_srcv.size() * sizeof(double), hipMemcpyHostToDevice, b->stream));
     HIPCHK(hipMemcpyAsync(b->d_args, b->h_args.data(), b->h_args.size(), hipMemcpyHostToDevice, b->stream));
     HIPCHK(hipMemcpyAsync(b->d_cum, b->h_cum.data(), b->h_cum.size() * sizeof(int), hipMemcpyHostToDevice, b->stream));
@@ -1207,6 +1208,7 @@ int batch_step(fdtd_batch* b, int nsteps, const long* n0, const double* const* s
     for (int d = 0; d < nd; ++d) {
         const fdtd_sim* m = b->doms[d];
         phase.push_back(m->mon_interval > 0 ? (int)(n0[d] % m->mon_interval) : -1);
+        phase.push_back(msteps[d]);
         if (b->prec == FDTD_FP16) phase.push_back((int)(n0[d] % m->fp16_interval));
     }
     if (b->use_graph) {
@@ -1217,7 +1219,7 @@ int batch_step(fdtd_batch* b, int nsteps, const long* n0, const double* const* s
             }
             hipGraph_t g;
             HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-            for (int s = 0; s < nsteps; ++s) batch_launch_step<T, R>(b, s, n0);
+            for (int s = 0; s < nsteps; ++s) batch_launch_step<T, R>(b, s, n0, msteps);
             HIPCHK(hipStreamEndCapture(b->stream, &g));
             const hipError_t e = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
@@ -1228,7 +1230,7 @@ int batch_step(fdtd_batch* b, int nsteps, const long* n0, const double* const* s
         }
         HIPCHK(hipGraphLaunch(b->gexec, b->stream));
     } else {
-        for (int s = 0; s < nsteps; ++s) batch_launch_step<T, R>(b, s, n0);
+        for (int s = 0; s < nsteps; ++s) batch_launch_step<T, R>(b, s, n0, msteps);
     }
     HIPCHK(hipGetLastError());
     if (ms) {
@@ -1248,7 +1250,8 @@ int batch_step(fdtd_batch* b, int nsteps, const long* n0, const double* const* s
     bool ok = true;
     for (int d = 0; d < nd; ++d) {
         fdtd_sim* s = b->doms[d];
-        const long n_end = n0[d] + nsteps;
+        const long n_end = n0[d] + msteps[d];
+        if (msteps[d] == 0) continue;
         for (Probe& p : s->probes) p.count = std::min(p.cap, (n_end - 1) / p.save_every + 1);
         s->checks.clear();
         const fdtdk::BatchDom& m = b->h_dom[d];
@@ -1350,18 +1353,25 @@ int fdtd_batch_set_graph(fdtd_batch_handle b, int on) {
     return FDTD_OK;
 }
 
-int fdtd_batch_step(fdtd_batch_handle b, int nsteps, const long* n0, const double* const* src_values, double* ms) {
+int fdtd_batch_step(fdtd_batch_handle b, int nsteps, const long* n0, const int* member_steps,
+                    const double* const* src_values, double* ms) {
     if (!b || nsteps < 0 || (!b->doms.empty() && !n0)) return fail(FDTD_EINVAL, "bad batch step arguments");
     if (nsteps == 0 || b->doms.empty()) {
         if (ms) *ms = 0;
         return FDTD_OK;
     }
-    for (size_t d = 0; d < b->doms.size(); ++d)
+    std::vector<int> ms_(b->doms.size(), nsteps);
+    for (size_t d = 0; d < b->doms.size(); ++d) {
         if (n0[d] < 0) return fail(FDTD_EINVAL, "negative step index");
+        if (member_steps) {
+            if (member_steps[d] < 0 || member_steps[d] > nsteps) return fail(FDTD_EINVAL, "member step count out of range");
+            ms_[d] = member_steps[d];
+        }
+    }
     switch (b->prec) {
-    case FDTD_FP64: return batch_step<double, double>(b, nsteps, n0, src_values, ms);
-    case FDTD_FP32: return batch_step<float, float>(b, nsteps, n0, src_values, ms);
-    default: return batch_step<uint16_t, float>(b, nsteps, n0, src_values, ms);
+    case FDTD_FP64: return batch_step<double, double>(b, nsteps, n0, ms_.data(), src_values, ms);
+    case FDTD_FP32: return batch_step<float, float>(b, nsteps, n0, ms_.data(), src_values, ms);
+    default: return batch_step<uint16_t, float>(b, nsteps, n0, ms_.data(), src_values, ms);
     }
 }
 

@@ -1148,7 +1148,7 @@ struct BatchDom {
     long N, ngain;   // cells, gain-box cells (monitor)
     long mon_first;  // first monitor slot of the block: ceil(n0 / interval)
     int interval;    // monitor cadence (0: off)
-    int pad;
+    int steps;       // steps this member takes in the block (it idles after)
 };
 struct BatchProbe {
     int dom, comp;
@@ -1182,7 +1182,8 @@ __device__ __forceinline__ int batch_domain(const int* __restrict__ cum, int ndo
 
 template <typename T, typename R, bool EPASS>
 __global__ void __launch_bounds__(256) k_batch_step(const StepArgs<T, R>* __restrict__ doms, const int* __restrict__ cum,
-                                                    int ndom, int chunk, BatchCtl* ctl) {
+                                                    const BatchDom* __restrict__ bd, int ndom, int chunk,
+                                                    BatchCtl* ctl) {
     __shared__ int first_s;
     const int tj = blockDim.y;
     const int ty = __builtin_amdgcn_readfirstlane(threadIdx.y);
@@ -1198,6 +1199,7 @@ __global__ void __launch_bounds__(256) k_batch_step(const StepArgs<T, R>* __rest
         int d = batch_domain(cum, ndom, first);
         for (int t = first; t < last; ++t) {
             while (t >= cum[d + 1]) ++d;
+            if (lstep >= bd[d].steps) continue;
             StepArgs<T, R> a = doms[d];
             const int lt = t - cum[d];
             const int nkt = (a.nz + 63) >> 6, njt = (a.ny + tj - 1) / tj;
@@ -1232,6 +1234,7 @@ __global__ void k_batch_probes(const StepArgs<T, R>* __restrict__ doms, const Ba
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= npr) return;
     const BatchProbe b = pr[p];
+    if (ctl->lstep >= bd[b.dom].steps) return;
     const long n = bd[b.dom].n0 + ctl->lstep;
     if (n % b.save_every != 0) return;
     const long slot = n / b.save_every;
@@ -1242,6 +1245,7 @@ template <typename T, typename R>
 __global__ void k_batch_dft(const StepArgs<T, R>* __restrict__ doms, const BatchDom* __restrict__ bd,
                             const BatchDft* __restrict__ dd, const BatchCtl* ctl) {
     const BatchDft& m = dd[blockIdx.y];
+    if (ctl->lstep >= bd[m.dom].steps) return;
     DftArgs d = m.d;
     const long n = bd[m.dom].n0 + ctl->lstep;
     const double t = (d.comp < 3) ? ((double)n + 1.0) * m.dt : ((double)n + 0.5) * m.dt;
@@ -1260,7 +1264,7 @@ __global__ void __launch_bounds__(256) k_batch_monitor(const StepArgs<T, R>* __r
                                                        const BatchCtl* ctl) {
     const int d = blockIdx.y;
     const BatchDom m = bd[d];
-    if (m.interval <= 0) return;
+    if (m.interval <= 0 || ctl->lstep >= m.steps) return;
     const long n = m.n0 + ctl->lstep;
     if (n % m.interval != 0) return;
     const long slot = n / m.interval - m.mon_first;

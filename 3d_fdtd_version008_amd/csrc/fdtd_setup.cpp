@@ -634,6 +634,7 @@ struct fdtd_scene {
     long n = 0;
     std::vector<double> vals;
     struct fdtd_scene_batch* batch = nullptr;  // the scene batch it belongs to
+    long stop = -1;                            // batch steps never take n past this (-1: no limit)
 };
 
 extern "C" {
@@ -787,6 +788,12 @@ int fdtd_scene_step_timed(fdtd_scene* sc, int nsteps, double* ms) {
 
 long fdtd_scene_steps_done(const fdtd_scene* sc) { return sc ? sc->n : -1; }
 
+int fdtd_scene_set_stop(fdtd_scene* sc, long n_stop) {
+    if (!sc || n_stop < -1) return FDTD_EINVAL;
+    sc->stop = n_stop;
+    return FDTD_OK;
+}
+
 fdtd_sim* fdtd_scene_handle(fdtd_scene* sc) { return sc ? sc->h : nullptr; }
 
 }  // extern "C"
@@ -863,21 +870,23 @@ int fdtd_scene_batch_step(fdtd_scene_batch* sb, int nsteps, double* ms) {
     if (!sb || nsteps < 0) return FDTD_EINVAL;
     const size_t nd = sb->scenes.size();
     std::vector<long> n0(nd);
+    std::vector<int> steps(nd);
     std::vector<const double*> sv(nd, nullptr);
     sb->vals.resize(nd);
     for (size_t d = 0; d < nd; ++d) {
         fdtd_scene* sc = sb->scenes[d];
         n0[d] = sc->n;
+        steps[d] = sc->stop < 0 ? nsteps : (int)std::max<long>(0, std::min<long>(nsteps, sc->stop - sc->n));
         const size_t ns = sc->setup->src.size();
         if (ns) {
-            sb->vals[d].resize((size_t)nsteps * ns);
-            fdtd_setup_source_values(sc->setup, sc->n, nsteps, sb->vals[d].data());
+            sb->vals[d].resize((size_t)std::max(steps[d], 1) * ns);
+            if (steps[d]) fdtd_setup_source_values(sc->setup, sc->n, steps[d], sb->vals[d].data());
             sv[d] = sb->vals[d].data();
         }
     }
-    const int rc = fdtd_batch_step(sb->b, nsteps, n0.data(), sv.data(), ms);
+    const int rc = fdtd_batch_step(sb->b, nsteps, n0.data(), steps.data(), sv.data(), ms);
     if (rc == FDTD_OK || rc == FDTD_EUNSTABLE)
-        for (fdtd_scene* sc : sb->scenes) sc->n += nsteps;
+        for (size_t d = 0; d < nd; ++d) sb->scenes[d]->n += steps[d];
     return rc;
 }
 

@@ -38,6 +38,7 @@ EXPORTS = [
     "fdtd_batch_set_tuning", "fdtd_batch_get_tuning", "fdtd_batch_set_graph", "fdtd_batch_step",
     "fdtd_scene_batch_create", "fdtd_scene_batch_destroy", "fdtd_scene_batch_add", "fdtd_scene_batch_remove",
     "fdtd_scene_batch_count", "fdtd_scene_batch_step", "fdtd_scene_batch_autotune", "fdtd_scene_batch_handle",
+    "fdtd_scene_set_stop",
 ]
 
 
@@ -113,6 +114,7 @@ def load() -> C.CDLL:
         "fdtd_scene_batch_step": (i, [vp, i, C.POINTER(d)]),
         "fdtd_scene_batch_autotune": (i, [vp, i, C.POINTER(i), C.POINTER(i), C.POINTER(i)]),
         "fdtd_scene_batch_handle": (vp, [vp]),
+        "fdtd_scene_set_stop": (i, [vp, l]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(lib, name)
@@ -215,7 +217,10 @@ class Scene:
         except Exception:
             pass
 
-    @property
+    def set_stop(self, n_stop: int):
+        """Batch steps never take this scene past step n_stop (-1: no limit)."""
+        _check(self.lib.fdtd_scene_set_stop(self.sc, n_stop))
+
     def steps_done(self) -> int:
         return self.lib.fdtd_scene_steps_done(self.sc)
 

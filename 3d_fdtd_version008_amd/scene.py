@@ -401,14 +401,13 @@ def precision_case(steps: int = 4000, shape=(512, 512, 384), n_cuboids: int = 64
     return sp
 
 
-def batch_case(index: int, seed: int = 1, steps: int = 2000, lo: int = 32, hi: int = 160) -> SceneSpec:
-    """One domain of the batching workload (seeded by (seed, index)): each
-    total axis log-uniform in [lo, hi] = [32, 160] cells, CPML 8..12 cells, a uniform mesh
-    (spacing U[10, 40] nm) or a z axis graded 10 -> 40 nm, 1..8 dielectric
-    blocks (eps_r U[1, 12]), an Ez point dipole (lambda U[0.8, 1.6] um), and in
-    a quarter of the domains a two-level gain slab with a DFT plane above it.
-    One Ez probe per domain."""
-    rng = np.random.default_rng([seed, index])
+def _random_domain(rng, lo: int, hi: int, gain: Optional[bool], nblocks: Optional[int], steps: int,
+                   name: str) -> SceneSpec:
+    """A seeded random domain: each total axis log-uniform in [lo, hi] cells,
+    CPML 8..12 cells, a uniform mesh (spacing U[10, 40] nm) or a z axis graded
+    10 -> 40 nm, `nblocks` dielectric blocks (eps_r U[1, 12]; None: 1..8), an
+    Ez point dipole (lambda U[0.8, 1.6] um), a two-level gain slab with a DFT
+    plane above it when `gain` (None: with probability 1/4), one Ez probe."""
     dims = [int(round(math.exp(rng.uniform(math.log(lo), math.log(hi))))) for _ in range(3)]
     npml = int(rng.integers(8, 13))
     graded = bool(rng.random() < 0.5)
@@ -423,13 +422,15 @@ def batch_case(index: int, seed: int = 1, steps: int = 2000, lo: int = 32, hi: i
     core_lo = [b[npml] for b in bounds]
     core_len = [b[n - npml] - b[npml] for b, n in zip(bounds, dims)]
     structures = []
-    for _ in range(int(rng.integers(1, 9))):
-        lo = [float(rng.uniform(0, 0.7)) * L for L in core_len]
-        hi = [lo[a] + float(rng.uniform(0.1, 0.3)) * core_len[a] for a in range(3)]
+    nb = int(rng.integers(1, 9)) if nblocks is None else nblocks
+    for _ in range(nb):
+        lo_ = [float(rng.uniform(0, 0.7)) * L for L in core_len]
+        hi_ = [lo_[a] + float(rng.uniform(0.1, 0.3)) * core_len[a] for a in range(3)]
         eps = float(rng.uniform(1.0, 12.0))
-        structures.append(Structure(SHAPE_BOX, math.sqrt(eps), (lo[0], hi[0], lo[1], hi[1], lo[2], hi[2])))
+        structures.append(Structure(SHAPE_BOX, math.sqrt(eps), (lo_[0], hi_[0], lo_[1], hi_[1], lo_[2], hi_[2])))
     tls, dfts = [], []
-    gain = bool(rng.random() < 0.25)
+    if gain is None:
+        gain = bool(rng.random() < 0.25)
     zmid = 0.5 * core_len[2]
     if gain:
         big = 1.0
@@ -450,4 +451,22 @@ def batch_case(index: int, seed: int = 1, steps: int = 2000, lo: int = 32, hi: i
         nx_core=dims[0] - 2 * npml, ny_core=dims[1] - 2 * npml, nz_core=dims[2] - 2 * npml, npml=npml,
         dx_uniform=float(dx[0]), dy_uniform=float(dy[0]), dz_uniform=float(dz.min()), dx=dx, dy=dy, dz=dz,
         S=0.99, structures=structures, tls_materials=tls, dipoles=[dip], probes=probes, dfts=dfts,
-        save_every=1, n_steps=steps, name=f"batch_{seed}_{index}_{dims[0]}x{dims[1]}x{dims[2]}")
+        save_every=1, n_steps=steps, name=f"{name}_{dims[0]}x{dims[1]}x{dims[2]}")
+
+
+def batch_case(index: int, seed: int = 1, steps: int = 2000, lo: int = 32, hi: int = 160) -> SceneSpec:
+    """One domain of the batching workload, seeded by (seed, index): axes
+    log-uniform in [lo, hi] = [32, 160] cells, 1..8 blocks, gain in a quarter
+    of the domains (see _random_domain)."""
+    rng = np.random.default_rng([seed, index])
+    return _random_domain(rng, lo, hi, None, None, steps, f"batch_{seed}_{index}")
+
+
+def serve_case(index: int, kind: int, seed: int = 1, lo: int = 48, hi: int = 224, steps: int = 500) -> SceneSpec:
+    """One request of the serving workload, seeded by (seed, index): kind 0 a
+    dielectric scene (one block), 1 dielectric + gain slab (one block), 2 a
+    multi-block scene (4..8 blocks); axes log-uniform in [lo, hi] cells."""
+    rng = np.random.default_rng([seed, 1000003 + index])
+    gain = kind == 1
+    nblocks = 1 if kind in (0, 1) else int(rng.integers(4, 9))
+    return _random_domain(rng, lo, hi, gain, nblocks, steps, f"serve_{seed}_{index}_k{kind}")

@@ -1,0 +1,268 @@
+"""Serving: a request loop over device-resident simulations.
+
+Requests (a SceneSpec, a storage precision and a step count) arrive over
+time.  Each is set up on the host (a thread pool overlaps this with device
+work), then admitted when a slot and enough device memory are free (the
+budget is checked against each scene's actual device footprint).  Admitted
+scenes join the batch of their precision and advance together, one tick
+(up to `tick` steps) at a time: every tick is one persistent H launch and one
+persistent E launch per precision for all resident requests, so requests that
+arrive while others run share the device with them (continuous batching).
+Finished requests are read back and freed at the end of the tick in which
+they complete, which makes room for waiting ones.
+
+A request's results do not depend on what else is resident: every member of a
+batch goes through the same per-cell operations as a solo run (fields and
+probe traces bitwise, DFT accumulators to 1e-12).
+
+No reference counterpart (the reference runs one simulation per process).
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import time
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional
+
+import numpy as np
+
+from . import runtime
+from . import scene as scn
+
+
+@dataclass
+class Request:
+    rid: int
+    spec: "scn.SceneSpec"
+    precision: str = "fp32"
+    steps: int = 500
+    arrival: float = 0.0          # seconds after the stream starts
+    keep_fields: bool = False     # read back the final fields too
+
+
+@dataclass
+class Result:
+    rid: int
+    precision: str
+    steps: int
+    cells: int
+    arrival: float
+    admitted: float
+    finished: float
+    device_bytes: int
+    probes: List[np.ndarray] = field(default_factory=list)
+    dfts: List[np.ndarray] = field(default_factory=list)
+    fields: Dict[str, np.ndarray] = field(default_factory=dict)
+
+    @property
+    def latency(self) -> float:
+        return self.finished - self.arrival
+
+
+@dataclass
+class _Live:
+    req: Request
+    setup: runtime.Setup
+    scene: runtime.Scene
+    remaining: int
+    admitted: float
+    device_bytes: int
+
+
+def estimate_device_bytes(setup: runtime.Setup, precision: str) -> int:
+    """Upper estimate of a scene's device footprint before it is created:
+    double-buffered fields, psi slabs (6 arrays per axis: E in place, H
+    double-buffered), material ids and axis tables, gain state in its
+    bounding box, probe and DFT buffers, and 1 MiB for small tables."""
+    s = {"fp64": 8, "fp32": 4, "fp16": 2}[precision]
+    nx, ny, nz = setup.shape
+    n = nx * ny * nz
+    np_ = setup.npml
+    slab = 2 * np_ * (ny * nz + nx * nz + nx * ny) if np_ else 0
+    gain = 0
+    if setup.tls_on:
+        rm = setup.array("region_map")
+        idx = np.argwhere(rm >= 0)
+        if len(idx):
+            ext = idx.max(axis=0) - idx.min(axis=0) + 1
+            gain = int(np.prod(ext)) * 33
+    probes = setup.n_channels * 8 * (setup.spec.n_steps + 16)
+    dfts = sum(len(d.freqs) * 16 * (d.box[1] - d.box[0]) * (d.box[3] - d.box[2]) * (d.box[5] - d.box[4])
+               for d in setup.spec.dfts)
+    return int(12 * s * n + 2 * n + 6 * s * slab + gain + probes + dfts + (1 << 20))
+
+
+class Server:
+    """Continuous-batching request loop on one device.
+
+    budget_bytes: device memory the resident requests may use together.
+    max_active: resident request slots.  tick: steps per batch call (a full
+    tick is replayed from a captured HIP graph when `graph`)."""
+
+    def __init__(self, budget_bytes: float = 200e9, max_active: int = 256, tick: int = 50, graph: bool = True,
+                 setup_workers: int = 8, tuning: Optional[tuple] = None):
+        self.budget = int(budget_bytes)
+        self.max_active = max_active
+        self.tick = tick
+        self.graph = graph
+        self.setup_workers = setup_workers
+        self.tuning = tuning          # (rows, wg_per_cu, chunk), e.g. from Batch.autotune
+        self.batches: Dict[str, runtime.Batch] = {}
+
+    def _batch(self, precision: str) -> runtime.Batch:
+        b = self.batches.get(precision)
+        if b is None:
+            b = runtime.Batch(precision)
+            b.set_graph(self.graph)
+            if self.tuning:
+                b.set_tuning(*self.tuning)
+            self.batches[precision] = b
+        return b
+
+    def run(self, requests: List[Request]) -> List[Result]:
+        """Replays the arrival times in real time and serves every request;
+        returns the results in completion order."""
+        reqs = sorted(requests, key=lambda r: r.arrival)
+        pool = cf.ThreadPoolExecutor(max_workers=self.setup_workers)
+        t0 = time.perf_counter()
+        now = lambda: time.perf_counter() - t0  # noqa: E731
+        nxt = 0                                   # next request not yet handed to setup
+        setting_up: List[cf.Future] = []
+        ready: List[tuple] = []                   # (request, setup, estimate)
+        live: List[_Live] = []
+        used = 0
+        done: List[Result] = []
+        try:
+            while nxt < len(reqs) or setting_up or ready or live:
+                t = now()
+                while nxt < len(reqs) and reqs[nxt].arrival <= t:
+                    r = reqs[nxt]
+                    setting_up.append(pool.submit(self._prepare, r))
+                    nxt += 1
+                still = []
+                for f in setting_up:
+                    if f.done():
+                        ready.append(f.result())
+                    else:
+                        still.append(f)
+                setting_up = still
+                # admission in arrival order while a slot and memory are free
+                ready.sort(key=lambda x: x[0].arrival)
+                waiting = []
+                for req, setup, est in ready:
+                    if len(live) < self.max_active and used + est <= self.budget:
+                        sc = runtime.Scene(setup, req.precision, probe_capacity=req.steps + 16)
+                        sc.set_stop(req.steps)
+                        actual = sc.device_bytes()
+                        self._batch(req.precision).add(sc)
+                        live.append(_Live(req, setup, sc, req.steps, now(), actual))
+                        used += actual
+                    else:
+                        waiting.append((req, setup, est))
+                ready = waiting
+                if not live:
+                    # idle: wait for the next arrival or a finished setup
+                    if setting_up:
+                        cf.wait(setting_up, timeout=0.05, return_when=cf.FIRST_COMPLETED)
+                    elif nxt < len(reqs):
+                        time.sleep(max(0.0, min(0.05, reqs[nxt].arrival - now())))
+                    continue
+                # one tick for every resident request; a request that reaches
+                # its step count mid-tick idles for the rest of it
+                for b in self.batches.values():
+                    if len(b):
+                        b.step(self.tick)
+                for lv in live:
+                    lv.remaining = lv.req.steps - lv.scene.steps_done()
+                t_end = now()
+                keep = []
+                for lv in live:
+                    if lv.scene.steps_done() >= lv.req.steps:
+                        done.append(self._retire(lv, t_end))
+                        used -= lv.device_bytes
+                    else:
+                        keep.append(lv)
+                live = keep
+        finally:
+            pool.shutdown(wait=True)
+            for b in self.batches.values():
+                b.close()
+            self.batches = {}
+        return done
+
+    @staticmethod
+    def _prepare(req: Request):
+        setup = runtime.Setup(req.spec)
+        return req, setup, estimate_device_bytes(setup, req.precision)
+
+    def _retire(self, lv: _Live, t_end: float) -> Result:
+        self.batches[lv.req.precision].remove(lv.scene)
+        st = lv.setup
+        res = Result(rid=lv.req.rid, precision=lv.req.precision, steps=lv.req.steps,
+                     cells=st.shape[0] * st.shape[1] * st.shape[2], arrival=lv.req.arrival, admitted=lv.admitted,
+                     finished=t_end, device_bytes=lv.device_bytes)
+        res.probes = [lv.scene.probe(c) for c in range(st.n_channels)]
+        res.dfts = [lv.scene.dft(m) for m in range(st.n_dfts)]
+        if lv.req.keep_fields:
+            res.fields = {f: lv.scene.field(f) for f in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")}
+        lv.scene.close()
+        return res
+
+
+def solve_alone(req: Request) -> Result:
+    """The same request served by itself (a batch of one), for equality
+    checks and the one-at-a-time baseline."""
+    t0 = time.perf_counter()
+    setup = runtime.Setup(req.spec)
+    sc = runtime.Scene(setup, req.precision, probe_capacity=req.steps + 16)
+    sc.set_stop(req.steps)
+    b = runtime.Batch(req.precision)
+    b.add(sc)
+    while sc.steps_done() < req.steps:
+        b.step(50)
+    res = Result(rid=req.rid, precision=req.precision, steps=req.steps,
+                 cells=setup.shape[0] * setup.shape[1] * setup.shape[2], arrival=0.0, admitted=0.0,
+                 finished=time.perf_counter() - t0, device_bytes=sc.device_bytes())
+    res.probes = [sc.probe(c) for c in range(setup.n_channels)]
+    res.dfts = [sc.dft(m) for m in range(setup.n_dfts)]
+    if req.keep_fields:
+        res.fields = {f: sc.field(f) for f in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")}
+    b.close()
+    sc.close()
+    return res
+
+
+def request_stream(n: int, rate: float, seed: int = 1, lo: int = 48, hi: int = 224, steps=(500, 4000),
+                   fp16_share: float = 0.5) -> List[Request]:
+    """The serving workload: Poisson arrivals at `rate` per second; each axis
+    of a request log-uniform in [lo, hi] total cells; steps uniform in
+    `steps`; 40% dielectric, 40% dielectric + gain, 20% multi-block scenes;
+    FP32 or FP16 storage per request."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for rid in range(n):
+        t += float(rng.exponential(1.0 / rate)) if rate > 0 else 0.0
+        kind = rng.choice(3, p=[0.4, 0.4, 0.2])
+        sp = scn.serve_case(rid, int(kind), seed=seed, lo=lo, hi=hi)
+        nst = int(rng.integers(steps[0], steps[1] + 1))
+        sp.n_steps = nst
+        prec = "fp16" if rng.random() < fp16_share else "fp32"
+        out.append(Request(rid, sp, prec, nst, t))
+    return out
+
+
+def summarize(results: List[Result]) -> dict:
+    lat = np.array([r.latency for r in results])
+    work = sum(r.cells * r.steps for r in results)
+    t_first = min(r.arrival for r in results)
+    t_last = max(r.finished for r in results)
+    return {
+        "requests": len(results),
+        "cell_updates": work,
+        "makespan_s": t_last - t_first,
+        "gcells_per_s": work / max(t_last - t_first, 1e-9) / 1e9,
+        "latency_p50_s": float(np.percentile(lat, 50)),
+        "latency_p99_s": float(np.percentile(lat, 99)),
+        "latency_max_s": float(lat.max()),
+    }

@@ -216,12 +216,15 @@ int fdtd_batch_get_tuning(fdtd_batch_handle b, int* rows, int* wg_per_cu, int* c
 /* on != 0: capture the launches of an fdtd_batch_step call in a HIP graph and
  * replay it for later calls with the same step count and membership. */
 int fdtd_batch_set_graph(fdtd_batch_handle b, int on);
-/* Advances every member nsteps steps.  n0[d] is member d's global step index
- * (slot order), src_values[d] its source values [nsteps][n_sources] as for
- * fdtd_step (may be NULL for a domain without sources).  ms (may be NULL)
- * receives the device time of the call.  Returns FDTD_EUNSTABLE if any
- * member fails the stability policy (fdtd_read_check on its handle). */
-int fdtd_batch_step(fdtd_batch_handle b, int nsteps, const long* n0, const double* const* src_values, double* ms);
+/* Runs a block of nsteps steps.  n0[d] is member d's global step index (slot
+ * order); member_steps[d] (0..nsteps; NULL = all nsteps) is how many of the
+ * block's steps member d takes (it idles for the rest, so members can finish
+ * mid-block); src_values[d] its source values [member_steps[d]][n_sources] as
+ * for fdtd_step (may be NULL for a domain without sources).  ms (may be NULL)
+ * receives the device time of the call.  Returns FDTD_EUNSTABLE if any member
+ * fails the stability policy (fdtd_read_check on its handle). */
+int fdtd_batch_step(fdtd_batch_handle b, int nsteps, const long* n0, const int* member_steps,
+                    const double* const* src_values, double* ms);
 
 #ifdef __cplusplus
 }

@@ -155,6 +155,9 @@ int  fdtd_scene_step(fdtd_scene* sc, int nsteps);
 /* fdtd_scene_step through fdtd_step_timed (per-kernel HIP-event times, ms[3]). */
 int  fdtd_scene_step_timed(fdtd_scene* sc, int nsteps, double* ms);
 long fdtd_scene_steps_done(const fdtd_scene* sc);
+/* Batch steps never advance the scene past step n_stop (-1: no limit): a
+ * batch call in which it reaches n_stop leaves it idle for the rest. */
+int  fdtd_scene_set_stop(fdtd_scene* sc, long n_stop);
 /* Device handle of the scene (fdtd_hip.h entry points). */
 struct fdtd_sim* fdtd_scene_handle(fdtd_scene* sc);
 

@@ -1,0 +1,52 @@
+"""Serving: continuous batching of a mixed FP32/FP16 request stream.
+
+Every request's probe traces and final fields are bitwise identical to the
+same request served alone (DFT accumulators to 1e-12 relative L2), while
+requests overlap on the device and admission is limited by slots and by the
+device-memory budget."""
+import importlib
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def test_served_results_equal_alone(pkg):
+    serve = importlib.import_module("3d_fdtd_version008_amd.serve")
+    reqs = serve.request_stream(10, rate=40.0, seed=2, lo=32, hi=48, steps=(60, 140))
+    for r in reqs:
+        r.keep_fields = True
+    # a budget for about three resident requests forces queueing by memory
+    est = [serve.estimate_device_bytes(pkg.Setup(r.spec), r.precision) for r in reqs]
+    srv = serve.Server(budget_bytes=3.2 * max(est), max_active=8, tick=20, graph=True, setup_workers=4)
+    served = {r.rid: r for r in srv.run(reqs)}
+    assert sorted(served) == [r.rid for r in reqs]
+    s = serve.summarize(list(served.values()))
+    _say(f"  served {s}")
+    # continuous batching: some request was admitted while another was resident
+    spans = sorted((x.admitted, x.finished) for x in served.values())
+    overlap = any(spans[q + 1][0] < spans[q][1] for q in range(len(spans) - 1))
+    assert overlap
+    # the budget bounded the resident footprint
+    for x in served.values():
+        live = sum(y.device_bytes for y in served.values() if y.admitted <= x.admitted < y.finished)
+        assert live <= srv.budget
+    for r in reqs:
+        alone = serve.solve_alone(r)
+        got = served[r.rid]
+        for a, b in zip(alone.probes, got.probes):
+            assert np.array_equal(a, b), f"request {r.rid} probe differs"
+        for f in alone.fields:
+            assert np.array_equal(alone.fields[f], got.fields[f]), f"request {r.rid} field {f} differs"
+        for a, b in zip(alone.dfts, got.dfts):
+            assert rel_l2(b, a) <= 1e-12
+        assert len(got.probes[0]) == r.steps
