@@ -90,7 +90,7 @@ struct fdtd_sim {
     void* F[6] = {};    // = F2[cur]
     void* psi[12] = {}; // = psi2[cur] (psi_E always psi2[0])
     int cur = 0;
-    int mode = 1;       // 0: H and E kernels, 1: fused step kernel
+    int mode = 0;       // 0: H and E kernels (default), 1: fused step kernel
     int ichunk = 8;     // x-planes per work-group of the fused kernel
     long psi_len[12] = {};
     void* inv[3] = {};

@@ -10,6 +10,13 @@ monitor).  Inputs are resident in HBM before the timed region.
 
 Prints ONE JSON line (rank 0, N=1).  --gpus N > 1 is not supported: this
 build runs on one GPU.
+
+Other forms (not the default line): --form precision (512x512x384 graded grid
+with 64 cuboids: FP32 and FP16-storage throughput and their error against the
+FP64 device path per probe and per DFT spectrum), --form batch (256 uneven
+domains in persistent batch launches against one-at-a-time solves), --form
+serve (a Poisson stream of mixed FP32/FP16 requests under continuous batching:
+throughput, p50/p99 latency, against one-at-a-time solves).
 """
 from __future__ import annotations
 
@@ -19,6 +26,8 @@ import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -85,26 +94,7 @@ def cpu_baseline(spec, steps: int):
                       f"(oracle/fdtd_oracle.c, OpenMP over i, serial CPML, J zero-fill, monitor every 10)"}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
-    ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--precision", default="fp32", choices=["fp64", "fp32", "fp16"])
-    ap.add_argument("--size", type=int, default=256)
-    ap.add_argument("--cpu-steps", type=int, default=10)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--mode", type=int, default=1, help="0: separate H/E kernels, 1: fused step kernel")
-    ap.add_argument("--ichunk", type=int, default=0, help="fused kernel x-planes per work-group; 0 = library default")
-    ap.add_argument("--copy-calib", type=int, default=0,
-                    help="also run one device-to-device copy of this many MiB (a known-size kernel for "
-                         "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py: --gpus {args.gpus} is not supported; this build measures one GPU (N=1).",
-              file=sys.stderr)
-        sys.exit(2)
-
+def plain(args):
     pkg = importlib.import_module("3d_fdtd_version008_amd")
     if args.copy_calib:
         copy_calibration(args.copy_calib << 20)
@@ -130,19 +120,18 @@ def main():
     if args.mode == 1:
         kname, kbytes, kt = "k_step_rows", step_bytes, t_h
         kernels = {kname: round(ms_h / args.steps, 4)}
-    elif t_e >= t_h:
-        kname, kbytes, kt = "k_update_E", e_bytes, t_e
     else:
-        kname, kbytes, kt = "k_update_H", h_bytes, t_h
-    if args.mode == 0:
         kernels = {"k_update_H": round(ms_h / args.steps, 4), "k_update_E": round(ms_e / args.steps, 4)}
+        if t_e >= t_h:
+            kname, kbytes, kt = "k_update_E", e_bytes, t_e
+        else:
+            kname, kbytes, kt = "k_update_H", h_bytes, t_h
     achieved = kbytes / kt / 1e9
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     if os.path.exists(tfile):
         try:
-            tj = json.load(open(tfile))
-            traffic = tj.get(f"{spec.name}:{args.precision}:{kname}")
+            traffic = json.load(open(tfile)).get(f"{spec.name}:{args.precision}:{kname}")
         except Exception:
             traffic = None
     line = {
@@ -161,8 +150,7 @@ def main():
         "config": {"workload": spec.name, "grid": list(setup.shape), "cells": N, "npml": setup.npml,
                    "precision": args.precision, "gain_cells": gain_cells, "materials": info["n_materials"],
                    "mat_id_bytes": info["mat_id_bytes"], "setup_s": round(t_setup, 2),
-                   "kernel_mode": ["h_e_kernels", "fused_rows"][args.mode],
-                   "planes_per_chunk": args.ichunk},
+                   "kernel_mode": ["h_e_kernels", "fused_rows"][args.mode], "planes_per_chunk": args.ichunk},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes_per_launch": kbytes, "avg_launch_us": round(kt * 1e6, 2)},
@@ -174,6 +162,184 @@ def main():
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(spec, args.cpu_steps)
     print(json.dumps(line), flush=True)
+
+
+def _probe_errors(a, b, spec):
+    """Relative L2 of each probe's recorded vector (all its components) and of
+    each DFT spectrum, scene a against reference scene b."""
+    out = {}
+    c = 0
+    for p, pr in enumerate(spec.probes):
+        idx = list(range(c, c + len(pr.components)))
+        c += len(pr.components)
+        x = np.stack([a.probe(q) for q in idx])
+        y = np.stack([b.probe(q) for q in idx])
+        out[f"probe{p}"] = float(np.linalg.norm(x - y) / max(np.linalg.norm(y), 1e-300))
+    for m in range(len(spec.dfts)):
+        x, y = a.dft(m), b.dft(m)
+        out[f"dft{m}"] = float(np.linalg.norm(x - y) / max(np.linalg.norm(y), 1e-300))
+    return out
+
+
+def precision_form(args):
+    """FP32 and FP16-storage throughput on the ~100M-cell precision geometry,
+    and their error against the FP64 device path (the oracle at this size,
+    itself pinned to the CPU oracle at smaller sizes) after args.steps steps."""
+    pkg = importlib.import_module("3d_fdtd_version008_amd")
+    spec = pkg.scene.precision_case(steps=args.steps)
+    t0 = time.time()
+    setup = pkg.Setup(spec)
+    t_setup = time.time() - t0
+    N = setup.shape[0] * setup.shape[1] * setup.shape[2]
+    rates, scenes = {}, {}
+    for prec in ("fp64", "fp32", "fp16"):
+        sc = pkg.Scene(setup, prec, probe_capacity=args.steps + 16)
+        sc.set_kernel_mode(args.mode, args.ichunk)
+        done = 0
+        if prec != "fp64":
+            sc.step(args.warmup)
+            sc.synchronize()
+            t1 = time.perf_counter()
+            sc.step(args.rate_steps)
+            sc.synchronize()
+            rates[prec] = N * args.rate_steps / (time.perf_counter() - t1) / 1e9
+            done = args.warmup + args.rate_steps
+            print(f"[precision] {prec} {rates[prec]:.2f} Gcell/s", file=sys.stderr, flush=True)
+        while done < args.steps:
+            k = min(500, args.steps - done)
+            sc.step(k)
+            done += k
+            print(f"[precision] {prec} {done}/{args.steps}", file=sys.stderr, flush=True)
+        scenes[prec] = sc
+    err = {p: _probe_errors(scenes[p], scenes["fp64"], spec) for p in ("fp32", "fp16")}
+    line = {"metric": "precision form: Gcell-updates/s and rel L2 vs FP64 per probe and per spectrum",
+            "form": "precision", "config": {"workload": spec.name, "grid": list(setup.shape), "cells": N,
+                                            "steps": args.steps, "setup_s": round(t_setup, 1),
+                                            "kernel_mode": args.mode},
+            "gcells_per_s": {k: round(v, 3) for k, v in rates.items()},
+            "fp16_over_fp32": round(rates["fp16"] / rates["fp32"], 3),
+            "rel_l2_vs_fp64": err}
+    for sc in scenes.values():
+        sc.close()
+    print(json.dumps(line), flush=True)
+
+
+def _solo_rate(pkg, specs, precision, steps, mode):
+    """One-at-a-time plain solves: scene creation plus stepping, one request
+    after another; returns (cell-updates, seconds)."""
+    work, secs = 0, 0.0
+    for sp in specs:
+        st = pkg.Setup(sp)
+        t0 = time.perf_counter()
+        sc = pkg.Scene(st, precision, probe_capacity=steps + 16)
+        sc.set_kernel_mode(mode)
+        sc.step(steps)
+        sc.synchronize()
+        secs += time.perf_counter() - t0
+        work += st.shape[0] * st.shape[1] * st.shape[2] * steps
+        sc.close()
+    return work, secs
+
+
+def batch_form(args):
+    """256 uneven domains advanced by persistent batch launches (HIP-graph
+    blocks of 50 steps, autotuned tiling) against one-at-a-time solves."""
+    pkg = importlib.import_module("3d_fdtd_version008_amd")
+    specs = [pkg.scene.batch_case(i, seed=1, steps=args.steps) for i in range(args.domains)]
+    t0 = time.time()
+    setups = [pkg.Setup(sp) for sp in specs]
+    t_setup = time.time() - t0
+    cells = sum(st.shape[0] * st.shape[1] * st.shape[2] for st in setups)
+    b = pkg.Batch(args.precision)
+    scenes = [pkg.Scene(st, args.precision, probe_capacity=args.steps + 200) for st in setups]
+    for sc in scenes:
+        b.add(sc)
+    tuning, taken = b.autotune(trial_steps=2)
+    b.set_graph(True)
+    b.step(50)
+    t1 = time.perf_counter()
+    dev_ms = 0.0
+    left = args.steps
+    while left > 0:
+        k = min(50, left)
+        dev_ms += b.step(k)
+        left -= k
+    wall = time.perf_counter() - t1
+    rate = cells * args.steps / wall / 1e9
+    b.close()
+    for sc in scenes:
+        sc.close()
+    sample = specs[:: max(1, len(specs) // args.solo_sample)][: args.solo_sample]
+    w, s = _solo_rate(pkg, sample, args.precision, args.solo_steps, 0)
+    solo = w / s / 1e9
+    print(json.dumps({"metric": "batch form: Gcell-updates/s of 256 uneven domains", "form": "batch",
+                      "value": round(rate, 3), "unit": "Gcell-updates/s", "steps": args.steps,
+                      "ms_per_step": round(wall / args.steps * 1e3, 4), "device_ms_per_step": round(dev_ms / args.steps, 4),
+                      "config": {"domains": len(specs), "cells": cells, "precision": args.precision,
+                                 "tuning_rows_wgpercu_chunk": list(tuning), "autotune_steps": taken,
+                                 "graph_block": 50, "setup_s": round(t_setup, 1)},
+                      "one_at_a_time": {"value": round(solo, 3), "sample_domains": len(sample),
+                                        "steps_each": args.solo_steps, "kernel_mode": "h_e_kernels"},
+                      "speedup_vs_one_at_a_time": round(rate / solo, 2)}), flush=True)
+
+
+def serve_form(args):
+    """A Poisson stream of mixed FP32/FP16 requests (48^3..224^3, 500..4000
+    steps, 40% dielectric, 40% gain, 20% multi-block) under continuous
+    batching with a 200 GB admission budget, against one-at-a-time solves."""
+    pkg = importlib.import_module("3d_fdtd_version008_amd")
+    serve = importlib.import_module("3d_fdtd_version008_amd.serve")
+    reqs = serve.request_stream(args.requests, rate=args.rate, seed=1)
+    srv = serve.Server(budget_bytes=200e9, max_active=256, tick=50, graph=True, setup_workers=8)
+    res = srv.run(reqs)
+    summ = serve.summarize(res)
+    sample = reqs[:: max(1, len(reqs) // args.solo_sample)][: args.solo_sample]
+    work, secs = 0, 0.0
+    for r in sample:
+        w, s = _solo_rate(pkg, [r.spec], "fp32", r.steps, 1)
+        work += w
+        secs += s
+    solo = work / secs / 1e9
+    print(json.dumps({"metric": "serve form: Gcell-updates/s, p50/p99 latency of a mixed request stream",
+                      "form": "serve", "value": round(summ["gcells_per_s"], 3), "unit": "Gcell-updates/s",
+                      "latency_p50_s": round(summ["latency_p50_s"], 3), "latency_p99_s": round(summ["latency_p99_s"], 3),
+                      "config": {"requests": args.requests, "arrival_rate_per_s": args.rate,
+                                 "fp16_share": 0.5, "budget_gb": 200, "tick_steps": 50,
+                                 "makespan_s": round(summ["makespan_s"], 2), "cell_updates": summ["cell_updates"]},
+                      "one_at_a_time": {"value": round(solo, 3), "sample_requests": len(sample),
+                                        "precision": "fp32", "kernel_mode": "fused_rows"},
+                      "speedup_vs_one_at_a_time": round(summ["gcells_per_s"] / solo, 2)}), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--form", default="plain", choices=["plain", "precision", "batch", "serve"])
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--precision", default="fp32", choices=["fp64", "fp32", "fp16"])
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mode", type=int, default=0, help="0: separate H/E kernels, 1: fused step kernel")
+    ap.add_argument("--ichunk", type=int, default=0, help="fused kernel x-planes per work-group; 0 = library default")
+    ap.add_argument("--rate-steps", type=int, default=200, help="precision form: timed steps per format")
+    ap.add_argument("--domains", type=int, default=256, help="batch form: domains")
+    ap.add_argument("--requests", type=int, default=64, help="serve form: requests")
+    ap.add_argument("--rate", type=float, default=20.0, help="serve form: arrivals per second")
+    ap.add_argument("--solo-sample", type=int, default=16, help="batch/serve forms: one-at-a-time sample size")
+    ap.add_argument("--solo-steps", type=int, default=200, help="batch form: steps per one-at-a-time solve")
+    ap.add_argument("--copy-calib", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (a known-size kernel for "
+                         "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: --gpus {args.gpus} is not supported; this build measures one GPU (N=1).",
+              file=sys.stderr)
+        sys.exit(2)
+    if args.steps is None:
+        args.steps = {"plain": 1000, "precision": 4000, "batch": 2000, "serve": 0}[args.form]
+    {"plain": plain, "precision": precision_form, "batch": batch_form, "serve": serve_form}[args.form](args)
 
 
 if __name__ == "__main__":

@@ -147,8 +147,8 @@ int fdtd_add_dft(fdtd_handle h, int component, const int box[6], int n_freqs, co
  * reference (FP64 only). */
 int fdtd_set_fp16_scales(fdtd_handle h, const double* scales18);
 int fdtd_get_fp16_scales(fdtd_handle h, double* scales18);
-/* Kernel schedule: mode 0 = separate H and E kernels (in place); mode 1 =
- * one fused step kernel (default) in which each wavefront owns one j-row and
+/* Kernel schedule: mode 0 = separate H and E kernels (in place, default);
+ * mode 1 = one fused step kernel in which each wavefront owns one j-row and
  * marches along x over planes_per_chunk planes (1..128; 0 keeps the current
  * value), double-buffering E, H and psi_H.  Grids the fused kernel does not
  * take (per-cell H coefficients, more than 16 sources or TLS regions, NzT >
