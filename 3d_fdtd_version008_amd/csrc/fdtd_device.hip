@@ -290,7 +290,7 @@ int setup_coeffs(fdtd_sim* s, const fdtd_grid_desc* g, const fdtd_material_desc*
 
 int setup_gain(fdtd_sim* s, const fdtd_tls_desc* t) {
     if (!t || t->n_regions <= 0 || !t->cell_region_map) return 0;
-    if (t->n_regions > 127) return fail(FDTD_EINVAL, "at most 127 TLS regions");
+    if (t->n_regions > fdtdk::STAGE_MAX_TLS) return fail(FDTD_EINVAL, "at most 16 TLS regions");
     if (!t->Ndip || !t->Ng0 || !t->Nu || !t->Ng || !t->Pz || !t->Pz_prev)
         return fail(FDTD_EINVAL, "TLS state array missing");
     int lo[3] = {s->nx, s->ny, s->nz}, hi[3] = {-1, -1, -1};
@@ -1074,6 +1074,11 @@ int batch_tables(fdtd_batch* b, int nsteps, const long* n0, const int* msteps) {
     const int nd = (int)b->doms.size();
     b->h_args.resize(sizeof(StepArgs<T, R>) * nd);
     StepArgs<T, R>* args = reinterpret_cast<StepArgs<T, R>*>(b->h_args.data());
+    if (int rc = b->ensure(&b->d_args, &b->args_cap, b->h_args.size())) return rc;
+    if (!b->d_ctl) {
+        size_t cap = 0;
+        if (int rc = b->ensure((void**)&b->d_ctl, &cap, sizeof(fdtdk::BatchCtl))) return rc;
+    }
     // source values: domain d at srcv_off[d], [nsteps][nsrc]
     b->srcv_off.assign(nd, 0);
     size_t nsv = 0;
@@ -1091,6 +1096,8 @@ int batch_tables(fdtd_batch* b, int nsteps, const long* n0, const int* msteps) {
         fdtd_sim* s = b->doms[d];
         args[d] = make_args<T, R>(s);
         args[d].srcv = b->d_srcv + b->srcv_off[d];
+        // pointers travel as offsets from the table (the kernel adds it back)
+        fdtdk::shift_args(args[d], (const char*)b->d_ctl, -1);
         const long tiles = (long)s->nx * ((s->ny + b->rows - 1) / b->rows) * ((s->nz + 63) / 64);
         if ((long)b->h_cum[d] + tiles > 0x7fffffffL) return fail(FDTD_EINVAL, "batch too large (tile count)");
         b->h_cum[d + 1] = b->h_cum[d] + (int)tiles;
@@ -1121,16 +1128,11 @@ int batch_tables(fdtd_batch* b, int nsteps, const long* n0, const int* msteps) {
     }
     b->slots = min_iv > 0 ? nsteps / min_iv + 2 : 0;
     int rc;
-    if ((rc = b->ensure(&b->d_args, &b->args_cap, b->h_args.size()))) return rc;
     if ((rc = b->ensure((void**)&b->d_cum, &b->cum_cap, b->h_cum.size() * sizeof(int)))) return rc;
     if ((rc = b->ensure((void**)&b->d_dom, &b->dom_cap, b->h_dom.size() * sizeof(fdtdk::BatchDom)))) return rc;
     if ((rc = b->ensure((void**)&b->d_probes, &b->probes_cap, b->h_probes.size() * sizeof(fdtdk::BatchProbe)))) return rc;
     if ((rc = b->ensure((void**)&b->d_dfts, &b->dfts_cap, b->h_dfts.size() * sizeof(fdtdk::BatchDft)))) return rc;
     if ((rc = b->ensure((void**)&b->d_mon, &b->mon_cap, (size_t)nd * std::max(b->slots, 1) * sizeof(MonOut)))) return rc;
-    if (!b->d_ctl) {
-        size_t cap = 0;
-        if ((rc = b->ensure((void**)&b->d_ctl, &cap, sizeof(fdtdk::BatchCtl)))) return rc;
-    }
     return 0;
 }
 
@@ -1143,18 +1145,20 @@ void batch_launch_step(fdtd_batch* b, int s, const long* n0, const int* msteps) 
     const dim3 blk(64, (unsigned)b->rows, 1);
     const int total = b->h_cum[nd];
     const int nwg = std::max(1, std::min(b->n_cu * b->wg_per_cu * (4 / b->rows), (total + b->chunk - 1) / b->chunk));
-    hipLaunchKernelGGL((fdtdk::k_batch_step<T, R, false>), dim3((unsigned)nwg), blk, 0, b->stream, args, b->d_cum,
-                       b->d_dom, nd, b->chunk, b->d_ctl);
-    hipLaunchKernelGGL((fdtdk::k_batch_step<T, R, true>), dim3((unsigned)nwg), blk, 0, b->stream, args, b->d_cum,
-                       b->d_dom, nd, b->chunk, b->d_ctl);
+    const char* anchor = (const char*)b->d_ctl;
+    hipLaunchKernelGGL((fdtdk::k_batch_step<T, R, false>), dim3((unsigned)nwg), blk, 0, b->stream, args, anchor,
+                       b->d_cum, b->d_dom, nd, b->chunk, b->d_ctl);
+    hipLaunchKernelGGL((fdtdk::k_batch_step<T, R, true>), dim3((unsigned)nwg), blk, 0, b->stream, args, anchor,
+                       b->d_cum, b->d_dom, nd, b->chunk, b->d_ctl);
     if (!b->h_probes.empty())
         hipLaunchKernelGGL((fdtdk::k_batch_probes<T, R>), dim3((unsigned)((b->h_probes.size() + 63) / 64)), dim3(64), 0,
-                           b->stream, args, b->d_dom, b->d_probes, (int)b->h_probes.size(), b->d_ctl);
+                           b->stream, args, anchor, b->d_dom, b->d_probes, (int)b->h_probes.size(), b->d_ctl);
     if (!b->h_dfts.empty()) {
         long mx = 1;
         for (const auto& m : b->h_dfts) mx = std::max(mx, m.d.ncell);
         hipLaunchKernelGGL((fdtdk::k_batch_dft<T, R>), dim3((unsigned)std::min<long>((mx + 255) / 256, 256),
-                           (unsigned)b->h_dfts.size()), dim3(256), 0, b->stream, args, b->d_dom, b->d_dfts, b->d_ctl);
+                           (unsigned)b->h_dfts.size()), dim3(256), 0, b->stream, args, anchor, b->d_dom, b->d_dfts,
+                           b->d_ctl);
     }
     bool mon = false;
     for (int d = 0; d < nd && !mon; ++d) {
@@ -1163,7 +1167,7 @@ void batch_launch_step(fdtd_batch* b, int s, const long* n0, const int* msteps) 
     }
     if (mon && b->slots > 0)
         hipLaunchKernelGGL((fdtdk::k_batch_monitor<T, R>), dim3(64, (unsigned)nd), dim3(256), 0, b->stream, args,
-                           b->d_dom, b->d_mon, b->slots, b->d_ctl);
+                           anchor, b->d_dom, b->d_mon, b->slots, b->d_ctl);
     if (sizeof(T) == 2)
         for (int d = 0; d < nd; ++d)
             if (s < msteps[d] && (n0[d] + s) % b->doms[d]->fp16_interval == 0) fp16_maintain(b->doms[d], b->stream);

@@ -206,6 +206,64 @@ __device__ __forceinline__ double* gain_arr(const StepArgs<T, R>& a, const Tabs<
     return q == 0 ? a.gNu : q == 1 ? a.gNg : q == 2 ? a.gPz : a.gPp;
 }
 
+constexpr int STAGE_MAX_TLS = 16;  // TLS regions staged in LDS
+constexpr int STAGE_MAX_SRC = 16;  // sources staged in LDS by the row kernel
+
+// Small read-only tables staged in LDS at kernel start: the FP16 scales and
+// the TLS coefficients.  As far as the compiler knows, a field store could
+// overwrite them in global memory, so it would reload them (and wait for the
+// load) at every use after a store; LDS copies cannot alias global stores.
+struct Staged {
+    float sc[54];
+    TLSCoef tl[STAGE_MAX_TLS];
+};
+template <typename T, typename R>
+__device__ __forceinline__ void stage_fill(const StepArgs<T, R>& a, Staged& st) {
+    const int t = threadIdx.x + threadIdx.y * blockDim.x;
+    if (sizeof(T) == 2 && t < 54) st.sc[t] = a.sc[t];
+    if (a.gain && t < a.ntls && t < STAGE_MAX_TLS) st.tl[t] = a.tls[t];
+}
+// Points the arguments and tables at the staged copies (call after a
+// barrier).  The library admits at most STAGE_MAX_TLS TLS regions.
+template <typename T, typename R>
+__device__ __forceinline__ void stage_use(StepArgs<T, R>& a, Tabs<R>& tb, Staged& st) {
+    if (sizeof(T) == 2) a.sc = st.sc;
+    tb.tls = st.tl;
+}
+
+// Arguments kept in a device-side table (batches) store every pointer as its
+// byte offset from an anchor address; the kernel adds the anchor (a kernel
+// argument, so known to address global memory) back, and the compiler then
+// emits global (not flat) accesses through them.  The anchor is not the
+// table itself, so stores through these pointers cannot alias the table and
+// the table is read through the scalar cache.  The host applies the same map
+// with the opposite sign.
+template <typename P>
+__host__ __device__ __forceinline__ void shift_ptr(P*& p, const char* base, long sign) {
+    if (sign > 0) p = (P*)((char*)base + (long)p);   // offset -> pointer (a GEP from base)
+    else p = (P*)((long)p - (long)base);             // pointer -> offset
+}
+template <typename T, typename R>
+__host__ __device__ __forceinline__ void shift_args(StepArgs<T, R>& a, const char* base, long sign) {
+    shift_ptr(a.Ex, base, sign); shift_ptr(a.Ey, base, sign); shift_ptr(a.Ez, base, sign);
+    shift_ptr(a.Hx, base, sign); shift_ptr(a.Hy, base, sign); shift_ptr(a.Hz, base, sign);
+    shift_ptr(a.Exw, base, sign); shift_ptr(a.Eyw, base, sign); shift_ptr(a.Ezw, base, sign);
+    shift_ptr(a.Hxw, base, sign); shift_ptr(a.Hyw, base, sign); shift_ptr(a.Hzw, base, sign);
+    shift_ptr(a.cpx, base, sign); shift_ptr(a.cpy, base, sign); shift_ptr(a.cpz, base, sign);
+    shift_ptr(a.sc, base, sign);
+    shift_ptr(a.matid, base, sign);
+    shift_ptr(a.lut, base, sign);
+    shift_ptr(a.src, base, sign);
+    shift_ptr(a.srcv, base, sign);
+    shift_ptr(a.greg, base, sign);
+    shift_ptr(a.gPz, base, sign); shift_ptr(a.gPp, base, sign); shift_ptr(a.gNu, base, sign); shift_ptr(a.gNg, base, sign);
+    shift_ptr(a.tls, base, sign);
+    for (int q = 0; q < 12; ++q) {
+        shift_ptr(a.psi[q], base, sign);
+        shift_ptr(a.psiw[q], base, sign);
+    }
+}
+
 // One CPML correction (src/boundary.hpp:182-223, 257-298) on an already
 // loaded psi:  psi = b*psi + c*d;  corr = d/kappa + psi;  returns (corr - d)
 // and the new psi in `pnew`.  FP64 divides as the reference does; the FP32
@@ -360,12 +418,19 @@ __device__ __forceinline__ H3<R> h_new(const StepArgs<T, R>& a, const Tabs<R>& t
 // blockDim = (64, BY); each thread owns one cell.
 template <typename T, typename R>
 __global__ void __launch_bounds__(256) k_update_H(StepArgs<T, R> a) {
+    Tabs<R> tb = global_tabs(a);
+    if constexpr (sizeof(T) == 2) {  // the H update reads only the FP16 scales
+        __shared__ Staged stg;
+        stage_fill(a, stg);
+        __syncthreads();
+        stage_use(a, tb, stg);
+    }
     const int k = blockIdx.x * 64 + threadIdx.x;
     // blockDim.x is one wavefront, so the row is uniform across it
     const int j = blockIdx.y * blockDim.y + __builtin_amdgcn_readfirstlane(threadIdx.y);
     const int i = blockIdx.z;
     if (k >= a.nz - 1 || j >= a.ny - 1 || i >= a.nx - 1) return;
-    (void)h_new<T, R>(a, global_tabs(a), i, j, k, true);
+    (void)h_new<T, R>(a, tb, i, j, k, true);
 }
 
 // Two-level polarization step (update_polarization_multi_region,
@@ -602,6 +667,13 @@ __device__ __forceinline__ void e_compute(const StepArgs<T, R>& a, const Tabs<R>
 // E update kernel of the two-kernel step: H^{n+1/2} neighbours from memory.
 template <typename T, typename R>
 __global__ void __launch_bounds__(256) k_update_E(StepArgs<T, R> a) {
+    __shared__ Staged stg;
+    if (sizeof(T) == 2 || a.gain) {  // same for the whole grid
+        stage_fill(a, stg);
+        __syncthreads();
+    }
+    Tabs<R> tb = global_tabs(a);
+    stage_use(a, tb, stg);  // without gain the TLS table is never read
     const int k = blockIdx.x * 64 + threadIdx.x;
     // blockDim.x is one wavefront, so the row is uniform across it
     const int j = blockIdx.y * blockDim.y + __builtin_amdgcn_readfirstlane(threadIdx.y);
@@ -617,11 +689,8 @@ __global__ void __launch_bounds__(256) k_update_E(StepArgs<T, R> a) {
         hz = ld<T, R>(a.Hz, id, a.sc[5]); hz_i = ld<T, R>(a.Hz, id - sI, a.sc[5]); hz_j = ld<T, R>(a.Hz, id - sJ, a.sc[5]);
     }
     const EIn<R> v = e_load<T, R>(a, i, j, k, e_mat(a, i, j, k));
-    e_compute<T, R>(a, global_tabs(a), i, j, k, v, hx, hy, hz, hx_j, hz_j, hx_k, hy_k, hy_i, hz_i);
+    e_compute<T, R>(a, tb, i, j, k, v, hx, hy, hz, hx_j, hz_j, hx_k, hy_k, hy_i, hz_i);
 }
-
-constexpr int STAGE_MAX_TLS = 16;  // TLS regions staged in LDS by the row kernel
-constexpr int STAGE_MAX_SRC = 16;  // sources staged in LDS by the row kernel
 
 // Row-marching step kernel: one launch advances H and E of the whole grid,
 // with no barrier and no data exchange between wavefronts.  A wavefront owns
@@ -1181,18 +1250,20 @@ __device__ __forceinline__ int batch_domain(const int* __restrict__ cum, int ndo
 }
 
 template <typename T, typename R, bool EPASS>
-__global__ void __launch_bounds__(256) k_batch_step(const StepArgs<T, R>* __restrict__ doms, const int* __restrict__ cum,
-                                                    const BatchDom* __restrict__ bd, int ndom, int chunk,
-                                                    BatchCtl* ctl) {
+__global__ void __launch_bounds__(256) k_batch_step(const StepArgs<T, R>* __restrict__ doms, const char* anchor,
+                                                    const int* __restrict__ cum, const BatchDom* __restrict__ bd,
+                                                    int ndom, int chunk, BatchCtl* ctl) {
     __shared__ int first_s;
+    __shared__ Staged stg;
     const int tj = blockDim.y;
     const int ty = __builtin_amdgcn_readfirstlane(threadIdx.y);
     const int total = cum[ndom];
     const int lstep = ctl->lstep;
+    int staged = -1;  // domain whose tables are in `stg`
     for (;;) {
         if (threadIdx.x == 0 && threadIdx.y == 0) first_s = atomicAdd(EPASS ? &ctl->ctr_e : &ctl->ctr_h, chunk);
         __syncthreads();
-        const int first = first_s;
+        const int first = __builtin_amdgcn_readfirstlane(first_s);  // one value for the work-group
         __syncthreads();
         if (first >= total) return;
         const int last = min(first + chunk, total);
@@ -1201,14 +1272,23 @@ __global__ void __launch_bounds__(256) k_batch_step(const StepArgs<T, R>* __rest
             while (t >= cum[d + 1]) ++d;
             if (lstep >= bd[d].steps) continue;
             StepArgs<T, R> a = doms[d];
+            shift_args(a, anchor, 1);
+            a.srcv += (long)lstep * a.nsrc;  // this step's source values
+            if (d != staged) {  // uniform across the work-group
+                __syncthreads();
+                stage_fill(a, stg);
+                __syncthreads();
+                staged = d;
+            }
+            Tabs<R> tb = global_tabs(a);
+            stage_use(a, tb, stg);
             const int lt = t - cum[d];
             const int nkt = (a.nz + 63) >> 6, njt = (a.ny + tj - 1) / tj;
             const int kt = lt % nkt, r = lt / nkt, jt = r % njt, i = r / njt;
             const int k = kt * 64 + threadIdx.x, j = jt * tj + ty;
             if (!EPASS) {
-                if (k < a.nz - 1 && j < a.ny - 1 && i < a.nx - 1) (void)h_new<T, R>(a, global_tabs(a), i, j, k, true);
+                if (k < a.nz - 1 && j < a.ny - 1 && i < a.nx - 1) (void)h_new<T, R>(a, tb, i, j, k, true);
             } else if (k < a.nz && j < a.ny) {
-                a.srcv += (long)lstep * a.nsrc;
                 const long id = ((long)i * a.ny + j) * a.nz + k;
                 const bool interior = (i > 0) & (j > 0) & (k > 0);
                 R hx = 0, hy = 0, hz = 0, hx_j = 0, hz_j = 0, hx_k = 0, hy_k = 0, hy_i = 0, hz_i = 0;
@@ -1222,15 +1302,16 @@ __global__ void __launch_bounds__(256) k_batch_step(const StepArgs<T, R>* __rest
                     hz_j = ld<T, R>(a.Hz, id - sJ, a.sc[5]);
                 }
                 const EIn<R> v = e_load<T, R>(a, i, j, k, e_mat(a, i, j, k));
-                e_compute<T, R>(a, global_tabs(a), i, j, k, v, hx, hy, hz, hx_j, hz_j, hx_k, hy_k, hy_i, hz_i);
+                e_compute<T, R>(a, tb, i, j, k, v, hx, hy, hz, hx_j, hz_j, hx_k, hy_k, hy_i, hz_i);
             }
         }
     }
 }
 
 template <typename T, typename R>
-__global__ void k_batch_probes(const StepArgs<T, R>* __restrict__ doms, const BatchDom* __restrict__ bd,
-                               const BatchProbe* __restrict__ pr, int npr, const BatchCtl* ctl) {
+__global__ void k_batch_probes(const StepArgs<T, R>* __restrict__ doms, const char* anchor,
+                               const BatchDom* __restrict__ bd, const BatchProbe* __restrict__ pr, int npr,
+                               const BatchCtl* ctl) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= npr) return;
     const BatchProbe b = pr[p];
@@ -1238,12 +1319,15 @@ __global__ void k_batch_probes(const StepArgs<T, R>* __restrict__ doms, const Ba
     const long n = bd[b.dom].n0 + ctl->lstep;
     if (n % b.save_every != 0) return;
     const long slot = n / b.save_every;
-    if (slot < b.cap) b.out[slot] = field_value<T, R>(doms[b.dom], b.comp, b.id);
+    if (slot >= b.cap) return;
+    StepArgs<T, R> a = doms[b.dom];
+    shift_args(a, anchor, 1);
+    b.out[slot] = field_value<T, R>(a, b.comp, b.id);
 }
 
 template <typename T, typename R>
-__global__ void k_batch_dft(const StepArgs<T, R>* __restrict__ doms, const BatchDom* __restrict__ bd,
-                            const BatchDft* __restrict__ dd, const BatchCtl* ctl) {
+__global__ void k_batch_dft(const StepArgs<T, R>* __restrict__ doms, const char* anchor,
+                            const BatchDom* __restrict__ bd, const BatchDft* __restrict__ dd, const BatchCtl* ctl) {
     const BatchDft& m = dd[blockIdx.y];
     if (ctl->lstep >= bd[m.dom].steps) return;
     DftArgs d = m.d;
@@ -1253,13 +1337,14 @@ __global__ void k_batch_dft(const StepArgs<T, R>* __restrict__ doms, const Batch
         d.wr[f] = cos(m.omega[f] * t) * m.dt;
         d.wi[f] = -sin(m.omega[f] * t) * m.dt;
     }
-    const StepArgs<T, R> a = doms[m.dom];
+    StepArgs<T, R> a = doms[m.dom];
+    shift_args(a, anchor, 1);
     for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < d.ncell; c += (long)gridDim.x * blockDim.x)
         dft_cell<T, R>(a, d, c);
 }
 
 template <typename T, typename R>
-__global__ void __launch_bounds__(256) k_batch_monitor(const StepArgs<T, R>* __restrict__ doms,
+__global__ void __launch_bounds__(256) k_batch_monitor(const StepArgs<T, R>* __restrict__ doms, const char* anchor,
                                                        const BatchDom* __restrict__ bd, MonOut* out, int slots_per_dom,
                                                        const BatchCtl* ctl) {
     const int d = blockIdx.y;
@@ -1269,7 +1354,9 @@ __global__ void __launch_bounds__(256) k_batch_monitor(const StepArgs<T, R>* __r
     if (n % m.interval != 0) return;
     const long slot = n / m.interval - m.mon_first;
     if (slot < 0 || slot >= slots_per_dom) return;
-    monitor_body<T, R>(doms[d], m.N, m.ngain, out + (long)d * slots_per_dom + slot, blockIdx.x, gridDim.x);
+    StepArgs<T, R> a = doms[d];
+    shift_args(a, anchor, 1);
+    monitor_body<T, R>(a, m.N, m.ngain, out + (long)d * slots_per_dom + slot, blockIdx.x, gridDim.x);
 }
 
 // ---- FP16 storage: adaptive per-component power-of-two scales ----
