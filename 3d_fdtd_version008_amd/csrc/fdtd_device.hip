@@ -410,7 +410,7 @@ int launch_step(fdtd_sim* s, long n, int sidx, hipEvent_t* ev) {
         a.srcv = s->d_srcv + (long)sidx * s->nsrc;
     } else {
         const dim3 blk(64, 4, 1);
-        const dim3 grd((unsigned)((s->nz + 63) / 64), (unsigned)((s->ny + 3) / 4), (unsigned)s->nx);
+        const dim3 grd((unsigned)(((s->nz + 63) / 64) * ((s->ny + 3) / 4) * s->nx));
         if (ev) (void)hipEventRecord(ev[0], s->stream);
         hipLaunchKernelGGL((fdtdk::k_update_H<T, R>), grd, blk, 0, s->stream, a);
         if (ev) (void)hipEventRecord(ev[1], s->stream);

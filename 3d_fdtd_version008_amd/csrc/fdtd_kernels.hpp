@@ -209,28 +209,6 @@ __device__ __forceinline__ double* gain_arr(const StepArgs<T, R>& a, const Tabs<
 constexpr int STAGE_MAX_TLS = 16;  // TLS regions staged in LDS
 constexpr int STAGE_MAX_SRC = 16;  // sources staged in LDS by the row kernel
 
-// Small read-only tables staged in LDS at kernel start: the FP16 scales and
-// the TLS coefficients.  As far as the compiler knows, a field store could
-// overwrite them in global memory, so it would reload them (and wait for the
-// load) at every use after a store; LDS copies cannot alias global stores.
-struct Staged {
-    float sc[54];
-    TLSCoef tl[STAGE_MAX_TLS];
-};
-template <typename T, typename R>
-__device__ __forceinline__ void stage_fill(const StepArgs<T, R>& a, Staged& st) {
-    const int t = threadIdx.x + threadIdx.y * blockDim.x;
-    if (sizeof(T) == 2 && t < 54) st.sc[t] = a.sc[t];
-    if (a.gain && t < a.ntls && t < STAGE_MAX_TLS) st.tl[t] = a.tls[t];
-}
-// Points the arguments and tables at the staged copies (call after a
-// barrier).  The library admits at most STAGE_MAX_TLS TLS regions.
-template <typename T, typename R>
-__device__ __forceinline__ void stage_use(StepArgs<T, R>& a, Tabs<R>& tb, Staged& st) {
-    if (sizeof(T) == 2) a.sc = st.sc;
-    tb.tls = st.tl;
-}
-
 // Arguments kept in a device-side table (batches) store every pointer as its
 // byte offset from an anchor address; the kernel adds the anchor (a kernel
 // argument, so known to address global memory) back, and the compiler then
@@ -414,21 +392,48 @@ __device__ __forceinline__ H3<R> h_new(const StepArgs<T, R>& a, const Tabs<R>& t
     return h_compute<T, R>(a, tb, i, j, k, h_load<T, R>(a, i, j, k), own);
 }
 
+// The FP16 scales copied into registers at kernel start, before any store:
+// afterwards a.sc points at the copy (fully promoted to registers, since every
+// index is a constant), so no scale is read from memory after a store.
+#define FDTD_SCALES_IN_REGISTERS(a)                       \
+    float scl_[54];                                       \
+    if constexpr (sizeof(T) == 2) {                       \
+        for (int q_ = 0; q_ < 54; ++q_) scl_[q_] = a.sc[q_]; \
+        a.sc = scl_;                                      \
+    }
+
+// Tile of a two-kernel launch.  The grid is 1-D over (k-block, j-block,
+// plane) tiles.  Work-group ids are dealt round-robin to the 8 XCDs, each
+// with its own L2; the remap hands every XCD one contiguous range of tiles
+// (whole runs of planes), so the neighbouring rows and the previous plane a
+// tile reads were loaded recently by the same XCD.
+struct Tile {
+    int i, j, k;
+};
+__device__ __forceinline__ Tile tile_of(int nz, int ny) {
+    const unsigned nb = gridDim.x, b = blockIdx.x;
+    const unsigned per = (nb + 7) / 8;
+    unsigned bb = (b % 8) * per + b / 8;
+    if (nb % 8 != 0) bb = b;  // the remap is a permutation only for multiples of 8
+    const int nkt = (nz + 63) >> 6, njt = (ny + (int)blockDim.y - 1) / (int)blockDim.y;
+    const int kt = (int)(bb % nkt);
+    const int r = (int)(bb / nkt);
+    Tile t;
+    t.k = kt * 64 + (int)threadIdx.x;
+    // blockDim.x is one wavefront, so the row is uniform across it
+    t.j = (r % njt) * (int)blockDim.y + __builtin_amdgcn_readfirstlane(threadIdx.y);
+    t.i = r / njt;
+    return t;
+}
+
 // H update kernel of the two-kernel step.  Threads cover the grid with
-// blockDim = (64, BY); each thread owns one cell.
+// blockDim = (64, BY); each thread owns one cell (tile_of).
 template <typename T, typename R>
 __global__ void __launch_bounds__(256) k_update_H(StepArgs<T, R> a) {
-    Tabs<R> tb = global_tabs(a);
-    if constexpr (sizeof(T) == 2) {  // the H update reads only the FP16 scales
-        __shared__ Staged stg;
-        stage_fill(a, stg);
-        __syncthreads();
-        stage_use(a, tb, stg);
-    }
-    const int k = blockIdx.x * 64 + threadIdx.x;
-    // blockDim.x is one wavefront, so the row is uniform across it
-    const int j = blockIdx.y * blockDim.y + __builtin_amdgcn_readfirstlane(threadIdx.y);
-    const int i = blockIdx.z;
+    FDTD_SCALES_IN_REGISTERS(a)
+    const Tabs<R> tb = global_tabs(a);
+    const Tile t = tile_of(a.nz, a.ny);
+    const int i = t.i, j = t.j, k = t.k;
     if (k >= a.nz - 1 || j >= a.ny - 1 || i >= a.nx - 1) return;
     (void)h_new<T, R>(a, tb, i, j, k, true);
 }
@@ -609,8 +614,13 @@ __device__ __forceinline__ void e_compute(const StepArgs<T, R>& a, const Tabs<R>
     const R bEx = v.cE[1], bEy = v.cE[3], bEz = v.cE[5];
     double Nu = v.Nu, Ng = v.Ng, Pz = v.Pz, Pp = v.Pp;
     R rz = cz - jz;
+    // the region's coefficients are read once, before any store (a read after
+    // a store would be repeated from memory: the compiler cannot rule out
+    // that the store changed them)
+    TLSCoef tc{};
     if (reg >= 0) {
-        const double dP = tls_polarize<sizeof(R) == 8>(tls[reg], Nu, Ng, (double)v.ez0, Pz, Pp);
+        tc = tls[reg];
+        const double dP = tls_polarize<sizeof(R) == 8>(tc, Nu, Ng, (double)v.ez0, Pz, Pp);
         rz = rz - (R)dP;
     }
     R ex, ey, ez;
@@ -656,7 +666,7 @@ __device__ __forceinline__ void e_compute(const StepArgs<T, R>& a, const Tabs<R>
     st<T, R>(a.Eyw, id, ey, a.sc[19]);
     st<T, R>(a.Ezw, id, ez, a.sc[20]);
     if (reg >= 0) {
-        const TLSCoef& c = tls[reg];
+        const TLSCoef& c = tc;
         // The reference stores Ez and reads it back for the population pass.
         const double ez_new = stored_value<T, R>(ez, a.sc[2], a.sc[20]);
         tls_populate(c, ez_new, (double)v.ez0, Pz, Pp, Nu, Ng);
@@ -667,17 +677,10 @@ __device__ __forceinline__ void e_compute(const StepArgs<T, R>& a, const Tabs<R>
 // E update kernel of the two-kernel step: H^{n+1/2} neighbours from memory.
 template <typename T, typename R>
 __global__ void __launch_bounds__(256) k_update_E(StepArgs<T, R> a) {
-    __shared__ Staged stg;
-    if (sizeof(T) == 2 || a.gain) {  // same for the whole grid
-        stage_fill(a, stg);
-        __syncthreads();
-    }
-    Tabs<R> tb = global_tabs(a);
-    stage_use(a, tb, stg);  // without gain the TLS table is never read
-    const int k = blockIdx.x * 64 + threadIdx.x;
-    // blockDim.x is one wavefront, so the row is uniform across it
-    const int j = blockIdx.y * blockDim.y + __builtin_amdgcn_readfirstlane(threadIdx.y);
-    const int i = blockIdx.z;
+    FDTD_SCALES_IN_REGISTERS(a)
+    const Tabs<R> tb = global_tabs(a);
+    const Tile t = tile_of(a.nz, a.ny);
+    const int i = t.i, j = t.j, k = t.k;
     if (k >= a.nz || j >= a.ny || i >= a.nx) return;
     const long id = ((long)i * a.ny + j) * a.nz + k;
     const bool interior = (i > 0) & (j > 0) & (k > 0);
@@ -1254,12 +1257,10 @@ __global__ void __launch_bounds__(256) k_batch_step(const StepArgs<T, R>* __rest
                                                     const int* __restrict__ cum, const BatchDom* __restrict__ bd,
                                                     int ndom, int chunk, BatchCtl* ctl) {
     __shared__ int first_s;
-    __shared__ Staged stg;
     const int tj = blockDim.y;
     const int ty = __builtin_amdgcn_readfirstlane(threadIdx.y);
     const int total = cum[ndom];
     const int lstep = ctl->lstep;
-    int staged = -1;  // domain whose tables are in `stg`
     for (;;) {
         if (threadIdx.x == 0 && threadIdx.y == 0) first_s = atomicAdd(EPASS ? &ctl->ctr_e : &ctl->ctr_h, chunk);
         __syncthreads();
@@ -1274,14 +1275,8 @@ __global__ void __launch_bounds__(256) k_batch_step(const StepArgs<T, R>* __rest
             StepArgs<T, R> a = doms[d];
             shift_args(a, anchor, 1);
             a.srcv += (long)lstep * a.nsrc;  // this step's source values
-            if (d != staged) {  // uniform across the work-group
-                __syncthreads();
-                stage_fill(a, stg);
-                __syncthreads();
-                staged = d;
-            }
-            Tabs<R> tb = global_tabs(a);
-            stage_use(a, tb, stg);
+            FDTD_SCALES_IN_REGISTERS(a)
+            const Tabs<R> tb = global_tabs(a);
             const int lt = t - cum[d];
             const int nkt = (a.nz + 63) >> 6, njt = (a.ny + tj - 1) / tj;
             const int kt = lt % nkt, r = lt / nkt, jt = r % njt, i = r / njt;
