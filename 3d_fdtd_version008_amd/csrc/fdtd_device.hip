@@ -1009,7 +1009,7 @@ struct fdtd_batch {
     int prec = 0;
     hipStream_t stream = nullptr;
     std::vector<fdtd_sim*> doms;
-    int rows = 4, wg_per_cu = 4, chunk = 4;
+    int rows = 4, wg_per_cu = 4, chunk = 4, planes = 1;
     int n_cu = 256;
     bool use_graph = false;
     // device tables (grown, never shrunk)
@@ -1098,7 +1098,8 @@ int batch_tables(fdtd_batch* b, int nsteps, const long* n0, const int* msteps) {
         args[d].srcv = b->d_srcv + b->srcv_off[d];
         // pointers travel as offsets from the table (the kernel adds it back)
         fdtdk::shift_args(args[d], (const char*)b->d_ctl, -1);
-        const long tiles = (long)s->nx * ((s->ny + b->rows - 1) / b->rows) * ((s->nz + 63) / 64);
+        const long tiles =
+            (long)((s->nx + b->planes - 1) / b->planes) * ((s->ny + b->rows - 1) / b->rows) * ((s->nz + 63) / 64);
         if ((long)b->h_cum[d] + tiles > 0x7fffffffL) return fail(FDTD_EINVAL, "batch too large (tile count)");
         b->h_cum[d + 1] = b->h_cum[d] + (int)tiles;
         fdtdk::BatchDom& m = b->h_dom[d];
@@ -1147,9 +1148,9 @@ void batch_launch_step(fdtd_batch* b, int s, const long* n0, const int* msteps) 
     const int nwg = std::max(1, std::min(b->n_cu * b->wg_per_cu * (4 / b->rows), (total + b->chunk - 1) / b->chunk));
     const char* anchor = (const char*)b->d_ctl;
     hipLaunchKernelGGL((fdtdk::k_batch_step<T, R, false>), dim3((unsigned)nwg), blk, 0, b->stream, args, anchor,
-                       b->d_cum, b->d_dom, nd, b->chunk, b->d_ctl);
+                       b->d_cum, b->d_dom, nd, b->chunk, b->planes, b->d_ctl);
     hipLaunchKernelGGL((fdtdk::k_batch_step<T, R, true>), dim3((unsigned)nwg), blk, 0, b->stream, args, anchor,
-                       b->d_cum, b->d_dom, nd, b->chunk, b->d_ctl);
+                       b->d_cum, b->d_dom, nd, b->chunk, b->planes, b->d_ctl);
     if (!b->h_probes.empty())
         hipLaunchKernelGGL((fdtdk::k_batch_probes<T, R>), dim3((unsigned)((b->h_probes.size() + 63) / 64)), dim3(64), 0,
                            b->stream, args, anchor, b->d_dom, b->d_probes, (int)b->h_probes.size(), b->d_ctl);
@@ -1333,21 +1334,26 @@ int fdtd_batch_remove(fdtd_batch_handle b, fdtd_handle h) {
 
 int fdtd_batch_count(fdtd_batch_handle b) { return b ? (int)b->doms.size() : fail(FDTD_EINVAL, "null batch"); }
 
-int fdtd_batch_set_tuning(fdtd_batch_handle b, int rows, int wg_per_cu, int chunk) {
-    if (!b || (rows != 1 && rows != 2 && rows != 4) || wg_per_cu < 1 || wg_per_cu > 16 || chunk < 1 || chunk > 64)
+int fdtd_batch_set_tuning(fdtd_batch_handle b, const int* tuning) {
+    if (!b || !tuning) return fail(FDTD_EINVAL, "null argument");
+    const int rows = tuning[0], wg_per_cu = tuning[1], chunk = tuning[2], planes = tuning[3];
+    if ((rows != 1 && rows != 2 && rows != 4) || wg_per_cu < 1 || wg_per_cu > 16 || chunk < 1 || chunk > 64 ||
+        planes < 1 || planes > 16)
         return fail(FDTD_EINVAL, "tuning out of range");
     b->rows = rows;
     b->wg_per_cu = wg_per_cu;
     b->chunk = chunk;
+    b->planes = planes;
     ++b->version;
     return FDTD_OK;
 }
 
-int fdtd_batch_get_tuning(fdtd_batch_handle b, int* rows, int* wg_per_cu, int* chunk) {
-    if (!b || !rows || !wg_per_cu || !chunk) return fail(FDTD_EINVAL, "null argument");
-    *rows = b->rows;
-    *wg_per_cu = b->wg_per_cu;
-    *chunk = b->chunk;
+int fdtd_batch_get_tuning(fdtd_batch_handle b, int* tuning) {
+    if (!b || !tuning) return fail(FDTD_EINVAL, "null argument");
+    tuning[0] = b->rows;
+    tuning[1] = b->wg_per_cu;
+    tuning[2] = b->chunk;
+    tuning[3] = b->planes;
     return FDTD_OK;
 }
 

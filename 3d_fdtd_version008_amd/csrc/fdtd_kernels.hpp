@@ -1202,8 +1202,8 @@ __global__ void __launch_bounds__(256) k_monitor(StepArgs<T, R> a, long N, long 
 // ---- Multi-domain (batch) step ----
 // Many independent domains of one precision advance together: per step one
 // persistent H launch and one persistent E launch cover every domain, their
-// work-groups taking `chunk` tiles (64 k-values by blockDim.y rows of one
-// x-plane) at a time from a device counter, and one launch each records the
+// work-groups taking `chunk` tiles (64 k-values by blockDim.y rows by
+// `planes` consecutive x-planes) at a time from a device counter, and one launch each records the
 // probes, accumulates the DFT monitors and reduces the stability monitor of
 // every domain.  Step-dependent inputs (source values, probe slots, DFT
 // phases, monitor slots) are indexed by a device-side step counter, so a block
@@ -1255,7 +1255,7 @@ __device__ __forceinline__ int batch_domain(const int* __restrict__ cum, int ndo
 template <typename T, typename R, bool EPASS>
 __global__ void __launch_bounds__(256) k_batch_step(const StepArgs<T, R>* __restrict__ doms, const char* anchor,
                                                     const int* __restrict__ cum, const BatchDom* __restrict__ bd,
-                                                    int ndom, int chunk, BatchCtl* ctl) {
+                                                    int ndom, int chunk, int planes, BatchCtl* ctl) {
     __shared__ int first_s;
     const int tj = blockDim.y;
     const int ty = __builtin_amdgcn_readfirstlane(threadIdx.y);
@@ -1277,27 +1277,31 @@ __global__ void __launch_bounds__(256) k_batch_step(const StepArgs<T, R>* __rest
             a.srcv += (long)lstep * a.nsrc;  // this step's source values
             FDTD_SCALES_IN_REGISTERS(a)
             const Tabs<R> tb = global_tabs(a);
+            // tile = 64 k-values x blockDim.y rows x `planes` consecutive planes
             const int lt = t - cum[d];
             const int nkt = (a.nz + 63) >> 6, njt = (a.ny + tj - 1) / tj;
-            const int kt = lt % nkt, r = lt / nkt, jt = r % njt, i = r / njt;
+            const int kt = lt % nkt, r = lt / nkt, jt = r % njt, i0 = (r / njt) * planes;
             const int k = kt * 64 + threadIdx.x, j = jt * tj + ty;
-            if (!EPASS) {
-                if (k < a.nz - 1 && j < a.ny - 1 && i < a.nx - 1) (void)h_new<T, R>(a, tb, i, j, k, true);
-            } else if (k < a.nz && j < a.ny) {
-                const long id = ((long)i * a.ny + j) * a.nz + k;
-                const bool interior = (i > 0) & (j > 0) & (k > 0);
-                R hx = 0, hy = 0, hz = 0, hx_j = 0, hz_j = 0, hx_k = 0, hy_k = 0, hy_i = 0, hz_i = 0;
-                if (interior) {
-                    const long sI = a.sI, sJ = a.sJ;
-                    hx = ld<T, R>(a.Hx, id, a.sc[3]); hx_j = ld<T, R>(a.Hx, id - sJ, a.sc[3]);
-                    hx_k = ld<T, R>(a.Hx, id - 1, a.sc[3]);
-                    hy = ld<T, R>(a.Hy, id, a.sc[4]); hy_i = ld<T, R>(a.Hy, id - sI, a.sc[4]);
-                    hy_k = ld<T, R>(a.Hy, id - 1, a.sc[4]);
-                    hz = ld<T, R>(a.Hz, id, a.sc[5]); hz_i = ld<T, R>(a.Hz, id - sI, a.sc[5]);
-                    hz_j = ld<T, R>(a.Hz, id - sJ, a.sc[5]);
+            const int i1 = min(i0 + planes, a.nx);
+            for (int i = i0; i < i1; ++i) {
+                if (!EPASS) {
+                    if (k < a.nz - 1 && j < a.ny - 1 && i < a.nx - 1) (void)h_new<T, R>(a, tb, i, j, k, true);
+                } else if (k < a.nz && j < a.ny) {
+                    const long id = ((long)i * a.ny + j) * a.nz + k;
+                    const bool interior = (i > 0) & (j > 0) & (k > 0);
+                    R hx = 0, hy = 0, hz = 0, hx_j = 0, hz_j = 0, hx_k = 0, hy_k = 0, hy_i = 0, hz_i = 0;
+                    if (interior) {
+                        const long sI = a.sI, sJ = a.sJ;
+                        hx = ld<T, R>(a.Hx, id, a.sc[3]); hx_j = ld<T, R>(a.Hx, id - sJ, a.sc[3]);
+                        hx_k = ld<T, R>(a.Hx, id - 1, a.sc[3]);
+                        hy = ld<T, R>(a.Hy, id, a.sc[4]); hy_i = ld<T, R>(a.Hy, id - sI, a.sc[4]);
+                        hy_k = ld<T, R>(a.Hy, id - 1, a.sc[4]);
+                        hz = ld<T, R>(a.Hz, id, a.sc[5]); hz_i = ld<T, R>(a.Hz, id - sI, a.sc[5]);
+                        hz_j = ld<T, R>(a.Hz, id - sJ, a.sc[5]);
+                    }
+                    const EIn<R> v = e_load<T, R>(a, i, j, k, e_mat(a, i, j, k));
+                    e_compute<T, R>(a, tb, i, j, k, v, hx, hy, hz, hx_j, hz_j, hx_k, hy_k, hy_i, hz_i);
                 }
-                const EIn<R> v = e_load<T, R>(a, i, j, k, e_mat(a, i, j, k));
-                e_compute<T, R>(a, tb, i, j, k, v, hx, hy, hz, hx_j, hz_j, hx_k, hy_k, hy_i, hz_i);
             }
         }
     }

@@ -812,8 +812,8 @@ struct TuneKey {
         return std::tie(prec, count, log2cells) < std::tie(o.prec, o.count, o.log2cells);
     }
 };
-std::map<TuneKey, std::array<int, 3>>& tune_cache() {
-    static std::map<TuneKey, std::array<int, 3>> m;
+std::map<TuneKey, std::array<int, 4>>& tune_cache() {
+    static std::map<TuneKey, std::array<int, 4>> m;
     return m;
 }
 std::mutex& tune_mutex() {
@@ -890,8 +890,8 @@ int fdtd_scene_batch_step(fdtd_scene_batch* sb, int nsteps, double* ms) {
     return rc;
 }
 
-int fdtd_scene_batch_autotune(fdtd_scene_batch* sb, int trial_steps, int* rows, int* wg_per_cu, int* chunk) {
-    if (!sb || trial_steps < 1 || !rows || !wg_per_cu || !chunk) return FDTD_EINVAL;
+int fdtd_scene_batch_autotune(fdtd_scene_batch* sb, int trial_steps, int* tuning) {
+    if (!sb || trial_steps < 1 || !tuning) return FDTD_EINVAL;
     long cells = 0;
     for (fdtd_scene* sc : sb->scenes) cells += (long)sc->setup->NxT * sc->setup->NyT * sc->setup->NzT;
     int l2 = 0;
@@ -901,9 +901,8 @@ int fdtd_scene_batch_autotune(fdtd_scene_batch* sb, int trial_steps, int* rows, 
         std::lock_guard<std::mutex> g(tune_mutex());
         auto it = tune_cache().find(key);
         if (it != tune_cache().end()) {
-            *rows = it->second[0]; *wg_per_cu = it->second[1]; *chunk = it->second[2];
-            const int rc = fdtd_batch_set_tuning(sb->b, *rows, *wg_per_cu, *chunk);
-            return rc ? rc : 0;
+            for (int q = 0; q < 4; ++q) tuning[q] = it->second[q];
+            return fdtd_batch_set_tuning(sb->b, tuning);
         }
     }
     int taken = 0;
@@ -913,24 +912,26 @@ int fdtd_scene_batch_autotune(fdtd_scene_batch* sb, int trial_steps, int* rows, 
     if (rc) return rc;
     taken += trial_steps;
     double best = 1e300;
-    std::array<int, 3> win{4, 4, 4};
-    for (int r : {1, 2, 4})
-        for (int w : {2, 4, 8})
-            for (int c : {1, 4, 16}) {
-                if ((rc = fdtd_batch_set_tuning(sb->b, r, w, c))) return rc;
-                if ((rc = fdtd_scene_batch_step(sb, trial_steps, &ms))) return rc;
-                taken += trial_steps;
-                if (ms < best) {
-                    best = ms;
-                    win = {r, w, c};
+    std::array<int, 4> win{4, 4, 4, 1};
+    for (int r : {1, 4})
+        for (int w : {4, 8})
+            for (int c : {4, 16})
+                for (int p : {1, 4, 8}) {
+                    const int cand[4] = {r, w, c, p};
+                    if ((rc = fdtd_batch_set_tuning(sb->b, cand))) return rc;
+                    if ((rc = fdtd_scene_batch_step(sb, trial_steps, &ms))) return rc;
+                    taken += trial_steps;
+                    if (ms < best) {
+                        best = ms;
+                        win = {r, w, c, p};
+                    }
                 }
-            }
-    if ((rc = fdtd_batch_set_tuning(sb->b, win[0], win[1], win[2]))) return rc;
+    if ((rc = fdtd_batch_set_tuning(sb->b, win.data()))) return rc;
     {
         std::lock_guard<std::mutex> g(tune_mutex());
         tune_cache()[key] = win;
     }
-    *rows = win[0]; *wg_per_cu = win[1]; *chunk = win[2];
+    for (int q = 0; q < 4; ++q) tuning[q] = win[q];
     return taken;
 }
 

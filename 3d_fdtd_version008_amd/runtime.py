@@ -103,8 +103,8 @@ def load() -> C.CDLL:
         "fdtd_get_fp16_scales": (i, [vp, vp]),
         "fdtd_scene_steps_done": (l, [vp]),
         "fdtd_scene_handle": (vp, [vp]),
-        "fdtd_batch_set_tuning": (i, [vp, i, i, i]),
-        "fdtd_batch_get_tuning": (i, [vp, C.POINTER(i), C.POINTER(i), C.POINTER(i)]),
+        "fdtd_batch_set_tuning": (i, [vp, vp]),
+        "fdtd_batch_get_tuning": (i, [vp, vp]),
         "fdtd_batch_set_graph": (i, [vp, i]),
         "fdtd_scene_batch_create": (i, [i, C.POINTER(vp)]),
         "fdtd_scene_batch_destroy": (i, [vp]),
@@ -112,7 +112,7 @@ def load() -> C.CDLL:
         "fdtd_scene_batch_remove": (i, [vp, vp]),
         "fdtd_scene_batch_count": (i, [vp]),
         "fdtd_scene_batch_step": (i, [vp, i, C.POINTER(d)]),
-        "fdtd_scene_batch_autotune": (i, [vp, i, C.POINTER(i), C.POINTER(i), C.POINTER(i)]),
+        "fdtd_scene_batch_autotune": (i, [vp, i, vp]),
         "fdtd_scene_batch_handle": (vp, [vp]),
         "fdtd_scene_set_stop": (i, [vp, l]),
     }
@@ -320,20 +320,22 @@ class Batch:
     def set_graph(self, on: bool = True):
         _check(self.lib.fdtd_batch_set_graph(self.b, int(on)))
 
-    def set_tuning(self, rows: int, wg_per_cu: int, chunk: int):
-        _check(self.lib.fdtd_batch_set_tuning(self.b, rows, wg_per_cu, chunk))
+    def set_tuning(self, rows: int, wg_per_cu: int, chunk: int, planes: int = 1):
+        t = (C.c_int * 4)(rows, wg_per_cu, chunk, planes)
+        _check(self.lib.fdtd_batch_set_tuning(self.b, t))
 
     def tuning(self):
-        r, w, c = C.c_int(), C.c_int(), C.c_int()
-        _check(self.lib.fdtd_batch_get_tuning(self.b, C.byref(r), C.byref(w), C.byref(c)))
-        return r.value, w.value, c.value
+        """(rows, wg_per_cu, chunk, planes) of the current tiling."""
+        t = (C.c_int * 4)()
+        _check(self.lib.fdtd_batch_get_tuning(self.b, t))
+        return tuple(t)
 
     def autotune(self, trial_steps: int = 5):
         """Times every candidate tiling on real steps and keeps the fastest;
-        returns ((rows, wg_per_cu, chunk), steps taken)."""
-        r, w, c = C.c_int(), C.c_int(), C.c_int()
-        taken = _check(self.lib.fdtd_scene_batch_autotune(self.sb, trial_steps, C.byref(r), C.byref(w), C.byref(c)))
-        return (r.value, w.value, c.value), taken
+        returns ((rows, wg_per_cu, chunk, planes), steps taken)."""
+        t = (C.c_int * 4)()
+        taken = _check(self.lib.fdtd_scene_batch_autotune(self.sb, trial_steps, t))
+        return tuple(t), taken
 
     def close(self):
         if getattr(self, "sb", None):

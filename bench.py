@@ -276,7 +276,7 @@ def batch_form(args):
                       "value": round(rate, 3), "unit": "Gcell-updates/s", "steps": args.steps,
                       "ms_per_step": round(wall / args.steps * 1e3, 4), "device_ms_per_step": round(dev_ms / args.steps, 4),
                       "config": {"domains": len(specs), "cells": cells, "precision": args.precision,
-                                 "tuning_rows_wgpercu_chunk": list(tuning), "autotune_steps": taken,
+                                 "tuning_rows_wgpercu_chunk_planes": list(tuning), "autotune_steps": taken,
                                  "graph_block": 50, "setup_s": round(t_setup, 1)},
                       "one_at_a_time": {"value": round(solo, 3), "sample_domains": len(sample),
                                         "steps_each": args.solo_steps, "kernel_mode": "h_e_kernels"},

@@ -173,12 +173,13 @@ int  fdtd_scene_batch_count(const fdtd_scene_batch* sb);
 /* ms (may be NULL) receives the device time of the call. */
 int  fdtd_scene_batch_step(fdtd_scene_batch* sb, int nsteps, double* ms);
 /* Autotuner: runs trial_steps real steps under each candidate tiling (rows
- * 1, 2, 4 x work-groups per CU 2, 4, 8 x tiles per counter increment 1, 4,
- * 16), keeps the fastest and returns it.  A winner is remembered per
- * (precision, member count, total cells rounded to a power of two) for the
- * life of the process; a later call with the same key applies it without
- * trials.  Returns the number of steps taken. */
-int  fdtd_scene_batch_autotune(fdtd_scene_batch* sb, int trial_steps, int* rows, int* wg_per_cu, int* chunk);
+ * 1, 4 x work-groups per CU 4, 8 x tiles per counter increment 4, 16 x
+ * planes per tile 1, 4, 8; after one warm-up trial), keeps the fastest and
+ * writes it to tuning[4] (fdtd_batch_set_tuning order).  A winner is
+ * remembered per (precision, member count, total cells rounded to a power of
+ * two) for the life of the process; a later call with the same key applies it
+ * without trials.  Returns the number of steps taken. */
+int  fdtd_scene_batch_autotune(fdtd_scene_batch* sb, int trial_steps, int* tuning);
 struct fdtd_batch* fdtd_scene_batch_handle(fdtd_scene_batch* sb);
 
 #ifdef __cplusplus

@@ -66,7 +66,7 @@ def test_batch_equals_alone(pkg, precision):
     setups = [pkg.Setup(sp) for sp in specs]
     _say(f"[batch] {precision} domains={[st.shape for st in setups]}")
     ref = _alone(pkg, setups, precision, 80)
-    for tuning, graph in [((4, 4, 4), False), ((1, 2, 1), False), ((2, 8, 16), True)]:
+    for tuning, graph in [((4, 4, 4, 1), False), ((1, 2, 1, 3), False), ((2, 8, 16, 8), True)]:
         _say(f"  tuning={tuning} graph={graph}")
         b = pkg.Batch(precision)
         scs = [pkg.Scene(st, precision) for st in setups]
@@ -132,9 +132,9 @@ def test_batch_autotune_keeps_results(pkg):
     scs = [pkg.Scene(st, "fp32") for st in setups]
     for sc in scs:
         b.add(sc)
-    (r, w, c), taken = b.autotune(trial_steps=2)
-    _say(f"  autotune -> rows={r} wg/cu={w} chunk={c} after {taken} steps")
-    assert taken == 2 * 28 and (r, w, c) == b.tuning()
+    tun, taken = b.autotune(trial_steps=2)
+    _say(f"  autotune -> rows, wg/cu, chunk, planes = {tun} after {taken} steps")
+    assert taken == 2 * 25 and tun == b.tuning()
     b.step(4)
     total = taken + 4
     ref = _alone(pkg, setups, "fp32", total)
@@ -145,5 +145,5 @@ def test_batch_autotune_keeps_results(pkg):
     scs2 = [pkg.Scene(st, "fp32") for st in setups]
     for sc in scs2:
         b2.add(sc)
-    (r2, w2, c2), taken2 = b2.autotune(trial_steps=2)
-    assert taken2 == 0 and (r2, w2, c2) == (r, w, c)
+    tun2, taken2 = b2.autotune(trial_steps=2)
+    assert taken2 == 0 and tun2 == tun

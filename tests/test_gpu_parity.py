@@ -79,7 +79,7 @@ def _compare(setup, sc, ref, tol, extra=(), vector=False):
 MODES = [(0, 0), (1, 16), (1, 7), (1, 128)]
 
 
-@pytest.mark.parametrize("mode", MODES, ids=lambda m: f"mode{m[0]}_ic{m[1]}")
+@pytest.mark.parametrize("mode", MODES, ids=lambda m: f"mode{m[0]}_c{m[1]}")
 def test_uniform_sphere_fp64_full(pkg, oracle_mod, mode):
     """The plain CPU case at full size: 64^3 core + 10-cell CPML, 1000 steps,
     for the two-kernel step and the fused step at several tile shapes."""
@@ -98,7 +98,7 @@ def test_psi_state_fp64(pkg, oracle_mod, mode):
         assert rel_l2(sc.field(name), ref.array(name)) <= 1e-12, name
 
 
-@pytest.mark.parametrize("mode", MODES, ids=lambda m: f"mode{m[0]}_ic{m[1]}")
+@pytest.mark.parametrize("mode", MODES, ids=lambda m: f"mode{m[0]}_c{m[1]}")
 def test_gain_slab_fp64(pkg, oracle_mod, mode):
     sp = pkg.scene.gain_slab_case(n_total=40, npml=6, steps=400, fine_len=100e-9, coarse=20e-9, fine=5e-9)
     setup, sc, ref = _run_pair(pkg, oracle_mod, sp, "fp64", mode=mode)
@@ -151,7 +151,7 @@ def test_step_kernels_agree_bitwise(pkg, precision):
 GOLDEN = __import__("os").path.join(__import__("os").path.dirname(__file__), "golden")
 
 
-@pytest.mark.parametrize("mode", [(0, 0), (1, 8)], ids=lambda m: f"mode{m[0]}")
+@pytest.mark.parametrize("mode", [(0, 0), (1, 8)], ids=lambda m: f"mode{m[0]}_c{m[1]}")
 @pytest.mark.parametrize("name", ["uniform_sphere", "gain_slab", "gain_clamp_dipole", "plane_waves", "pec_cavity",
                                   "graded_cuboids"])
 def test_fp64_matches_reference_golden(pkg, name, mode):
