@@ -1008,30 +1008,6 @@ __global__ void __launch_bounds__(64 * ROW_WAVES) k_step_rows(StepArgs<T, R> a, 
     }
 }
 
-// PECBoundary::apply_after_E (src/boundary.hpp:55-74): zero E on the six faces.
-template <typename T>
-__global__ void k_pec_faces(T* Ex, T* Ey, T* Ez, int nx, int ny, int nz) {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long fxy = (long)nx * ny, fxz = (long)nx * nz, fyz = (long)ny * nz;
-    long id;
-    if (t < 2 * fyz) {
-        const long q = t % fyz;
-        const int i = t < fyz ? 0 : nx - 1;
-        id = ((long)i * ny + q / nz) * nz + q % nz;
-    } else if (t < 2 * fyz + 2 * fxz) {
-        const long u = t - 2 * fyz, q = u % fxz;
-        const int j = u < fxz ? 0 : ny - 1;
-        id = ((q / nz) * ny + j) * nz + q % nz;
-    } else if (t < 2 * fyz + 2 * fxz + 2 * fxy) {
-        const long u = t - 2 * fyz - 2 * fxz, q = u % fxy;
-        const int k = u < fxy ? 0 : nz - 1;
-        id = q * nz + k;
-    } else {
-        return;
-    }
-    Ex[id] = (T)0; Ey[id] = (T)0; Ez[id] = (T)0;
-}
-
 struct ProbeSlot {
     long id;
     int comp;
