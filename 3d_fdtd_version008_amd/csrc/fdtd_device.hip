@@ -485,7 +485,7 @@ fdtdk::ArrSet fp16_arrays(const fdtd_sim* s) {
 // power-of-two rescaling (exact), entirely on the device.
 void fp16_maintain(fdtd_sim* s, hipStream_t st) {
     const fdtdk::ArrSet a = fp16_arrays(s);
-    hipLaunchKernelGGL(fdtdk::k_fp16_maxima, dim3(512), dim3(256), 0, st, a, s->d_maxbits);
+    hipLaunchKernelGGL(fdtdk::k_fp16_maxima, dim3(256, 18), dim3(256), 0, st, a, s->d_maxbits);
     hipLaunchKernelGGL(fdtdk::k_fp16_plan, dim3(1), dim3(64), 0, st, s->d_sc, s->d_factor, s->d_maxbits);
     hipLaunchKernelGGL(fdtdk::k_fp16_rescale, dim3(512), dim3(256), 0, st, a, (const float*)s->d_factor);
     fdtdk::ArrSet o{};  // the other buffer holds the never-updated cells too

@@ -278,22 +278,10 @@ struct HIn {
     bool upd;
 };
 
+// Per-cell H inputs other than the fields: H coefficients (when they vary
+// per cell) and psi_H on the slabs.
 template <typename T, typename R>
-__device__ __forceinline__ HIn<R> h_load(const StepArgs<T, R>& a, int i, int j, int k) {
-    HIn<R> v;
-    const long id = ((long)i * a.ny + j) * a.nz + k;
-    v.hx = ld<T, R>(a.Hx, id, a.sc[3]);
-    v.hy = ld<T, R>(a.Hy, id, a.sc[4]);
-    v.hz = ld<T, R>(a.Hz, id, a.sc[5]);
-    v.upd = (i < a.nx - 1) & (j < a.ny - 1) & (k < a.nz - 1);
-    for (int q = 0; q < 6; ++q) v.p[q] = 0;
-    for (int q = 0; q < 6; ++q) v.cH[q] = 0;
-    v.ex = v.ex_j = v.ex_k = v.ey = v.ey_i = v.ey_k = v.ez = v.ez_i = v.ez_j = 0;
-    if (!v.upd) return v;
-    const long sI = a.sI, sJ = a.sJ;
-    v.ex = ld<T, R>(a.Ex, id, a.sc[0]); v.ex_j = ld<T, R>(a.Ex, id + sJ, a.sc[0]); v.ex_k = ld<T, R>(a.Ex, id + 1, a.sc[0]);
-    v.ey = ld<T, R>(a.Ey, id, a.sc[1]); v.ey_i = ld<T, R>(a.Ey, id + sI, a.sc[1]); v.ey_k = ld<T, R>(a.Ey, id + 1, a.sc[1]);
-    v.ez = ld<T, R>(a.Ez, id, a.sc[2]); v.ez_i = ld<T, R>(a.Ez, id + sI, a.sc[2]); v.ez_j = ld<T, R>(a.Ez, id + sJ, a.sc[2]);
+__device__ __forceinline__ void h_load_rest(const StepArgs<T, R>& a, int i, int j, int k, long id, HIn<R>& v) {
     if (!a.h_uniform) {
         const int m = mat_of<R>(a.matid, a.mid16, id);
         for (int q = 0; q < 6; ++q) v.cH[q] = a.lut[(6 + q) * a.nmat + m];
@@ -317,6 +305,25 @@ __device__ __forceinline__ HIn<R> h_load(const StepArgs<T, R>& a, int i, int j, 
             v.p[4] = ld<T, R>(a.psi[10], q, a.sc[16]);
         }
     }
+}
+
+template <typename T, typename R>
+__device__ __forceinline__ HIn<R> h_load(const StepArgs<T, R>& a, int i, int j, int k) {
+    HIn<R> v;
+    const long id = ((long)i * a.ny + j) * a.nz + k;
+    v.hx = ld<T, R>(a.Hx, id, a.sc[3]);
+    v.hy = ld<T, R>(a.Hy, id, a.sc[4]);
+    v.hz = ld<T, R>(a.Hz, id, a.sc[5]);
+    v.upd = (i < a.nx - 1) & (j < a.ny - 1) & (k < a.nz - 1);
+    for (int q = 0; q < 6; ++q) v.p[q] = 0;
+    for (int q = 0; q < 6; ++q) v.cH[q] = 0;
+    v.ex = v.ex_j = v.ex_k = v.ey = v.ey_i = v.ey_k = v.ez = v.ez_i = v.ez_j = 0;
+    if (!v.upd) return v;
+    const long sI = a.sI, sJ = a.sJ;
+    v.ex = ld<T, R>(a.Ex, id, a.sc[0]); v.ex_j = ld<T, R>(a.Ex, id + sJ, a.sc[0]); v.ex_k = ld<T, R>(a.Ex, id + 1, a.sc[0]);
+    v.ey = ld<T, R>(a.Ey, id, a.sc[1]); v.ey_i = ld<T, R>(a.Ey, id + sI, a.sc[1]); v.ey_k = ld<T, R>(a.Ey, id + 1, a.sc[1]);
+    v.ez = ld<T, R>(a.Ez, id, a.sc[2]); v.ez_i = ld<T, R>(a.Ez, id + sI, a.sc[2]); v.ez_j = ld<T, R>(a.Ez, id + sJ, a.sc[2]);
+    h_load_rest<T, R>(a, i, j, k, id, v);
     return v;
 }
 
@@ -328,7 +335,7 @@ __device__ __forceinline__ HIn<R> h_load(const StepArgs<T, R>& a, int i, int j, 
 // stored value.  Returns the values a later read of the stored H gives.
 template <typename T, typename R>
 __device__ __forceinline__ H3<R> h_compute(const StepArgs<T, R>& a, const Tabs<R>& tb, int i, int j, int k,
-                                           const HIn<R>& v, bool own) {
+                                           const HIn<R>& v, bool own, bool store_fields = true) {
     const Axes<R>& ax = tb.ax;
     if (!v.upd) return H3<R>{v.hx, v.hy, v.hz};
     const long id = ((long)i * a.ny + j) * a.nz + k;
@@ -378,7 +385,7 @@ __device__ __forceinline__ H3<R> h_compute(const StepArgs<T, R>& a, const Tabs<R
             }
         }
     }
-    if (own) {
+    if (own && store_fields) {
         st<T, R>(a.Hxw, id, hx, a.sc[21]);
         st<T, R>(a.Hyw, id, hy, a.sc[22]);
         st<T, R>(a.Hzw, id, hz, a.sc[23]);
@@ -488,71 +495,157 @@ struct EIn {
     int reg;
     long g;
     double Nu, Ng, Pz, Pp;
+    // axis tables at the cell: inverse spacings, and the CPML E rows
+    // (kappa, 1/kappa, bE, cE) at i, j and k where the cell is in that slab
+    R idx, idy, idz;
+    R kx[4], ky[4], kz[4];
+    R jx, jy, jz;  // source current J at the cell
+    TLSCoef tc;    // TLS coefficients of region `reg` (when reg >= 0)
 };
 
-// Material id of an E cell (the LUT row its coefficients come from).
-template <typename T, typename R>
-__device__ __forceinline__ int e_mat(const StepArgs<T, R>& a, int i, int j, int k) {
-    if (!((i > 0) & (j > 0) & (k > 0))) return 0;
-    return mat_of<R>(a.matid, a.mid16, ((long)i * a.ny + j) * a.nz + k);
+// Source current at cell (i, j, k): the sum over the footprints that hold it.
+// Footprints and values are read for every source (uniform addresses, so
+// through the scalar cache) and the cell's share is selected without a
+// branch, so no read waits behind the cell's vector loads.
+template <typename R>
+__device__ __forceinline__ void e_sources(const Tabs<R>& tb, int nsrc, int i, int j, int k, R& jx, R& jy, R& jz) {
+    jx = jy = jz = 0;
+    for (int s = 0; s < nsrc; ++s) {
+        const SourceBox b = tb.src[s];
+        const R val = (R)tb.srcv[s];
+        const bool hit = (i >= b.i0) & (i <= b.i1) & (j >= b.j0) & (j <= b.j1) & (k >= b.k0) & (k <= b.k1);
+        const R add = hit ? val : (R)0;
+        jx += b.comp == 0 ? add : (R)0;
+        jy += b.comp == 1 ? add : (R)0;
+        jz += b.comp == 2 ? add : (R)0;
+    }
+}
+// Axis-table values of an E cell (px, py, pz: the slabs whose rows are read).
+template <typename R>
+__device__ __forceinline__ void e_axes(const Axes<R>& ax, int i, int j, int k, bool px, bool py, bool pz, EIn<R>& v) {
+    v.idx = ax.X(0, i);
+    v.idy = ax.Y(0, j);
+    v.idz = ax.Z(0, k);
+    for (int q = 0; q < 4; ++q) v.kx[q] = v.ky[q] = v.kz[q] = 0;
+    if (px) { v.kx[0] = ax.X(1, i); v.kx[1] = ax.X(6, i); v.kx[2] = ax.X(2, i); v.kx[3] = ax.X(3, i); }
+    if (py) { v.ky[0] = ax.Y(1, j); v.ky[1] = ax.Y(6, j); v.ky[2] = ax.Y(2, j); v.ky[3] = ax.Y(3, j); }
+    if (pz) { v.kz[0] = ax.Z(1, k); v.kz[1] = ax.Z(6, k); v.kz[2] = ax.Z(2, k); v.kz[3] = ax.Z(3, k); }
 }
 
-// Gathers the E inputs of a cell whose material id `m` is already known (the
-// fused kernel loads ids one plane earlier, so this gather never waits on a
-// load issued in the same pass).  Gain state is loaded for every cell of the
-// gain box (zero outside the regions), so no load depends on the region id.
+// The compiler cannot see through this (no instruction is emitted): a value
+// that passes through it is not a constant, so operations on a variable that
+// is either such a default or a loaded value are not folded into the branch
+// that loads it (where they would wait for the load).
+template <typename X>
+__device__ __forceinline__ X opaque(X x) {
+    asm("" : "+v"(x));
+    return x;
+}
+// Storage value to compute type (the conversion half of ld).
 template <typename T, typename R>
-__device__ __forceinline__ EIn<R> e_load(const StepArgs<T, R>& a, int i, int j, int k, int m) {
+__device__ __forceinline__ R cv(T x, R scale) {
+    if constexpr (sizeof(T) == 2) {
+        return (R)__half2float(__ushort_as_half(x)) * scale;
+    } else {
+        (void)scale;
+        return (R)x;
+    }
+}
+
+// Inputs of the E update of cell (i, j, k) for a wavefront that covers 64
+// consecutive k of one (i, j) row: i, j and kw (the k of the wavefront's
+// first lane) are uniform across it.  Every input is requested
+// before the first one is used, so a wavefront makes one trip to memory plus
+// one to the caches: the material and region ids first (the gathers after
+// them wait for these alone; loads complete in issue order), then H^{n+1/2}
+// and E^n, psi_E, the gain state and the z-axis table.  Wave-uniform
+// conditions select which of these a wavefront loads; within a wavefront
+// every lane loads, with indices clamped into bounds where a lane's cell is
+// outside the slab or the gain box.  Values loaded under a condition start
+// from opaque defaults and are converted after the last load is issued, so
+// no arithmetic on them is placed (and waits) inside the branch.  X/Y axis
+// values, source footprints and the coefficients of a single TLS region are
+// uniform and come through the scalar cache.
+// Returns H^{n+1/2} at the cell and at its -x, -y, -z neighbours in h[9]:
+// hx hy hz hx_j hz_j hx_k hy_k hy_i hz_i.
+template <typename T, typename R>
+__device__ __forceinline__ EIn<R> e_gather(const StepArgs<T, R>& a, const Tabs<R>& tb, int i, int j, int k, int kw,
+                                           R* h) {
     EIn<R> v;
     const long id = ((long)i * a.ny + j) * a.nz + k;
-    v.reg = -1;
-    v.g = 0;
-    v.Nu = v.Ng = v.Pz = v.Pp = 0;
-    if (a.gain) {
-        const int gi = i - a.g0x, gj = j - a.g0y, gk = k - a.g0z;
-        if ((unsigned)gi < (unsigned)a.gnx && (unsigned)gj < (unsigned)a.gny && (unsigned)gk < (unsigned)a.gnz) {
-            v.g = ((long)gi * a.gny + gj) * a.gnz + gk;
-            v.reg = a.greg[v.g];
-            v.Nu = a.gNu[v.g]; v.Ng = a.gNg[v.g]; v.Pz = a.gPz[v.g]; v.Pp = a.gPp[v.g];
-        }
-    }
-    for (int q = 0; q < 6; ++q) v.p[q] = 0;
-    v.ex0 = v.ey0 = 0;
+    const int np = a.np, nx = a.nx, ny = a.ny, nz = a.nz;
+    // ids
+    const int m = mat_of<R>(a.matid, a.mid16, id);
+    const int gi = i - a.g0x, gj = j - a.g0y, gk = k - a.g0z;
+    const bool growu = a.gain && ((unsigned)gi < (unsigned)a.gnx) & ((unsigned)gj < (unsigned)a.gny) &
+                       (kw < a.g0z + a.gnz) & (kw + 63 >= a.g0z);
+    const bool gin = growu && (unsigned)gk < (unsigned)a.gnz;
+    const long g = ((long)gi * a.gny + gj) * a.gnz + (gin ? gk : 0);
+    int r8 = opaque(-1);
+    if (growu) r8 = a.greg[g];
+    // fields.  On the rows i == 0 or j == 0, which the reference does not
+    // update, the -x/-y offsets are taken as 0 and on the first row id - 1 as
+    // id, so every address is in bounds; those values are never used (nor
+    // is the k == 0 lane's H(k-1)).
+    const long sI = i > 0 ? a.sI : 0, sJ = j > 0 ? (long)a.sJ : 0;
+    const long km = id > 0 ? 1 : 0;
     v.ez0 = ld<T, R>(a.Ez, id, a.sc[2]);
-    for (int q = 0; q < 6; ++q) v.cE[q] = 0;
-    const bool interior = (i > 0) & (j > 0) & (k > 0);
-    if (!interior) return v;
+    h[0] = ld<T, R>(a.Hx, id, a.sc[3]); h[3] = ld<T, R>(a.Hx, id - sJ, a.sc[3]); h[5] = ld<T, R>(a.Hx, id - km, a.sc[3]);
+    h[1] = ld<T, R>(a.Hy, id, a.sc[4]); h[7] = ld<T, R>(a.Hy, id - sI, a.sc[4]); h[6] = ld<T, R>(a.Hy, id - km, a.sc[4]);
+    h[2] = ld<T, R>(a.Hz, id, a.sc[5]); h[8] = ld<T, R>(a.Hz, id - sI, a.sc[5]); h[4] = ld<T, R>(a.Hz, id - sJ, a.sc[5]);
     v.ex0 = ld<T, R>(a.Ex, id, a.sc[0]);
     v.ey0 = ld<T, R>(a.Ey, id, a.sc[1]);
-    {
-        v.cE[1] = a.lut[1 * a.nmat + m];
-        v.cE[3] = a.lut[3 * a.nmat + m];
-        v.cE[5] = a.lut[5 * a.nmat + m];
-        if (!a.lossless) {
-            v.cE[0] = a.lut[0 * a.nmat + m];
-            v.cE[2] = a.lut[2 * a.nmat + m];
-            v.cE[4] = a.lut[4 * a.nmat + m];
-        }
+    // psi_E on the slabs
+    T praw[6];
+    for (int q = 0; q < 6; ++q) praw[q] = opaque((T)0);
+    const bool px = a.cpml && in_pml(i, nx, np), py = a.cpml && in_pml(j, ny, np);
+    const bool pzw = a.cpml && ((kw < np) | (kw + 63 >= nz - np));
+    if (py) {
+        const long q = psi_y(i, j, k, nx, ny, nz, np);
+        praw[0] = a.psi[0][q];
+        praw[5] = a.psi[5][q];
     }
-    if (a.cpml) {
-        const int np = a.np, nx = a.nx, ny = a.ny, nz = a.nz;
-        const bool px = in_pml(i, nx, np), py = in_pml(j, ny, np), pz = in_pml(k, nz, np);
-        if (py) {
-            const long q = psi_y(i, j, k, nx, ny, nz, np);
-            v.p[0] = ld<T, R>(a.psi[0], q, a.sc[6]);
-            v.p[5] = ld<T, R>(a.psi[5], q, a.sc[11]);
-        }
-        if (pz) {
-            const long q = psi_z(i, j, k, nx, ny, nz, np);
-            v.p[1] = ld<T, R>(a.psi[1], q, a.sc[7]);
-            v.p[2] = ld<T, R>(a.psi[2], q, a.sc[8]);
-        }
-        if (px) {
-            const long q = psi_x(i, j, k, nx, ny, nz, np);
-            v.p[3] = ld<T, R>(a.psi[3], q, a.sc[9]);
-            v.p[4] = ld<T, R>(a.psi[4], q, a.sc[10]);
-        }
+    if (pzw) {
+        const long q = psi_z(i, j, in_pml(k, nz, np) ? k : 0, nx, ny, nz, np);
+        praw[1] = a.psi[1][q];
+        praw[2] = a.psi[2][q];
     }
+    if (px) {
+        const long q = psi_x(i, j, k, nx, ny, nz, np);
+        praw[3] = a.psi[3][q];
+        praw[4] = a.psi[4][q];
+    }
+    // gain state
+    double gs[4];
+    for (int q = 0; q < 4; ++q) gs[q] = opaque(0.0);
+    if (growu) { gs[0] = a.gNu[g]; gs[1] = a.gNg[g]; gs[2] = a.gPz[g]; gs[3] = a.gPp[g]; }
+    // axis tables
+    const Axes<R>& ax = tb.ax;
+    v.idx = ax.X(0, i);
+    v.idy = ax.Y(0, j);
+    v.idz = ax.Z(0, k);
+    for (int q = 0; q < 4; ++q) v.kx[q] = v.ky[q] = 0;
+    if (px) { v.kx[0] = ax.X(1, i); v.kx[1] = ax.X(6, i); v.kx[2] = ax.X(2, i); v.kx[3] = ax.X(3, i); }
+    if (py) { v.ky[0] = ax.Y(1, j); v.ky[1] = ax.Y(6, j); v.ky[2] = ax.Y(2, j); v.ky[3] = ax.Y(3, j); }
+    for (int q = 0; q < 4; ++q) v.kz[q] = opaque((R)0);
+    if (pzw) { v.kz[0] = ax.Z(1, k); v.kz[1] = ax.Z(6, k); v.kz[2] = ax.Z(2, k); v.kz[3] = ax.Z(3, k); }
+    e_sources(tb, a.nsrc, i, j, k, v.jx, v.jy, v.jz);
+    if (a.gain && a.ntls == 1) v.tc = tb.tls[0];
+    // the gathers that wait for the ids
+    v.cE[1] = a.lut[1 * a.nmat + m];
+    v.cE[3] = a.lut[3 * a.nmat + m];
+    v.cE[5] = a.lut[5 * a.nmat + m];
+    v.cE[0] = v.cE[2] = v.cE[4] = 0;
+    if (!a.lossless) {
+        v.cE[0] = a.lut[0 * a.nmat + m];
+        v.cE[2] = a.lut[2 * a.nmat + m];
+        v.cE[4] = a.lut[4 * a.nmat + m];
+    }
+    if (a.gain && a.ntls != 1) v.tc = tb.tls[r8 < 0 ? 0 : r8];
+    v.reg = gin ? r8 : -1;
+    v.g = g;
+    v.Nu = gs[0]; v.Ng = gs[1]; v.Pz = gs[2]; v.Pp = gs[3];
+    for (int q = 0; q < 6; ++q) v.p[q] = cv<T, R>(praw[q], a.sc[6 + q]);
     return v;
 }
 
@@ -568,12 +661,15 @@ __device__ __forceinline__ EIn<R> e_load(const StepArgs<T, R>& a, int i, int j, 
 // reference where Ez_old is only written on updated cells.  With a PEC
 // boundary the six faces are written as zero (PECBoundary::apply_after_E,
 // src/boundary.hpp:55-74).
+//
+// Returns the cell's E after the step: its new value, zero on a PEC face, and
+// for a cell the reference does not update, v.ex0/ey0/ez0 (so a caller that
+// stores fields itself, store_fields = false, passes the loaded values
+// there).
 template <typename T, typename R>
-__device__ __forceinline__ void e_compute(const StepArgs<T, R>& a, const Tabs<R>& tb, int i, int j, int k,
-                                          const EIn<R>& v, R hx, R hy, R hz, R hx_j, R hz_j, R hx_k, R hy_k, R hy_i,
-                                          R hz_i) {
-    const Axes<R>& ax = tb.ax;
-    const TLSCoef* tls = tb.tls;
+__device__ __forceinline__ H3<R> e_compute(const StepArgs<T, R>& a, const Tabs<R>& tb, int i, int j, int k,
+                                           const EIn<R>& v, R hx, R hy, R hz, R hx_j, R hz_j, R hx_k, R hy_k, R hy_i,
+                                           R hz_i, bool store_fields = true) {
     const long id = ((long)i * a.ny + j) * a.nz + k;
     const int reg = v.reg;
     const long g = v.g;
@@ -581,7 +677,7 @@ __device__ __forceinline__ void e_compute(const StepArgs<T, R>& a, const Tabs<R>
     const bool pec_face = a.pec && (!interior || i == a.nx - 1 || j == a.ny - 1 || k == a.nz - 1);
     if (!interior) {
         if (reg >= 0) {
-            const TLSCoef c = tls[reg];
+            const TLSCoef& c = v.tc;
             double Nu = v.Nu, Ng = v.Ng, Pz = v.Pz, Pp = v.Pp;
             const double ez = (double)v.ez0;
             (void)tls_polarize<sizeof(R) == 8>(c, Nu, Ng, ez, Pz, Pp);
@@ -589,37 +685,26 @@ __device__ __forceinline__ void e_compute(const StepArgs<T, R>& a, const Tabs<R>
             gain_arr(a, tb, 0)[g] = Nu; gain_arr(a, tb, 1)[g] = Ng; gain_arr(a, tb, 2)[g] = Pz; gain_arr(a, tb, 3)[g] = Pp;
         }
         if (pec_face) {
-            st<T, R>(a.Exw, id, (R)0, a.sc[18]);
-            st<T, R>(a.Eyw, id, (R)0, a.sc[19]);
-            st<T, R>(a.Ezw, id, (R)0, a.sc[20]);
+            if (store_fields) {
+                st<T, R>(a.Exw, id, (R)0, a.sc[18]);
+                st<T, R>(a.Eyw, id, (R)0, a.sc[19]);
+                st<T, R>(a.Ezw, id, (R)0, a.sc[20]);
+            }
+            return H3<R>{0, 0, 0};
         }
-        return;
+        return H3<R>{v.ex0, v.ey0, v.ez0};
     }
-    const R idx = ax.X(0, i), idy = ax.Y(0, j), idz = ax.Z(0, k);
+    const R idx = v.idx, idy = v.idy, idz = v.idz;
     const R dHz_y = (hz - hz_j) * idy, dHy_z = (hy - hy_k) * idz;
     const R dHx_z = (hx - hx_k) * idz, dHz_x = (hz - hz_i) * idx;
     const R dHy_x = (hy - hy_i) * idx, dHx_y = (hx - hx_j) * idy;
     const R cx = dHz_y - dHy_z, cy = dHx_z - dHz_x, cz = dHy_x - dHx_y;
-    R jx = 0, jy = 0, jz = 0;
-    for (int s = 0; s < a.nsrc; ++s) {
-        // i and j are uniform across the wavefront: test them (and read the
-        // value) only for boxes the row crosses
-        const SourceBox b = tb.src[s];
-        if (i >= b.i0 && i <= b.i1 && j >= b.j0 && j <= b.j1) {
-            const R val = (R)tb.srcv[s];
-            const R add = (k >= b.k0 && k <= b.k1) ? val : (R)0;
-            if (b.comp == 0) jx += add; else if (b.comp == 1) jy += add; else jz += add;
-        }
-    }
+    const R jx = v.jx, jy = v.jy, jz = v.jz;
     const R bEx = v.cE[1], bEy = v.cE[3], bEz = v.cE[5];
     double Nu = v.Nu, Ng = v.Ng, Pz = v.Pz, Pp = v.Pp;
     R rz = cz - jz;
-    // the region's coefficients are read once, before any store (a read after
-    // a store would be repeated from memory: the compiler cannot rule out
-    // that the store changed them)
-    TLSCoef tc{};
+    const TLSCoef& tc = v.tc;
     if (reg >= 0) {
-        tc = tls[reg];
         const double dP = tls_polarize<sizeof(R) == 8>(tc, Nu, Ng, (double)v.ez0, Pz, Pp);
         rz = rz - (R)dP;
     }
@@ -638,12 +723,12 @@ __device__ __forceinline__ void e_compute(const StepArgs<T, R>& a, const Tabs<R>
         const bool px = in_pml(i, nx, np), py = in_pml(j, ny, np), pz = in_pml(k, nz, np);
         if (px | py | pz) {
             R pn[6];
-            if (py) ex += bEx * cpml_term<R>(v.p[0], dHz_y, ax.Y(1, j), ax.Y(6, j), ax.Y(2, j), ax.Y(3, j), pn[0]);
-            if (pz) ex += bEx * (-cpml_term<R>(v.p[1], dHy_z, ax.Z(1, k), ax.Z(6, k), ax.Z(2, k), ax.Z(3, k), pn[1]));
-            if (pz) ey += bEy * cpml_term<R>(v.p[2], dHx_z, ax.Z(1, k), ax.Z(6, k), ax.Z(2, k), ax.Z(3, k), pn[2]);
-            if (px) ey += bEy * (-cpml_term<R>(v.p[3], dHz_x, ax.X(1, i), ax.X(6, i), ax.X(2, i), ax.X(3, i), pn[3]));
-            if (px) ez += bEz * cpml_term<R>(v.p[4], dHy_x, ax.X(1, i), ax.X(6, i), ax.X(2, i), ax.X(3, i), pn[4]);
-            if (py) ez += bEz * (-cpml_term<R>(v.p[5], dHx_y, ax.Y(1, j), ax.Y(6, j), ax.Y(2, j), ax.Y(3, j), pn[5]));
+            if (py) ex += bEx * cpml_term<R>(v.p[0], dHz_y, v.ky[0], v.ky[1], v.ky[2], v.ky[3], pn[0]);
+            if (pz) ex += bEx * (-cpml_term<R>(v.p[1], dHy_z, v.kz[0], v.kz[1], v.kz[2], v.kz[3], pn[1]));
+            if (pz) ey += bEy * cpml_term<R>(v.p[2], dHx_z, v.kz[0], v.kz[1], v.kz[2], v.kz[3], pn[2]);
+            if (px) ey += bEy * (-cpml_term<R>(v.p[3], dHz_x, v.kx[0], v.kx[1], v.kx[2], v.kx[3], pn[3]));
+            if (px) ez += bEz * cpml_term<R>(v.p[4], dHy_x, v.kx[0], v.kx[1], v.kx[2], v.kx[3], pn[4]);
+            if (py) ez += bEz * (-cpml_term<R>(v.p[5], dHx_y, v.ky[0], v.ky[1], v.ky[2], v.ky[3], pn[5]));
             if (py) {
                 const long q = psi_y(i, j, k, nx, ny, nz, np);
                 st<T, R>(psi_rd(a, tb, 0), q, pn[0], a.sc[24]);
@@ -662,9 +747,11 @@ __device__ __forceinline__ void e_compute(const StepArgs<T, R>& a, const Tabs<R>
         }
     }
     if (pec_face) { ex = 0; ey = 0; ez = 0; }
-    st<T, R>(a.Exw, id, ex, a.sc[18]);
-    st<T, R>(a.Eyw, id, ey, a.sc[19]);
-    st<T, R>(a.Ezw, id, ez, a.sc[20]);
+    if (store_fields) {
+        st<T, R>(a.Exw, id, ex, a.sc[18]);
+        st<T, R>(a.Eyw, id, ey, a.sc[19]);
+        st<T, R>(a.Ezw, id, ez, a.sc[20]);
+    }
     if (reg >= 0) {
         const TLSCoef& c = tc;
         // The reference stores Ez and reads it back for the population pass.
@@ -672,6 +759,16 @@ __device__ __forceinline__ void e_compute(const StepArgs<T, R>& a, const Tabs<R>
         tls_populate(c, ez_new, (double)v.ez0, Pz, Pp, Nu, Ng);
         gain_arr(a, tb, 0)[g] = Nu; gain_arr(a, tb, 1)[g] = Ng; gain_arr(a, tb, 2)[g] = Pz; gain_arr(a, tb, 3)[g] = Pp;
     }
+    return H3<R>{ex, ey, ez};
+}
+
+// E update of one cell (i, j and kw uniform across the wavefront, as for
+// e_gather).
+template <typename T, typename R>
+__device__ __forceinline__ void e_cell(const StepArgs<T, R>& a, const Tabs<R>& tb, int i, int j, int k, int kw) {
+    R h[9];
+    const EIn<R> v = e_gather<T, R>(a, tb, i, j, k, kw, h);
+    e_compute<T, R>(a, tb, i, j, k, v, h[0], h[1], h[2], h[3], h[4], h[5], h[6], h[7], h[8]);
 }
 
 // E update kernel of the two-kernel step: H^{n+1/2} neighbours from memory.
@@ -681,18 +778,9 @@ __global__ void __launch_bounds__(256) k_update_E(StepArgs<T, R> a) {
     const Tabs<R> tb = global_tabs(a);
     const Tile t = tile_of(a.nz, a.ny);
     const int i = t.i, j = t.j, k = t.k;
+    const int kw = __builtin_amdgcn_readfirstlane(k);  // lane 0: the tile's first k
     if (k >= a.nz || j >= a.ny || i >= a.nx) return;
-    const long id = ((long)i * a.ny + j) * a.nz + k;
-    const bool interior = (i > 0) & (j > 0) & (k > 0);
-    R hx = 0, hy = 0, hz = 0, hx_j = 0, hz_j = 0, hx_k = 0, hy_k = 0, hy_i = 0, hz_i = 0;
-    if (interior) {
-        const long sI = a.sI, sJ = a.sJ;
-        hx = ld<T, R>(a.Hx, id, a.sc[3]); hx_j = ld<T, R>(a.Hx, id - sJ, a.sc[3]); hx_k = ld<T, R>(a.Hx, id - 1, a.sc[3]);
-        hy = ld<T, R>(a.Hy, id, a.sc[4]); hy_i = ld<T, R>(a.Hy, id - sI, a.sc[4]); hy_k = ld<T, R>(a.Hy, id - 1, a.sc[4]);
-        hz = ld<T, R>(a.Hz, id, a.sc[5]); hz_i = ld<T, R>(a.Hz, id - sI, a.sc[5]); hz_j = ld<T, R>(a.Hz, id - sJ, a.sc[5]);
-    }
-    const EIn<R> v = e_load<T, R>(a, i, j, k, e_mat(a, i, j, k));
-    e_compute<T, R>(a, tb, i, j, k, v, hx, hy, hz, hx_j, hz_j, hx_k, hy_k, hy_i, hz_i);
+    e_cell<T, R>(a, tb, i, j, k, kw);
 }
 
 // Row-marching step kernel: one launch advances H and E of the whole grid,
@@ -997,6 +1085,11 @@ __global__ void __launch_bounds__(64 * ROW_WAVES) k_step_rows(StepArgs<T, R> a, 
                 ev.reg = cur.reg;
                 ev.g = ev.reg >= 0 ? ((long)(i - s.g0x) * s.gny + (j - s.g0y)) * s.gnz + (k - s.g0z) : 0;
                 ev.Nu = cur.Nu; ev.Ng = cur.Ng; ev.Pz = cur.Pz; ev.Pp = cur.Pp;
+                const bool cp = s.cpml != 0;
+                e_axes(tb.ax, i, j, k, cp && in_pml(i, s.nx, s.np), cp && in_pml(j, s.ny, s.np),
+                       cp && in_pml(k, s.nz, s.np), ev);
+                e_sources(tb, s.nsrc, i, j, k, ev.jx, ev.jy, ev.jz);
+                if (ev.reg >= 0) ev.tc = tb.tls[ev.reg];
                 e_compute<T, R>(s, tb, i, j, k, ev, h.x, h.y, h.z, gx, gz, hxk, hyk, hyp, hzp);
             }
             put(0, c, cur.eyn, 1);
@@ -1263,20 +1356,7 @@ __global__ void __launch_bounds__(256) k_batch_step(const StepArgs<T, R>* __rest
                 if (!EPASS) {
                     if (k < a.nz - 1 && j < a.ny - 1 && i < a.nx - 1) (void)h_new<T, R>(a, tb, i, j, k, true);
                 } else if (k < a.nz && j < a.ny) {
-                    const long id = ((long)i * a.ny + j) * a.nz + k;
-                    const bool interior = (i > 0) & (j > 0) & (k > 0);
-                    R hx = 0, hy = 0, hz = 0, hx_j = 0, hz_j = 0, hx_k = 0, hy_k = 0, hy_i = 0, hz_i = 0;
-                    if (interior) {
-                        const long sI = a.sI, sJ = a.sJ;
-                        hx = ld<T, R>(a.Hx, id, a.sc[3]); hx_j = ld<T, R>(a.Hx, id - sJ, a.sc[3]);
-                        hx_k = ld<T, R>(a.Hx, id - 1, a.sc[3]);
-                        hy = ld<T, R>(a.Hy, id, a.sc[4]); hy_i = ld<T, R>(a.Hy, id - sI, a.sc[4]);
-                        hy_k = ld<T, R>(a.Hy, id - 1, a.sc[4]);
-                        hz = ld<T, R>(a.Hz, id, a.sc[5]); hz_i = ld<T, R>(a.Hz, id - sI, a.sc[5]);
-                        hz_j = ld<T, R>(a.Hz, id - sJ, a.sc[5]);
-                    }
-                    const EIn<R> v = e_load<T, R>(a, i, j, k, e_mat(a, i, j, k));
-                    e_compute<T, R>(a, tb, i, j, k, v, hx, hy, hz, hx_j, hz_j, hx_k, hy_k, hy_i, hz_i);
+                    e_cell<T, R>(a, tb, i, j, k, kt * 64);
                 }
             }
         }
@@ -1341,21 +1421,40 @@ struct ArrSet {
     const uint16_t* p[18];
     long n[18];
 };
+// |x| of an FP16 value is its bit pattern with the sign bit cleared, and for
+// non-negative FP16 values the integer order of the patterns is the numeric
+// order (NaN/Inf above every finite value), so the maximum is taken on the
+// 16-bit patterns, eight per 16-byte load.  blockIdx.y is the array.
 __global__ void __launch_bounds__(256) k_fp16_maxima(ArrSet s, unsigned* maxbits) {
-    for (int c = 0; c < 18; ++c) {
-        if (!s.p[c]) continue;
-        unsigned m = 0;
-        const long stride = (long)gridDim.x * blockDim.x;
-        for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < s.n[c]; i += stride) {
-            const float v = fabsf(__half2float(reinterpret_cast<const __half*>(s.p[c])[i]));
-            const unsigned b = __float_as_uint(v);
-            m = b > m ? b : m;
+    const int c = blockIdx.y;
+    const uint16_t* p = s.p[c];
+    if (!p) return;
+    const long n = s.n[c];
+    const long n8 = n / 8;  // hipMalloc'ed arrays are 16-byte aligned
+    const uint4* v = reinterpret_cast<const uint4*>(p);
+    unsigned m = 0;
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n8; i += stride) {
+        const uint4 w = v[i];
+        const unsigned x[4] = {w.x & 0x7fff7fffu, w.y & 0x7fff7fffu, w.z & 0x7fff7fffu, w.w & 0x7fff7fffu};
+        for (int q = 0; q < 4; ++q) {
+            const unsigned lo = x[q] & 0xffffu, hi = x[q] >> 16;
+            m = lo > m ? lo : m;
+            m = hi > m ? hi : m;
         }
-        for (int off = 32; off > 0; off >>= 1) {
-            const unsigned o = __shfl_down(m, off);
-            m = o > m ? o : m;
-        }
-        if ((threadIdx.x & 63) == 0 && m) atomicMax(&maxbits[c], m);
+    }
+    for (long i = n8 * 8 + (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const unsigned b = p[i] & 0x7fffu;
+        m = b > m ? b : m;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned o = __shfl_down(m, off);
+        m = o > m ? o : m;
+    }
+    if ((threadIdx.x & 63) == 0 && m) {
+        // as the float bit pattern of |value| (what the plan reads)
+        const float f = __half2float(__ushort_as_half((unsigned short)m));
+        atomicMax(&maxbits[c], __float_as_uint(f));
     }
 }
 // Decide new scales: keep the largest stored magnitude near 2^8 (2^8 x
