@@ -50,6 +50,37 @@ __device__ __forceinline__ void st(T* p, long i, R v, R inv_scale) {
         p[i] = (T)v;
     }
 }
+// The compiler cannot see through this (no instruction is emitted): a value
+// that passes through it is not a constant, so operations on a variable that
+// is either such a default or a loaded value are not folded into the branch
+// that loads it (where they would wait for the load).
+template <typename X>
+__device__ __forceinline__ X opaque(X x) {
+    asm("" : "+v"(x));
+    return x;
+}
+// Register type of a raw storage value: FP16 bits are held zero-extended in
+// 32 bits (the load extends them; packing two into one register would need
+// the loaded value at once).
+template <typename T>
+struct Raw {
+    using type = T;
+};
+template <>
+struct Raw<uint16_t> {
+    using type = unsigned;
+};
+// Storage value to compute type (the conversion half of ld).
+template <typename T, typename R>
+__device__ __forceinline__ R cv(typename Raw<T>::type x, R scale) {
+    if constexpr (sizeof(T) == 2) {
+        return (R)__half2float(__ushort_as_half((unsigned short)x)) * scale;
+    } else {
+        (void)scale;
+        return (R)x;
+    }
+}
+
 // The value a store followed by a load would return (rounding to storage).
 template <typename T, typename R>
 __device__ __forceinline__ double stored_value(R v, R scale, R inv_scale) {
@@ -274,9 +305,35 @@ template <typename R>
 struct HIn {
     R ex, ex_j, ex_k, ey, ey_i, ey_k, ez, ez_i, ez_j, hx, hy, hz;
     R p[6];  // psi_H: Hx_y, Hx_z, Hy_z, Hy_x, Hz_x, Hz_y (where in a slab)
-    R cH[6]; // aHx bHx aHy bHy aHz bHz (when H coefficients vary per cell)
+    R cH[6]; // aHx bHx aHy bHy aHz bHz
     bool upd;
+    // axis tables at the cell: inverse spacings, and the CPML H rows
+    // (kappa, 1/kappa, bH, cH) at i, j and k where the cell is in that slab
+    R idx, idy, idz;
+    R kx[4], ky[4], kz[4];
 };
+
+// Axis-table values and H coefficients of a cell, read from `tb` (for callers
+// that hold the tables in LDS; h_gather reads them with the other inputs).
+// With cell-invariant H coefficients cH comes from tb.hco, else v.cH must
+// already hold the cell's row.
+template <typename T, typename R>
+__device__ __forceinline__ void h_tabs(const StepArgs<T, R>& a, const Tabs<R>& tb, int i, int j, int k, HIn<R>& v) {
+    const Axes<R>& ax = tb.ax;
+    v.idx = ax.X(0, i);
+    v.idy = ax.Y(0, j);
+    v.idz = ax.Z(0, k);
+    const bool c = a.cpml != 0;
+    const bool px = c && in_pml(i, a.nx, a.np), py = c && in_pml(j, a.ny, a.np), pz = c && in_pml(k, a.nz, a.np);
+    for (int q = 0; q < 4; ++q) v.kx[q] = v.ky[q] = v.kz[q] = 0;
+    if (px) { v.kx[0] = ax.X(1, i); v.kx[1] = ax.X(6, i); v.kx[2] = ax.X(4, i); v.kx[3] = ax.X(5, i); }
+    if (py) { v.ky[0] = ax.Y(1, j); v.ky[1] = ax.Y(6, j); v.ky[2] = ax.Y(4, j); v.ky[3] = ax.Y(5, j); }
+    if (pz) { v.kz[0] = ax.Z(1, k); v.kz[1] = ax.Z(6, k); v.kz[2] = ax.Z(4, k); v.kz[3] = ax.Z(5, k); }
+    if (a.h_uniform) {
+        const int nm = tb.hco == a.lut + 6 * a.nmat ? a.nmat : 1;
+        for (int q = 0; q < 6; ++q) v.cH[q] = tb.hco[q * nm];
+    }
+}
 
 // Per-cell H inputs other than the fields: H coefficients (when they vary
 // per cell) and psi_H on the slabs.
@@ -308,7 +365,7 @@ __device__ __forceinline__ void h_load_rest(const StepArgs<T, R>& a, int i, int 
 }
 
 template <typename T, typename R>
-__device__ __forceinline__ HIn<R> h_load(const StepArgs<T, R>& a, int i, int j, int k) {
+__device__ __forceinline__ HIn<R> h_load(const StepArgs<T, R>& a, const Tabs<R>& tb, int i, int j, int k) {
     HIn<R> v;
     const long id = ((long)i * a.ny + j) * a.nz + k;
     v.hx = ld<T, R>(a.Hx, id, a.sc[3]);
@@ -324,6 +381,68 @@ __device__ __forceinline__ HIn<R> h_load(const StepArgs<T, R>& a, int i, int j, 
     v.ey = ld<T, R>(a.Ey, id, a.sc[1]); v.ey_i = ld<T, R>(a.Ey, id + sI, a.sc[1]); v.ey_k = ld<T, R>(a.Ey, id + 1, a.sc[1]);
     v.ez = ld<T, R>(a.Ez, id, a.sc[2]); v.ez_i = ld<T, R>(a.Ez, id + sI, a.sc[2]); v.ez_j = ld<T, R>(a.Ez, id + sJ, a.sc[2]);
     h_load_rest<T, R>(a, i, j, k, id, v);
+    h_tabs<T, R>(a, tb, i, j, k, v);
+    return v;
+}
+
+// Inputs of the H update of cell (i, j, k), one the reference updates
+// (i < NxT-1, j < NyT-1, k < NzT-1), for a wavefront that covers 64
+// consecutive k of one (i, j) row (i, j and kw, the k of its first lane,
+// uniform across it).  As in e_gather, every input is requested before the
+// first one is used: the material id first (when H coefficients vary per
+// cell), then E^n and H^{n-1/2}, psi_H and the z-axis table, with clamped
+// indices under wave-uniform conditions and opaque defaults; X/Y axis values
+// and cell-invariant coefficients come through the scalar cache.
+template <typename T, typename R>
+__device__ __forceinline__ HIn<R> h_gather(const StepArgs<T, R>& a, const Tabs<R>& tb, int i, int j, int k, int kw) {
+    HIn<R> v;
+    const long id = ((long)i * a.ny + j) * a.nz + k;
+    const int np = a.np, nx = a.nx, ny = a.ny, nz = a.nz;
+    int m = 0;
+    if (!a.h_uniform) m = mat_of<R>(a.matid, a.mid16, id);
+    v.upd = true;
+    const long sI = a.sI, sJ = a.sJ;
+    v.hx = ld<T, R>(a.Hx, id, a.sc[3]);
+    v.hy = ld<T, R>(a.Hy, id, a.sc[4]);
+    v.hz = ld<T, R>(a.Hz, id, a.sc[5]);
+    v.ex = ld<T, R>(a.Ex, id, a.sc[0]); v.ex_j = ld<T, R>(a.Ex, id + sJ, a.sc[0]); v.ex_k = ld<T, R>(a.Ex, id + 1, a.sc[0]);
+    v.ey = ld<T, R>(a.Ey, id, a.sc[1]); v.ey_i = ld<T, R>(a.Ey, id + sI, a.sc[1]); v.ey_k = ld<T, R>(a.Ey, id + 1, a.sc[1]);
+    v.ez = ld<T, R>(a.Ez, id, a.sc[2]); v.ez_i = ld<T, R>(a.Ez, id + sI, a.sc[2]); v.ez_j = ld<T, R>(a.Ez, id + sJ, a.sc[2]);
+    using RawT = typename Raw<T>::type;
+    RawT praw[6];
+    for (int q = 0; q < 6; ++q) praw[q] = opaque((RawT)0);
+    const bool px = a.cpml && in_pml(i, nx, np), py = a.cpml && in_pml(j, ny, np);
+    const bool pzw = a.cpml && ((kw < np) | (kw + 63 >= nz - np));
+    if (py) {
+        const long q = psi_y(i, j, k, nx, ny, nz, np);
+        praw[0] = a.psi[6][q];
+        praw[5] = a.psi[11][q];
+    }
+    if (pzw) {
+        const long q = psi_z(i, j, in_pml(k, nz, np) ? k : 0, nx, ny, nz, np);
+        praw[1] = a.psi[7][q];
+        praw[2] = a.psi[8][q];
+    }
+    if (px) {
+        const long q = psi_x(i, j, k, nx, ny, nz, np);
+        praw[3] = a.psi[9][q];
+        praw[4] = a.psi[10][q];
+    }
+    const Axes<R>& ax = tb.ax;
+    v.idx = ax.X(0, i);
+    v.idy = ax.Y(0, j);
+    v.idz = ax.Z(0, k);
+    for (int q = 0; q < 4; ++q) v.kx[q] = v.ky[q] = 0;
+    if (px) { v.kx[0] = ax.X(1, i); v.kx[1] = ax.X(6, i); v.kx[2] = ax.X(4, i); v.kx[3] = ax.X(5, i); }
+    if (py) { v.ky[0] = ax.Y(1, j); v.ky[1] = ax.Y(6, j); v.ky[2] = ax.Y(4, j); v.ky[3] = ax.Y(5, j); }
+    for (int q = 0; q < 4; ++q) v.kz[q] = opaque((R)0);
+    if (pzw) { v.kz[0] = ax.Z(1, k); v.kz[1] = ax.Z(6, k); v.kz[2] = ax.Z(4, k); v.kz[3] = ax.Z(5, k); }
+    if (a.h_uniform) {
+        for (int q = 0; q < 6; ++q) v.cH[q] = a.lut[(6 + q) * a.nmat];
+    } else {
+        for (int q = 0; q < 6; ++q) v.cH[q] = a.lut[(6 + q) * a.nmat + m];
+    }
+    for (int q = 0; q < 6; ++q) v.p[q] = cv<T, R>(praw[q], a.sc[12 + q]);
     return v;
 }
 
@@ -336,22 +455,14 @@ __device__ __forceinline__ HIn<R> h_load(const StepArgs<T, R>& a, int i, int j, 
 template <typename T, typename R>
 __device__ __forceinline__ H3<R> h_compute(const StepArgs<T, R>& a, const Tabs<R>& tb, int i, int j, int k,
                                            const HIn<R>& v, bool own, bool store_fields = true) {
-    const Axes<R>& ax = tb.ax;
     if (!v.upd) return H3<R>{v.hx, v.hy, v.hz};
     const long id = ((long)i * a.ny + j) * a.nz + k;
-    const R idx = ax.X(0, i), idy = ax.Y(0, j), idz = ax.Z(0, k);
+    const R idx = v.idx, idy = v.idy, idz = v.idz;
     const R dEz_y = (v.ez_j - v.ez) * idy, dEy_z = (v.ey_k - v.ey) * idz;
     const R dEx_z = (v.ex_k - v.ex) * idz, dEz_x = (v.ez_i - v.ez) * idx;
     const R dEy_x = (v.ey_i - v.ey) * idx, dEx_y = (v.ex_j - v.ex) * idy;
     const R cx = dEz_y - dEy_z, cy = dEx_z - dEz_x, cz = dEy_x - dEx_y;
-    R aHx, bHx, aHy, bHy, aHz, bHz;
-    if (a.h_uniform) {  // cell-invariant
-        const int nm = tb.hco == a.lut + 6 * a.nmat ? a.nmat : 1;
-        aHx = tb.hco[0]; bHx = tb.hco[nm]; aHy = tb.hco[2 * nm];
-        bHy = tb.hco[3 * nm]; aHz = tb.hco[4 * nm]; bHz = tb.hco[5 * nm];
-    } else {
-        aHx = v.cH[0]; bHx = v.cH[1]; aHy = v.cH[2]; bHy = v.cH[3]; aHz = v.cH[4]; bHz = v.cH[5];
-    }
+    const R aHx = v.cH[0], bHx = v.cH[1], aHy = v.cH[2], bHy = v.cH[3], aHz = v.cH[4], bHz = v.cH[5];
     R hx = aHx * v.hx - bHx * cx;
     R hy = aHy * v.hy - bHy * cy;
     R hz = aHz * v.hz - bHz * cz;
@@ -360,12 +471,12 @@ __device__ __forceinline__ H3<R> h_compute(const StepArgs<T, R>& a, const Tabs<R
         const bool px = in_pml(i, nx, np), py = in_pml(j, ny, np), pz = in_pml(k, nz, np);
         if (px | py | pz) {
             R pn[6];
-            if (py) hx -= bHx * cpml_term<R>(v.p[0], dEz_y, ax.Y(1, j), ax.Y(6, j), ax.Y(4, j), ax.Y(5, j), pn[0]);
-            if (pz) hx -= bHx * (-cpml_term<R>(v.p[1], dEy_z, ax.Z(1, k), ax.Z(6, k), ax.Z(4, k), ax.Z(5, k), pn[1]));
-            if (pz) hy -= bHy * cpml_term<R>(v.p[2], dEx_z, ax.Z(1, k), ax.Z(6, k), ax.Z(4, k), ax.Z(5, k), pn[2]);
-            if (px) hy -= bHy * (-cpml_term<R>(v.p[3], dEz_x, ax.X(1, i), ax.X(6, i), ax.X(4, i), ax.X(5, i), pn[3]));
-            if (px) hz -= bHz * cpml_term<R>(v.p[4], dEy_x, ax.X(1, i), ax.X(6, i), ax.X(4, i), ax.X(5, i), pn[4]);
-            if (py) hz -= bHz * (-cpml_term<R>(v.p[5], dEx_y, ax.Y(1, j), ax.Y(6, j), ax.Y(4, j), ax.Y(5, j), pn[5]));
+            if (py) hx -= bHx * cpml_term<R>(v.p[0], dEz_y, v.ky[0], v.ky[1], v.ky[2], v.ky[3], pn[0]);
+            if (pz) hx -= bHx * (-cpml_term<R>(v.p[1], dEy_z, v.kz[0], v.kz[1], v.kz[2], v.kz[3], pn[1]));
+            if (pz) hy -= bHy * cpml_term<R>(v.p[2], dEx_z, v.kz[0], v.kz[1], v.kz[2], v.kz[3], pn[2]);
+            if (px) hy -= bHy * (-cpml_term<R>(v.p[3], dEz_x, v.kx[0], v.kx[1], v.kx[2], v.kx[3], pn[3]));
+            if (px) hz -= bHz * cpml_term<R>(v.p[4], dEy_x, v.kx[0], v.kx[1], v.kx[2], v.kx[3], pn[4]);
+            if (py) hz -= bHz * (-cpml_term<R>(v.p[5], dEx_y, v.ky[0], v.ky[1], v.ky[2], v.ky[3], pn[5]));
             if (own) {
                 if (py) {
                     const long q = psi_y(i, j, k, nx, ny, nz, np);
@@ -396,7 +507,7 @@ __device__ __forceinline__ H3<R> h_compute(const StepArgs<T, R>& a, const Tabs<R
 
 template <typename T, typename R>
 __device__ __forceinline__ H3<R> h_new(const StepArgs<T, R>& a, const Tabs<R>& tb, int i, int j, int k, bool own) {
-    return h_compute<T, R>(a, tb, i, j, k, h_load<T, R>(a, i, j, k), own);
+    return h_compute<T, R>(a, tb, i, j, k, h_load<T, R>(a, tb, i, j, k), own);
 }
 
 // The FP16 scales copied into registers at kernel start, before any store:
@@ -441,8 +552,9 @@ __global__ void __launch_bounds__(256) k_update_H(StepArgs<T, R> a) {
     const Tabs<R> tb = global_tabs(a);
     const Tile t = tile_of(a.nz, a.ny);
     const int i = t.i, j = t.j, k = t.k;
+    const int kw = __builtin_amdgcn_readfirstlane(k);  // lane 0: the tile's first k
     if (k >= a.nz - 1 || j >= a.ny - 1 || i >= a.nx - 1) return;
-    (void)h_new<T, R>(a, tb, i, j, k, true);
+    (void)h_compute<T, R>(a, tb, i, j, k, h_gather<T, R>(a, tb, i, j, k, kw), true);
 }
 
 // Two-level polarization step (update_polarization_multi_region,
@@ -532,26 +644,6 @@ __device__ __forceinline__ void e_axes(const Axes<R>& ax, int i, int j, int k, b
     if (pz) { v.kz[0] = ax.Z(1, k); v.kz[1] = ax.Z(6, k); v.kz[2] = ax.Z(2, k); v.kz[3] = ax.Z(3, k); }
 }
 
-// The compiler cannot see through this (no instruction is emitted): a value
-// that passes through it is not a constant, so operations on a variable that
-// is either such a default or a loaded value are not folded into the branch
-// that loads it (where they would wait for the load).
-template <typename X>
-__device__ __forceinline__ X opaque(X x) {
-    asm("" : "+v"(x));
-    return x;
-}
-// Storage value to compute type (the conversion half of ld).
-template <typename T, typename R>
-__device__ __forceinline__ R cv(T x, R scale) {
-    if constexpr (sizeof(T) == 2) {
-        return (R)__half2float(__ushort_as_half(x)) * scale;
-    } else {
-        (void)scale;
-        return (R)x;
-    }
-}
-
 // Inputs of the E update of cell (i, j, k) for a wavefront that covers 64
 // consecutive k of one (i, j) row: i, j and kw (the k of the wavefront's
 // first lane) are uniform across it.  Every input is requested
@@ -596,8 +688,9 @@ __device__ __forceinline__ EIn<R> e_gather(const StepArgs<T, R>& a, const Tabs<R
     v.ex0 = ld<T, R>(a.Ex, id, a.sc[0]);
     v.ey0 = ld<T, R>(a.Ey, id, a.sc[1]);
     // psi_E on the slabs
-    T praw[6];
-    for (int q = 0; q < 6; ++q) praw[q] = opaque((T)0);
+    using RawT = typename Raw<T>::type;
+    RawT praw[6];
+    for (int q = 0; q < 6; ++q) praw[q] = opaque((RawT)0);
     const bool px = a.cpml && in_pml(i, nx, np), py = a.cpml && in_pml(j, ny, np);
     const bool pzw = a.cpml && ((kw < np) | (kw + 63 >= nz - np));
     if (py) {
@@ -1056,6 +1149,7 @@ __global__ void __launch_bounds__(64 * ROW_WAVES) k_step_rows(StepArgs<T, R> a, 
             hv.hx = cur.hx; hv.hy = cur.hy; hv.hz = cur.hz;
             for (int q = 0; q < 6; ++q) { hv.p[q] = cur.ph[q]; hv.cH[q] = 0; }
             hv.upd = hup;
+            h_tabs<T, R>(s, tb, i, j, k, hv);
             const H3<R> h = h_compute<T, R>(s, tb, i, j, k, hv, kin);
             // Hx, Hz of the cell at j-1
             R gx = 0, gz = 0;
@@ -1069,6 +1163,7 @@ __global__ void __launch_bounds__(64 * ROW_WAVES) k_step_rows(StepArgs<T, R> a, 
                 gv.p[3] = 0; gv.p[4] = cur.qh[2]; gv.p[5] = cur.qh[3];
                 for (int q = 0; q < 6; ++q) gv.cH[q] = 0;
                 gv.upd = kin & (i < s.nx - 1) & (k < s.nz - 1);
+                h_tabs<T, R>(s, tb, i, j - 1, k, gv);
                 const H3<R> g = h_compute<T, R>(s, tb, i, j - 1, k, gv, false);
                 gx = g.x;
                 gz = g.z;
@@ -1354,7 +1449,8 @@ __global__ void __launch_bounds__(256) k_batch_step(const StepArgs<T, R>* __rest
             const int i1 = min(i0 + planes, a.nx);
             for (int i = i0; i < i1; ++i) {
                 if (!EPASS) {
-                    if (k < a.nz - 1 && j < a.ny - 1 && i < a.nx - 1) (void)h_new<T, R>(a, tb, i, j, k, true);
+                    if (k < a.nz - 1 && j < a.ny - 1 && i < a.nx - 1)
+                        (void)h_compute<T, R>(a, tb, i, j, k, h_gather<T, R>(a, tb, i, j, k, kt * 64), true);
                 } else if (k < a.nz && j < a.ny) {
                     e_cell<T, R>(a, tb, i, j, k, kt * 64);
                 }
