@@ -92,6 +92,7 @@ struct fdtd_sim {
     int cur = 0;
     int mode = 0;       // 0: H and E kernels (default), 1: fused step kernel
     int ichunk = 8;     // x-planes per work-group of the fused kernel
+    int kwid = 0;       // k-width of a wavefront's cells in the H and E kernels (16, 32, 64; 0: by precision)
     long psi_len[12] = {};
     void* inv[3] = {};
     void* cp[3] = {};
@@ -193,6 +194,8 @@ StepArgs<T, R> make_args(const fdtd_sim* s) {
     a.gPz = s->gPz; a.gPp = s->gPp; a.gNu = s->gNu; a.gNg = s->gNg;
     a.tls = s->d_tls;
     a.ntls = (int)s->h_tls.size();
+    a.tk = fdtdk::make_fastdiv((unsigned)((s->nz + 63) / 64));
+    a.tj = fdtdk::make_fastdiv((unsigned)((s->ny + fdtdk::TILE_ROWS - 1) / fdtdk::TILE_ROWS));
     return a;
 }
 
@@ -409,13 +412,24 @@ int launch_step(fdtd_sim* s, long n, int sidx, hipEvent_t* ev) {
         a = make_args<T, R>(s);
         a.srcv = s->d_srcv + (long)sidx * s->nsrc;
     } else {
-        const dim3 blk(64, 4, 1);
-        const dim3 grd((unsigned)(((s->nz + 63) / 64) * ((s->ny + 3) / 4) * s->nx));
+        const dim3 blk(64 * fdtdk::TILE_ROWS);
+        const dim3 grd(a.tk.d * a.tj.d * (unsigned)s->nx);
+        // narrow wavefronts pay off where instruction issue, not HBM, bounds
+        // the kernels (FP16 storage)
+        const int kwid = s->kwid ? s->kwid : (s->prec == FDTD_FP16 ? 32 : 64);
         if (ev) (void)hipEventRecord(ev[0], s->stream);
-        hipLaunchKernelGGL((fdtdk::k_update_H<T, R>), grd, blk, 0, s->stream, a);
+        switch (kwid) {
+        case 64: hipLaunchKernelGGL((fdtdk::k_update_H<T, R, 64>), grd, blk, 0, s->stream, a); break;
+        case 32: hipLaunchKernelGGL((fdtdk::k_update_H<T, R, 32>), grd, blk, 0, s->stream, a); break;
+        default: hipLaunchKernelGGL((fdtdk::k_update_H<T, R, 16>), grd, blk, 0, s->stream, a); break;
+        }
         if (ev) (void)hipEventRecord(ev[1], s->stream);
         if (ev) (void)hipEventRecord(ev[2], s->stream);
-        hipLaunchKernelGGL((fdtdk::k_update_E<T, R>), grd, blk, 0, s->stream, a);
+        switch (kwid) {
+        case 64: hipLaunchKernelGGL((fdtdk::k_update_E<T, R, 64>), grd, blk, 0, s->stream, a); break;
+        case 32: hipLaunchKernelGGL((fdtdk::k_update_E<T, R, 32>), grd, blk, 0, s->stream, a); break;
+        default: hipLaunchKernelGGL((fdtdk::k_update_E<T, R, 16>), grd, blk, 0, s->stream, a); break;
+        }
         if (ev) (void)hipEventRecord(ev[3], s->stream);
     }
     // probes: PointFieldDetector records when n % save_every == 0
@@ -772,12 +786,14 @@ int fdtd_get_fp16_scales(fdtd_handle h, double* scales18) {
     return FDTD_OK;
 }
 
-int fdtd_set_kernel_mode(fdtd_handle h, int mode, int planes_per_chunk) {
+int fdtd_set_kernel_mode(fdtd_handle h, int mode, int param) {
     if (!h || mode < 0 || mode > 1) return fail(FDTD_EINVAL, "mode must be 0 (H and E kernels) or 1 (fused step kernel)");
-    if (planes_per_chunk < 0 || planes_per_chunk > fdtdk::ROW_MAX_CHUNK)
-        return fail(FDTD_EINVAL, "planes_per_chunk must be in [0, 128]");
+    if (mode == 0 && param != 0 && param != 16 && param != 32 && param != 64)
+        return fail(FDTD_EINVAL, "mode 0: the wavefront k-width must be 16, 32 or 64 (0 keeps it)");
+    if (mode == 1 && (param < 0 || param > fdtdk::ROW_MAX_CHUNK))
+        return fail(FDTD_EINVAL, "mode 1: planes_per_chunk must be in [0, 128]");
     h->mode = mode;
-    if (planes_per_chunk) h->ichunk = planes_per_chunk;
+    if (param) (mode == 0 ? h->kwid : h->ichunk) = param;
     return FDTD_OK;
 }
 

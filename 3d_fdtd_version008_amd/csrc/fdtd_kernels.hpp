@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 #include <hip/hip_fp16.h>
 #include <stdint.h>
+#include "fdtd_fastdiv.hpp"
 
 namespace fdtdk {
 
@@ -138,6 +139,9 @@ struct StepArgs {
     double *gPz, *gPp, *gNu, *gNg;
     const TLSCoef* tls;
     int ntls;
+    // tiles of the two-kernel grid: k-tiles of 64 cells and j-tiles of
+    // TILE_ROWS rows, as divisors of the tile index
+    FastDiv tk, tj;
 };
 
 __device__ __forceinline__ bool in_pml(int i, int n, int np) {
@@ -386,15 +390,17 @@ __device__ __forceinline__ HIn<R> h_load(const StepArgs<T, R>& a, const Tabs<R>&
 }
 
 // Inputs of the H update of cell (i, j, k), one the reference updates
-// (i < NxT-1, j < NyT-1, k < NzT-1), for a wavefront that covers 64
-// consecutive k of one (i, j) row (i, j and kw, the k of its first lane,
-// uniform across it).  As in e_gather, every input is requested before the
+// (i < NxT-1, j < NyT-1, k < NzT-1), for a wavefront that covers KW
+// consecutive k of 64/KW consecutive rows of plane i (i, kw and jw, the k and
+// j of its first lane, uniform across it).  As in e_gather, every input is requested before the
 // first one is used: the material id first (when H coefficients vary per
 // cell), then E^n and H^{n-1/2}, psi_H and the z-axis table, with clamped
 // indices under wave-uniform conditions and opaque defaults; X/Y axis values
 // and cell-invariant coefficients come through the scalar cache.
-template <typename T, typename R>
-__device__ __forceinline__ HIn<R> h_gather(const StepArgs<T, R>& a, const Tabs<R>& tb, int i, int j, int k, int kw) {
+template <typename T, typename R, int KW = 64, bool CP = true>
+__device__ __forceinline__ HIn<R> h_gather(const StepArgs<T, R>& a, const Tabs<R>& tb, int i, int j, int k, int kw,
+                                           int jw) {
+    constexpr int RW = 64 / KW;  // rows per wavefront
     HIn<R> v;
     const long id = ((long)i * a.ny + j) * a.nz + k;
     const int np = a.np, nx = a.nx, ny = a.ny, nz = a.nz;
@@ -411,10 +417,11 @@ __device__ __forceinline__ HIn<R> h_gather(const StepArgs<T, R>& a, const Tabs<R
     using RawT = typename Raw<T>::type;
     RawT praw[6];
     for (int q = 0; q < 6; ++q) praw[q] = opaque((RawT)0);
-    const bool px = a.cpml && in_pml(i, nx, np), py = a.cpml && in_pml(j, ny, np);
-    const bool pzw = a.cpml && ((kw < np) | (kw + 63 >= nz - np));
-    if (py) {
-        const long q = psi_y(i, j, k, nx, ny, nz, np);
+    const bool px = CP && a.cpml && in_pml(i, nx, np);
+    const bool pyw = CP && a.cpml && (in_pml(jw, ny, np) | in_pml(jw + RW - 1, ny, np));
+    const bool pzw = CP && a.cpml && ((kw < np) | (kw + KW - 1 >= nz - np));
+    if (pyw) {
+        const long q = psi_y(i, in_pml(j, ny, np) ? j : 0, k, nx, ny, nz, np);
         praw[0] = a.psi[6][q];
         praw[5] = a.psi[11][q];
     }
@@ -432,9 +439,10 @@ __device__ __forceinline__ HIn<R> h_gather(const StepArgs<T, R>& a, const Tabs<R
     v.idx = ax.X(0, i);
     v.idy = ax.Y(0, j);
     v.idz = ax.Z(0, k);
-    for (int q = 0; q < 4; ++q) v.kx[q] = v.ky[q] = 0;
+    for (int q = 0; q < 4; ++q) v.kx[q] = 0;
     if (px) { v.kx[0] = ax.X(1, i); v.kx[1] = ax.X(6, i); v.kx[2] = ax.X(4, i); v.kx[3] = ax.X(5, i); }
-    if (py) { v.ky[0] = ax.Y(1, j); v.ky[1] = ax.Y(6, j); v.ky[2] = ax.Y(4, j); v.ky[3] = ax.Y(5, j); }
+    for (int q = 0; q < 4; ++q) v.ky[q] = opaque((R)0);
+    if (pyw) { v.ky[0] = ax.Y(1, j); v.ky[1] = ax.Y(6, j); v.ky[2] = ax.Y(4, j); v.ky[3] = ax.Y(5, j); }
     for (int q = 0; q < 4; ++q) v.kz[q] = opaque((R)0);
     if (pzw) { v.kz[0] = ax.Z(1, k); v.kz[1] = ax.Z(6, k); v.kz[2] = ax.Z(4, k); v.kz[3] = ax.Z(5, k); }
     if (a.h_uniform) {
@@ -452,7 +460,7 @@ __device__ __forceinline__ HIn<R> h_gather(const StepArgs<T, R>& a, const Tabs<R
 // `own` (halo recomputation by a neighbouring tile must not store).  Cells the
 // reference never updates (i = NxT-1, j = NyT-1 or k = NzT-1) return their
 // stored value.  Returns the values a later read of the stored H gives.
-template <typename T, typename R>
+template <typename T, typename R, bool CP = true>
 __device__ __forceinline__ H3<R> h_compute(const StepArgs<T, R>& a, const Tabs<R>& tb, int i, int j, int k,
                                            const HIn<R>& v, bool own, bool store_fields = true) {
     if (!v.upd) return H3<R>{v.hx, v.hy, v.hz};
@@ -466,7 +474,7 @@ __device__ __forceinline__ H3<R> h_compute(const StepArgs<T, R>& a, const Tabs<R
     R hx = aHx * v.hx - bHx * cx;
     R hy = aHy * v.hy - bHy * cy;
     R hz = aHz * v.hz - bHz * cz;
-    if (a.cpml) {
+    if (CP && a.cpml) {
         const int np = a.np, nx = a.nx, ny = a.ny, nz = a.nz;
         const bool px = in_pml(i, nx, np), py = in_pml(j, ny, np), pz = in_pml(k, nz, np);
         if (px | py | pz) {
@@ -521,40 +529,50 @@ __device__ __forceinline__ H3<R> h_new(const StepArgs<T, R>& a, const Tabs<R>& t
     }
 
 // Tile of a two-kernel launch.  The grid is 1-D over (k-block, j-block,
-// plane) tiles.  Work-group ids are dealt round-robin to the 8 XCDs, each
-// with its own L2; the remap hands every XCD one contiguous range of tiles
-// (whole runs of planes), so the neighbouring rows and the previous plane a
-// tile reads were loaded recently by the same XCD.
+// plane) tiles of 64 x TILE_ROWS x 1 cells, one 256-thread work-group each.
+// Work-group ids are dealt round-robin to the 8 XCDs, each with its own L2;
+// the remap hands every XCD one contiguous range of tiles (whole runs of
+// planes), so the neighbouring rows and the previous plane a tile reads were
+// loaded recently by the same XCD.  The tile index is split with precomputed
+// divisors (a.tk, a.tj).  Inside the tile each wavefront covers KW
+// consecutive k of 64/KW consecutive rows; (kw, jw) is its first cell.
+constexpr int TILE_ROWS = 4;
 struct Tile {
-    int i, j, k;
+    int i, j, k, kw, jw;
 };
-__device__ __forceinline__ Tile tile_of(int nz, int ny) {
+template <int KW, typename T, typename R>
+__device__ __forceinline__ Tile tile_of(const StepArgs<T, R>& a) {
+    static_assert(KW == 16 || KW == 32 || KW == 64, "wavefront k-width");
+    constexpr int G = 64 / KW;  // k-groups per tile = rows per wavefront
     const unsigned nb = gridDim.x, b = blockIdx.x;
     const unsigned per = (nb + 7) / 8;
     unsigned bb = (b % 8) * per + b / 8;
     if (nb % 8 != 0) bb = b;  // the remap is a permutation only for multiples of 8
-    const int nkt = (nz + 63) >> 6, njt = (ny + (int)blockDim.y - 1) / (int)blockDim.y;
-    const int kt = (int)(bb % nkt);
-    const int r = (int)(bb / nkt);
+    const unsigned r = fdiv(bb, a.tk);
+    const unsigned kt = bb - r * a.tk.d;
+    const unsigned i = fdiv(r, a.tj);
+    const unsigned jt = r - i * a.tj.d;
+    const int w = (int)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wavefront in the work-group
+    const int l = (int)(threadIdx.x & 63);
     Tile t;
-    t.k = kt * 64 + (int)threadIdx.x;
-    // blockDim.x is one wavefront, so the row is uniform across it
-    t.j = (r % njt) * (int)blockDim.y + __builtin_amdgcn_readfirstlane(threadIdx.y);
-    t.i = r / njt;
+    t.kw = (int)kt * 64 + (w % G) * KW;
+    t.jw = (int)jt * TILE_ROWS + (w / G) * G;
+    t.k = t.kw + l % KW;
+    t.j = t.jw + l / KW;
+    t.i = (int)i;
     return t;
 }
 
-// H update kernel of the two-kernel step.  Threads cover the grid with
-// blockDim = (64, BY); each thread owns one cell (tile_of).
-template <typename T, typename R>
+// H update kernel of the two-kernel step: one 256-thread work-group per
+// tile, each thread one cell (tile_of).
+template <typename T, typename R, int KW>
 __global__ void __launch_bounds__(256) k_update_H(StepArgs<T, R> a) {
     FDTD_SCALES_IN_REGISTERS(a)
     const Tabs<R> tb = global_tabs(a);
-    const Tile t = tile_of(a.nz, a.ny);
+    const Tile t = tile_of<KW>(a);
     const int i = t.i, j = t.j, k = t.k;
-    const int kw = __builtin_amdgcn_readfirstlane(k);  // lane 0: the tile's first k
     if (k >= a.nz - 1 || j >= a.ny - 1 || i >= a.nx - 1) return;
-    (void)h_compute<T, R>(a, tb, i, j, k, h_gather<T, R>(a, tb, i, j, k, kw), true);
+    (void)h_compute<T, R>(a, tb, i, j, k, h_gather<T, R, KW>(a, tb, i, j, k, t.kw, t.jw), true);
 }
 
 // Two-level polarization step (update_polarization_multi_region,
@@ -644,9 +662,9 @@ __device__ __forceinline__ void e_axes(const Axes<R>& ax, int i, int j, int k, b
     if (pz) { v.kz[0] = ax.Z(1, k); v.kz[1] = ax.Z(6, k); v.kz[2] = ax.Z(2, k); v.kz[3] = ax.Z(3, k); }
 }
 
-// Inputs of the E update of cell (i, j, k) for a wavefront that covers 64
-// consecutive k of one (i, j) row: i, j and kw (the k of the wavefront's
-// first lane) are uniform across it.  Every input is requested
+// Inputs of the E update of cell (i, j, k) for a wavefront that covers KW
+// consecutive k of 64/KW consecutive rows of plane i: i, kw and jw (the k
+// and j of the wavefront's first lane) are uniform across it.  Every input is requested
 // before the first one is used, so a wavefront makes one trip to memory plus
 // one to the caches: the material and region ids first (the gathers after
 // them wait for these alone; loads complete in issue order), then H^{n+1/2}
@@ -656,23 +674,27 @@ __device__ __forceinline__ void e_axes(const Axes<R>& ax, int i, int j, int k, b
 // outside the slab or the gain box.  Values loaded under a condition start
 // from opaque defaults and are converted after the last load is issued, so
 // no arithmetic on them is placed (and waits) inside the branch.  X/Y axis
-// values, source footprints and the coefficients of a single TLS region are
-// uniform and come through the scalar cache.
+// values (Y too when a wavefront holds one row), source footprints and the
+// coefficients of a single TLS region are uniform and come through the scalar
+// cache.  Narrow wavefronts (KW < 64) confine the CPML z slabs and a z-normal
+// gain slab to fewer wavefronts: a wavefront pays for the slab code when any
+// of its lanes is in the slab.
 // Returns H^{n+1/2} at the cell and at its -x, -y, -z neighbours in h[9]:
 // hx hy hz hx_j hz_j hx_k hy_k hy_i hz_i.
-template <typename T, typename R>
+template <typename T, typename R, int KW = 64, bool CP = true, bool GN = true>
 __device__ __forceinline__ EIn<R> e_gather(const StepArgs<T, R>& a, const Tabs<R>& tb, int i, int j, int k, int kw,
-                                           R* h) {
+                                           int jw, R* h) {
+    constexpr int RW = 64 / KW;  // rows per wavefront
     EIn<R> v;
     const long id = ((long)i * a.ny + j) * a.nz + k;
     const int np = a.np, nx = a.nx, ny = a.ny, nz = a.nz;
     // ids
     const int m = mat_of<R>(a.matid, a.mid16, id);
     const int gi = i - a.g0x, gj = j - a.g0y, gk = k - a.g0z;
-    const bool growu = a.gain && ((unsigned)gi < (unsigned)a.gnx) & ((unsigned)gj < (unsigned)a.gny) &
-                       (kw < a.g0z + a.gnz) & (kw + 63 >= a.g0z);
-    const bool gin = growu && (unsigned)gk < (unsigned)a.gnz;
-    const long g = ((long)gi * a.gny + gj) * a.gnz + (gin ? gk : 0);
+    const bool growu = GN && a.gain && ((unsigned)gi < (unsigned)a.gnx) & (jw < a.g0y + a.gny) & (jw + RW - 1 >= a.g0y) &
+                       (kw < a.g0z + a.gnz) & (kw + KW - 1 >= a.g0z);
+    const bool gin = growu && ((unsigned)gj < (unsigned)a.gny) & ((unsigned)gk < (unsigned)a.gnz);
+    const long g = ((long)gi * a.gny + (gin ? gj : 0)) * a.gnz + (gin ? gk : 0);
     int r8 = opaque(-1);
     if (growu) r8 = a.greg[g];
     // fields.  On the rows i == 0 or j == 0, which the reference does not
@@ -691,10 +713,11 @@ __device__ __forceinline__ EIn<R> e_gather(const StepArgs<T, R>& a, const Tabs<R
     using RawT = typename Raw<T>::type;
     RawT praw[6];
     for (int q = 0; q < 6; ++q) praw[q] = opaque((RawT)0);
-    const bool px = a.cpml && in_pml(i, nx, np), py = a.cpml && in_pml(j, ny, np);
-    const bool pzw = a.cpml && ((kw < np) | (kw + 63 >= nz - np));
-    if (py) {
-        const long q = psi_y(i, j, k, nx, ny, nz, np);
+    const bool px = CP && a.cpml && in_pml(i, nx, np);
+    const bool pyw = CP && a.cpml && (in_pml(jw, ny, np) | in_pml(jw + RW - 1, ny, np));
+    const bool pzw = CP && a.cpml && ((kw < np) | (kw + KW - 1 >= nz - np));
+    if (pyw) {
+        const long q = psi_y(i, in_pml(j, ny, np) ? j : 0, k, nx, ny, nz, np);
         praw[0] = a.psi[0][q];
         praw[5] = a.psi[5][q];
     }
@@ -717,13 +740,14 @@ __device__ __forceinline__ EIn<R> e_gather(const StepArgs<T, R>& a, const Tabs<R
     v.idx = ax.X(0, i);
     v.idy = ax.Y(0, j);
     v.idz = ax.Z(0, k);
-    for (int q = 0; q < 4; ++q) v.kx[q] = v.ky[q] = 0;
+    for (int q = 0; q < 4; ++q) v.kx[q] = 0;
     if (px) { v.kx[0] = ax.X(1, i); v.kx[1] = ax.X(6, i); v.kx[2] = ax.X(2, i); v.kx[3] = ax.X(3, i); }
-    if (py) { v.ky[0] = ax.Y(1, j); v.ky[1] = ax.Y(6, j); v.ky[2] = ax.Y(2, j); v.ky[3] = ax.Y(3, j); }
+    for (int q = 0; q < 4; ++q) v.ky[q] = opaque((R)0);
+    if (pyw) { v.ky[0] = ax.Y(1, j); v.ky[1] = ax.Y(6, j); v.ky[2] = ax.Y(2, j); v.ky[3] = ax.Y(3, j); }
     for (int q = 0; q < 4; ++q) v.kz[q] = opaque((R)0);
     if (pzw) { v.kz[0] = ax.Z(1, k); v.kz[1] = ax.Z(6, k); v.kz[2] = ax.Z(2, k); v.kz[3] = ax.Z(3, k); }
     e_sources(tb, a.nsrc, i, j, k, v.jx, v.jy, v.jz);
-    if (a.gain && a.ntls == 1) v.tc = tb.tls[0];
+    if (GN && a.gain && a.ntls == 1) v.tc = tb.tls[0];
     // the gathers that wait for the ids
     v.cE[1] = a.lut[1 * a.nmat + m];
     v.cE[3] = a.lut[3 * a.nmat + m];
@@ -734,7 +758,7 @@ __device__ __forceinline__ EIn<R> e_gather(const StepArgs<T, R>& a, const Tabs<R
         v.cE[2] = a.lut[2 * a.nmat + m];
         v.cE[4] = a.lut[4 * a.nmat + m];
     }
-    if (a.gain && a.ntls != 1) v.tc = tb.tls[r8 < 0 ? 0 : r8];
+    if (GN && a.gain && a.ntls != 1) v.tc = tb.tls[r8 < 0 ? 0 : r8];
     v.reg = gin ? r8 : -1;
     v.g = g;
     v.Nu = gs[0]; v.Ng = gs[1]; v.Pz = gs[2]; v.Pp = gs[3];
@@ -759,7 +783,7 @@ __device__ __forceinline__ EIn<R> e_gather(const StepArgs<T, R>& a, const Tabs<R
 // for a cell the reference does not update, v.ex0/ey0/ez0 (so a caller that
 // stores fields itself, store_fields = false, passes the loaded values
 // there).
-template <typename T, typename R>
+template <typename T, typename R, bool CP = true>
 __device__ __forceinline__ H3<R> e_compute(const StepArgs<T, R>& a, const Tabs<R>& tb, int i, int j, int k,
                                            const EIn<R>& v, R hx, R hy, R hz, R hx_j, R hz_j, R hx_k, R hy_k, R hy_i,
                                            R hz_i, bool store_fields = true) {
@@ -811,7 +835,7 @@ __device__ __forceinline__ H3<R> e_compute(const StepArgs<T, R>& a, const Tabs<R
         ey = v.cE[2] * v.ey0 + bEy * (cy - jy);
         ez = v.cE[4] * v.ez0 + bEz * rz;
     }
-    if (a.cpml) {
+    if (CP && a.cpml) {
         const int np = a.np, nx = a.nx, ny = a.ny, nz = a.nz;
         const bool px = in_pml(i, nx, np), py = in_pml(j, ny, np), pz = in_pml(k, nz, np);
         if (px | py | pz) {
@@ -855,25 +879,26 @@ __device__ __forceinline__ H3<R> e_compute(const StepArgs<T, R>& a, const Tabs<R
     return H3<R>{ex, ey, ez};
 }
 
-// E update of one cell (i, j and kw uniform across the wavefront, as for
-// e_gather).
-template <typename T, typename R>
-__device__ __forceinline__ void e_cell(const StepArgs<T, R>& a, const Tabs<R>& tb, int i, int j, int k, int kw) {
+// E update of one cell (i, kw and jw uniform across the wavefront, as for
+// e_gather).  CP and GN compile the CPML and gain code in (the default) or out.
+template <typename T, typename R, int KW = 64, bool CP = true, bool GN = true>
+__device__ __forceinline__ void e_cell(const StepArgs<T, R>& a, const Tabs<R>& tb, int i, int j, int k, int kw, int jw) {
     R h[9];
-    const EIn<R> v = e_gather<T, R>(a, tb, i, j, k, kw, h);
-    e_compute<T, R>(a, tb, i, j, k, v, h[0], h[1], h[2], h[3], h[4], h[5], h[6], h[7], h[8]);
+    const EIn<R> v = e_gather<T, R, KW, CP, GN>(a, tb, i, j, k, kw, jw, h);
+    e_compute<T, R, CP>(a, tb, i, j, k, v, h[0], h[1], h[2], h[3], h[4], h[5], h[6], h[7], h[8]);
 }
 
+
+
 // E update kernel of the two-kernel step: H^{n+1/2} neighbours from memory.
-template <typename T, typename R>
+template <typename T, typename R, int KW>
 __global__ void __launch_bounds__(256) k_update_E(StepArgs<T, R> a) {
     FDTD_SCALES_IN_REGISTERS(a)
     const Tabs<R> tb = global_tabs(a);
-    const Tile t = tile_of(a.nz, a.ny);
+    const Tile t = tile_of<KW>(a);
     const int i = t.i, j = t.j, k = t.k;
-    const int kw = __builtin_amdgcn_readfirstlane(k);  // lane 0: the tile's first k
     if (k >= a.nz || j >= a.ny || i >= a.nx) return;
-    e_cell<T, R>(a, tb, i, j, k, kw);
+    e_cell<T, R, KW>(a, tb, i, j, k, t.kw, t.jw);
 }
 
 // Row-marching step kernel: one launch advances H and E of the whole grid,
@@ -1450,9 +1475,9 @@ __global__ void __launch_bounds__(256) k_batch_step(const StepArgs<T, R>* __rest
             for (int i = i0; i < i1; ++i) {
                 if (!EPASS) {
                     if (k < a.nz - 1 && j < a.ny - 1 && i < a.nx - 1)
-                        (void)h_compute<T, R>(a, tb, i, j, k, h_gather<T, R>(a, tb, i, j, k, kt * 64), true);
+                        (void)h_compute<T, R>(a, tb, i, j, k, h_gather<T, R>(a, tb, i, j, k, kt * 64, j), true);
                 } else if (k < a.nz && j < a.ny) {
-                    e_cell<T, R>(a, tb, i, j, k, kt * 64);
+                    e_cell<T, R>(a, tb, i, j, k, kt * 64, j);
                 }
             }
         }

@@ -236,9 +236,10 @@ class Scene:
         _check(self.lib.fdtd_scene_step_timed(self.sc, nsteps, ms))
         return tuple(ms)
 
-    def set_kernel_mode(self, mode: int, planes_per_chunk: int = 0):
-        """0: separate H/E kernels; 1: fused step kernel (x-planes per work-group)."""
-        _check(self.lib.fdtd_set_kernel_mode(self.h, mode, planes_per_chunk))
+    def set_kernel_mode(self, mode: int, param: int = 0):
+        """0: separate H/E kernels, `param` = wavefront k-width (16, 32, 64;
+        0 keeps it); 1: fused step kernel, `param` = x-planes per work-group."""
+        _check(self.lib.fdtd_set_kernel_mode(self.h, mode, param))
 
     def synchronize(self):
         _check(self.lib.fdtd_synchronize(self.h))

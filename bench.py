@@ -104,7 +104,7 @@ def plain(args):
     t_setup = time.time() - t0
     sc = pkg.Scene(setup, args.precision, probe_capacity=args.steps + args.warmup + 16)
     info = sc.info()
-    sc.set_kernel_mode(args.mode, args.ichunk)
+    sc.set_kernel_mode(args.mode, args.kwidth if args.mode == 0 else args.ichunk)
     sc.step(args.warmup)
     sc.synchronize()
     t1 = time.perf_counter()
@@ -150,7 +150,8 @@ def plain(args):
         "config": {"workload": spec.name, "grid": list(setup.shape), "cells": N, "npml": setup.npml,
                    "precision": args.precision, "gain_cells": gain_cells, "materials": info["n_materials"],
                    "mat_id_bytes": info["mat_id_bytes"], "setup_s": round(t_setup, 2),
-                   "kernel_mode": ["h_e_kernels", "fused_rows"][args.mode], "planes_per_chunk": args.ichunk},
+                   "kernel_mode": ["h_e_kernels", "fused_rows"][args.mode], "planes_per_chunk": args.ichunk,
+                   "wavefront_kwidth": (args.kwidth or (32 if args.precision == "fp16" else 64)) if args.mode == 0 else None},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes_per_launch": kbytes, "avg_launch_us": round(kt * 1e6, 2)},
@@ -194,7 +195,7 @@ def precision_form(args):
     rates, scenes = {}, {}
     for prec in ("fp64", "fp32", "fp16"):
         sc = pkg.Scene(setup, prec, probe_capacity=args.steps + 16)
-        sc.set_kernel_mode(args.mode, args.ichunk)
+        sc.set_kernel_mode(args.mode, args.kwidth if args.mode == 0 else args.ichunk)
         done = 0
         if prec != "fp64":
             sc.step(args.warmup)
@@ -323,6 +324,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mode", type=int, default=0, help="0: separate H/E kernels, 1: fused step kernel")
     ap.add_argument("--ichunk", type=int, default=0, help="fused kernel x-planes per work-group; 0 = library default")
+    ap.add_argument("--kwidth", type=int, default=0, help="H/E kernels: wavefront k-width (16, 32, 64); 0 = library default")
     ap.add_argument("--rate-steps", type=int, default=200, help="precision form: timed steps per format")
     ap.add_argument("--domains", type=int, default=256, help="batch form: domains")
     ap.add_argument("--requests", type=int, default=64, help="serve form: requests")

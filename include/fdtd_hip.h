@@ -147,14 +147,18 @@ int fdtd_add_dft(fdtd_handle h, int component, const int box[6], int n_freqs, co
  * reference (FP64 only). */
 int fdtd_set_fp16_scales(fdtd_handle h, const double* scales18);
 int fdtd_get_fp16_scales(fdtd_handle h, double* scales18);
-/* Kernel schedule: mode 0 = separate H and E kernels (in place, default);
+/* Kernel schedule: mode 0 = separate H and E kernels (in place, default),
+ * `param` = the k-width of the cells one wavefront updates (16, 32 or 64
+ * consecutive k of 64/param rows; by default 32 for FP16 storage and 64
+ * otherwise; 0 keeps the current value);
  * mode 1 = one fused step kernel in which each wavefront owns one j-row and
- * marches along x over planes_per_chunk planes (1..128; 0 keeps the current
- * value), double-buffering E, H and psi_H.  Grids the fused kernel does not
- * take (per-cell H coefficients, more than 16 sources or TLS regions, NzT >
- * 1024, field arrays of 4 GiB or more) run mode 0.  Results are identical;
- * only the memory traffic differs.  No reference counterpart. */
-int fdtd_set_kernel_mode(fdtd_handle h, int mode, int planes_per_chunk);
+ * marches along x over `param` planes (1..128; 0 keeps the current value),
+ * double-buffering E, H and psi_H.  Grids the fused kernel does not take
+ * (per-cell H coefficients, more than 16 sources or TLS regions, NzT > 1024,
+ * field arrays of 4 GiB or more) run mode 0.  Results are identical; only
+ * the instruction count and memory traffic differ.  No reference
+ * counterpart. */
+int fdtd_set_kernel_mode(fdtd_handle h, int mode, int param);
 /* Stability monitor cadence (STABILITY_MONITOR_INTERVAL, default 10; 0 = off). */
 int fdtd_set_monitor(fdtd_handle h, int interval);
 

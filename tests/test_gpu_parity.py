@@ -76,7 +76,10 @@ def _compare(setup, sc, ref, tol, extra=(), vector=False):
     assert not bad, f"exceeds {tol}: {bad}"
 
 
-MODES = [(0, 0), (1, 16), (1, 7), (1, 128)]
+# (0, w): H and E kernels with wavefronts of w consecutive k; (1, c): fused
+# step kernel with c x-planes per work-group
+MODES = [(0, 16), (0, 32), (0, 64), (1, 16), (1, 7), (1, 128)]
+MODES2 = [(0, 16), (1, 16)]
 
 
 @pytest.mark.parametrize("mode", MODES, ids=lambda m: f"mode{m[0]}_c{m[1]}")
@@ -90,7 +93,7 @@ def test_uniform_sphere_fp64_full(pkg, oracle_mod, mode):
     assert len(sc.probe(0)) == sp.n_steps
 
 
-@pytest.mark.parametrize("mode", MODES[:2], ids=lambda m: f"mode{m[0]}")
+@pytest.mark.parametrize("mode", MODES2, ids=lambda m: f"mode{m[0]}")
 def test_psi_state_fp64(pkg, oracle_mod, mode):
     sp = pkg.scene.uniform_sphere_case(core=16, npml=6, steps=150)
     setup, sc, ref = _run_pair(pkg, oracle_mod, sp, "fp64", mode=mode)
@@ -112,14 +115,14 @@ def test_gain_cuboids_nonuniform_fp64(pkg, oracle_mod):
     _compare(setup, sc, ref, 1e-12, extra=("Nu", "Ng", "Pz"))
 
 
-@pytest.mark.parametrize("mode", MODES[:2], ids=lambda m: f"mode{m[0]}")
+@pytest.mark.parametrize("mode", MODES2, ids=lambda m: f"mode{m[0]}")
 def test_fp32_uniform(pkg, oracle_mod, mode):
     sp = pkg.scene.uniform_sphere_case(core=32, npml=8, steps=400)
     setup, sc, ref = _run_pair(pkg, oracle_mod, sp, "fp32", mode=mode)
     _compare(setup, sc, ref, 2e-4, vector=True)
 
 
-@pytest.mark.parametrize("mode", MODES[:2], ids=lambda m: f"mode{m[0]}")
+@pytest.mark.parametrize("mode", MODES2, ids=lambda m: f"mode{m[0]}")
 def test_fp32_gain(pkg, oracle_mod, mode):
     sp = pkg.scene.gain_slab_case(n_total=40, npml=6, steps=400, fine_len=100e-9, coarse=20e-9, fine=5e-9)
     setup, sc, ref = _run_pair(pkg, oracle_mod, sp, "fp32", mode=mode)
@@ -129,20 +132,21 @@ def test_fp32_gain(pkg, oracle_mod, mode):
 
 @pytest.mark.parametrize("precision", ["fp32", "fp16"])
 def test_step_kernels_agree_bitwise(pkg, precision):
-    """The two-kernel step and the fused step kernel run the same operations
-    per value, so their results are identical in every storage format."""
+    """The two-kernel step (at every wavefront k-width) and the fused step
+    kernel run the same operations per value, so their results are identical
+    in every storage format."""
     sp = pkg.scene.gain_slab_case(n_total=44, npml=6, steps=120, fine_len=100e-9, coarse=20e-9, fine=5e-9,
                                   n_cuboids=4, seed=5)
     setup = pkg.Setup(sp)
     runs = {}
-    for mode in [(0, 0), (1, 16), (1, 5)]:
+    for mode in [(0, 16), (0, 32), (0, 64), (1, 16), (1, 5)]:
         _say(f"[case] {sp.name} {precision} mode={mode}")
         sc = pkg.Scene(setup, precision)
         sc.set_kernel_mode(*mode)
         sc.step(sp.n_steps)
         runs[mode] = {n: sc.field(n) for n in FIELDS + ("Nu", "Pz") + tuple(pkg.runtime.PSI_NAMES)}
         sc.close()
-    base = runs[(0, 0)]
+    base = runs[(0, 16)]
     for mode, got in runs.items():
         diff = [n for n in base if not np.array_equal(base[n], got[n])]
         assert not diff, f"mode {mode} differs in {diff}"
@@ -151,7 +155,7 @@ def test_step_kernels_agree_bitwise(pkg, precision):
 GOLDEN = __import__("os").path.join(__import__("os").path.dirname(__file__), "golden")
 
 
-@pytest.mark.parametrize("mode", [(0, 0), (1, 8)], ids=lambda m: f"mode{m[0]}_c{m[1]}")
+@pytest.mark.parametrize("mode", [(0, 16), (0, 64), (1, 8)], ids=lambda m: f"mode{m[0]}_c{m[1]}")
 @pytest.mark.parametrize("name", ["uniform_sphere", "gain_slab", "gain_clamp_dipole", "plane_waves", "pec_cavity",
                                   "graded_cuboids"])
 def test_fp64_matches_reference_golden(pkg, name, mode):
