@@ -18,6 +18,7 @@
 
 #include "../../include/fdtd_hip.h"
 #include "fdtd_kernels.hpp"
+#include "fdtd_bake.hpp"
 
 using fdtdk::MonOut;
 using fdtdk::SourceBox;
@@ -81,6 +82,7 @@ struct fdtd_sim {
     size_t esz = 8;     // bytes per stored field value
     size_t rsz = 8;     // bytes per compute value
     std::vector<void*> allocs;
+    std::vector<Key12> h_tuples;  // the coefficient table in FP64, by material id
     size_t bytes = 0;
     // Fields and psi_H are double-buffered: the fused step reads buffer `cur`
     // and writes buffer cur^1; the two-kernel step updates buffer `cur` in
@@ -199,45 +201,13 @@ StepArgs<T, R> make_args(const fdtd_sim* s) {
     return a;
 }
 
-template <typename R>
-int setup_coeffs(fdtd_sim* s, const fdtd_grid_desc* g, const fdtd_material_desc* m, const fdtd_boundary_desc* bc) {
-    // per-axis tables [7][n]: inverse spacing, kappa, bE, cE, bH, cH, 1/kappa
-    const int dims[3] = {s->nx, s->ny, s->nz};
-    const double* inv[3] = {g->inv_dx, g->inv_dy, g->inv_dz};
-    for (int a = 0; a < 3; ++a) {
-        const int n = dims[a];
-        std::vector<R> v((size_t)7 * n);
-        for (int i = 0; i < n; ++i) v[(size_t)5 * n + i] = 0;
-        for (int i = 0; i < n; ++i) {
-            if (s->bc_type == 1) {
-                const double* K[3] = {bc->kx, bc->ky, bc->kz};
-                const double* BE[3] = {bc->bEx, bc->bEy, bc->bEz};
-                const double* CE[3] = {bc->cEx, bc->cEy, bc->cEz};
-                const double* BH[3] = {bc->bHx, bc->bHy, bc->bHz};
-                const double* CH[3] = {bc->cHx, bc->cHy, bc->cHz};
-                v[(size_t)n + i] = (R)K[a][i];
-                v[(size_t)2 * n + i] = (R)BE[a][i];
-                v[(size_t)3 * n + i] = (R)CE[a][i];
-                v[(size_t)4 * n + i] = (R)BH[a][i];
-                v[(size_t)5 * n + i] = (R)CH[a][i];
-            } else {
-                v[(size_t)n + i] = 1; v[(size_t)2 * n + i] = 1; v[(size_t)3 * n + i] = 0;
-                v[(size_t)4 * n + i] = 1; v[(size_t)5 * n + i] = 0;
-            }
-            v[i] = (R)inv[a][i];
-            const double kap = s->bc_type == 1 ? (a == 0 ? bc->kx : a == 1 ? bc->ky : bc->kz)[i] : 1.0;
-            v[(size_t)6 * n + i] = (R)(1.0 / kap);
-        }
-        R* d;
-        if (int rc = s->upload(&d, v.data(), v.size())) return rc;
-        s->cp[a] = d;
-    }
-    // Material tuples -> ids + exact LUT.
+// Material ids and distinct tuples (first-occurrence order) from the 12
+// host MaterialGrids arrays.
+int ids_from_arrays(fdtd_sim* s, const fdtd_material_desc* m, std::vector<Key12>& tuples) {
     const double* arrs[12] = {m->aEx, m->bEx, m->aEy, m->bEy, m->aEz, m->bEz, m->aHx, m->bHx, m->aHy, m->bHy, m->aHz, m->bHz};
     for (int q = 0; q < 12; ++q)
         if (!arrs[q]) return fail(FDTD_EINVAL, "material array missing");
     std::unordered_map<Key12, int, Key12Hash> ids;
-    std::vector<Key12> tuples;
     std::vector<uint16_t> mid((size_t)s->N);
     Key12 last{};
     int last_id = -1;
@@ -275,6 +245,226 @@ int setup_coeffs(fdtd_sim* s, const fdtd_grid_desc* g, const fdtd_material_desc*
         if (int rc = s->upload(&d, m8.data(), m8.size())) return rc;
         s->matid = d;
     }
+    return 0;
+}
+
+// Device memory that lives for one setup call.
+struct TempBuf {
+    std::vector<void*> p;
+    ~TempBuf() {
+        for (void* q : p) (void)hipFree(q);
+    }
+    template <typename X>
+    int get(X** out, size_t n) {
+        void* q = nullptr;
+        if (hipMalloc(&q, std::max<size_t>(n * sizeof(X), 8)) != hipSuccess)
+            return fail(FDTD_ENOMEM, "hipMalloc of a bake buffer failed");
+        p.push_back(q);
+        *out = (X*)q;
+        return 0;
+    }
+    template <typename X>
+    int put(X** out, const X* src, size_t n) {
+        if (int rc = get(out, n)) return rc;
+        if (n && hipMemcpy(*out, src, n * sizeof(X), hipMemcpyHostToDevice) != hipSuccess)
+            return fail(FDTD_EDEVICE, "bake upload failed");
+        return 0;
+    }
+};
+
+// StructureScene::bake (src/structure_material.hpp:211-288) on the device:
+// per-cell material ids and the distinct coefficient tuples (the pure tuples
+// of the materials some cell lies wholly in, in material order, then those of
+// interface cells in cell order).
+template <typename ID>
+int finish_ids(fdtd_sim* s, TempBuf& tmp, const uint16_t* ids16, const std::vector<uint16_t>& remap,
+               const std::vector<long long>& icell, const std::vector<uint16_t>& iid) {
+    void* d;
+    if (int rc = s->alloc(&d, (size_t)s->N * sizeof(ID))) return rc;
+    uint16_t* dremap;
+    if (int rc = tmp.put(&dremap, remap.data(), remap.size())) return rc;
+    hipLaunchKernelGGL(fdtdk::k_ids_remap<ID>, dim3(2048), dim3(256), 0, s->stream, ids16, (const uint16_t*)dremap, (ID*)d,
+                       (long long)s->N);
+    HIPCHK(hipGetLastError());
+    if (!icell.empty()) {
+        long long* dc;
+        uint16_t* di;
+        if (int rc = tmp.put(&dc, icell.data(), icell.size())) return rc;
+        if (int rc = tmp.put(&di, iid.data(), iid.size())) return rc;
+        hipLaunchKernelGGL(fdtdk::k_bake_scatter<ID>, dim3((unsigned)((icell.size() + 255) / 256)), dim3(256), 0, s->stream,
+                           (ID*)d, (const long long*)dc, (const uint16_t*)di, (long long)icell.size());
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->matid = d;
+    return 0;
+}
+
+int bake_on_device(fdtd_sim* s, const fdtd_bake_desc* bd, std::vector<Key12>& tuples) {
+    if (!bd->x_bounds || !bd->y_bounds || !bd->z_bounds || !bd->dx || !bd->dy || !bd->dz || bd->n_shapes < 0 ||
+        bd->n_shapes > 65533 || (bd->n_shapes > 0 && !bd->shapes))
+        return fail(FDTD_EINVAL, "incomplete bake description (or more than 65533 structures)");
+    int ns = (int)std::round(std::cbrt((double)bd->subpixel_samples));
+    if (ns < 1) ns = 1;
+    const int nsh = bd->n_shapes;
+    // material properties, row 0 = background, row q + 1 = structure q
+    std::vector<double> props((size_t)4 * (nsh + 1));
+    props[0] = bd->bg_eps_r; props[1] = bd->bg_mu_r; props[2] = bd->bg_sigma; props[3] = bd->bg_sigma_m;
+    std::vector<fdtdk::BakeShape> sh((size_t)nsh);
+    for (int q = 0; q < nsh; ++q) {
+        const fdtd_bake_shape& b = bd->shapes[q];
+        if (b.type < 0 || b.type > 2) return fail(FDTD_EINVAL, "unknown bake shape type");
+        sh[q].type = b.type;
+        for (int r = 0; r < 6; ++r) sh[q].p[r] = b.p[r];
+        double* pr = &props[(size_t)4 * (q + 1)];
+        pr[0] = b.eps_r; pr[1] = b.mu_r; pr[2] = b.sigma; pr[3] = b.sigma_m;
+    }
+    TempBuf tmp;
+    fdtdk::BakeArgs a{};
+    a.nx = s->nx; a.ny = s->ny; a.nz = s->nz; a.ns = ns; a.nsh = nsh;
+    a.dt = bd->dt; a.eps0 = bd->eps0; a.mu0 = bd->mu0;
+    int rc = 0;
+    double *xb, *yb, *zb, *dx, *dy, *dz, *dprops;
+    fdtdk::BakeShape* dsh;
+    uint16_t* ids16;
+    uint8_t* dused;
+    unsigned long long* dcount;
+    if ((rc = tmp.put(&xb, bd->x_bounds, (size_t)s->nx + 1)) || (rc = tmp.put(&yb, bd->y_bounds, (size_t)s->ny + 1)) ||
+        (rc = tmp.put(&zb, bd->z_bounds, (size_t)s->nz + 1)) || (rc = tmp.put(&dx, bd->dx, (size_t)s->nx)) ||
+        (rc = tmp.put(&dy, bd->dy, (size_t)s->ny)) || (rc = tmp.put(&dz, bd->dz, (size_t)s->nz)) ||
+        (rc = tmp.put(&dprops, props.data(), props.size())) || (rc = tmp.put(&dsh, sh.data(), sh.size())) ||
+        (rc = tmp.get(&ids16, (size_t)s->N)) || (rc = tmp.get(&dused, (size_t)nsh + 1)) || (rc = tmp.get(&dcount, 1)))
+        return rc;
+    a.xb = xb; a.yb = yb; a.zb = zb; a.dx = dx; a.dy = dy; a.dz = dz;
+    a.props = dprops; a.sh = dsh;
+    a.ids = ids16; a.used = dused; a.count = dcount;
+    // interface list: sized for a sixteenth of the cells, grown once if short
+    unsigned long long cap = (unsigned long long)std::max<long>(1L << 20, s->N / 16), count = 0;
+    std::vector<long long> hc;
+    std::vector<double> ht;
+    std::vector<uint8_t> used((size_t)nsh + 1);
+    bool done = false;
+    for (int pass = 0; pass < 2 && !done; ++pass) {
+        TempBuf lst;
+        long long* icell;
+        double* itup;
+        if ((rc = lst.get(&icell, (size_t)cap)) || (rc = lst.get(&itup, (size_t)cap * 12))) return rc;
+        a.icell = icell; a.itup = itup; a.cap = cap;
+        HIPCHK(hipMemsetAsync(dcount, 0, sizeof(unsigned long long), s->stream));
+        HIPCHK(hipMemsetAsync(dused, 0, (size_t)nsh + 1, s->stream));
+        const dim3 grid((unsigned)((s->nz + 255) / 256), (unsigned)s->ny, (unsigned)s->nx);
+        hipLaunchKernelGGL(fdtdk::k_bake, grid, dim3(256), 0, s->stream, a);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(&count, dcount, sizeof(count), hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+        if (count > cap) {
+            cap = count;
+            continue;
+        }
+        hc.resize((size_t)count);
+        ht.resize((size_t)count * 12);
+        if (count) {
+            HIPCHK(hipMemcpy(hc.data(), icell, count * sizeof(long long), hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(ht.data(), itup, count * 12 * sizeof(double), hipMemcpyDeviceToHost));
+        }
+        HIPCHK(hipMemcpy(used.data(), dused, used.size(), hipMemcpyDeviceToHost));
+        done = true;
+    }
+    if (!done) return fail(FDTD_EDEVICE, "bake interface list overflow");
+    // pure tuples of the used materials: the per-component averages of ns^3
+    // equal samples
+    std::unordered_map<Key12, int, Key12Hash> ids;
+    std::vector<uint16_t> remap((size_t)nsh + 1, 0);
+    const int cnt = ns * ns * ns;
+    for (int mtl = 0; mtl <= nsh; ++mtl) {
+        if (!used[(size_t)mtl]) continue;
+        const double* pr = &props[(size_t)4 * mtl];
+        double eps_sum = 0, mu_sum = 0, sg_sum = 0, sgm_sum = 0;
+        for (int c = 0; c < cnt; ++c) {
+            eps_sum += pr[0] * bd->eps0;
+            mu_sum += pr[1] * bd->mu0;
+            sg_sum += pr[2];
+            sgm_sum += pr[3];
+        }
+        const double eps = eps_sum / cnt, mu = mu_sum / cnt, sigma = sg_sum / cnt, sigm = sgm_sum / cnt;
+        const double c1 = sigma * bd->dt / (2 * eps), c2 = sigm * bd->dt / (2 * mu);
+        const double aE = (1 - c1) / (1 + c1), bE = (bd->dt / eps) / (1 + c1);
+        const double aH = (1 - c2) / (1 + c2), bH = (bd->dt / mu) / (1 + c2);
+        Key12 k;
+        for (int c = 0; c < 3; ++c) {
+            k.v[2 * c] = aE; k.v[2 * c + 1] = bE;
+            k.v[6 + 2 * c] = aH; k.v[7 + 2 * c] = bH;
+        }
+        auto it = ids.find(k);
+        if (it == ids.end()) {
+            it = ids.emplace(k, (int)tuples.size()).first;
+            tuples.push_back(k);
+        }
+        remap[(size_t)mtl] = (uint16_t)it->second;
+    }
+    // interface tuples, deduplicated in cell order
+    std::vector<size_t> ord(hc.size());
+    for (size_t t = 0; t < ord.size(); ++t) ord[t] = t;
+    std::sort(ord.begin(), ord.end(), [&](size_t x, size_t y) { return hc[x] < hc[y]; });
+    std::vector<long long> sc(hc.size());
+    std::vector<uint16_t> si(hc.size());
+    for (size_t r = 0; r < ord.size(); ++r) {
+        Key12 k;
+        std::memcpy(k.v, &ht[ord[r] * 12], sizeof(k.v));
+        auto it = ids.find(k);
+        if (it == ids.end()) {
+            if (tuples.size() >= 65536) return fail(FDTD_EINVAL, "more than 65536 distinct material coefficient tuples");
+            it = ids.emplace(k, (int)tuples.size()).first;
+            tuples.push_back(k);
+        }
+        sc[r] = hc[ord[r]];
+        si[r] = (uint16_t)it->second;
+    }
+    s->nmat = (int)tuples.size();
+    s->mid16 = s->nmat > 256;
+    return s->mid16 ? finish_ids<uint16_t>(s, tmp, ids16, remap, sc, si) : finish_ids<uint8_t>(s, tmp, ids16, remap, sc, si);
+}
+
+template <typename R>
+int setup_coeffs(fdtd_sim* s, const fdtd_grid_desc* g, const fdtd_material_desc* m, const fdtd_boundary_desc* bc) {
+    // per-axis tables [7][n]: inverse spacing, kappa, bE, cE, bH, cH, 1/kappa
+    const int dims[3] = {s->nx, s->ny, s->nz};
+    const double* inv[3] = {g->inv_dx, g->inv_dy, g->inv_dz};
+    for (int a = 0; a < 3; ++a) {
+        const int n = dims[a];
+        std::vector<R> v((size_t)7 * n);
+        for (int i = 0; i < n; ++i) v[(size_t)5 * n + i] = 0;
+        for (int i = 0; i < n; ++i) {
+            if (s->bc_type == 1) {
+                const double* K[3] = {bc->kx, bc->ky, bc->kz};
+                const double* BE[3] = {bc->bEx, bc->bEy, bc->bEz};
+                const double* CE[3] = {bc->cEx, bc->cEy, bc->cEz};
+                const double* BH[3] = {bc->bHx, bc->bHy, bc->bHz};
+                const double* CH[3] = {bc->cHx, bc->cHy, bc->cHz};
+                v[(size_t)n + i] = (R)K[a][i];
+                v[(size_t)2 * n + i] = (R)BE[a][i];
+                v[(size_t)3 * n + i] = (R)CE[a][i];
+                v[(size_t)4 * n + i] = (R)BH[a][i];
+                v[(size_t)5 * n + i] = (R)CH[a][i];
+            } else {
+                v[(size_t)n + i] = 1; v[(size_t)2 * n + i] = 1; v[(size_t)3 * n + i] = 0;
+                v[(size_t)4 * n + i] = 1; v[(size_t)5 * n + i] = 0;
+            }
+            v[i] = (R)inv[a][i];
+            const double kap = s->bc_type == 1 ? (a == 0 ? bc->kx : a == 1 ? bc->ky : bc->kz)[i] : 1.0;
+            v[(size_t)6 * n + i] = (R)(1.0 / kap);
+        }
+        R* d;
+        if (int rc = s->upload(&d, v.data(), v.size())) return rc;
+        s->cp[a] = d;
+    }
+    // Material tuples -> ids + exact LUT.
+    std::vector<Key12> tuples;
+    if (m->bake && !m->aEx) {
+        if (int rc = bake_on_device(s, m->bake, tuples)) return rc;
+    } else if (int rc = ids_from_arrays(s, m, tuples)) {
+        return rc;
+    }
     std::vector<R> lut((size_t)12 * s->nmat);
     bool lossless = true, hu = true;
     for (int t = 0; t < s->nmat; ++t) {
@@ -288,6 +478,7 @@ int setup_coeffs(fdtd_sim* s, const fdtd_grid_desc* g, const fdtd_material_desc*
     R* d;
     if (int rc = s->upload(&d, lut.data(), lut.size())) return rc;
     s->lut = d;
+    s->h_tuples = std::move(tuples);
     return 0;
 }
 
@@ -646,7 +837,6 @@ int fdtd_create(const fdtd_grid_desc* g, const fdtd_material_desc* m, const fdtd
         delete s;
         return fail(FDTD_EDEVICE, "hipStreamCreate failed");
     }
-    s->mode = 1;
     for (int b = 0; b < 2; ++b)
         for (int q = 0; q < 6 && !rc; ++q) rc = s->alloc(&s->F2[b][q], (size_t)s->N * s->esz);
     if (!rc && s->bc_type == 1) {
@@ -710,6 +900,21 @@ int fdtd_destroy(fdtd_handle h) {
 int fdtd_device_bytes(fdtd_handle h, size_t* bytes) {
     if (!h || !bytes) return fail(FDTD_EINVAL, "null argument");
     *bytes = h->bytes;
+    return FDTD_OK;
+}
+
+int fdtd_download_material(fdtd_handle h, int q, double* out) {
+    if (!h || !out || q < 0 || q > 11) return fail(FDTD_EINVAL, "bad material array request");
+    if (h->batch) return fail(FDTD_EINVAL, "domain is a batch member");
+    std::vector<uint8_t> raw((size_t)h->N * (h->mid16 ? 2 : 1));
+    HIPCHK(hipMemcpyAsync(raw.data(), h->matid, raw.size(), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const uint16_t* m16 = reinterpret_cast<const uint16_t*>(raw.data());
+    for (long id = 0; id < h->N; ++id) {
+        const int t = h->mid16 ? m16[id] : raw[(size_t)id];
+        if (t >= (int)h->h_tuples.size()) return fail(FDTD_EDEVICE, "material id out of range");
+        out[id] = h->h_tuples[(size_t)t].v[q];
+    }
     return FDTD_OK;
 }
 

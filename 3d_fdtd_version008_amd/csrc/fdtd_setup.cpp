@@ -153,6 +153,8 @@ struct fdtd_setup {
     std::vector<double> m[12];  // aEx bEx aEy bEy aEz bEz aHx bHx aHy bHy aHz bHz
     std::vector<Shape> shapes;
     double bg_eps = 1, bg_mu = 1, bg_sig = 0, bg_sigm = 0;
+    int subpixel = 8;
+    bool device_bake = false;  // bake on the device at scene creation
     // TLS
     bool tls_on = false;
     std::vector<Region> regs;
@@ -397,7 +399,9 @@ int fdtd_setup_build(const fdtd_scene_spec* sp, fdtd_setup** out) {
         }
         S->shapes.push_back(sh);
     }
-    bake(S, sp->subpixel_samples > 0 ? sp->subpixel_samples : 8);
+    S->subpixel = sp->subpixel_samples > 0 ? sp->subpixel_samples : 8;
+    S->device_bake = sp->device_bake != 0;
+    if (!S->device_bake) bake(S, S->subpixel);
 
     // TLS, src/3D_FDTD_version008.cpp:100-172
     for (const Shape& sh : S->shapes)
@@ -560,7 +564,10 @@ long fdtd_setup_array(const fdtd_setup* s, const char* name, double* out) {
     const std::vector<double>* v = nullptr;
     static const char* mn[12] = {"aEx", "bEx", "aEy", "bEy", "aEz", "bEz", "aHx", "bHx", "aHy", "bHy", "aHz", "bHz"};
     for (int q = 0; q < 12; ++q)
-        if (n == mn[q]) v = &s->m[q];
+        if (n == mn[q]) {
+            if (s->m[q].empty()) return -1;  // baked on the device
+            v = &s->m[q];
+        }
     const struct { const char* n; const std::vector<double>* v; } tab[] = {
         {"dx", &s->gs.dx}, {"dy", &s->gs.dy}, {"dz", &s->gs.dz}, {"inv_dx", &s->gs.inv_dx}, {"inv_dy", &s->gs.inv_dy},
         {"inv_dz", &s->gs.inv_dz}, {"x_bounds", &s->gs.x_bounds}, {"y_bounds", &s->gs.y_bounds},
@@ -647,8 +654,28 @@ int fdtd_scene_create(const fdtd_setup* S, int precision, long probe_capacity, f
     g.inv_dx = S->gs.inv_dx.data(); g.inv_dy = S->gs.inv_dy.data(); g.inv_dz = S->gs.inv_dz.data();
     g.dt = S->dt;
     fdtd_material_desc m{};
-    const double** mp[12] = {&m.aEx, &m.bEx, &m.aEy, &m.bEy, &m.aEz, &m.bEz, &m.aHx, &m.bHx, &m.aHy, &m.bHy, &m.aHz, &m.bHz};
-    for (int q = 0; q < 12; ++q) *mp[q] = S->m[q].data();
+    fdtd_bake_desc bd{};
+    std::vector<fdtd_bake_shape> bsh;
+    if (S->device_bake) {
+        bd.x_bounds = S->gs.x_bounds.data(); bd.y_bounds = S->gs.y_bounds.data(); bd.z_bounds = S->gs.z_bounds.data();
+        bd.dx = S->gs.dx.data(); bd.dy = S->gs.dy.data(); bd.dz = S->gs.dz.data();
+        bd.bg_eps_r = S->bg_eps; bd.bg_mu_r = S->bg_mu; bd.bg_sigma = S->bg_sig; bd.bg_sigma_m = S->bg_sigm;
+        for (const Shape& sh : S->shapes) {
+            fdtd_bake_shape b{};
+            b.type = sh.type;
+            for (int r = 0; r < 6; ++r) b.p[r] = sh.p[r];
+            b.eps_r = sh.eps_r; b.mu_r = sh.mu_r; b.sigma = sh.sigma; b.sigma_m = sh.sigma_m;
+            bsh.push_back(b);
+        }
+        bd.n_shapes = (int)bsh.size();
+        bd.shapes = bsh.data();
+        bd.subpixel_samples = S->subpixel;
+        bd.dt = S->dt; bd.eps0 = EPS0; bd.mu0 = MU0;
+        m.bake = &bd;
+    } else {
+        const double** mp[12] = {&m.aEx, &m.bEx, &m.aEy, &m.bEy, &m.aEz, &m.bEz, &m.aHx, &m.bHx, &m.aHy, &m.bHy, &m.aHz, &m.bHz};
+        for (int q = 0; q < 12; ++q) *mp[q] = S->m[q].data();
+    }
     fdtd_boundary_desc b{};
     b.type = S->bc_type;
     b.npml = (int)S->npml;

@@ -22,12 +22,14 @@ PRECISION = {"fp64": FP64, "fp32": FP32, "fp16": FP16}
 FIELD_ID = {"Ex": 0, "Ey": 1, "Ez": 2, "Hx": 3, "Hy": 4, "Hz": 5, "Nu": 6, "Ng": 7, "Pz": 8, "Pz_prev": 9}
 PSI_NAMES = ["psi_Ex_y", "psi_Ex_z", "psi_Ey_z", "psi_Ey_x", "psi_Ez_x", "psi_Ez_y",
              "psi_Hx_y", "psi_Hx_z", "psi_Hy_z", "psi_Hy_x", "psi_Hz_x", "psi_Hz_y"]
+MATERIAL_NAMES = ["aEx", "bEx", "aEy", "bEy", "aEz", "bEz", "aHx", "bHx", "aHy", "bHy", "aHz", "bHz"]
 for _q, _n in enumerate(PSI_NAMES):
     FIELD_ID[_n] = 10 + _q
 
 # The C ABI's exported entry points (include/fdtd_hip.h, include/fdtd_scene.h).
 EXPORTS = [
-    "fdtd_create", "fdtd_destroy", "fdtd_last_error", "fdtd_device_bytes", "fdtd_set_sources", "fdtd_add_probe",
+    "fdtd_create", "fdtd_destroy", "fdtd_last_error", "fdtd_device_bytes", "fdtd_download_material",
+    "fdtd_set_sources", "fdtd_add_probe",
     "fdtd_add_dft", "fdtd_set_monitor", "fdtd_step", "fdtd_synchronize", "fdtd_download", "fdtd_upload",
     "fdtd_read_probe", "fdtd_read_dft", "fdtd_read_check", "fdtd_info", "fdtd_step_timed",
     "fdtd_setup_build", "fdtd_setup_free", "fdtd_setup_info", "fdtd_setup_dt", "fdtd_setup_array",
@@ -74,6 +76,7 @@ def load() -> C.CDLL:
         "fdtd_last_error": (C.c_char_p, []),
         "fdtd_destroy": (i, [vp]),
         "fdtd_device_bytes": (i, [vp, C.POINTER(C.c_size_t)]),
+        "fdtd_download_material": (i, [vp, i, C.c_void_p]),
         "fdtd_set_sources": (i, [vp, i, vp]),
         "fdtd_add_probe": (i, [vp, i, i, i, i, i, l]),
         "fdtd_add_dft": (i, [vp, i, vp, i, vp]),
@@ -285,6 +288,13 @@ class Scene:
         b = C.c_size_t()
         _check(self.lib.fdtd_device_bytes(self.h, C.byref(b)))
         return b.value
+
+    def material(self, name: str) -> np.ndarray:
+        """MaterialGrids array `name` (aEx ... bHz) as the device holds it."""
+        q = MATERIAL_NAMES.index(name)
+        out = np.empty(self.shape, dtype=np.float64)
+        _check(self.lib.fdtd_download_material(self.h, q, out.ctypes.data_as(C.c_void_p)))
+        return out
 
 
 class Batch:

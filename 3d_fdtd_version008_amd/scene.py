@@ -84,6 +84,7 @@ class CSceneSpec(C.Structure):
         ("n_dfts", C.c_int), ("dfts", C.POINTER(_DFT)),
         ("subpixel_samples", C.c_int),
         ("stability_interval", C.c_int),
+        ("device_bake", C.c_int),
     ]
 
 
@@ -186,6 +187,7 @@ class SceneSpec:
     dfts: List[DFTMonitor] = field(default_factory=list)
     subpixel_samples: int = 8
     stability_interval: int = 10
+    device_bake: int = 0            # 1: bake materials on the device at scene creation
     n_steps: int = 100
     name: str = "scene"
 
@@ -262,6 +264,7 @@ class SceneSpec:
         s.n_dfts, s.dfts = arr(_DFT, self.dfts, f_dft)
         s.subpixel_samples = self.subpixel_samples
         s.stability_interval = self.stability_interval
+        s.device_bake = self.device_bake
         s._keep = keep  # keep arrays alive with the struct
         return s
 

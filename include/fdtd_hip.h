@@ -61,12 +61,44 @@ typedef struct {
     double dt;
 } fdtd_grid_desc;
 
+/* One structure of StructureScene::items (src/structure_material.hpp:
+ * 96-100, shapes :44-93) for the device bake: shape type (0 box, 1 sphere,
+ * 2 z-cylinder, as Box / Sphere / CylinderZ), its parameters in total-grid
+ * physical coordinates (box: x0 x1 y0 y1 z0 z1 with half-open extents;
+ * sphere: cx cy cz r; cylinder: cx cy r z0 z1) and its Material
+ * (:17-22). */
+typedef struct {
+    int type;
+    double p[6];
+    double eps_r, mu_r, sigma, sigma_m;
+} fdtd_bake_shape;
+
+/* Inputs of StructureScene::bake (src/structure_material.hpp:211-288) run on
+ * the device: GridSpacing bounds (n+1 values per axis) and spacings, the
+ * background Material, the structures in insertion order (later ones
+ * override earlier ones), subpixel_samples (rounded to s^3 samples per Yee
+ * component, as the reference does), dt and the eps0 / mu0 the host uses. */
+typedef struct {
+    const double *x_bounds, *y_bounds, *z_bounds;
+    const double *dx, *dy, *dz;
+    double bg_eps_r, bg_mu_r, bg_sigma, bg_sigma_m;
+    int n_shapes;
+    const fdtd_bake_shape* shapes;
+    int subpixel_samples;
+    double dt, eps0, mu0;
+} fdtd_bake_desc;
+
 /* MaterialGrids, src/global_function.hpp:70-90 (as filled by
  * StructureScene::bake, src/structure_material.hpp:211-288): 12 arrays of
- * nx*ny*nz values in idx3 order. */
+ * nx*ny*nz values in idx3 order.  Alternatively, with the 12 arrays NULL and
+ * `bake` set, fdtd_create runs the bake itself on the device (every value
+ * bitwise the host bake's) and no N-sized material array exists on the host:
+ * cells whose subpixel samples all fall in one material take that
+ * material's coefficients; the others are computed per cell. */
 typedef struct {
     const double *aEx, *bEx, *aEy, *bEy, *aEz, *bEz;
     const double *aHx, *bHx, *aHy, *bHy, *aHz, *bHz;
+    const fdtd_bake_desc* bake;
 } fdtd_material_desc;
 
 /* Boundary, src/boundary.hpp:16-136.  type FDTD_BC_PEC (0) or FDTD_BC_CPML (1).
@@ -125,6 +157,12 @@ int fdtd_destroy(fdtd_handle h);
 const char* fdtd_last_error(void);
 /* Device bytes held by the handle (admission control). */
 int fdtd_device_bytes(fdtd_handle h, size_t* bytes);
+/* MaterialGrids array q (0..11: aEx bEx aEy bEy aEz bEz aHx bHx aHy bHy aHz
+ * bHz) as the handle holds it, expanded from the per-cell material ids and
+ * the exact FP64 coefficient table into nx*ny*nz values (idx3 order).  The
+ * counterpart of reading mg.<array> after bake_and_bind
+ * (src/params.hpp:512-516). */
+int fdtd_download_material(fdtd_handle h, int q, double* out);
 
 /* ---- sources, probes, monitors ---- */
 int fdtd_set_sources(fdtd_handle h, int n, const fdtd_source_desc* src);

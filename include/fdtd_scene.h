@@ -120,6 +120,12 @@ typedef struct {
 
     int    subpixel_samples;          /* bake_and_bind argument, 8 in the reference */
     int    stability_interval;        /* STABILITY_MONITOR_INTERVAL, 10 */
+
+    /* 1: StructureScene::bake runs on the device when a scene is created
+     * (fdtd_create's bake path, bitwise the host bake's values); the setup
+     * then keeps no N-sized material array and fdtd_setup_array returns -1
+     * for "aEx" ... "bHz" (fdtd_download_material reads them back). */
+    int    device_bake;
 } fdtd_scene_spec;
 
 /* ---- host setup (no device needed) ----
