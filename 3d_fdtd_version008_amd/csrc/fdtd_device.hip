@@ -83,6 +83,9 @@ struct fdtd_sim {
     size_t rsz = 8;     // bytes per compute value
     std::vector<void*> allocs;
     std::vector<Key12> h_tuples;  // the coefficient table in FP64, by material id
+    // steps enqueued by fdtd_step_async whose monitor records are not read yet
+    bool pending = false;
+    std::vector<long> pending_mon;
     size_t bytes = 0;
     // Fields and psi_H are double-buffered: the fused step reads buffer `cur`
     // and writes buffer cur^1; the two-kernel step updates buffer `cur` in
@@ -1008,9 +1011,48 @@ int fdtd_set_monitor(fdtd_handle h, int interval) {
     return FDTD_OK;
 }
 
-static int step_impl(fdtd_handle h, long n0, int nsteps, const double* src_values, double* ms) {
+// Reads the monitor records of the steps last enqueued and applies the
+// stability policy (StabilityMonitor::check_stability, src/stability_monitor.hpp:42-144).
+static int step_finish(fdtd_handle h) {
+    if (!h->pending) return FDTD_OK;
+    h->pending = false;
+    const std::vector<long> mon_steps = std::move(h->pending_mon);
+    h->pending_mon.clear();
+    h->checks.clear();
+    if (mon_steps.empty()) {
+        HIPCHK(hipStreamSynchronize(h->stream));
+        HIPCHK(hipGetLastError());
+        return FDTD_OK;
+    }
+    std::vector<MonOut> mo(mon_steps.size());
+    HIPCHK(hipMemcpyAsync(mo.data(), h->d_mon, mo.size() * sizeof(MonOut), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    bool ok = true;
+    long fail_step = -1;
+    for (size_t q = 0; q < mo.size(); ++q) {
+        fdtd_check_record r{};
+        const unsigned long long inf = 0x7ff0000000000000ull;
+        r.step = mon_steps[q];
+        r.nan_E = mo[q].maxE >= inf; r.nan_H = mo[q].maxH >= inf;
+        r.nan_P = mo[q].maxP >= inf; r.nan_N = mo[q].nanN >= inf;
+        r.max_E = bits_to_double(mo[q].maxE);
+        r.max_H = bits_to_double(mo[q].maxH);
+        r.max_P = bits_to_double(mo[q].maxP);
+        r.pop_total = mo[q].pop_total;
+        r.pop_initial = mo[q].pop_init;
+        h->checks.push_back(r);
+        if (ok && !apply_policy(h, r)) { ok = false; fail_step = r.step; }
+    }
+    if (!ok) return fail(FDTD_EUNSTABLE, "simulation unstable at step " + std::to_string(fail_step));
+    return FDTD_OK;
+}
+
+// Enqueues nsteps steps on the handle's stream (and, with ms, the timing
+// events); the monitor records are read by step_finish.
+static int step_launch(fdtd_handle h, long n0, int nsteps, const double* src_values, std::vector<hipEvent_t>* ev) {
     if (!h || nsteps < 0 || n0 < 0) return fail(FDTD_EINVAL, "bad step arguments");
     if (h->batch) return fail(FDTD_EINVAL, "domain is a batch member: step it through its batch");
+    if (int rc = step_finish(h)) return rc;  // the monitor buffer is reused
     if (nsteps == 0) return FDTD_OK;
     if (h->nsrc > 0) {
         if (!src_values) return fail(FDTD_EINVAL, "src_values required");
@@ -1037,21 +1079,28 @@ static int step_impl(fdtd_handle h, long n0, int nsteps, const double* src_value
     }
     if (!mon_steps.empty()) HIPCHK(hipMemsetAsync(h->d_mon, 0, mon_steps.size() * sizeof(MonOut), h->stream));
     int slot = 0;
-    std::vector<hipEvent_t> ev;
-    if (ms) {
-        ev.resize((size_t)nsteps * 4 + 2);
-        for (auto& e : ev) HIPCHK(hipEventCreate(&e));
-        HIPCHK(hipEventRecord(ev[0], h->stream));
+    if (ev) {
+        ev->resize((size_t)nsteps * 4 + 2);
+        for (auto& e : *ev) HIPCHK(hipEventCreate(&e));
+        HIPCHK(hipEventRecord((*ev)[0], h->stream));
     }
     for (int s = 0; s < nsteps; ++s) {
         const long n = n0 + s;
-        dispatch_step(h, n, s, ms ? &ev[1 + (size_t)s * 4] : nullptr);
+        dispatch_step(h, n, s, ev ? &(*ev)[1 + (size_t)s * 4] : nullptr);
         if (h->mon_interval > 0 && n % h->mon_interval == 0) dispatch_monitor(h, h->d_mon + slot++);
         if (h->prec == FDTD_FP16 && n % h->fp16_interval == 0) fp16_maintain(h, h->stream);
     }
     HIPCHK(hipGetLastError());
-    if (ms) {
-        HIPCHK(hipEventRecord(ev.back(), h->stream));
+    if (ev) HIPCHK(hipEventRecord(ev->back(), h->stream));
+    h->pending = true;
+    h->pending_mon = std::move(mon_steps);
+    return FDTD_OK;
+}
+
+static int step_impl(fdtd_handle h, long n0, int nsteps, const double* src_values, double* ms) {
+    std::vector<hipEvent_t> ev;
+    if (int rc = step_launch(h, n0, nsteps, src_values, ms ? &ev : nullptr)) return rc;
+    if (ms && !ev.empty()) {
         HIPCHK(hipEventSynchronize(ev.back()));
         double th = 0, te = 0;
         for (int s = 0; s < nsteps; ++s) {
@@ -1066,34 +1115,30 @@ static int step_impl(fdtd_handle h, long n0, int nsteps, const double* src_value
         ms[0] = th; ms[1] = te; ms[2] = tot;
         for (auto& e : ev) (void)hipEventDestroy(e);
     }
-    h->checks.clear();
-    if (!mon_steps.empty()) {
-        std::vector<MonOut> mo(mon_steps.size());
-        HIPCHK(hipMemcpyAsync(mo.data(), h->d_mon, mo.size() * sizeof(MonOut), hipMemcpyDeviceToHost, h->stream));
-        HIPCHK(hipStreamSynchronize(h->stream));
-        bool ok = true;
-        long fail_step = -1;
-        for (size_t q = 0; q < mo.size(); ++q) {
-            fdtd_check_record r{};
-            const unsigned long long inf = 0x7ff0000000000000ull;
-            r.step = mon_steps[q];
-            r.nan_E = mo[q].maxE >= inf; r.nan_H = mo[q].maxH >= inf;
-            r.nan_P = mo[q].maxP >= inf; r.nan_N = mo[q].nanN >= inf;
-            r.max_E = bits_to_double(mo[q].maxE);
-            r.max_H = bits_to_double(mo[q].maxH);
-            r.max_P = bits_to_double(mo[q].maxP);
-            r.pop_total = mo[q].pop_total;
-            r.pop_initial = mo[q].pop_init;
-            h->checks.push_back(r);
-            if (ok && !apply_policy(h, r)) { ok = false; fail_step = r.step; }
-        }
-        if (!ok) return fail(FDTD_EUNSTABLE, "simulation unstable at step " + std::to_string(fail_step));
-    }
-    return FDTD_OK;
+    return step_finish(h);
 }
 
 int fdtd_step(fdtd_handle h, long n0, int nsteps, const double* src_values) {
     return step_impl(h, n0, nsteps, src_values, nullptr);
+}
+
+int fdtd_step_async(fdtd_handle h, long n0, int nsteps, const double* src_values) {
+    return step_launch(h, n0, nsteps, src_values, nullptr);
+}
+
+int fdtd_step_query(fdtd_handle h) {
+    if (!h) return fail(FDTD_EINVAL, "null handle");
+    const hipError_t e = hipStreamQuery(h->stream);
+    if (e == hipSuccess) return 1;
+    if (e == hipErrorNotReady) return 0;
+    return fail(FDTD_EDEVICE, std::string("hipStreamQuery: ") + hipGetErrorString(e));
+}
+
+int fdtd_step_wait(fdtd_handle h) {
+    if (!h) return fail(FDTD_EINVAL, "null handle");
+    if (int rc = step_finish(h)) return rc;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return FDTD_OK;
 }
 
 int fdtd_step_timed(fdtd_handle h, long n0, int nsteps, const double* src_values, double* ms) {
@@ -1382,10 +1427,12 @@ void batch_launch_step(fdtd_batch* b, int s, const long* n0, const int* msteps) 
                            (unsigned)b->h_dfts.size()), dim3(256), 0, b->stream, args, anchor, b->d_dom, b->d_dfts,
                            b->d_ctl);
     }
+    // the monitor kernel itself skips members that are not on a monitor step
+    // or have stopped, so the launch depends on the step phases only
     bool mon = false;
     for (int d = 0; d < nd && !mon; ++d) {
         const int iv = b->doms[d]->mon_interval;
-        mon = iv > 0 && s < msteps[d] && (n0[d] + s) % iv == 0;
+        mon = iv > 0 && (n0[d] + s) % iv == 0;
     }
     if (mon && b->slots > 0)
         hipLaunchKernelGGL((fdtdk::k_batch_monitor<T, R>), dim3(64, (unsigned)nd), dim3(256), 0, b->stream, args,
@@ -1429,13 +1476,18 @@ int batch_step(fdtd_batch* b, int nsteps, const long* n0, const int* msteps, con
         HIPCHK(hipEventRecord(e0, b->stream));
     }
     // The monitor and FP16 rescaling run on each member's own cadence: a
-    // captured block is valid for the same step phases only.
+    // captured block is valid for the same step phases only.  Members that
+    // stop inside the block idle on the device (their step count is in the
+    // domain table), so the count enters the key only where the host decides
+    // per member (FP16 rescaling).
     std::vector<int> phase;
     for (int d = 0; d < nd; ++d) {
         const fdtd_sim* m = b->doms[d];
         phase.push_back(m->mon_interval > 0 ? (int)(n0[d] % m->mon_interval) : -1);
-        phase.push_back(msteps[d]);
-        if (b->prec == FDTD_FP16) phase.push_back((int)(n0[d] % m->fp16_interval));
+        if (b->prec == FDTD_FP16) {
+            phase.push_back(msteps[d]);
+            phase.push_back((int)(n0[d] % m->fp16_interval));
+        }
     }
     if (b->use_graph) {
         if (!b->gexec || b->g_nsteps != nsteps || b->g_version != b->version || b->g_phase != phase) {
@@ -1535,6 +1587,7 @@ int fdtd_batch_add(fdtd_batch_handle b, fdtd_handle h) {
     if (h->batch) return fail(FDTD_EINVAL, "domain already belongs to a batch");
     if (h->prec != b->prec) return fail(FDTD_EINVAL, "domain precision differs from the batch's");
     // the batch steps in place in whichever buffer holds the current state
+    if (int rc = step_finish(h)) return rc;
     HIPCHK(hipStreamSynchronize(h->stream));
     h->batch = b;
     b->doms.push_back(h);

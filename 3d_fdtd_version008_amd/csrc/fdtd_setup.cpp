@@ -813,6 +813,26 @@ int fdtd_scene_step_timed(fdtd_scene* sc, int nsteps, double* ms) {
     return rc;
 }
 
+int fdtd_scene_step_async(fdtd_scene* sc, int nsteps) {
+    if (!sc || nsteps < 0) return FDTD_EINVAL;
+    const size_t ns = sc->setup->src.size();
+    sc->vals.resize((size_t)nsteps * std::max<size_t>(ns, 1));
+    if (ns) fdtd_setup_source_values(sc->setup, sc->n, nsteps, sc->vals.data());
+    const int rc = fdtd_step_async(sc->h, sc->n, nsteps, ns ? sc->vals.data() : nullptr);
+    if (rc == FDTD_OK) sc->n += nsteps;
+    return rc;
+}
+
+int fdtd_scene_step_wait(fdtd_scene* sc) {
+    if (!sc) return FDTD_EINVAL;
+    return fdtd_step_wait(sc->h);
+}
+
+int fdtd_scene_step_query(fdtd_scene* sc) {
+    if (!sc) return FDTD_EINVAL;
+    return fdtd_step_query(sc->h);
+}
+
 long fdtd_scene_steps_done(const fdtd_scene* sc) { return sc ? sc->n : -1; }
 
 int fdtd_scene_set_stop(fdtd_scene* sc, long n_stop) {

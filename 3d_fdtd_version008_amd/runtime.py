@@ -34,7 +34,9 @@ EXPORTS = [
     "fdtd_read_probe", "fdtd_read_dft", "fdtd_read_check", "fdtd_info", "fdtd_step_timed",
     "fdtd_setup_build", "fdtd_setup_free", "fdtd_setup_info", "fdtd_setup_dt", "fdtd_setup_array",
     "fdtd_setup_source_values", "fdtd_setup_probe", "fdtd_scene_create", "fdtd_scene_destroy", "fdtd_scene_step",
-    "fdtd_scene_steps_done", "fdtd_scene_handle", "fdtd_scene_step_timed", "fdtd_set_fp16_scales",
+    "fdtd_scene_steps_done", "fdtd_scene_handle", "fdtd_scene_step_timed", "fdtd_scene_step_async",
+    "fdtd_scene_step_wait", "fdtd_scene_step_query", "fdtd_step_async", "fdtd_step_wait", "fdtd_step_query",
+    "fdtd_set_fp16_scales",
     "fdtd_get_fp16_scales", "fdtd_set_kernel_mode",
     "fdtd_batch_create", "fdtd_batch_destroy", "fdtd_batch_add", "fdtd_batch_remove", "fdtd_batch_count",
     "fdtd_batch_set_tuning", "fdtd_batch_get_tuning", "fdtd_batch_set_graph", "fdtd_batch_step",
@@ -100,6 +102,12 @@ def load() -> C.CDLL:
         "fdtd_scene_destroy": (i, [vp]),
         "fdtd_scene_step": (i, [vp, i]),
         "fdtd_scene_step_timed": (i, [vp, i, vp]),
+        "fdtd_scene_step_async": (i, [vp, i]),
+        "fdtd_scene_step_wait": (i, [vp]),
+        "fdtd_step_async": (i, [vp, l, i, vp]),
+        "fdtd_step_wait": (i, [vp]),
+        "fdtd_step_query": (i, [vp]),
+        "fdtd_scene_step_query": (i, [vp]),
         "fdtd_step_timed": (i, [vp, l, i, vp, vp]),
         "fdtd_set_fp16_scales": (i, [vp, vp]),
         "fdtd_set_kernel_mode": (i, [vp, i, i]),
@@ -238,6 +246,20 @@ class Scene:
         ms = (C.c_double * 3)()
         _check(self.lib.fdtd_scene_step_timed(self.sc, nsteps, ms))
         return tuple(ms)
+
+    def step_async(self, nsteps: int):
+        """Enqueues nsteps steps and returns at once (see step_wait)."""
+        _check(self.lib.fdtd_scene_step_async(self.sc, nsteps))
+
+    def step_wait(self):
+        """Waits for the enqueued steps and applies the stability check."""
+        _check(self.lib.fdtd_scene_step_wait(self.sc))
+
+    def step_ready(self) -> bool:
+        """True when the enqueued steps have all run (never blocks)."""
+        r = self.lib.fdtd_scene_step_query(self.sc)
+        _check(min(r, 0))
+        return r == 1
 
     def set_kernel_mode(self, mode: int, param: int = 0):
         """0: separate H/E kernels, `param` = wavefront k-width (16, 32, 64;

@@ -3,17 +3,30 @@
 Requests (a SceneSpec, a storage precision and a step count) arrive over
 time.  Each is set up on the host (a thread pool overlaps this with device
 work), then admitted when a slot and enough device memory are free (the
-budget is checked against each scene's actual device footprint).  Admitted
-scenes join the batch of their precision and advance together, one tick
-(up to `tick` steps) at a time: every tick is one persistent H launch and one
-persistent E launch per precision for all resident requests, so requests that
-arrive while others run share the device with them (continuous batching).
+budget is checked against each scene's actual device footprint; materials
+are baked on the device at admission).  Admitted requests advance together,
+one tick (up to `tick` steps) at a time, so requests that arrive while others
+run share the device with them (continuous batching).  Two engines:
+
+- "streams" (default): every resident request is a plain simulation on its
+  own HIP stream.  The loop polls the streams (fdtd_step_query, which never
+  blocks) and gives a request its next tick as soon as its previous one has
+  run, so the streams stay busy, their kernels overlap on the device and each
+  request runs the plain H and E kernels; admission and retirement happen
+  while the other requests keep running.
+- "batch": resident requests join the batch of their precision; a tick is
+  one persistent H launch and one persistent E launch per precision per step
+  for all of them (replayed from a captured HIP graph).
+- "hybrid": requests below `small_cells` cells go to the batches (a few
+  launches per step for all of them), larger ones to their own streams; the
+  batches run while the streams' steps are in flight.
+
 Finished requests are read back and freed at the end of the tick in which
 they complete, which makes room for waiting ones.
 
-A request's results do not depend on what else is resident: every member of a
-batch goes through the same per-cell operations as a solo run (fields and
-probe traces bitwise, DFT accumulators to 1e-12).
+A request's results do not depend on what else is resident: it goes through
+the same per-cell operations as a solo run (fields and probe traces bitwise,
+DFT accumulators to 1e-12).
 
 No reference counterpart (the reference runs one simulation per process).
 """
@@ -67,6 +80,11 @@ class _Live:
     remaining: int
     admitted: float
     device_bytes: int
+    batched: bool = False  # stepped through the batch of its precision
+
+    @property
+    def cells(self) -> int:
+        return self.setup.shape[0] * self.setup.shape[1] * self.setup.shape[2]
 
 
 def estimate_device_bytes(setup: runtime.Setup, precision: str) -> int:
@@ -100,14 +118,22 @@ class Server:
     tick is replayed from a captured HIP graph when `graph`)."""
 
     def __init__(self, budget_bytes: float = 200e9, max_active: int = 256, tick: int = 50, graph: bool = True,
-                 setup_workers: int = 8, tuning: Optional[tuple] = None):
+                 setup_workers: int = 8, tuning: Optional[tuple] = None, engine: str = "streams",
+                 small_cells: int = 2_000_000, enqueue_workers: int = 4):
+        if engine not in ("streams", "batch", "hybrid"):
+            raise ValueError("engine must be 'streams', 'batch' or 'hybrid'")
+        self.engine = engine
+        self.small_cells = small_cells
+        self.enqueue_workers = enqueue_workers
         self.budget = int(budget_bytes)
         self.max_active = max_active
         self.tick = tick
         self.graph = graph
         self.setup_workers = setup_workers
-        self.tuning = tuning          # (rows, wg_per_cu, chunk), e.g. from Batch.autotune
+        self.tuning = tuning          # (rows, wg_per_cu, chunk, planes), e.g. from Batch.autotune
         self.batches: Dict[str, runtime.Batch] = {}
+        # wall seconds of the last run by activity, and its tick count
+        self.stats: Dict[str, float] = {}
 
     def _batch(self, precision: str) -> runtime.Batch:
         b = self.batches.get(precision)
@@ -122,8 +148,92 @@ class Server:
     def run(self, requests: List[Request]) -> List[Result]:
         """Replays the arrival times in real time and serves every request;
         returns the results in completion order."""
+        if self.engine == "streams":
+            return self._run_streams(requests)
+        return self._run_ticks(requests)
+
+    def _run_streams(self, requests: List[Request]) -> List[Result]:
         reqs = sorted(requests, key=lambda r: r.arrival)
         pool = cf.ThreadPoolExecutor(max_workers=self.setup_workers)
+        t0 = time.perf_counter()
+        now = lambda: time.perf_counter() - t0  # noqa: E731
+        nxt = 0
+        setting_up: List[cf.Future] = []
+        ready: List[tuple] = []
+        live: List[_Live] = []
+        used = 0
+        done: List[Result] = []
+        st = {"enqueue_s": 0.0, "admit_s": 0.0, "retire_s": 0.0, "idle_s": 0.0, "ticks": 0, "polls": 0}
+        self.stats = st
+
+        def enqueue(lv: _Live):
+            te = now()
+            lv.scene.step_async(min(self.tick, lv.req.steps - lv.scene.steps_done()))
+            st["enqueue_s"] += now() - te
+            st["ticks"] += 1
+
+        try:
+            while nxt < len(reqs) or setting_up or ready or live:
+                t = now()
+                while nxt < len(reqs) and reqs[nxt].arrival <= t:
+                    setting_up.append(pool.submit(self._prepare, reqs[nxt]))
+                    nxt += 1
+                still = []
+                for f in setting_up:
+                    if f.done():
+                        ready.append(f.result())
+                    else:
+                        still.append(f)
+                setting_up = still
+                ready.sort(key=lambda x: x[0].arrival)
+                waiting = []
+                for req, setup, est in ready:
+                    if len(live) < self.max_active and used + est <= self.budget:
+                        ta = now()
+                        sc = runtime.Scene(setup, req.precision, probe_capacity=req.steps + 16)
+                        lv = _Live(req, setup, sc, req.steps, now(), sc.device_bytes())
+                        live.append(lv)
+                        used += lv.device_bytes
+                        st["admit_s"] += now() - ta
+                        enqueue(lv)
+                    else:
+                        waiting.append((req, setup, est))
+                ready = waiting
+                progressed = False
+                keep = []
+                for lv in live:
+                    st["polls"] += 1
+                    if not lv.scene.step_ready():
+                        keep.append(lv)
+                        continue
+                    progressed = True
+                    lv.scene.step_wait()  # the steps have run: applies the stability policy
+                    if lv.scene.steps_done() >= lv.req.steps:
+                        tr = now()
+                        done.append(self._retire(lv, tr))
+                        used -= lv.device_bytes
+                        st["retire_s"] += now() - tr
+                    else:
+                        enqueue(lv)
+                        keep.append(lv)
+                live = keep
+                if not progressed:
+                    ti = now()
+                    if not live and setting_up:
+                        cf.wait(setting_up, timeout=0.05, return_when=cf.FIRST_COMPLETED)
+                    elif not live and nxt < len(reqs):
+                        time.sleep(max(0.0, min(0.05, reqs[nxt].arrival - now())))
+                    else:
+                        time.sleep(0.0002)
+                    st["idle_s"] += now() - ti
+        finally:
+            pool.shutdown(wait=True)
+        return done
+
+    def _run_ticks(self, requests: List[Request]) -> List[Result]:
+        reqs = sorted(requests, key=lambda r: r.arrival)
+        pool = cf.ThreadPoolExecutor(max_workers=self.setup_workers)
+        enq = cf.ThreadPoolExecutor(max_workers=max(1, self.enqueue_workers))
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0  # noqa: E731
         nxt = 0                                   # next request not yet handed to setup
@@ -132,6 +242,8 @@ class Server:
         live: List[_Live] = []
         used = 0
         done: List[Result] = []
+        st = {"step_s": 0.0, "device_ms": 0.0, "admit_s": 0.0, "retire_s": 0.0, "idle_s": 0.0, "ticks": 0}
+        self.stats = st
         try:
             while nxt < len(reqs) or setting_up or ready or live:
                 t = now()
@@ -151,27 +263,46 @@ class Server:
                 waiting = []
                 for req, setup, est in ready:
                     if len(live) < self.max_active and used + est <= self.budget:
+                        ta = now()
                         sc = runtime.Scene(setup, req.precision, probe_capacity=req.steps + 16)
                         sc.set_stop(req.steps)
                         actual = sc.device_bytes()
-                        self._batch(req.precision).add(sc)
-                        live.append(_Live(req, setup, sc, req.steps, now(), actual))
+                        cells = setup.shape[0] * setup.shape[1] * setup.shape[2]
+                        batched = self.engine == "batch" or (self.engine == "hybrid" and cells < self.small_cells)
+                        if batched:
+                            self._batch(req.precision).add(sc)
+                        live.append(_Live(req, setup, sc, req.steps, now(), actual, batched))
                         used += actual
+                        st["admit_s"] += now() - ta
                     else:
                         waiting.append((req, setup, est))
                 ready = waiting
                 if not live:
                     # idle: wait for the next arrival or a finished setup
+                    ti = now()
                     if setting_up:
                         cf.wait(setting_up, timeout=0.05, return_when=cf.FIRST_COMPLETED)
                     elif nxt < len(reqs):
                         time.sleep(max(0.0, min(0.05, reqs[nxt].arrival - now())))
+                    st["idle_s"] += now() - ti
                     continue
                 # one tick for every resident request; a request that reaches
                 # its step count mid-tick idles for the rest of it
+                ts = now()
+                # requests on their own streams: enqueue (largest first, so
+                # their kernels start while the host enqueues the rest), run
+                # the batches meanwhile, then wait
+                solo = sorted((x for x in live if not x.batched), key=lambda x: -x.cells)
+                if solo:
+                    list(enq.map(lambda x: x.scene.step_async(min(self.tick, x.req.steps - x.scene.steps_done())),
+                                 solo))
                 for b in self.batches.values():
                     if len(b):
-                        b.step(self.tick)
+                        st["device_ms"] += b.step(self.tick)
+                if solo:
+                    list(enq.map(lambda x: x.scene.step_wait(), solo))
+                st["step_s"] += now() - ts
+                st["ticks"] += 1
                 for lv in live:
                     lv.remaining = lv.req.steps - lv.scene.steps_done()
                 t_end = now()
@@ -183,8 +314,10 @@ class Server:
                     else:
                         keep.append(lv)
                 live = keep
+                st["retire_s"] += now() - t_end
         finally:
             pool.shutdown(wait=True)
+            enq.shutdown(wait=True)
             for b in self.batches.values():
                 b.close()
             self.batches = {}
@@ -196,7 +329,8 @@ class Server:
         return req, setup, estimate_device_bytes(setup, req.precision)
 
     def _retire(self, lv: _Live, t_end: float) -> Result:
-        self.batches[lv.req.precision].remove(lv.scene)
+        if lv.batched:
+            self.batches[lv.req.precision].remove(lv.scene)
         st = lv.setup
         res = Result(rid=lv.req.rid, precision=lv.req.precision, steps=lv.req.steps,
                      cells=st.shape[0] * st.shape[1] * st.shape[2], arrival=lv.req.arrival, admitted=lv.admitted,
@@ -209,17 +343,21 @@ class Server:
         return res
 
 
-def solve_alone(req: Request) -> Result:
-    """The same request served by itself (a batch of one), for equality
-    checks and the one-at-a-time baseline."""
+def solve_alone(req: Request, engine: str = "streams") -> Result:
+    """The same request served by itself (a plain solve, or a batch of one
+    for the batch engine), for equality checks."""
     t0 = time.perf_counter()
     setup = runtime.Setup(req.spec)
     sc = runtime.Scene(setup, req.precision, probe_capacity=req.steps + 16)
     sc.set_stop(req.steps)
-    b = runtime.Batch(req.precision)
-    b.add(sc)
-    while sc.steps_done() < req.steps:
-        b.step(50)
+    b = None
+    if engine == "batch":
+        b = runtime.Batch(req.precision)
+        b.add(sc)
+        while sc.steps_done() < req.steps:
+            b.step(50)
+    else:
+        sc.step(req.steps)
     res = Result(rid=req.rid, precision=req.precision, steps=req.steps,
                  cells=setup.shape[0] * setup.shape[1] * setup.shape[2], arrival=0.0, admitted=0.0,
                  finished=time.perf_counter() - t0, device_bytes=sc.device_bytes())
@@ -227,7 +365,8 @@ def solve_alone(req: Request) -> Result:
     res.dfts = [sc.dft(m) for m in range(setup.n_dfts)]
     if req.keep_fields:
         res.fields = {f: sc.field(f) for f in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")}
-    b.close()
+    if b is not None:
+        b.close()
     sc.close()
     return res
 
@@ -247,6 +386,7 @@ def request_stream(n: int, rate: float, seed: int = 1, lo: int = 48, hi: int = 2
         sp = scn.serve_case(rid, int(kind), seed=seed, lo=lo, hi=hi)
         nst = int(rng.integers(steps[0], steps[1] + 1))
         sp.n_steps = nst
+        sp.device_bake = 1  # materials baked on the device at admission
         prec = "fp16" if rng.random() < fp16_share else "fp32"
         out.append(Request(rid, sp, prec, nst, t))
     return out

@@ -209,6 +209,20 @@ int fdtd_set_monitor(fdtd_handle h, int interval);
  * the reference's stability policy (NaN/Inf or more than 10 threshold
  * warnings); fdtd_read_check gives the records. */
 int fdtd_step(fdtd_handle h, long n0, int nsteps, const double* src_values);
+/* fdtd_step in two halves, so that a caller can keep many handles' steps in
+ * flight at once (each handle has its own stream): fdtd_step_async enqueues
+ * the steps and returns without waiting for the device (src_values is read
+ * before it returns); fdtd_step_wait waits for them and applies the
+ * stability policy to their monitor records (FDTD_EUNSTABLE as fdtd_step).
+ * An enqueue on a handle with steps still pending first completes those.
+ * Results are those of fdtd_step.  No reference counterpart (the reference
+ * steps one simulation synchronously). */
+int fdtd_step_async(fdtd_handle h, long n0, int nsteps, const double* src_values);
+int fdtd_step_wait(fdtd_handle h);
+/* 1 when the handle's enqueued steps have all run on the device (a
+ * following fdtd_step_wait returns without waiting), 0 while some are still
+ * in flight; never blocks. */
+int fdtd_step_query(fdtd_handle h);
 int fdtd_synchronize(fdtd_handle h);
 /* fdtd_step that also times, with HIP events on the handle's stream, every
  * H-kernel and E-kernel launch: ms[0] = sum of H kernel times, ms[1] = sum of

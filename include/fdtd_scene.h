@@ -160,6 +160,12 @@ int  fdtd_scene_destroy(fdtd_scene* sc);
 int  fdtd_scene_step(fdtd_scene* sc, int nsteps);
 /* fdtd_scene_step through fdtd_step_timed (per-kernel HIP-event times, ms[3]). */
 int  fdtd_scene_step_timed(fdtd_scene* sc, int nsteps, double* ms);
+/* fdtd_scene_step through fdtd_step_async / fdtd_step_wait: enqueue, and
+ * wait for (and check) what was enqueued.  steps_done counts enqueued steps. */
+int  fdtd_scene_step_async(fdtd_scene* sc, int nsteps);
+int  fdtd_scene_step_wait(fdtd_scene* sc);
+/* fdtd_step_query of the scene's simulation (1 idle, 0 busy). */
+int  fdtd_scene_step_query(fdtd_scene* sc);
 long fdtd_scene_steps_done(const fdtd_scene* sc);
 /* Batch steps never advance the scene past step n_stop (-1: no limit): a
  * batch call in which it reaches n_stop leaves it idle for the rest. */
