@@ -112,6 +112,11 @@ struct fdtd_sim {
     int8_t* greg = nullptr;
     double *gPz = nullptr, *gPp = nullptr, *gNu = nullptr, *gNg = nullptr;
     TLSCoef* d_tls = nullptr;
+    // TLS diagnostics recorder: every tls_every steps (0: off), up to
+    // tls_cap records of 5 values at d_tlsrec; FP64 spacings for dV
+    int tls_every = 0;
+    long tls_cap = 0, tls_count = 0, tls_gmon = -1;
+    double *d_tlsrec = nullptr, *d_tlspart = nullptr, *d_dxyz = nullptr;
     std::vector<TLSCoef> h_tls;
     // sources
     SourceBox* d_src = nullptr;
@@ -588,6 +593,8 @@ void set_cur(fdtd_sim* s, int c) {
     for (int q = 0; q < 12; ++q) s->psi[q] = (q < 6) ? s->psi2[0][q] : s->psi2[c][q];
 }
 
+constexpr int TLS_PARTS = 256;  // blocks of the TLS diagnostic reduction
+
 template <typename T, typename R>
 int launch_step(fdtd_sim* s, long n, int sidx, hipEvent_t* ev) {
     StepArgs<T, R> a = make_args<T, R>(s);
@@ -656,6 +663,16 @@ int launch_step(fdtd_sim* s, long n, int sidx, hipEvent_t* ev) {
         }
         da.acc = d.d_acc;
         hipLaunchKernelGGL((fdtdk::k_dft<T, R>), dim3((unsigned)((d.ncell + 255) / 256)), dim3(256), 0, s->stream, a, da);
+    }
+    // TLS diagnostics after the populations (src/3D_FDTD_version008.cpp:330-336)
+    if (s->tls_every > 0 && n % s->tls_every == 0 && s->tls_count < s->tls_cap) {
+        const double* dx = s->d_dxyz;
+        hipLaunchKernelGGL((fdtdk::k_tls_partial<T, R>), dim3(TLS_PARTS), dim3(256), 0, s->stream, a, dx, dx + s->nx,
+                           dx + s->nx + s->ny, s->d_tlspart);
+        hipLaunchKernelGGL(fdtdk::k_tls_final, dim3(1), dim3(64), 0, s->stream, (const double*)s->d_tlspart, TLS_PARTS,
+                           (const int8_t*)s->greg, (const double*)s->gPz, (const double*)s->gPp,
+                           (const fdtdk::TLSCoef*)s->d_tls, s->tls_gmon, s->d_tlsrec + 5 * s->tls_count);
+        s->tls_count++;
     }
     return 0;
 }
@@ -903,6 +920,48 @@ int fdtd_destroy(fdtd_handle h) {
 int fdtd_device_bytes(fdtd_handle h, size_t* bytes) {
     if (!h || !bytes) return fail(FDTD_EINVAL, "null argument");
     *bytes = h->bytes;
+    return FDTD_OK;
+}
+
+static int step_finish(fdtd_handle h);
+
+int fdtd_add_tls_monitor(fdtd_handle h, const double* dx, const double* dy, const double* dz, int i, int j, int k,
+                         int save_every, long capacity) {
+    if (!h || !dx || !dy || !dz) return fail(FDTD_EINVAL, "null argument");
+    if (!h->gain) return fail(FDTD_EINVAL, "no TLS region on this handle");
+    if (h->batch) return fail(FDTD_EINVAL, "domain is a batch member");
+    if (save_every < 1 || capacity < 1) return fail(FDTD_EINVAL, "bad TLS monitor arguments");
+    if (i < 0 || j < 0 || k < 0 || i >= h->nx || j >= h->ny || k >= h->nz) return fail(FDTD_EINVAL, "monitor cell outside grid");
+    if (int rc = step_finish(h)) return rc;
+    std::vector<double> d((size_t)h->nx + h->ny + h->nz);
+    std::copy(dx, dx + h->nx, d.begin());
+    std::copy(dy, dy + h->ny, d.begin() + h->nx);
+    std::copy(dz, dz + h->nz, d.begin() + h->nx + h->ny);
+    if (int rc = h->upload(&h->d_dxyz, d.data(), d.size())) return rc;
+    void* p;
+    if (int rc = h->alloc(&p, (size_t)capacity * 5 * sizeof(double))) return rc;
+    h->d_tlsrec = (double*)p;
+    if (int rc = h->alloc(&p, (size_t)TLS_PARTS * 6 * sizeof(double))) return rc;
+    h->d_tlspart = (double*)p;
+    const int gi = i - h->g0[0], gj = j - h->g0[1], gk = k - h->g0[2];
+    h->tls_gmon = (gi >= 0 && gj >= 0 && gk >= 0 && gi < h->gn[0] && gj < h->gn[1] && gk < h->gn[2])
+                      ? ((long)gi * h->gn[1] + gj) * h->gn[2] + gk
+                      : -1;
+    h->tls_every = save_every;
+    h->tls_cap = capacity;
+    h->tls_count = 0;
+    return FDTD_OK;
+}
+
+int fdtd_read_tls_monitor(fdtd_handle h, double* out, long max, long* count) {
+    if (!h || !count || (max > 0 && !out)) return fail(FDTD_EINVAL, "null argument");
+    if (int rc = step_finish(h)) return rc;
+    const long n = std::min(max, h->tls_count);
+    if (n > 0) {
+        HIPCHK(hipMemcpyAsync(out, h->d_tlsrec, (size_t)n * 5 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
+    *count = h->tls_count;
     return FDTD_OK;
 }
 

@@ -1257,6 +1257,93 @@ __device__ __forceinline__ double field_value(const StepArgs<T, R>& a, int comp,
     return 0.0;
 }
 
+// ---- TLS diagnostics (PopulationTimeSeriesDetector::record,
+// PolarizationDiagnostics::record, src/two_level_detectors.hpp:70-103,
+// 230-257) ----
+// The reference sums over the region cells in idx3 order.  A parallel sum
+// cannot keep that order, so the terms are accumulated in double-double
+// (Knuth's TwoSum: hi + lo carries about 106 bits) and the block partials
+// are merged in a fixed order: the result is the correctly rounded sum of
+// the terms (short of contrived cancellation), the same on every run, and
+// within a few ulps per term of the reference's sequential sum.
+struct DD {
+    double hi, lo;
+};
+__device__ __forceinline__ void dd_add(DD& s, double x) {
+    const double t = s.hi + x;
+    const double bp = t - s.hi;
+    const double e = (s.hi - (t - bp)) + (x - bp);
+    s.hi = t;
+    s.lo += e;
+}
+__device__ __forceinline__ DD dd_merge(const DD& a, const DD& b) {
+    const double t = a.hi + b.hi;
+    const double bp = t - a.hi;
+    const double e = (a.hi - (t - bp)) + (b.hi - bp);
+    const double lo = (a.lo + b.lo) + e;
+    const double hi = t + lo;
+    return DD{hi, lo - (hi - t)};
+}
+
+// Block partials of (sum Nu dV, sum Ng dV, sum -0.5 Pz Ez dV) over the region
+// cells of the gain box, dV = dx[i]*dy[j]*dz[k]; part[block][3][2].
+template <typename T, typename R>
+__global__ void __launch_bounds__(256) k_tls_partial(StepArgs<T, R> a, const double* dx, const double* dy,
+                                                     const double* dz, double* part) {
+    __shared__ DD sh[3][256];
+    DD s[3] = {{0, 0}, {0, 0}, {0, 0}};
+    const long total = (long)a.gnx * a.gny * a.gnz;
+    for (long g = (long)blockIdx.x * 256 + threadIdx.x; g < total; g += (long)gridDim.x * 256) {
+        if (a.greg[g] < 0) continue;
+        const long r = g / a.gnz;
+        const int k = a.g0z + (int)(g - r * a.gnz);
+        const int j = a.g0y + (int)(r % a.gny);
+        const int i = a.g0x + (int)(r / a.gny);
+        const long id = ((long)i * a.ny + j) * a.nz + k;
+        const double dV = dx[i] * dy[j] * dz[k];
+        const double ez = (double)ld<T, R>(a.Ez, id, a.sc[2]);
+        dd_add(s[0], a.gNu[g] * dV);
+        dd_add(s[1], a.gNg[g] * dV);
+        dd_add(s[2], -0.5 * a.gPz[g] * ez * dV);
+    }
+    for (int q = 0; q < 3; ++q) sh[q][threadIdx.x] = s[q];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int q = 0; q < 3; ++q) sh[q][threadIdx.x] = dd_merge(sh[q][threadIdx.x], sh[q][threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 3; ++q) {
+            part[(long)blockIdx.x * 6 + 2 * q] = sh[q][0].hi;
+            part[(long)blockIdx.x * 6 + 2 * q + 1] = sh[q][0].lo;
+        }
+}
+
+// One record: rec[0..4] = total Nu, total Ng, interaction energy, Pz and
+// dPz/dt at the monitor cell (gain-box index gmon, -1 outside the box; zero
+// where the cell is not a region cell, as the reference's zero-initialised
+// state arrays are there).  One thread merges the partials in block order.
+__global__ void k_tls_final(const double* part, int nparts, const int8_t* greg, const double* gPz, const double* gPp,
+                            const TLSCoef* tls, long gmon, double* rec) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    for (int q = 0; q < 3; ++q) {
+        DD s{0, 0};
+        for (int b = 0; b < nparts; ++b) s = dd_merge(s, DD{part[(long)b * 6 + 2 * q], part[(long)b * 6 + 2 * q + 1]});
+        rec[q] = s.hi + s.lo;
+    }
+    double pz = 0, dpz = 0;
+    if (gmon >= 0) {
+        const int reg = greg[gmon];
+        if (reg >= 0) {
+            pz = gPz[gmon];
+            dpz = (gPz[gmon] - gPp[gmon]) * tls[reg].inv_dt;
+        }
+    }
+    rec[3] = pz;
+    rec[4] = dpz;
+}
+
 // PointFieldDetector::record_after_E / get_field_value
 // (src/detectors/point_field_detector.hpp:111-124, src/detectors/idetector.hpp:79-109).
 template <typename T, typename R>

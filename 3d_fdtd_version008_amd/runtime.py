@@ -29,6 +29,7 @@ for _q, _n in enumerate(PSI_NAMES):
 # The C ABI's exported entry points (include/fdtd_hip.h, include/fdtd_scene.h).
 EXPORTS = [
     "fdtd_create", "fdtd_destroy", "fdtd_last_error", "fdtd_device_bytes", "fdtd_download_material",
+    "fdtd_add_tls_monitor", "fdtd_read_tls_monitor",
     "fdtd_set_sources", "fdtd_add_probe",
     "fdtd_add_dft", "fdtd_set_monitor", "fdtd_step", "fdtd_synchronize", "fdtd_download", "fdtd_upload",
     "fdtd_read_probe", "fdtd_read_dft", "fdtd_read_check", "fdtd_info", "fdtd_step_timed",
@@ -79,6 +80,8 @@ def load() -> C.CDLL:
         "fdtd_destroy": (i, [vp]),
         "fdtd_device_bytes": (i, [vp, C.POINTER(C.c_size_t)]),
         "fdtd_download_material": (i, [vp, i, C.c_void_p]),
+        "fdtd_add_tls_monitor": (i, [vp, vp, vp, vp, i, i, i, i, l]),
+        "fdtd_read_tls_monitor": (i, [vp, vp, l, C.POINTER(C.c_long)]),
         "fdtd_set_sources": (i, [vp, i, vp]),
         "fdtd_add_probe": (i, [vp, i, i, i, i, i, l]),
         "fdtd_add_dft": (i, [vp, i, vp, i, vp]),
@@ -284,6 +287,22 @@ class Scene:
         out = np.empty(cnt.value, dtype=np.float64)
         if cnt.value:
             _check(self.lib.fdtd_read_probe(self.h, channel, out.ctypes.data_as(C.c_void_p), cnt.value, C.byref(cnt)))
+        return out
+
+    def add_tls_monitor(self, i: int, j: int, k: int, save_every: int = 1, capacity: int = 100000):
+        """Records the TLS diagnostics (total Nu, total Ng, interaction energy,
+        Pz and dPz/dt at cell (i, j, k)) every save_every steps on the device."""
+        self._dxyz = [np.ascontiguousarray(self.setup.array(n, shaped=False)) for n in ("dx", "dy", "dz")]
+        p = [a.ctypes.data_as(C.c_void_p) for a in self._dxyz]
+        _check(self.lib.fdtd_add_tls_monitor(self.h, p[0], p[1], p[2], i, j, k, save_every, capacity))
+
+    def tls_records(self) -> np.ndarray:
+        """(records, 5): total Nu, total Ng, U_int, Pz and dPz/dt at the cell."""
+        cnt = C.c_long()
+        _check(self.lib.fdtd_read_tls_monitor(self.h, None, 0, C.byref(cnt)))
+        out = np.empty((cnt.value, 5), dtype=np.float64)
+        if cnt.value:
+            _check(self.lib.fdtd_read_tls_monitor(self.h, out.ctypes.data_as(C.c_void_p), cnt.value, C.byref(cnt)))
         return out
 
     def dft(self, m: int) -> np.ndarray:

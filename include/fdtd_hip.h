@@ -242,6 +242,25 @@ int fdtd_read_probe(fdtd_handle h, int probe, double* out, long max, long* count
 /* Accumulators of DFT monitor m, layout [freq][cell][re, im], cell in box
  * order (i, j, k with k fastest). */
 int fdtd_read_dft(fdtd_handle h, int m, double* out);
+/* TLS diagnostics as device reductions: every save_every steps (after the
+ * population update, where the reference records its TLS detectors,
+ * src/3D_FDTD_version008.cpp:330-336) one record of 5 values is kept, up to
+ * `capacity` records:
+ *   [0] total Nu  = sum Nu dV     (PopulationTimeSeriesDetector::record,
+ *   [1] total Ng  = sum Ng dV      src/two_level_detectors.hpp:70-103)
+ *   [2] U_int     = sum -0.5 Pz Ez dV   (PolarizationDiagnostics::record,
+ *   [3] Pz at cell (i, j, k)             src/two_level_detectors.hpp:230-257)
+ *   [4] dPz/dt at cell (i, j, k)
+ * over the TLS region cells, dV = dx[i]*dy[j]*dz[k] from the GridSpacing
+ * arrays passed here.  The reference sums sequentially; the device sums in
+ * double-double and rounds once, so a total is the correctly rounded sum of
+ * the same terms (deterministic, within a few ulps per term of the
+ * sequential sum).  Inversion, inversion fraction and the TLSGlobalHistory
+ * percentages follow from [0] and [1] on the host.  Single handles only. */
+int fdtd_add_tls_monitor(fdtd_handle h, const double* dx, const double* dy, const double* dz, int i, int j, int k,
+                         int save_every, long capacity);
+/* Copies up to max records (5 values each) into out; *count = records taken. */
+int fdtd_read_tls_monitor(fdtd_handle h, double* out, long max, long* count);
 /* Monitor records of the last fdtd_step call (up to max); *count = number. */
 int fdtd_read_check(fdtd_handle h, fdtd_check_record* out, int max, int* count);
 
