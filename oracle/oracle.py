@@ -56,6 +56,12 @@ def _lib(impl: str):
         "get_probe": (C.c_long, [vp, C.c_int, C.c_int, vp]),
         "set_array": (C.c_int, [vp, C.c_char_p, vp]),
     }
+    if impl == "ref":
+        d, i, l = C.c_double, C.c_int, C.c_long
+        sig.update({
+            "add_point_detector": (i, [vp, C.c_char_p, C.c_char_p, d, d, d, i, vp, i, l, i, d, d, d]),
+            "add_movie": (i, [vp, C.c_char_p, C.c_char_p, i, i, d, i, l, i, l, l, l, d, d, d, d, d, d]),
+        })
     if impl == "oracle":
         sig.update({
             "get_region": (C.c_int, [vp, C.c_int, vp]),
@@ -96,6 +102,19 @@ class CpuSim:
         if self.h:
             self.f["destroy"](self.h)
             self.h = None
+
+    def add_point_detector(self, out_root, name, x, y, z, components, save_every, n_steps, float64=True,
+                           domain_min=(0.0, 0.0, 0.0)):
+        """The reference's PointFieldDetector writing its own files (ref only)."""
+        comps = (C.c_int * len(components))(*components)
+        self.f["add_point_detector"](self.h, out_root.encode(), name.encode(), x, y, z, len(components),
+                                     C.cast(comps, C.c_void_p), save_every, n_steps, int(float64), *domain_min)
+
+    def add_movie(self, out_root, name, component, plane, position, save_every, n_steps, float64, n_phys, L_phys,
+                  domain_min=(0.0, 0.0, 0.0)):
+        """The reference's FieldMovie2D writing its own files (ref only)."""
+        self.f["add_movie"](self.h, out_root.encode(), name.encode(), component, plane, position, save_every, n_steps,
+                            int(float64), *n_phys, *L_phys, *domain_min)
 
     def __del__(self):
         try:

@@ -26,6 +26,8 @@
 #include "sources/dipole_source.hpp"
 #include "sources/plane_wave_source.hpp"
 #include "detectors/idetector.hpp"
+#include "detectors/point_field_detector.hpp"
+#include "detectors/field_movie_2d.hpp"
 #include "stability_monitor.hpp"
 
 #include "../include/fdtd_scene.h"
@@ -63,6 +65,8 @@ struct ref_sim {
     long fail_step{-1};
     double last_max_E{0};
     size_t n{0};
+    // the reference's own file-writing detectors, recorded after the E update
+    std::vector<std::unique_ptr<Detectors::IDetector>> dets;
 };
 
 extern "C" {
@@ -253,6 +257,7 @@ int ref_step(ref_sim* S, int nsteps) {
                                                                 S->Ey, S->Ez, S->Hx, S->Hy, S->Hz));
             }
         }
+        for (auto& d : S->dets) d->record_after_E(n, S->dt, S->Ex, S->Ey, S->Ez, S->Hx, S->Hy, S->Hz);
         if (S->tls_on) update_populations_multi_region(NxT, NyT, NzT, S->dt, S->tls_manager, S->Ez, S->tls_state);
         S->n++;
         if (n % S->stab_interval == 0) {
@@ -266,7 +271,53 @@ int ref_step(ref_sim* S, int nsteps) {
     return rc;
 }
 
-void ref_destroy(ref_sim* S) { delete S; }
+void ref_destroy(ref_sim* S) {
+    for (auto& d : S->dets) d->finalize();
+    delete S;
+}
+
+// PointFieldDetector (src/detectors/point_field_detector.hpp:183-196)
+// writing <out_root>/<name>/{<comp>_ts.bin, metadata.json}.
+int ref_add_point_detector(ref_sim* S, const char* out_root, const char* name, double x, double y, double z,
+                           int ncomp, const int* comps, int save_every, long n_steps, int float64, double dmx,
+                           double dmy, double dmz) {
+    NullBuf nb;
+    std::streambuf* old_out = std::cout.rdbuf(&nb);
+    Detectors::PointFieldDetectorConfig c;
+    c.name = name;
+    c.x = x; c.y = y; c.z = z;
+    c.components.clear();
+    for (int q = 0; q < ncomp; ++q) c.components.push_back(static_cast<Detectors::FieldComponent>(comps[q]));
+    c.save_every = (std::size_t)save_every;
+    c.n_steps = (std::size_t)n_steps;
+    c.write_float64 = float64 != 0;
+    S->dets.push_back(Detectors::make_point_field_detector(out_root, c, S->NxT, S->NyT, S->NzT, S->npml, S->gs, S->dt,
+                                                           dmx, dmy, dmz));
+    std::cout.rdbuf(old_out);
+    return 0;
+}
+
+// FieldMovie2D (src/detectors/field_movie_2d.hpp:216-229) writing
+// <out_root>/<name>/{frame_NNNN.raw, metadata.json}.
+int ref_add_movie(ref_sim* S, const char* out_root, const char* name, int comp, int plane, double pos,
+                  int save_every, long n_steps, int float64, long nx_phys, long ny_phys, long nz_phys, double Lx,
+                  double Ly, double Lz, double dmx, double dmy, double dmz) {
+    NullBuf nb;
+    std::streambuf* old_out = std::cout.rdbuf(&nb);
+    Detectors::FieldMovie2DConfig c;
+    c.name = name;
+    c.component = static_cast<Detectors::FieldComponent>(comp);
+    c.slice_plane = plane;
+    c.slice_position = pos;
+    c.save_every = (std::size_t)save_every;
+    c.n_steps = (std::size_t)n_steps;
+    c.write_float64 = float64 != 0;
+    S->dets.push_back(Detectors::make_field_movie_2d(out_root, c, S->NxT, S->NyT, S->NzT, (std::size_t)nx_phys,
+                                                     (std::size_t)ny_phys, (std::size_t)nz_phys, S->npml, S->gs, Lx, Ly,
+                                                     Lz, dmx, dmy, dmz));
+    std::cout.rdbuf(old_out);
+    return 0;
+}
 
 void ref_info(const ref_sim* S, long* out) {
     out[0] = (long)S->NxT; out[1] = (long)S->NyT; out[2] = (long)S->NzT; out[3] = (long)S->npml;
