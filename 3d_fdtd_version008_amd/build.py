@@ -1,7 +1,8 @@
 """Builds the gfx950 library lib3dfdtd_hip.so in-tree.
 
-The host setup (fdtd_setup.cpp) is compiled by g++ with -ffp-contract=off so
-its coefficient arrays keep the reference's rounding; the device library
+The host setup (fdtd_setup.cpp) and the auto-mesher (fdtd_automesh.cpp) are
+compiled by g++ with -ffp-contract=off so their arrays keep the reference's
+rounding; the device library
 (fdtd_device.hip) by hipcc for gfx950 with packed-FP32 instruction selection
 disabled.
 """
@@ -33,11 +34,13 @@ def build(verbose: bool = False) -> str:
     os.makedirs(tmp, exist_ok=True)
     setup_o = os.path.join(tmp, "fdtd_setup.o")
     dev_o = os.path.join(tmp, "fdtd_device.o")
-    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-pthread", "-c",
-          os.path.join(CSRC, "fdtd_setup.cpp"), "-o", setup_o])
+    mesh_o = os.path.join(tmp, "fdtd_automesh.o")
+    for src, obj in (("fdtd_setup.cpp", setup_o), ("fdtd_automesh.cpp", mesh_o)):
+        _run(["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-pthread", "-c",
+              os.path.join(CSRC, src), "-o", obj])
     _run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, "fdtd_device.hip"), "-o", dev_o])
     tmp_lib = LIB + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", dev_o, setup_o, "-o", tmp_lib, "-pthread"])
+    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", dev_o, setup_o, mesh_o, "-o", tmp_lib, "-pthread"])
     os.replace(tmp_lib, LIB)
     if verbose:
         print("built", LIB)

@@ -333,9 +333,16 @@ int fdtd_setup_build(const fdtd_scene_spec* sp, fdtd_setup** out) {
         g.dy.assign(nyg, sp->dy_uniform);
         g.dz.assign(nzg, sp->dz_uniform);
     }
-    g.x_bounds = GridSpacing::bounds(g.dx);
-    g.y_bounds = GridSpacing::bounds(g.dy);
-    g.z_bounds = GridSpacing::bounds(g.dz);
+    const bool lines = sp->dx && sp->dy && sp->dz && sp->x_bounds && sp->y_bounds && sp->z_bounds;
+    if (lines) {
+        g.x_bounds.assign(sp->x_bounds, sp->x_bounds + nxg + 1);
+        g.y_bounds.assign(sp->y_bounds, sp->y_bounds + nyg + 1);
+        g.z_bounds.assign(sp->z_bounds, sp->z_bounds + nzg + 1);
+    } else {
+        g.x_bounds = GridSpacing::bounds(g.dx);
+        g.y_bounds = GridSpacing::bounds(g.dy);
+        g.z_bounds = GridSpacing::bounds(g.dz);
+    }
     auto inv = [](const std::vector<double>& d) {
         std::vector<double> v(d.size());
         for (size_t i = 0; i < d.size(); ++i) v[i] = 1.0 / d[i];

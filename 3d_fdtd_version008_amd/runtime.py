@@ -43,7 +43,7 @@ EXPORTS = [
     "fdtd_batch_set_tuning", "fdtd_batch_get_tuning", "fdtd_batch_set_graph", "fdtd_batch_step",
     "fdtd_scene_batch_create", "fdtd_scene_batch_destroy", "fdtd_scene_batch_add", "fdtd_scene_batch_remove",
     "fdtd_scene_batch_count", "fdtd_scene_batch_step", "fdtd_scene_batch_autotune", "fdtd_scene_batch_handle",
-    "fdtd_scene_set_stop",
+    "fdtd_scene_set_stop", "fdtd_auto_mesh_axis", "fdtd_structure_mesh_bounds",
 ]
 
 

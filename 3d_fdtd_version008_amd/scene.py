@@ -85,6 +85,8 @@ class CSceneSpec(C.Structure):
         ("subpixel_samples", C.c_int),
         ("stability_interval", C.c_int),
         ("device_bake", C.c_int),
+        ("x_bounds", C.POINTER(C.c_double)), ("y_bounds", C.POINTER(C.c_double)),
+        ("z_bounds", C.POINTER(C.c_double)),
     ]
 
 
@@ -188,6 +190,9 @@ class SceneSpec:
     subpixel_samples: int = 8
     stability_interval: int = 10
     device_bake: int = 0            # 1: bake materials on the device at scene creation
+    x_bounds: Optional[np.ndarray] = None  # grid lines of an auto-meshed grid (with dx/dy/dz)
+    y_bounds: Optional[np.ndarray] = None
+    z_bounds: Optional[np.ndarray] = None
     n_steps: int = 100
     name: str = "scene"
 
@@ -209,7 +214,7 @@ class SceneSpec:
         keep = []
         s.nx_core, s.ny_core, s.nz_core, s.npml = self.nx_core, self.ny_core, self.nz_core, self.npml
         s.dx_uniform, s.dy_uniform, s.dz_uniform = self.dx_uniform, self.dy_uniform, self.dz_uniform
-        for nm in ("dx", "dy", "dz"):
+        for nm in ("dx", "dy", "dz", "x_bounds", "y_bounds", "z_bounds"):
             arr = getattr(self, nm)
             if arr is not None:
                 a = np.ascontiguousarray(arr, dtype=np.float64)

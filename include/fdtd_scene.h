@@ -126,6 +126,16 @@ typedef struct {
      * then keeps no N-sized material array and fdtd_setup_array returns -1
      * for "aEx" ... "bHz" (fdtd_download_material reads them back). */
     int    device_bake;
+
+    /* Optional grid lines (length n*_core + 2*npml + 1 per axis), used with
+     * dx/dy/dz: the lines of an auto-meshed grid as make_context keeps them
+     * (GridSpacing::*_bounds from AutoMeshGenerator::generate,
+     * src/params.hpp:209-211, see fdtd_mesh.h).  Each width is the difference
+     * of two neighbouring lines, so running sums of the widths rebuild the
+     * lines except where a line is below half its successor (the first cells
+     * of an axis); passing them keeps the grid exact there too.  NULL: running
+     * sums (build_cumulative_bounds, src/global_function.hpp:160-167). */
+    const double *x_bounds, *y_bounds, *z_bounds;
 } fdtd_scene_spec;
 
 /* ---- host setup (no device needed) ----

@@ -801,9 +801,16 @@ oracle_sim* oracle_create(const fdtd_scene_spec* sp) {
         S->dz = filled(nzg, sp->dz_uniform);
     }
     S->nx_grid = nxg; S->ny_grid = nyg; S->nz_grid = nzg;
-    S->xb = cumulative_bounds(S->dx, nxg);
-    S->yb = cumulative_bounds(S->dy, nyg);
-    S->zb = cumulative_bounds(S->dz, nzg);
+    if (sp->dx && sp->dy && sp->dz && sp->x_bounds && sp->y_bounds && sp->z_bounds) {
+        /* auto-meshed lines as given (GridSpacing after AutoMeshGenerator::generate) */
+        S->xb = zalloc(nxg + 1); memcpy(S->xb, sp->x_bounds, (nxg + 1) * sizeof(double));
+        S->yb = zalloc(nyg + 1); memcpy(S->yb, sp->y_bounds, (nyg + 1) * sizeof(double));
+        S->zb = zalloc(nzg + 1); memcpy(S->zb, sp->z_bounds, (nzg + 1) * sizeof(double));
+    } else {
+        S->xb = cumulative_bounds(S->dx, nxg);
+        S->yb = cumulative_bounds(S->dy, nyg);
+        S->zb = cumulative_bounds(S->dz, nzg);
+    }
     /* GridSpacing::compute_inverses, src/global_function.hpp:98-105 */
     S->inv_dx = zalloc(nxg); S->inv_dy = zalloc(nyg); S->inv_dz = zalloc(nzg);
     for (size_t i = 0; i < nxg; ++i) S->inv_dx[i] = 1.0 / S->dx[i];

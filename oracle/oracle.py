@@ -202,3 +202,35 @@ class CpuSim:
 
 def threads() -> int:
     return _lib("oracle")["threads"]()
+
+
+REF_MESH_SO = os.path.join(HERE, "_ref", "libref_mesh.so")
+REF_MESH_TESTS = (os.path.join(HERE, "_ref", "test_auto_mesh"), os.path.join(HERE, "_ref", "verify_mesh_override"))
+
+
+def ref_auto_mesh(cfg, domain, npml, boxes=(), background_n=1.0, cap=20000):
+    """The reference's own AutoMeshGenerator::generate (compiled from the
+    reference tree into _ref/libref_mesh.so): x, y and z grid lines for an
+    AutoMeshConfig-like `cfg` (attributes as in automesh.AutoMeshConfig)."""
+    if "mesh" not in _libs:
+        lib = C.CDLL(REF_MESH_SO)
+        f = lib.ref_auto_mesh
+        d = C.c_double
+        f.restype = C.c_int
+        f.argtypes = [C.c_void_p, d, d, d, C.c_long, C.c_int, C.c_void_p, C.c_int, C.c_void_p, d, C.c_void_p,
+                      C.c_long, C.c_void_p]
+        _libs["mesh"] = f
+    f = _libs["mesh"]
+    c = np.array([cfg.lambda_min, cfg.f_max, cfg.mesh_accuracy, cfg.ppw_override, cfg.dx_min, cfg.dx_max,
+                  cfg.max_grading_ratio, float(bool(cfg.pml_graded)), float(bool(cfg.override_enabled))])
+    st = np.array([[b.x_min, b.x_max, b.y_min, b.y_max, b.z_min, b.z_max, b.n] for b in boxes],
+                  dtype=np.float64).reshape(-1, 7)
+    ov = np.array([[float(bool(o.enabled)), o.x0, o.x1, o.y0, o.y1, o.z0, o.z1, o.dx, o.dy, o.dz, o.priority]
+                   for o in cfg.overrides], dtype=np.float64).reshape(-1, 11)
+    out = np.zeros(3 * cap)
+    counts = np.zeros(3, dtype=np.int64)
+    rc = f(c.ctypes.data, float(domain[0]), float(domain[1]), float(domain[2]), int(npml), len(st),
+           st.ctypes.data, len(ov), ov.ctypes.data, float(background_n), out.ctypes.data, cap, counts.ctypes.data)
+    if rc != 0:
+        raise RuntimeError(f"ref_auto_mesh returned {rc}")
+    return [out[a * cap:a * cap + counts[a]].copy() for a in range(3)]

@@ -90,9 +90,15 @@ ref_sim* ref_create(const fdtd_scene_spec* sp) {
         g.dy.assign(nyg, sp->dy_uniform);
         g.dz.assign(nzg, sp->dz_uniform);
     }
-    build_cumulative_bounds(g.dx, g.x_bounds);
-    build_cumulative_bounds(g.dy, g.y_bounds);
-    build_cumulative_bounds(g.dz, g.z_bounds);
+    if (sp->dx && sp->dy && sp->dz && sp->x_bounds && sp->y_bounds && sp->z_bounds) {
+        g.x_bounds.assign(sp->x_bounds, sp->x_bounds + nxg + 1);
+        g.y_bounds.assign(sp->y_bounds, sp->y_bounds + nyg + 1);
+        g.z_bounds.assign(sp->z_bounds, sp->z_bounds + nzg + 1);
+    } else {
+        build_cumulative_bounds(g.dx, g.x_bounds);
+        build_cumulative_bounds(g.dy, g.y_bounds);
+        build_cumulative_bounds(g.dz, g.z_bounds);
+    }
     g.compute_inverses();
     real dx_min = *std::min_element(g.dx.begin(), g.dx.end());
     real dy_min = *std::min_element(g.dy.begin(), g.dy.end());

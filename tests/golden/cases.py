@@ -3,7 +3,8 @@
 Each case is small enough for a fixture of well under 1 MB and exercises one
 part of the reference path: the uniform CPML sphere case, a graded-mesh gain
 slab (TLS polarization + populations), population clamp, plane-wave sources
-along each axis, a PEC cavity, and every probe component.  The fixtures hold
+along each axis, a PEC cavity, every probe component, and an auto-meshed
+grid.  The fixtures hold
 the compiled reference's outputs (see make_golden.py).
 """
 import importlib
@@ -82,7 +83,31 @@ def graded_cuboids():
     return sp.copy(name="g_graded_cuboids")
 
 
+def auto_mesh_inputs(name):
+    """(base spec, AutoMeshConfig, lambda0, physical domain) of an auto-meshed
+    case: a dielectric z-cylinder and an override box on a 320 nm cube,
+    lambda0 1 um, PPW 10, dx_min / dx_max left to make_context's defaults
+    (lambda0/100, lambda0/4)."""
+    assert name == "auto_mesh_cylinder"
+    am = importlib.import_module("3d_fdtd_version008_amd.automesh")
+    base = S.uniform_sphere_case(core=10, npml=4, d=40e-9, steps=300)
+    base.structures = [S.Structure(S.SHAPE_CYLINDER_Z, 2.0, (160e-9, 160e-9, 70e-9, 90e-9, 230e-9, 0))]
+    base.dipoles[0].tau = 1e-15
+    cfg = am.AutoMeshConfig(mesh_accuracy=2, dx_min=0, dx_max=0, override_enabled=True,
+                            overrides=[am.MeshOverride(110e-9, 210e-9, 120e-9, 200e-9, 130e-9, 190e-9, 20e-9, 20e-9,
+                                                       12e-9)])
+    return base, cfg, 1000e-9, (320e-9, 320e-9, 320e-9)
+
+
+def auto_mesh_cylinder():
+    """A grid from the auto-mesher (MeshMode::AUTO_NONUNIFORM); the fixture
+    also holds the lines the reference's own generator returns for it."""
+    am = importlib.import_module("3d_fdtd_version008_amd.automesh")
+    base, cfg, lambda0, domain = auto_mesh_inputs("auto_mesh_cylinder")
+    return am.auto_mesh_spec(base, lambda0, cfg, domain).copy(name="g_auto_mesh_cylinder")
+
+
 CASES = {f.__name__: f for f in (uniform_sphere, gain_slab, gain_clamp_dipole, plane_waves, pec_cavity,
-                                  graded_cuboids)}
+                                  graded_cuboids, auto_mesh_cylinder)}
 FIELDS = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
 TLS_FIELDS = ("Nu", "Ng", "Pz", "Pz_prev")

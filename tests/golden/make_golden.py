@@ -5,7 +5,7 @@ oracle/Makefile, driven in the reference's serial step order).
 Each fixture holds, after the case's step count: the six fields, the TLS
 state (when the case has gain), every probe trace, dt, the grid shape, and the
 stability verdict.  Run from the repository root:
-    make -C oracle && python tests/golden/make_golden.py
+    make -C oracle && python tests/golden/make_golden.py [case ...]
 """
 import os
 import sys
@@ -22,11 +22,32 @@ import oracle  # noqa: E402
 import cases  # noqa: E402
 
 
+def reference_lines(name, sp):
+    """For an auto-meshed case: the lines of the reference's own generator
+    (oracle/_ref/libref_mesh.so) for the case's inputs, which must be the
+    lines the case was built with."""
+    import dataclasses
+    import importlib
+
+    am = importlib.import_module("3d_fdtd_version008_amd.automesh")
+    base, cfg, lambda0, domain = cases.auto_mesh_inputs(name)
+    cfg = dataclasses.replace(cfg, lambda_min=lambda0)
+    cfg.dx_min = cfg.dx_min if cfg.dx_min > 0 else lambda0 / 100.0
+    cfg.dx_max = cfg.dx_max if cfg.dx_max > 0 else lambda0 / 4.0
+    boxes = [am.structure_box(s) for s in base.structures]
+    lines = oracle.ref_auto_mesh(cfg, domain, base.npml, boxes, 1.0)
+    for a, mine in enumerate((sp.x_bounds, sp.y_bounds, sp.z_bounds)):
+        assert np.array_equal(lines[a], mine), f"{name}: auto-mesher differs from the reference's on axis {a}"
+    return {f"{ax}_bounds": lines[a] for a, ax in enumerate("xyz")}
+
+
 def fixture(name, make):
     sp = make()
+    extra = reference_lines(name, sp) if sp.x_bounds is not None else {}
     sim = oracle.CpuSim(sp, "ref")
     sim.step(sp.n_steps)
     out = {"dt": np.array(sim.dt), "shape": np.array(sim.shape), "steps": np.array(sp.n_steps)}
+    out.update(extra)
     for f in cases.FIELDS:
         out[f] = sim.array(f)
     if sim.info()[5]:
@@ -50,5 +71,7 @@ def fixture(name, make):
 if __name__ == "__main__":
     if not oracle.available("ref"):
         sys.exit("oracle/_ref/libref.so is not built (needs the reference tree; run make -C oracle)")
+    only = sys.argv[1:]  # optional case names
     for name, make in cases.CASES.items():
-        fixture(name, make)
+        if not only or name in only:
+            fixture(name, make)

@@ -157,7 +157,7 @@ GOLDEN = __import__("os").path.join(__import__("os").path.dirname(__file__), "go
 
 @pytest.mark.parametrize("mode", [(0, 16), (0, 64), (1, 8)], ids=lambda m: f"mode{m[0]}_c{m[1]}")
 @pytest.mark.parametrize("name", ["uniform_sphere", "gain_slab", "gain_clamp_dipole", "plane_waves", "pec_cavity",
-                                  "graded_cuboids"])
+                                  "graded_cuboids", "auto_mesh_cylinder"])
 def test_fp64_matches_reference_golden(pkg, name, mode):
     """The device FP64 path against the compiled reference's own outputs."""
     sys.path.insert(0, GOLDEN)
