@@ -97,7 +97,8 @@ struct fdtd_sim {
     int cur = 0;
     int mode = 0;       // 0: H and E kernels (default), 1: fused step kernel
     int ichunk = 8;     // x-planes per work-group of the fused kernel
-    int kwid = 0;       // k-width of a wavefront's cells in the H and E kernels (16, 32, 64; 0: by precision)
+    int kwid_h = 0;     // k-width of a wavefront's cells in the H kernel (16, 32, 64; 0: by precision)
+    int kwid_e = 0;     // the same for the E kernel
     long psi_len[12] = {};
     void* inv[3] = {};
     void* cp[3] = {};
@@ -617,7 +618,9 @@ int launch_step(fdtd_sim* s, long n, int sidx, hipEvent_t* ev) {
         const dim3 grd(a.tk.d * a.tj.d * (unsigned)s->nx);
         // narrow wavefronts pay off where instruction issue, not HBM, bounds
         // the kernels (FP16 storage)
-        const int kwid = s->kwid ? s->kwid : (s->prec == FDTD_FP16 ? 32 : 64);
+        const int kw_def = 32;  // measured best or tied for every storage format
+        const int kwid = s->kwid_h ? s->kwid_h : kw_def;
+        const int kwid_e = s->kwid_e ? s->kwid_e : kw_def;
         if (ev) (void)hipEventRecord(ev[0], s->stream);
         switch (kwid) {
         case 64: hipLaunchKernelGGL((fdtdk::k_update_H<T, R, 64>), grd, blk, 0, s->stream, a); break;
@@ -626,7 +629,7 @@ int launch_step(fdtd_sim* s, long n, int sidx, hipEvent_t* ev) {
         }
         if (ev) (void)hipEventRecord(ev[1], s->stream);
         if (ev) (void)hipEventRecord(ev[2], s->stream);
-        switch (kwid) {
+        switch (kwid_e) {
         case 64: hipLaunchKernelGGL((fdtdk::k_update_E<T, R, 64>), grd, blk, 0, s->stream, a); break;
         case 32: hipLaunchKernelGGL((fdtdk::k_update_E<T, R, 32>), grd, blk, 0, s->stream, a); break;
         default: hipLaunchKernelGGL((fdtdk::k_update_E<T, R, 16>), grd, blk, 0, s->stream, a); break;
@@ -646,9 +649,14 @@ int launch_step(fdtd_sim* s, long n, int sidx, hipEvent_t* ev) {
             pa.n = 0;
         }
     }
-    if (pa.n) hipLaunchKernelGGL((fdtdk::k_probes<T, R>), dim3(1), dim3(64), 0, s->stream, a, pa);
+    // the remaining probes ride in the first DFT launch
+    if (pa.n && s->dfts.empty()) {
+        hipLaunchKernelGGL((fdtdk::k_probes<T, R>), dim3(1), dim3(64), 0, s->stream, a, pa);
+        pa.n = 0;
+    }
     for (auto& d : s->dfts) {
-        fdtdk::DftArgs da{};
+        fdtdk::DftLaunch L{};
+        fdtdk::DftArgs& da = L.d;
         da.comp = d.comp;
         da.i0 = d.box[0]; da.j0 = d.box[2]; da.k0 = d.box[4];
         da.ny_b = d.box[3] - d.box[2];
@@ -662,7 +670,12 @@ int launch_step(fdtd_sim* s, long n, int sidx, hipEvent_t* ev) {
             da.wi[f] = -std::sin(w * t) * s->dt;
         }
         da.acc = d.d_acc;
-        hipLaunchKernelGGL((fdtdk::k_dft<T, R>), dim3((unsigned)((d.ncell + 255) / 256)), dim3(256), 0, s->stream, a, da);
+        L.fz = fdtdk::make_fastdiv((unsigned)da.nz_b);
+        L.fy = fdtdk::make_fastdiv((unsigned)da.ny_b);
+        L.bpf = (unsigned)((d.ncell + 255) / 256);
+        const unsigned nb = L.bpf * (unsigned)d.nf + (pa.n ? 1u : 0u);
+        hipLaunchKernelGGL((fdtdk::k_observe<T, R>), dim3(nb), dim3(256), 0, s->stream, a, L, pa);
+        pa.n = 0;
     }
     // TLS diagnostics after the populations (src/3D_FDTD_version008.cpp:330-336)
     if (s->tls_every > 0 && n % s->tls_every == 0 && s->tls_count < s->tls_cap) {
@@ -680,8 +693,19 @@ int launch_step(fdtd_sim* s, long n, int sidx, hipEvent_t* ev) {
 template <typename T, typename R>
 void launch_monitor(fdtd_sim* s, MonOut* slot) {
     StepArgs<T, R> a = make_args<T, R>(s);
-    const long nb = std::min<long>(std::max<long>((s->N + 2047) / 2048, 1), 2048);
-    hipLaunchKernelGGL((fdtdk::k_monitor<T, R>), dim3((unsigned)nb), dim3(256), 0, s->stream, a, s->N, s->gain ? s->gbox : 0L, slot);
+    const void* f[6] = {a.Ex, a.Ey, a.Ez, a.Hx, a.Hy, a.Hz};
+    bool aligned = true;
+    for (const void* p : f) aligned = aligned && ((uintptr_t)p % 16 == 0);
+    const long ng = s->gain ? s->gbox : 0L;
+    if (aligned) {
+        // about four 16-byte vectors per thread and array
+        const long nv = s->N / (16 / (long)sizeof(T));
+        const long nb = std::min<long>(std::max<long>((nv + 1023) / 1024, 1), 8192);
+        hipLaunchKernelGGL((fdtdk::k_monitor<T, R, true>), dim3((unsigned)nb), dim3(256), 0, s->stream, a, s->N, ng, slot);
+    } else {
+        const long nb = std::min<long>(std::max<long>((s->N + 2047) / 2048, 1), 2048);
+        hipLaunchKernelGGL((fdtdk::k_monitor<T, R, false>), dim3((unsigned)nb), dim3(256), 0, s->stream, a, s->N, ng, slot);
+    }
 }
 
 int dispatch_step(fdtd_sim* s, long n, int sidx, hipEvent_t* ev) {
@@ -1021,6 +1045,7 @@ int fdtd_add_dft(fdtd_handle h, int component, const int box[6], int n_freqs, co
     d.nf = n_freqs;
     for (int f = 0; f < n_freqs; ++f) d.freqs[f] = freqs[f];
     d.ncell = (long)(box[1] - box[0]) * (box[3] - box[2]) * (box[5] - box[4]);
+    if (d.ncell >= (1L << 31)) return fail(FDTD_EINVAL, "DFT box of 2^31 cells or more");
     const size_t b = (size_t)d.ncell * n_freqs * 2 * sizeof(double);
     if (int rc = h->alloc(reinterpret_cast<void**>(&d.d_acc), b)) return rc;
     HIPCHK(hipMemsetAsync(d.d_acc, 0, b, h->stream));
@@ -1055,12 +1080,19 @@ int fdtd_get_fp16_scales(fdtd_handle h, double* scales18) {
 
 int fdtd_set_kernel_mode(fdtd_handle h, int mode, int param) {
     if (!h || mode < 0 || mode > 1) return fail(FDTD_EINVAL, "mode must be 0 (H and E kernels) or 1 (fused step kernel)");
-    if (mode == 0 && param != 0 && param != 16 && param != 32 && param != 64)
-        return fail(FDTD_EINVAL, "mode 0: the wavefront k-width must be 16, 32 or 64 (0 keeps it)");
+    auto width_ok = [](int w) { return w == 16 || w == 32 || w == 64; };
+    const int wh = param < 256 ? param : (param & 255), we = param < 256 ? param : (param >> 8);
+    if (mode == 0 && param != 0 && !(param < 65536 && width_ok(wh) && width_ok(we)))
+        return fail(FDTD_EINVAL, "mode 0: the wavefront k-widths must be 16, 32 or 64 (0 keeps them)");
     if (mode == 1 && (param < 0 || param > fdtdk::ROW_MAX_CHUNK))
         return fail(FDTD_EINVAL, "mode 1: planes_per_chunk must be in [0, 128]");
     h->mode = mode;
-    if (param) (mode == 0 ? h->kwid : h->ichunk) = param;
+    if (param && mode == 0) {
+        h->kwid_h = wh;
+        h->kwid_e = we;
+    } else if (param) {
+        h->ichunk = param;
+    }
     return FDTD_OK;
 }
 

@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 #include <hip/hip_fp16.h>
 #include <stdint.h>
+#include <type_traits>
 #include "fdtd_fastdiv.hpp"
 
 namespace fdtdk {
@@ -1233,6 +1234,10 @@ struct ProbeArgs {
 
 template <typename T, typename R>
 __device__ __forceinline__ double field_value(const StepArgs<T, R>& a, int comp, long id) {
+    if (comp < 6) {  // a raw component: one load
+        const T* p = comp == 0 ? a.Ex : comp == 1 ? a.Ey : comp == 2 ? a.Ez : comp == 3 ? a.Hx : comp == 4 ? a.Hy : a.Hz;
+        return (double)ld<T, R>(p, id, a.sc[comp]);
+    }
     const double ex = (double)ld<T, R>(a.Ex, id, a.sc[0]), ey = (double)ld<T, R>(a.Ey, id, a.sc[1]),
                  ez = (double)ld<T, R>(a.Ez, id, a.sc[2]);
     const double hx = (double)ld<T, R>(a.Hx, id, a.sc[3]), hy = (double)ld<T, R>(a.Hy, id, a.sc[4]),
@@ -1376,10 +1381,36 @@ __device__ __forceinline__ void dft_cell(const StepArgs<T, R>& a, const DftArgs&
     }
 }
 
+// One step's observers on one domain in one launch: DftMonitor accumulation
+// (one thread per cell and frequency: blocks [f * bpf, (f + 1) * bpf) hold
+// frequency f) and, in the last block when pa.n > 0, the point probes.  Each
+// accumulator gets the same += F * w as in dft_cell.
+struct DftLaunch {
+    DftArgs d;
+    FastDiv fz, fy;  // divisions by nz_b and ny_b of the box
+    unsigned bpf;    // blocks per frequency
+};
+
 template <typename T, typename R>
-__global__ void k_dft(StepArgs<T, R> a, DftArgs d) {
-    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c < d.ncell) dft_cell<T, R>(a, d, c);
+__global__ void __launch_bounds__(256) k_observe(StepArgs<T, R> a, DftLaunch L, ProbeArgs pa) {
+    const unsigned nb_dft = L.bpf * (unsigned)L.d.nf;
+    if (blockIdx.x >= nb_dft) {
+        const int t = threadIdx.x;
+        if (t < pa.n) *pa.p[t].out = field_value<T, R>(a, pa.p[t].comp, pa.p[t].id);
+        return;
+    }
+    const unsigned f = blockIdx.x / L.bpf;
+    const unsigned c = (blockIdx.x - f * L.bpf) * blockDim.x + threadIdx.x;
+    if (c >= (unsigned)L.d.ncell) return;
+    const unsigned r = fdiv(c, L.fz);
+    const unsigned ii = fdiv(r, L.fy);
+    const int kk = (int)(c - r * (unsigned)L.d.nz_b), jj = (int)(r - ii * (unsigned)L.d.ny_b);
+    const long id = ((long)(L.d.i0 + (int)ii) * a.ny + (L.d.j0 + jj)) * a.nz + (L.d.k0 + kk);
+    double* p = L.d.acc + ((long)f * L.d.ncell + c) * 2;
+    const double re = p[0], im = p[1];
+    const double F = field_value<T, R>(a, L.d.comp, id);
+    p[0] = re + F * L.d.wr[f];
+    p[1] = im + F * L.d.wi[f];
 }
 
 // Stability reduction (StabilityMonitor::check_stability inputs,
@@ -1399,20 +1430,96 @@ __device__ __forceinline__ unsigned long long absbits(const T* p, long i, R s) {
     return (unsigned long long)__double_as_longlong(v);
 }
 
+// |raw| of a stored element: the sign bit cleared.  For the stored formats
+// (IEEE binary64/32/16) the integer order of these patterns is the numeric
+// order of |value|, with every NaN/Inf above every finite value, and the
+// conversion to the FP64 value (times a positive scale for FP16 storage) is
+// non-decreasing; so the largest pattern, converted once, is what absbits
+// would give as the maximum over the elements.
+template <typename T>
+using RawU = typename std::conditional<sizeof(T) == 8, unsigned long long, unsigned>::type;
+
+template <typename T>
+__device__ __forceinline__ RawU<T> raw_absmax16(const uint4 q) {
+    if constexpr (sizeof(T) == 8) {
+        const unsigned long long x = (((unsigned long long)q.y << 32) | q.x) & 0x7fffffffffffffffull;
+        const unsigned long long y = (((unsigned long long)q.w << 32) | q.z) & 0x7fffffffffffffffull;
+        return x > y ? x : y;
+    } else if constexpr (sizeof(T) == 4) {
+        return max(max(q.x & 0x7fffffffu, q.y & 0x7fffffffu), max(q.z & 0x7fffffffu, q.w & 0x7fffffffu));
+    } else {  // eight binary16 patterns, two per word
+        unsigned m = 0;
+        const unsigned w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+        for (int t = 0; t < 4; ++t) m = max(m, max(w[t] & 0x7fffu, (w[t] >> 16) & 0x7fffu));
+        return m;
+    }
+}
+
 template <typename T, typename R>
+__device__ __forceinline__ unsigned long long raw_to_absbits(RawU<T> m, R scale) {
+    T t;
+    if constexpr (sizeof(T) == 2) {
+        const uint16_t b = (uint16_t)m;
+        __builtin_memcpy(&t, &b, 2);
+    } else {
+        __builtin_memcpy(&t, &m, sizeof(T));
+    }
+    return absbits<T, R>(&t, 0, scale);
+}
+
+template <typename T, typename R, bool VEC = false>
 __device__ __forceinline__ void monitor_body(const StepArgs<T, R>& a, long N, long ngain, MonOut* out, int bid,
                                              int nblk) {
     unsigned long long mE = 0, mH = 0, mP = 0, nN = 0;
     double pt = 0.0, pi = 0.0;
     const long stride = (long)nblk * blockDim.x;
-    for (long id = (long)bid * blockDim.x + threadIdx.x; id < N; id += stride) {
-        unsigned long long v;
-        v = absbits<T, R>(a.Ex, id, a.sc[0]); mE = v > mE ? v : mE;
-        v = absbits<T, R>(a.Ey, id, a.sc[1]); mE = v > mE ? v : mE;
-        v = absbits<T, R>(a.Ez, id, a.sc[2]); mE = v > mE ? v : mE;
-        v = absbits<T, R>(a.Hx, id, a.sc[3]); mH = v > mH ? v : mH;
-        v = absbits<T, R>(a.Hy, id, a.sc[4]); mH = v > mH ? v : mH;
-        v = absbits<T, R>(a.Hz, id, a.sc[5]); mH = v > mH ? v : mH;
+    if constexpr (VEC) {
+        // 16-byte loads (the arrays are 16-byte aligned), per-array maxima of
+        // |raw|, one conversion per array at the end
+        constexpr int V = 16 / (int)sizeof(T);
+        const long nv = N / V;
+        const T* f[6] = {a.Ex, a.Ey, a.Ez, a.Hx, a.Hy, a.Hz};
+        RawU<T> m[6] = {0, 0, 0, 0, 0, 0};
+        for (long v = (long)bid * blockDim.x + threadIdx.x; v < nv; v += stride) {
+            uint4 q[6];
+#pragma unroll
+            for (int c = 0; c < 6; ++c) q[c] = reinterpret_cast<const uint4*>(f[c])[v];
+#pragma unroll
+            for (int c = 0; c < 6; ++c) {
+                const RawU<T> x = raw_absmax16<T>(q[c]);
+                m[c] = x > m[c] ? x : m[c];
+            }
+        }
+        for (long id = nv * V + (long)bid * blockDim.x + threadIdx.x; id < N; id += stride) {
+#pragma unroll
+            for (int c = 0; c < 6; ++c) {
+                RawU<T> x;
+                if constexpr (sizeof(T) == 2) {
+                    x = (RawU<T>)(reinterpret_cast<const uint16_t*>(f[c])[id] & 0x7fffu);
+                } else {
+                    __builtin_memcpy(&x, f[c] + id, sizeof(T));
+                    x &= sizeof(T) == 8 ? (RawU<T>)0x7fffffffffffffffull : (RawU<T>)0x7fffffffu;
+                }
+                m[c] = x > m[c] ? x : m[c];
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            const unsigned long long v = raw_to_absbits<T, R>(m[c], a.sc[c]);
+            if (c < 3) mE = v > mE ? v : mE;
+            else mH = v > mH ? v : mH;
+        }
+    } else {
+        for (long id = (long)bid * blockDim.x + threadIdx.x; id < N; id += stride) {
+            unsigned long long v;
+            v = absbits<T, R>(a.Ex, id, a.sc[0]); mE = v > mE ? v : mE;
+            v = absbits<T, R>(a.Ey, id, a.sc[1]); mE = v > mE ? v : mE;
+            v = absbits<T, R>(a.Ez, id, a.sc[2]); mE = v > mE ? v : mE;
+            v = absbits<T, R>(a.Hx, id, a.sc[3]); mH = v > mH ? v : mH;
+            v = absbits<T, R>(a.Hy, id, a.sc[4]); mH = v > mH ? v : mH;
+            v = absbits<T, R>(a.Hz, id, a.sc[5]); mH = v > mH ? v : mH;
+        }
     }
     for (long g = (long)bid * blockDim.x + threadIdx.x; g < ngain; g += stride) {
         const int r = a.greg[g];
@@ -1470,9 +1577,9 @@ __device__ __forceinline__ void monitor_body(const StepArgs<T, R>& a, long N, lo
     }
 }
 
-template <typename T, typename R>
+template <typename T, typename R, bool VEC>
 __global__ void __launch_bounds__(256) k_monitor(StepArgs<T, R> a, long N, long ngain, MonOut* out) {
-    monitor_body<T, R>(a, N, ngain, out, blockIdx.x, gridDim.x);
+    monitor_body<T, R, VEC>(a, N, ngain, out, blockIdx.x, gridDim.x);
 }
 
 // ---- Multi-domain (batch) step ----

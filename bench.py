@@ -94,17 +94,27 @@ def cpu_baseline(spec, steps: int):
                       f"(oracle/fdtd_oracle.c, OpenMP over i, serial CPML, J zero-fill, monitor every 10)"}
 
 
+def _kw_param(args) -> int:
+    """fdtd_set_kernel_mode's mode-0 parameter: one width, or H + 256 * E."""
+    if args.kwidth_e and args.kwidth_e != args.kwidth:
+        return (args.kwidth or 32) + 256 * args.kwidth_e
+    return args.kwidth
+
+
 def plain(args):
     pkg = importlib.import_module("3d_fdtd_version008_amd")
     if args.copy_calib:
         copy_calibration(args.copy_calib << 20)
-    spec = pkg.scene.gain_slab_case(n_total=args.size, npml=12, steps=5000)
+    if args.workload == "c3":
+        spec = pkg.scene.precision_case(steps=5000)
+    else:
+        spec = pkg.scene.gain_slab_case(n_total=args.size, npml=12, steps=5000)
     t0 = time.time()
     setup = pkg.Setup(spec)
     t_setup = time.time() - t0
     sc = pkg.Scene(setup, args.precision, probe_capacity=args.steps + args.warmup + 16)
     info = sc.info()
-    sc.set_kernel_mode(args.mode, args.kwidth if args.mode == 0 else args.ichunk)
+    sc.set_kernel_mode(args.mode, _kw_param(args) if args.mode == 0 else args.ichunk)
     sc.step(args.warmup)
     sc.synchronize()
     t1 = time.perf_counter()
@@ -151,7 +161,8 @@ def plain(args):
                    "precision": args.precision, "gain_cells": gain_cells, "materials": info["n_materials"],
                    "mat_id_bytes": info["mat_id_bytes"], "setup_s": round(t_setup, 2),
                    "kernel_mode": ["h_e_kernels", "fused_rows"][args.mode], "planes_per_chunk": args.ichunk,
-                   "wavefront_kwidth": (args.kwidth or (32 if args.precision == "fp16" else 64)) if args.mode == 0 else None},
+                   "wavefront_kwidth": _kw_param(args) or 32
+                   if args.mode == 0 else None},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes_per_launch": kbytes, "avg_launch_us": round(kt * 1e6, 2)},
@@ -195,7 +206,7 @@ def precision_form(args):
     rates, scenes = {}, {}
     for prec in ("fp64", "fp32", "fp16"):
         sc = pkg.Scene(setup, prec, probe_capacity=args.steps + 16)
-        sc.set_kernel_mode(args.mode, args.kwidth if args.mode == 0 else args.ichunk)
+        sc.set_kernel_mode(args.mode, _kw_param(args) if args.mode == 0 else args.ichunk)
         done = 0
         if prec != "fp64":
             sc.step(args.warmup)
@@ -328,11 +339,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--precision", default="fp32", choices=["fp64", "fp32", "fp16"])
     ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3"],
+                    help="plain form: c2 = size^3 gain slab (default), c3 = 512x512x384 with 64 cuboids")
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mode", type=int, default=0, help="0: separate H/E kernels, 1: fused step kernel")
     ap.add_argument("--ichunk", type=int, default=0, help="fused kernel x-planes per work-group; 0 = library default")
     ap.add_argument("--kwidth", type=int, default=0, help="H/E kernels: wavefront k-width (16, 32, 64); 0 = library default")
+    ap.add_argument("--kwidth-e", type=int, default=0, help="E kernel's own wavefront k-width; 0 = same as --kwidth")
     ap.add_argument("--rate-steps", type=int, default=200, help="precision form: timed steps per format")
     ap.add_argument("--domains", type=int, default=256, help="batch form: domains")
     ap.add_argument("--requests", type=int, default=64, help="serve form: requests")

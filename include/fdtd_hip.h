@@ -187,8 +187,9 @@ int fdtd_set_fp16_scales(fdtd_handle h, const double* scales18);
 int fdtd_get_fp16_scales(fdtd_handle h, double* scales18);
 /* Kernel schedule: mode 0 = separate H and E kernels (in place, default),
  * `param` = the k-width of the cells one wavefront updates (16, 32 or 64
- * consecutive k of 64/param rows; by default 32 for FP16 storage and 64
- * otherwise; 0 keeps the current value);
+ * consecutive k of 64/param rows; 32 by default; 0 keeps the current
+ * value), for both kernels, or
+ * width_H + 256 * width_E to set them apart;
  * mode 1 = one fused step kernel in which each wavefront owns one j-row and
  * marches along x over `param` planes (1..128; 0 keeps the current value),
  * double-buffering E, H and psi_H.  Grids the fused kernel does not take
