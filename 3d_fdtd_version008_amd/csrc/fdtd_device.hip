@@ -131,6 +131,7 @@ struct fdtd_sim {
     int mon_interval = 10;
     MonOut* d_mon = nullptr;
     int mon_cap = 0;
+    MonOut* d_monpart = nullptr;  // per-block partial records of the monitor reduction
     std::vector<fdtd_check_record> checks;
     double prev_max_E = 0.0;
     size_t inst_count = 0;
@@ -595,6 +596,7 @@ void set_cur(fdtd_sim* s, int c) {
 }
 
 constexpr int TLS_PARTS = 256;  // blocks of the TLS diagnostic reduction
+constexpr long MONPART_MAX = 2048;  // blocks (and partial records) of the monitor reduction
 
 template <typename T, typename R>
 int launch_step(fdtd_sim* s, long n, int sidx, hipEvent_t* ev) {
@@ -697,11 +699,16 @@ void launch_monitor(fdtd_sim* s, MonOut* slot) {
     bool aligned = true;
     for (const void* p : f) aligned = aligned && ((uintptr_t)p % 16 == 0);
     const long ng = s->gain ? s->gbox : 0L;
-    if (aligned) {
-        // about four 16-byte vectors per thread and array
+    if (aligned && s->d_monpart) {
+        // at least four 16-byte vectors per thread and array, at most
+        // MONPART_MAX blocks: block records (no same-address atomics),
+        // merged by one small launch
         const long nv = s->N / (16 / (long)sizeof(T));
-        const long nb = std::min<long>(std::max<long>((nv + 1023) / 1024, 1), 8192);
-        hipLaunchKernelGGL((fdtdk::k_monitor<T, R, true>), dim3((unsigned)nb), dim3(256), 0, s->stream, a, s->N, ng, slot);
+        const long nb = std::min<long>(std::max<long>((nv + 1023) / 1024, 1), MONPART_MAX);
+        hipLaunchKernelGGL((fdtdk::k_monitor<T, R, true>), dim3((unsigned)nb), dim3(256), 0, s->stream, a, s->N, ng,
+                           s->d_monpart);
+        hipLaunchKernelGGL(fdtdk::k_monitor_final, dim3(1), dim3(64), 0, s->stream, (const MonOut*)s->d_monpart,
+                           (int)nb, slot);
     } else {
         const long nb = std::min<long>(std::max<long>((s->N + 2047) / 2048, 1), 2048);
         hipLaunchKernelGGL((fdtdk::k_monitor<T, R, false>), dim3((unsigned)nb), dim3(256), 0, s->stream, a, s->N, ng, slot);
@@ -925,6 +932,11 @@ int fdtd_create(const fdtd_grid_desc* g, const fdtd_material_desc* m, const fdtd
         for (int q = 0; q < 18; ++q) sc[36 + q] = std::ldexp(sc[q], -6);  // floors
         if (hipMemcpy(s->d_sc, sc, sizeof(sc), hipMemcpyHostToDevice) != hipSuccess) rc = fail(FDTD_EDEVICE, "scale init failed");
         if (!rc && hipMemset(s->d_maxbits, 0, 18 * sizeof(unsigned)) != hipSuccess) rc = fail(FDTD_EDEVICE, "scale init failed");
+    }
+    if (!rc) {  // monitor block partials (allocated here so the footprint is known at creation)
+        void* p = nullptr;
+        rc = s->alloc(&p, (size_t)MONPART_MAX * sizeof(MonOut));
+        s->d_monpart = (MonOut*)p;
     }
     if (!rc && hipStreamSynchronize(s->stream) != hipSuccess) rc = fail(FDTD_EDEVICE, "initialisation failed");
     if (rc) {

@@ -1424,6 +1424,42 @@ struct MonOut {
     double pop_total, pop_init;
 };
 
+// One monitor record per work-group: wavefront reduction (64 lanes), then
+// across the work-group's wavefronts in LDS in wavefront order.  Every thread
+// of the work-group calls it; thread 0 gets the record.
+__device__ __forceinline__ MonOut block_record(unsigned long long mE, unsigned long long mH, unsigned long long mP,
+                                               unsigned long long nN, double pt, double pi) {
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned long long o;
+        o = __shfl_down(mE, off); mE = o > mE ? o : mE;
+        o = __shfl_down(mH, off); mH = o > mH ? o : mH;
+        o = __shfl_down(mP, off); mP = o > mP ? o : mP;
+        o = __shfl_down(nN, off); nN = o > nN ? o : nN;
+        pt += __shfl_down(pt, off);
+        pi += __shfl_down(pi, off);
+    }
+    __shared__ unsigned long long su[4][4];
+    __shared__ double sd[2][4];
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        su[0][w] = mE; su[1][w] = mH; su[2][w] = mP; su[3][w] = nN;
+        sd[0][w] = pt; sd[1][w] = pi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = (int)(blockDim.x >> 6);
+        for (int q = 1; q < nw; ++q) {
+            mE = su[0][q] > mE ? su[0][q] : mE;
+            mH = su[1][q] > mH ? su[1][q] : mH;
+            mP = su[2][q] > mP ? su[2][q] : mP;
+            nN = su[3][q] > nN ? su[3][q] : nN;
+            pt += sd[0][q];
+            pi += sd[1][q];
+        }
+    }
+    return MonOut{mE, mH, mP, nN, pt, pi};
+}
+
 template <typename T, typename R>
 __device__ __forceinline__ unsigned long long absbits(const T* p, long i, R s) {
     const double v = fabs((double)ld<T, R>(p, i, s));
@@ -1539,8 +1575,40 @@ __device__ __forceinline__ void monitor_body(const StepArgs<T, R>& a, long N, lo
             pi += a.tls[r].N0;
         }
     }
-    // wavefront reduction (64 lanes), then across the block's wavefronts in
-    // LDS: one set of atomics per block.
+    const MonOut rec = block_record(mE, mH, mP, nN, pt, pi);
+    if (threadIdx.x == 0) {
+        if constexpr (VEC) {
+            out[bid] = rec;  // one partial record per block; k_monitor_final merges them
+        } else {
+            if (rec.maxE) atomicMax(&out->maxE, rec.maxE);
+            if (rec.maxH) atomicMax(&out->maxH, rec.maxH);
+            if (rec.maxP) atomicMax(&out->maxP, rec.maxP);
+            if (rec.nanN) atomicMax(&out->nanN, rec.nanN);
+            if (rec.pop_total != 0.0) atomicAdd(&out->pop_total, rec.pop_total);
+            if (rec.pop_init != 0.0) atomicAdd(&out->pop_init, rec.pop_init);
+        }
+    }
+}
+
+template <typename T, typename R, bool VEC>
+__global__ void __launch_bounds__(256) k_monitor(StepArgs<T, R> a, long N, long ngain, MonOut* out) {
+    monitor_body<T, R, VEC>(a, N, ngain, out, blockIdx.x, gridDim.x);
+}
+
+// Merges the per-block monitor records in a fixed order (so the population
+// sums are the same on every run) into the step's record.
+__global__ void __launch_bounds__(64) k_monitor_final(const MonOut* __restrict__ part, int n, MonOut* slot) {
+    unsigned long long mE = 0, mH = 0, mP = 0, nN = 0;
+    double pt = 0.0, pi = 0.0;
+    for (int q = threadIdx.x; q < n; q += 64) {
+        const MonOut r = part[q];
+        mE = r.maxE > mE ? r.maxE : mE;
+        mH = r.maxH > mH ? r.maxH : mH;
+        mP = r.maxP > mP ? r.maxP : mP;
+        nN = r.nanN > nN ? r.nanN : nN;
+        pt += r.pop_total;
+        pi += r.pop_init;
+    }
     for (int off = 32; off > 0; off >>= 1) {
         unsigned long long o;
         o = __shfl_down(mE, off); mE = o > mE ? o : mE;
@@ -1550,36 +1618,7 @@ __device__ __forceinline__ void monitor_body(const StepArgs<T, R>& a, long N, lo
         pt += __shfl_down(pt, off);
         pi += __shfl_down(pi, off);
     }
-    __shared__ unsigned long long su[4][4];
-    __shared__ double sd[2][4];
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) {
-        su[0][w] = mE; su[1][w] = mH; su[2][w] = mP; su[3][w] = nN;
-        sd[0][w] = pt; sd[1][w] = pi;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const int nw = (int)(blockDim.x >> 6);
-        for (int q = 1; q < nw; ++q) {
-            mE = su[0][q] > mE ? su[0][q] : mE;
-            mH = su[1][q] > mH ? su[1][q] : mH;
-            mP = su[2][q] > mP ? su[2][q] : mP;
-            nN = su[3][q] > nN ? su[3][q] : nN;
-            pt += sd[0][q];
-            pi += sd[1][q];
-        }
-        if (mE) atomicMax(&out->maxE, mE);
-        if (mH) atomicMax(&out->maxH, mH);
-        if (mP) atomicMax(&out->maxP, mP);
-        if (nN) atomicMax(&out->nanN, nN);
-        if (pt != 0.0) atomicAdd(&out->pop_total, pt);
-        if (pi != 0.0) atomicAdd(&out->pop_init, pi);
-    }
-}
-
-template <typename T, typename R, bool VEC>
-__global__ void __launch_bounds__(256) k_monitor(StepArgs<T, R> a, long N, long ngain, MonOut* out) {
-    monitor_body<T, R, VEC>(a, N, ngain, out, blockIdx.x, gridDim.x);
+    if (threadIdx.x == 0) *slot = MonOut{mE, mH, mP, nN, pt, pi};
 }
 
 // ---- Multi-domain (batch) step ----

@@ -161,7 +161,7 @@ def plain(args):
                    "precision": args.precision, "gain_cells": gain_cells, "materials": info["n_materials"],
                    "mat_id_bytes": info["mat_id_bytes"], "setup_s": round(t_setup, 2),
                    "kernel_mode": ["h_e_kernels", "fused_rows"][args.mode], "planes_per_chunk": args.ichunk,
-                   "wavefront_kwidth": _kw_param(args) or 32
+                   "wavefront_kwidth": {"H": args.kwidth or 32, "E": args.kwidth_e or args.kwidth or 32}
                    if args.mode == 0 else None},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
