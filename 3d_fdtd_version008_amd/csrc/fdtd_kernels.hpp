@@ -52,6 +52,31 @@ __device__ __forceinline__ void st(T* p, long i, R v, R inv_scale) {
         p[i] = (T)v;
     }
 }
+// Cell-indexed access with index type IX: long (64-bit element offsets) or
+// unsigned, where the host guarantees every array the kernel touches is below
+// 4 GiB: the byte offset is then formed in 32 bits, so an access is a uniform
+// base plus one 32-bit offset register (neighbours at +1 become immediate
+// offsets) instead of a 64-bit address per access.
+template <typename IX>
+constexpr bool is_off32() { return sizeof(IX) == 4; }
+template <typename P, typename IX>
+__device__ __forceinline__ P* at(P* p, IX i) {
+    if constexpr (is_off32<IX>()) {
+        using C = typename std::conditional<std::is_const<P>::value, const char, char>::type;
+        return reinterpret_cast<P*>(reinterpret_cast<C*>(p) + (unsigned)i * (unsigned)sizeof(P));
+    } else {
+        return p + (long)i;
+    }
+}
+template <typename T, typename R, typename IX>
+__device__ __forceinline__ R ldx(const T* p, IX i, R scale) {
+    return ld<T, R>(at(p, i), 0L, scale);
+}
+template <typename T, typename R, typename IX>
+__device__ __forceinline__ void stx(T* p, IX i, R v, R inv_scale) {
+    st<T, R>(at(p, i), 0L, v, inv_scale);
+}
+
 // The compiler cannot see through this (no instruction is emitted): a value
 // that passes through it is not a constant, so operations on a variable that
 // is either such a default or a loaded value are not folded into the branch
@@ -150,24 +175,27 @@ __device__ __forceinline__ bool in_pml(int i, int n, int np) {
     return d < np;
 }
 // psi slab offsets
-__device__ __forceinline__ long psi_x(int i, int j, int k, int nx, int ny, int nz, int np) {
+template <typename IX = long>
+__device__ __forceinline__ IX psi_x(int i, int j, int k, int nx, int ny, int nz, int np) {
     int ii = i < np ? i : i - (nx - 2 * np);
-    return ((long)ii * ny + j) * nz + k;
+    return ((IX)ii * ny + j) * nz + k;
 }
-__device__ __forceinline__ long psi_y(int i, int j, int k, int nx, int ny, int nz, int np) {
+template <typename IX = long>
+__device__ __forceinline__ IX psi_y(int i, int j, int k, int nx, int ny, int nz, int np) {
     (void)nx;
     int jj = j < np ? j : j - (ny - 2 * np);
-    return ((long)i * (2 * np) + jj) * nz + k;
+    return ((IX)i * (2 * np) + jj) * nz + k;
 }
-__device__ __forceinline__ long psi_z(int i, int j, int k, int nx, int ny, int nz, int np) {
+template <typename IX = long>
+__device__ __forceinline__ IX psi_z(int i, int j, int k, int nx, int ny, int nz, int np) {
     (void)nx;
     int kk = k < np ? k : k - (nz - 2 * np);
-    return ((long)i * ny + j) * (2 * np) + kk;
+    return ((IX)i * ny + j) * (2 * np) + kk;
 }
 
-template <typename R>
-__device__ __forceinline__ int mat_of(const void* m, int mid16, long id) {
-    return mid16 ? (int)reinterpret_cast<const uint16_t*>(m)[id] : (int)reinterpret_cast<const uint8_t*>(m)[id];
+template <typename R, typename IX = long>
+__device__ __forceinline__ int mat_of(const void* m, int mid16, IX id) {
+    return mid16 ? (int)*at(reinterpret_cast<const uint16_t*>(m), id) : (int)*at(reinterpret_cast<const uint8_t*>(m), id);
 }
 
 // Where a kernel reads the per-axis tables from: global memory, or the
@@ -398,23 +426,23 @@ __device__ __forceinline__ HIn<R> h_load(const StepArgs<T, R>& a, const Tabs<R>&
 // cell), then E^n and H^{n-1/2}, psi_H and the z-axis table, with clamped
 // indices under wave-uniform conditions and opaque defaults; X/Y axis values
 // and cell-invariant coefficients come through the scalar cache.
-template <typename T, typename R, int KW = 64, bool CP = true>
+template <typename T, typename R, int KW = 64, bool CP = true, typename IX = long>
 __device__ __forceinline__ HIn<R> h_gather(const StepArgs<T, R>& a, const Tabs<R>& tb, int i, int j, int k, int kw,
                                            int jw) {
     constexpr int RW = 64 / KW;  // rows per wavefront
     HIn<R> v;
-    const long id = ((long)i * a.ny + j) * a.nz + k;
+    const IX id = ((IX)i * a.ny + j) * a.nz + k;
     const int np = a.np, nx = a.nx, ny = a.ny, nz = a.nz;
     int m = 0;
-    if (!a.h_uniform) m = mat_of<R>(a.matid, a.mid16, id);
+    if (!a.h_uniform) m = mat_of<R, IX>(a.matid, a.mid16, id);
     v.upd = true;
-    const long sI = a.sI, sJ = a.sJ;
-    v.hx = ld<T, R>(a.Hx, id, a.sc[3]);
-    v.hy = ld<T, R>(a.Hy, id, a.sc[4]);
-    v.hz = ld<T, R>(a.Hz, id, a.sc[5]);
-    v.ex = ld<T, R>(a.Ex, id, a.sc[0]); v.ex_j = ld<T, R>(a.Ex, id + sJ, a.sc[0]); v.ex_k = ld<T, R>(a.Ex, id + 1, a.sc[0]);
-    v.ey = ld<T, R>(a.Ey, id, a.sc[1]); v.ey_i = ld<T, R>(a.Ey, id + sI, a.sc[1]); v.ey_k = ld<T, R>(a.Ey, id + 1, a.sc[1]);
-    v.ez = ld<T, R>(a.Ez, id, a.sc[2]); v.ez_i = ld<T, R>(a.Ez, id + sI, a.sc[2]); v.ez_j = ld<T, R>(a.Ez, id + sJ, a.sc[2]);
+    const IX sI = (IX)a.sI, sJ = (IX)a.sJ;
+    v.hx = ldx<T, R>(a.Hx, id, a.sc[3]);
+    v.hy = ldx<T, R>(a.Hy, id, a.sc[4]);
+    v.hz = ldx<T, R>(a.Hz, id, a.sc[5]);
+    v.ex = ldx<T, R>(a.Ex, id, a.sc[0]); v.ex_j = ldx<T, R>(a.Ex, id + sJ, a.sc[0]); v.ex_k = ldx<T, R>(a.Ex, id + 1, a.sc[0]);
+    v.ey = ldx<T, R>(a.Ey, id, a.sc[1]); v.ey_i = ldx<T, R>(a.Ey, id + sI, a.sc[1]); v.ey_k = ldx<T, R>(a.Ey, id + 1, a.sc[1]);
+    v.ez = ldx<T, R>(a.Ez, id, a.sc[2]); v.ez_i = ldx<T, R>(a.Ez, id + sI, a.sc[2]); v.ez_j = ldx<T, R>(a.Ez, id + sJ, a.sc[2]);
     using RawT = typename Raw<T>::type;
     RawT praw[6];
     for (int q = 0; q < 6; ++q) praw[q] = opaque((RawT)0);
@@ -422,19 +450,19 @@ __device__ __forceinline__ HIn<R> h_gather(const StepArgs<T, R>& a, const Tabs<R
     const bool pyw = CP && a.cpml && (in_pml(jw, ny, np) | in_pml(jw + RW - 1, ny, np));
     const bool pzw = CP && a.cpml && ((kw < np) | (kw + KW - 1 >= nz - np));
     if (pyw) {
-        const long q = psi_y(i, in_pml(j, ny, np) ? j : 0, k, nx, ny, nz, np);
-        praw[0] = a.psi[6][q];
-        praw[5] = a.psi[11][q];
+        const IX q = psi_y<IX>(i, in_pml(j, ny, np) ? j : 0, k, nx, ny, nz, np);
+        praw[0] = *at(a.psi[6], q);
+        praw[5] = *at(a.psi[11], q);
     }
     if (pzw) {
-        const long q = psi_z(i, j, in_pml(k, nz, np) ? k : 0, nx, ny, nz, np);
-        praw[1] = a.psi[7][q];
-        praw[2] = a.psi[8][q];
+        const IX q = psi_z<IX>(i, j, in_pml(k, nz, np) ? k : 0, nx, ny, nz, np);
+        praw[1] = *at(a.psi[7], q);
+        praw[2] = *at(a.psi[8], q);
     }
     if (px) {
-        const long q = psi_x(i, j, k, nx, ny, nz, np);
-        praw[3] = a.psi[9][q];
-        praw[4] = a.psi[10][q];
+        const IX q = psi_x<IX>(i, j, k, nx, ny, nz, np);
+        praw[3] = *at(a.psi[9], q);
+        praw[4] = *at(a.psi[10], q);
     }
     const Axes<R>& ax = tb.ax;
     v.idx = ax.X(0, i);
@@ -461,11 +489,11 @@ __device__ __forceinline__ HIn<R> h_gather(const StepArgs<T, R>& a, const Tabs<R
 // `own` (halo recomputation by a neighbouring tile must not store).  Cells the
 // reference never updates (i = NxT-1, j = NyT-1 or k = NzT-1) return their
 // stored value.  Returns the values a later read of the stored H gives.
-template <typename T, typename R, bool CP = true>
+template <typename T, typename R, bool CP = true, typename IX = long>
 __device__ __forceinline__ H3<R> h_compute(const StepArgs<T, R>& a, const Tabs<R>& tb, int i, int j, int k,
                                            const HIn<R>& v, bool own, bool store_fields = true) {
     if (!v.upd) return H3<R>{v.hx, v.hy, v.hz};
-    const long id = ((long)i * a.ny + j) * a.nz + k;
+    const IX id = ((IX)i * a.ny + j) * a.nz + k;
     const R idx = v.idx, idy = v.idy, idz = v.idz;
     const R dEz_y = (v.ez_j - v.ez) * idy, dEy_z = (v.ey_k - v.ey) * idz;
     const R dEx_z = (v.ex_k - v.ex) * idz, dEz_x = (v.ez_i - v.ez) * idx;
@@ -488,27 +516,27 @@ __device__ __forceinline__ H3<R> h_compute(const StepArgs<T, R>& a, const Tabs<R
             if (py) hz -= bHz * (-cpml_term<R>(v.p[5], dEx_y, v.ky[0], v.ky[1], v.ky[2], v.ky[3], pn[5]));
             if (own) {
                 if (py) {
-                    const long q = psi_y(i, j, k, nx, ny, nz, np);
-                    st<T, R>(psi_wr(a, tb, 6), q, pn[0], a.sc[30]);
-                    st<T, R>(psi_wr(a, tb, 11), q, pn[5], a.sc[35]);
+                    const IX q = psi_y<IX>(i, j, k, nx, ny, nz, np);
+                    stx<T, R>(psi_wr(a, tb, 6), q, pn[0], a.sc[30]);
+                    stx<T, R>(psi_wr(a, tb, 11), q, pn[5], a.sc[35]);
                 }
                 if (pz) {
-                    const long q = psi_z(i, j, k, nx, ny, nz, np);
-                    st<T, R>(psi_wr(a, tb, 7), q, pn[1], a.sc[31]);
-                    st<T, R>(psi_wr(a, tb, 8), q, pn[2], a.sc[32]);
+                    const IX q = psi_z<IX>(i, j, k, nx, ny, nz, np);
+                    stx<T, R>(psi_wr(a, tb, 7), q, pn[1], a.sc[31]);
+                    stx<T, R>(psi_wr(a, tb, 8), q, pn[2], a.sc[32]);
                 }
                 if (px) {
-                    const long q = psi_x(i, j, k, nx, ny, nz, np);
-                    st<T, R>(psi_wr(a, tb, 9), q, pn[3], a.sc[33]);
-                    st<T, R>(psi_wr(a, tb, 10), q, pn[4], a.sc[34]);
+                    const IX q = psi_x<IX>(i, j, k, nx, ny, nz, np);
+                    stx<T, R>(psi_wr(a, tb, 9), q, pn[3], a.sc[33]);
+                    stx<T, R>(psi_wr(a, tb, 10), q, pn[4], a.sc[34]);
                 }
             }
         }
     }
     if (own && store_fields) {
-        st<T, R>(a.Hxw, id, hx, a.sc[21]);
-        st<T, R>(a.Hyw, id, hy, a.sc[22]);
-        st<T, R>(a.Hzw, id, hz, a.sc[23]);
+        stx<T, R>(a.Hxw, id, hx, a.sc[21]);
+        stx<T, R>(a.Hyw, id, hy, a.sc[22]);
+        stx<T, R>(a.Hzw, id, hz, a.sc[23]);
     }
     return H3<R>{(R)stored_value<T, R>(hx, a.sc[3], a.sc[21]), (R)stored_value<T, R>(hy, a.sc[4], a.sc[22]),
                  (R)stored_value<T, R>(hz, a.sc[5], a.sc[23])};
@@ -566,14 +594,14 @@ __device__ __forceinline__ Tile tile_of(const StepArgs<T, R>& a) {
 
 // H update kernel of the two-kernel step: one 256-thread work-group per
 // tile, each thread one cell (tile_of).
-template <typename T, typename R, int KW>
+template <typename T, typename R, int KW, typename IX = long>
 __global__ void __launch_bounds__(256) k_update_H(StepArgs<T, R> a) {
     FDTD_SCALES_IN_REGISTERS(a)
     const Tabs<R> tb = global_tabs(a);
     const Tile t = tile_of<KW>(a);
     const int i = t.i, j = t.j, k = t.k;
     if (k >= a.nz - 1 || j >= a.ny - 1 || i >= a.nx - 1) return;
-    (void)h_compute<T, R>(a, tb, i, j, k, h_gather<T, R, KW>(a, tb, i, j, k, t.kw, t.jw), true);
+    (void)h_compute<T, R, true, IX>(a, tb, i, j, k, h_gather<T, R, KW, true, IX>(a, tb, i, j, k, t.kw, t.jw), true);
 }
 
 // Two-level polarization step (update_polarization_multi_region,
@@ -682,34 +710,34 @@ __device__ __forceinline__ void e_axes(const Axes<R>& ax, int i, int j, int k, b
 // of its lanes is in the slab.
 // Returns H^{n+1/2} at the cell and at its -x, -y, -z neighbours in h[9]:
 // hx hy hz hx_j hz_j hx_k hy_k hy_i hz_i.
-template <typename T, typename R, int KW = 64, bool CP = true, bool GN = true>
+template <typename T, typename R, int KW = 64, bool CP = true, bool GN = true, typename IX = long>
 __device__ __forceinline__ EIn<R> e_gather(const StepArgs<T, R>& a, const Tabs<R>& tb, int i, int j, int k, int kw,
                                            int jw, R* h) {
     constexpr int RW = 64 / KW;  // rows per wavefront
     EIn<R> v;
-    const long id = ((long)i * a.ny + j) * a.nz + k;
+    const IX id = ((IX)i * a.ny + j) * a.nz + k;
     const int np = a.np, nx = a.nx, ny = a.ny, nz = a.nz;
     // ids
-    const int m = mat_of<R>(a.matid, a.mid16, id);
+    const int m = mat_of<R, IX>(a.matid, a.mid16, id);
     const int gi = i - a.g0x, gj = j - a.g0y, gk = k - a.g0z;
     const bool growu = GN && a.gain && ((unsigned)gi < (unsigned)a.gnx) & (jw < a.g0y + a.gny) & (jw + RW - 1 >= a.g0y) &
                        (kw < a.g0z + a.gnz) & (kw + KW - 1 >= a.g0z);
     const bool gin = growu && ((unsigned)gj < (unsigned)a.gny) & ((unsigned)gk < (unsigned)a.gnz);
-    const long g = ((long)gi * a.gny + (gin ? gj : 0)) * a.gnz + (gin ? gk : 0);
+    const IX g = ((IX)gi * a.gny + (gin ? gj : 0)) * a.gnz + (gin ? gk : 0);
     int r8 = opaque(-1);
-    if (growu) r8 = a.greg[g];
+    if (growu) r8 = *at(a.greg, g);
     // fields.  On the rows i == 0 or j == 0, which the reference does not
     // update, the -x/-y offsets are taken as 0 and on the first row id - 1 as
     // id, so every address is in bounds; those values are never used (nor
     // is the k == 0 lane's H(k-1)).
-    const long sI = i > 0 ? a.sI : 0, sJ = j > 0 ? (long)a.sJ : 0;
-    const long km = id > 0 ? 1 : 0;
-    v.ez0 = ld<T, R>(a.Ez, id, a.sc[2]);
-    h[0] = ld<T, R>(a.Hx, id, a.sc[3]); h[3] = ld<T, R>(a.Hx, id - sJ, a.sc[3]); h[5] = ld<T, R>(a.Hx, id - km, a.sc[3]);
-    h[1] = ld<T, R>(a.Hy, id, a.sc[4]); h[7] = ld<T, R>(a.Hy, id - sI, a.sc[4]); h[6] = ld<T, R>(a.Hy, id - km, a.sc[4]);
-    h[2] = ld<T, R>(a.Hz, id, a.sc[5]); h[8] = ld<T, R>(a.Hz, id - sI, a.sc[5]); h[4] = ld<T, R>(a.Hz, id - sJ, a.sc[5]);
-    v.ex0 = ld<T, R>(a.Ex, id, a.sc[0]);
-    v.ey0 = ld<T, R>(a.Ey, id, a.sc[1]);
+    const IX sI = i > 0 ? (IX)a.sI : 0, sJ = j > 0 ? (IX)a.sJ : 0;
+    const IX km = id > 0 ? 1 : 0;
+    v.ez0 = ldx<T, R>(a.Ez, id, a.sc[2]);
+    h[0] = ldx<T, R>(a.Hx, id, a.sc[3]); h[3] = ldx<T, R>(a.Hx, id - sJ, a.sc[3]); h[5] = ldx<T, R>(a.Hx, id - km, a.sc[3]);
+    h[1] = ldx<T, R>(a.Hy, id, a.sc[4]); h[7] = ldx<T, R>(a.Hy, id - sI, a.sc[4]); h[6] = ldx<T, R>(a.Hy, id - km, a.sc[4]);
+    h[2] = ldx<T, R>(a.Hz, id, a.sc[5]); h[8] = ldx<T, R>(a.Hz, id - sI, a.sc[5]); h[4] = ldx<T, R>(a.Hz, id - sJ, a.sc[5]);
+    v.ex0 = ldx<T, R>(a.Ex, id, a.sc[0]);
+    v.ey0 = ldx<T, R>(a.Ey, id, a.sc[1]);
     // psi_E on the slabs
     using RawT = typename Raw<T>::type;
     RawT praw[6];
@@ -718,24 +746,24 @@ __device__ __forceinline__ EIn<R> e_gather(const StepArgs<T, R>& a, const Tabs<R
     const bool pyw = CP && a.cpml && (in_pml(jw, ny, np) | in_pml(jw + RW - 1, ny, np));
     const bool pzw = CP && a.cpml && ((kw < np) | (kw + KW - 1 >= nz - np));
     if (pyw) {
-        const long q = psi_y(i, in_pml(j, ny, np) ? j : 0, k, nx, ny, nz, np);
-        praw[0] = a.psi[0][q];
-        praw[5] = a.psi[5][q];
+        const IX q = psi_y<IX>(i, in_pml(j, ny, np) ? j : 0, k, nx, ny, nz, np);
+        praw[0] = *at(a.psi[0], q);
+        praw[5] = *at(a.psi[5], q);
     }
     if (pzw) {
-        const long q = psi_z(i, j, in_pml(k, nz, np) ? k : 0, nx, ny, nz, np);
-        praw[1] = a.psi[1][q];
-        praw[2] = a.psi[2][q];
+        const IX q = psi_z<IX>(i, j, in_pml(k, nz, np) ? k : 0, nx, ny, nz, np);
+        praw[1] = *at(a.psi[1], q);
+        praw[2] = *at(a.psi[2], q);
     }
     if (px) {
-        const long q = psi_x(i, j, k, nx, ny, nz, np);
-        praw[3] = a.psi[3][q];
-        praw[4] = a.psi[4][q];
+        const IX q = psi_x<IX>(i, j, k, nx, ny, nz, np);
+        praw[3] = *at(a.psi[3], q);
+        praw[4] = *at(a.psi[4], q);
     }
     // gain state
     double gs[4];
     for (int q = 0; q < 4; ++q) gs[q] = opaque(0.0);
-    if (growu) { gs[0] = a.gNu[g]; gs[1] = a.gNg[g]; gs[2] = a.gPz[g]; gs[3] = a.gPp[g]; }
+    if (growu) { gs[0] = *at(a.gNu, g); gs[1] = *at(a.gNg, g); gs[2] = *at(a.gPz, g); gs[3] = *at(a.gPp, g); }
     // axis tables
     const Axes<R>& ax = tb.ax;
     v.idx = ax.X(0, i);
@@ -761,7 +789,7 @@ __device__ __forceinline__ EIn<R> e_gather(const StepArgs<T, R>& a, const Tabs<R
     }
     if (GN && a.gain && a.ntls != 1) v.tc = tb.tls[r8 < 0 ? 0 : r8];
     v.reg = gin ? r8 : -1;
-    v.g = g;
+    v.g = (long)g;
     v.Nu = gs[0]; v.Ng = gs[1]; v.Pz = gs[2]; v.Pp = gs[3];
     for (int q = 0; q < 6; ++q) v.p[q] = cv<T, R>(praw[q], a.sc[6 + q]);
     return v;
@@ -784,13 +812,13 @@ __device__ __forceinline__ EIn<R> e_gather(const StepArgs<T, R>& a, const Tabs<R
 // for a cell the reference does not update, v.ex0/ey0/ez0 (so a caller that
 // stores fields itself, store_fields = false, passes the loaded values
 // there).
-template <typename T, typename R, bool CP = true>
+template <typename T, typename R, bool CP = true, typename IX = long>
 __device__ __forceinline__ H3<R> e_compute(const StepArgs<T, R>& a, const Tabs<R>& tb, int i, int j, int k,
                                            const EIn<R>& v, R hx, R hy, R hz, R hx_j, R hz_j, R hx_k, R hy_k, R hy_i,
                                            R hz_i, bool store_fields = true) {
-    const long id = ((long)i * a.ny + j) * a.nz + k;
+    const IX id = ((IX)i * a.ny + j) * a.nz + k;
     const int reg = v.reg;
-    const long g = v.g;
+    const IX g = (IX)v.g;
     const bool interior = (i > 0) & (j > 0) & (k > 0);
     const bool pec_face = a.pec && (!interior || i == a.nx - 1 || j == a.ny - 1 || k == a.nz - 1);
     if (!interior) {
@@ -800,13 +828,13 @@ __device__ __forceinline__ H3<R> e_compute(const StepArgs<T, R>& a, const Tabs<R
             const double ez = (double)v.ez0;
             (void)tls_polarize<sizeof(R) == 8>(c, Nu, Ng, ez, Pz, Pp);
             tls_populate(c, ez, 0.0, Pz, Pp, Nu, Ng);
-            gain_arr(a, tb, 0)[g] = Nu; gain_arr(a, tb, 1)[g] = Ng; gain_arr(a, tb, 2)[g] = Pz; gain_arr(a, tb, 3)[g] = Pp;
+            *at(gain_arr(a, tb, 0), g) = Nu; *at(gain_arr(a, tb, 1), g) = Ng; *at(gain_arr(a, tb, 2), g) = Pz; *at(gain_arr(a, tb, 3), g) = Pp;
         }
         if (pec_face) {
             if (store_fields) {
-                st<T, R>(a.Exw, id, (R)0, a.sc[18]);
-                st<T, R>(a.Eyw, id, (R)0, a.sc[19]);
-                st<T, R>(a.Ezw, id, (R)0, a.sc[20]);
+                stx<T, R>(a.Exw, id, (R)0, a.sc[18]);
+                stx<T, R>(a.Eyw, id, (R)0, a.sc[19]);
+                stx<T, R>(a.Ezw, id, (R)0, a.sc[20]);
             }
             return H3<R>{0, 0, 0};
         }
@@ -848,58 +876,58 @@ __device__ __forceinline__ H3<R> e_compute(const StepArgs<T, R>& a, const Tabs<R
             if (px) ez += bEz * cpml_term<R>(v.p[4], dHy_x, v.kx[0], v.kx[1], v.kx[2], v.kx[3], pn[4]);
             if (py) ez += bEz * (-cpml_term<R>(v.p[5], dHx_y, v.ky[0], v.ky[1], v.ky[2], v.ky[3], pn[5]));
             if (py) {
-                const long q = psi_y(i, j, k, nx, ny, nz, np);
-                st<T, R>(psi_rd(a, tb, 0), q, pn[0], a.sc[24]);
-                st<T, R>(psi_rd(a, tb, 5), q, pn[5], a.sc[29]);
+                const IX q = psi_y<IX>(i, j, k, nx, ny, nz, np);
+                stx<T, R>(psi_rd(a, tb, 0), q, pn[0], a.sc[24]);
+                stx<T, R>(psi_rd(a, tb, 5), q, pn[5], a.sc[29]);
             }
             if (pz) {
-                const long q = psi_z(i, j, k, nx, ny, nz, np);
-                st<T, R>(psi_rd(a, tb, 1), q, pn[1], a.sc[25]);
-                st<T, R>(psi_rd(a, tb, 2), q, pn[2], a.sc[26]);
+                const IX q = psi_z<IX>(i, j, k, nx, ny, nz, np);
+                stx<T, R>(psi_rd(a, tb, 1), q, pn[1], a.sc[25]);
+                stx<T, R>(psi_rd(a, tb, 2), q, pn[2], a.sc[26]);
             }
             if (px) {
-                const long q = psi_x(i, j, k, nx, ny, nz, np);
-                st<T, R>(psi_rd(a, tb, 3), q, pn[3], a.sc[27]);
-                st<T, R>(psi_rd(a, tb, 4), q, pn[4], a.sc[28]);
+                const IX q = psi_x<IX>(i, j, k, nx, ny, nz, np);
+                stx<T, R>(psi_rd(a, tb, 3), q, pn[3], a.sc[27]);
+                stx<T, R>(psi_rd(a, tb, 4), q, pn[4], a.sc[28]);
             }
         }
     }
     if (pec_face) { ex = 0; ey = 0; ez = 0; }
     if (store_fields) {
-        st<T, R>(a.Exw, id, ex, a.sc[18]);
-        st<T, R>(a.Eyw, id, ey, a.sc[19]);
-        st<T, R>(a.Ezw, id, ez, a.sc[20]);
+        stx<T, R>(a.Exw, id, ex, a.sc[18]);
+        stx<T, R>(a.Eyw, id, ey, a.sc[19]);
+        stx<T, R>(a.Ezw, id, ez, a.sc[20]);
     }
     if (reg >= 0) {
         const TLSCoef& c = tc;
         // The reference stores Ez and reads it back for the population pass.
         const double ez_new = stored_value<T, R>(ez, a.sc[2], a.sc[20]);
         tls_populate(c, ez_new, (double)v.ez0, Pz, Pp, Nu, Ng);
-        gain_arr(a, tb, 0)[g] = Nu; gain_arr(a, tb, 1)[g] = Ng; gain_arr(a, tb, 2)[g] = Pz; gain_arr(a, tb, 3)[g] = Pp;
+        *at(gain_arr(a, tb, 0), g) = Nu; *at(gain_arr(a, tb, 1), g) = Ng; *at(gain_arr(a, tb, 2), g) = Pz; *at(gain_arr(a, tb, 3), g) = Pp;
     }
     return H3<R>{ex, ey, ez};
 }
 
 // E update of one cell (i, kw and jw uniform across the wavefront, as for
 // e_gather).  CP and GN compile the CPML and gain code in (the default) or out.
-template <typename T, typename R, int KW = 64, bool CP = true, bool GN = true>
+template <typename T, typename R, int KW = 64, bool CP = true, bool GN = true, typename IX = long>
 __device__ __forceinline__ void e_cell(const StepArgs<T, R>& a, const Tabs<R>& tb, int i, int j, int k, int kw, int jw) {
     R h[9];
-    const EIn<R> v = e_gather<T, R, KW, CP, GN>(a, tb, i, j, k, kw, jw, h);
-    e_compute<T, R, CP>(a, tb, i, j, k, v, h[0], h[1], h[2], h[3], h[4], h[5], h[6], h[7], h[8]);
+    const EIn<R> v = e_gather<T, R, KW, CP, GN, IX>(a, tb, i, j, k, kw, jw, h);
+    e_compute<T, R, CP, IX>(a, tb, i, j, k, v, h[0], h[1], h[2], h[3], h[4], h[5], h[6], h[7], h[8]);
 }
 
 
 
 // E update kernel of the two-kernel step: H^{n+1/2} neighbours from memory.
-template <typename T, typename R, int KW>
+template <typename T, typename R, int KW, typename IX = long>
 __global__ void __launch_bounds__(256) k_update_E(StepArgs<T, R> a) {
     FDTD_SCALES_IN_REGISTERS(a)
     const Tabs<R> tb = global_tabs(a);
     const Tile t = tile_of<KW>(a);
     const int i = t.i, j = t.j, k = t.k;
     if (k >= a.nz || j >= a.ny || i >= a.nx) return;
-    e_cell<T, R, KW>(a, tb, i, j, k, t.kw, t.jw);
+    e_cell<T, R, KW, true, true, IX>(a, tb, i, j, k, t.kw, t.jw);
 }
 
 // Row-marching step kernel: one launch advances H and E of the whole grid,

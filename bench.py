@@ -95,10 +95,12 @@ def cpu_baseline(spec, steps: int):
 
 
 def _kw_param(args) -> int:
-    """fdtd_set_kernel_mode's mode-0 parameter: one width, or H + 256 * E."""
+    """fdtd_set_kernel_mode's mode-0 parameter: one width, or H + 256 * E,
+    plus 131072 for 64-bit element offsets."""
+    o64 = 0x20000 if getattr(args, "offsets64", False) else 0
     if args.kwidth_e and args.kwidth_e != args.kwidth:
-        return (args.kwidth or 32) + 256 * args.kwidth_e
-    return args.kwidth
+        return (args.kwidth or 32) + 256 * args.kwidth_e + o64
+    return (args.kwidth or (32 if o64 else 0)) + o64
 
 
 def plain(args):
@@ -347,6 +349,7 @@ def main():
     ap.add_argument("--ichunk", type=int, default=0, help="fused kernel x-planes per work-group; 0 = library default")
     ap.add_argument("--kwidth", type=int, default=0, help="H/E kernels: wavefront k-width (16, 32, 64); 0 = library default")
     ap.add_argument("--kwidth-e", type=int, default=0, help="E kernel's own wavefront k-width; 0 = same as --kwidth")
+    ap.add_argument("--offsets64", action="store_true", help="H/E kernels with 64-bit element offsets")
     ap.add_argument("--rate-steps", type=int, default=200, help="precision form: timed steps per format")
     ap.add_argument("--domains", type=int, default=256, help="batch form: domains")
     ap.add_argument("--requests", type=int, default=64, help="serve form: requests")

@@ -189,7 +189,9 @@ int fdtd_get_fp16_scales(fdtd_handle h, double* scales18);
  * `param` = the k-width of the cells one wavefront updates (16, 32 or 64
  * consecutive k of 64/param rows; 32 by default; 0 keeps the current
  * value), for both kernels, or
- * width_H + 256 * width_E to set them apart;
+ * width_H + 256 * width_E to set them apart (plus 131072: 64-bit element
+ * offsets even where 32-bit byte offsets, the default with FP16 storage
+ * when every array is below 4 GiB, would do);
  * mode 1 = one fused step kernel in which each wavefront owns one j-row and
  * marches along x over `param` planes (1..128; 0 keeps the current value),
  * double-buffering E, H and psi_H.  Grids the fused kernel does not take

@@ -132,14 +132,15 @@ def test_fp32_gain(pkg, oracle_mod, mode):
 
 @pytest.mark.parametrize("precision", ["fp32", "fp16"])
 def test_step_kernels_agree_bitwise(pkg, precision):
-    """The two-kernel step (at every wavefront k-width) and the fused step
+    """The two-kernel step (at every wavefront k-width, with 32-bit and 64-bit
+    element offsets: 0x20000 + width selects the latter) and the fused step
     kernel run the same operations per value, so their results are identical
     in every storage format."""
     sp = pkg.scene.gain_slab_case(n_total=44, npml=6, steps=120, fine_len=100e-9, coarse=20e-9, fine=5e-9,
                                   n_cuboids=4, seed=5)
     setup = pkg.Setup(sp)
     runs = {}
-    for mode in [(0, 16), (0, 32), (0, 64), (1, 16), (1, 5)]:
+    for mode in [(0, 16), (0, 32), (0, 64), (0, 0x20000 + 32), (0, 0x20000 + 16 + 256 * 64), (1, 16), (1, 5)]:
         _say(f"[case] {sp.name} {precision} mode={mode}")
         sc = pkg.Scene(setup, precision)
         sc.set_kernel_mode(*mode)
