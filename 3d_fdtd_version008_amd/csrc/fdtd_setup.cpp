@@ -585,6 +585,29 @@ long fdtd_setup_array(const fdtd_setup* s, const char* name, double* out) {
         {"Ndip", &s->Ndip}, {"Ng0", &s->Ng0}, {"Nu", &s->Nu}, {"Ng", &s->Ng}, {"Pz", &s->Pz}, {"Pz_prev", &s->Pz_prev}};
     for (const auto& t : tab)
         if (n == t.n) v = t.v;
+    if (n == "n_grid") {
+        // make_n_grid_from_scene (src/structure_material.hpp:291-310): the
+        // index sqrt(eps_r mu_r) of the last structure containing each cell
+        // centre, the background's elsewhere (what the mesh detector slices)
+        const auto& xb = s->gs.x_bounds;
+        const auto& yb = s->gs.y_bounds;
+        const auto& zb = s->gs.z_bounds;
+        if (out)
+            for (size_t i = 0; i < s->NxT; ++i)
+                for (size_t j = 0; j < s->NyT; ++j)
+                    for (size_t k = 0; k < s->NzT; ++k) {
+                        const double x = 0.5 * (xb[i] + xb[i + 1]), y = 0.5 * (yb[j] + yb[j + 1]),
+                                     z = 0.5 * (zb[k] + zb[k + 1]);
+                        double eps = s->bg_eps, mu = s->bg_mu;
+                        for (const Shape& sh : s->shapes)
+                            if (sh.contains(x, y, z)) {
+                                eps = sh.eps_r;
+                                mu = sh.mu_r;
+                            }
+                        out[(i * s->NyT + j) * s->NzT + k] = std::sqrt(eps * mu);
+                    }
+        return (long)s->N;
+    }
     if (n == "region_map" && s->tls_on) {
         if (out)
             for (size_t i = 0; i < s->region_map.size(); ++i) out[i] = (double)s->region_map[i];

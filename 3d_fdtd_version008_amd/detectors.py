@@ -154,6 +154,24 @@ def write_point_probe(root: str, name: str, setup, cell, components: Sequence[in
     return d
 
 
+def _slice_index(setup, plane: int, position: float, domain_min=(0.0, 0.0, 0.0)):
+    """Slice index and its cell-centre coordinate for a plane (0 XY, 1 XZ,
+    2 YZ) at a physical position, as FieldMovie2D / MeshDetector::initialize
+    place it (src/detectors/field_movie_2d.hpp:70-92): physical_to_index of
+    position - domain_min along the normal, clamped to [1, N-2]."""
+    nx, ny, nz = setup.shape
+    npml = setup.npml
+    xb, yb, zb = (setup.array(n, shaped=False) for n in ("x_bounds", "y_bounds", "z_bounds"))
+    b, n, dm = ((zb, nz, domain_min[2]), (yb, ny, domain_min[1]), (xb, nx, domain_min[0]))[plane]
+    idx = max(1, min(physical_to_index(b, npml, position - dm), n - 2))
+    return idx, cell_center_physical(b, npml, idx)
+
+
+def _plane_of(v: np.ndarray, plane: int, index: int) -> np.ndarray:
+    """The slice in the detectors' loop order (dim1 outer, dim2 inner)."""
+    return np.ascontiguousarray((v[:, :, index], v[:, index, :], v[index, :, :])[plane])
+
+
 # ---- FieldMovie2D (src/detectors/field_movie_2d.hpp:41-213) ----
 
 class FieldMovie2DWriter:
@@ -172,10 +190,7 @@ class FieldMovie2DWriter:
         nx, ny, nz = setup.shape
         npml = setup.npml
         xb, yb, zb = (setup.array(n, shaped=False) for n in ("x_bounds", "y_bounds", "z_bounds"))
-        b, n, dm = ((zb, nz, domain_min[2]), (yb, ny, domain_min[1]), (xb, nx, domain_min[0]))[slice_plane]
-        idx = physical_to_index(b, npml, slice_position - dm)
-        self.index = max(1, min(idx, n - 2))
-        coord = cell_center_physical(b, npml, self.index)
+        self.index, coord = _slice_index(setup, slice_plane, slice_position, domain_min)
         pname = ("XY", "XZ", "YZ")[slice_plane]
         dims = ((nx, ny, "x", "y"), (nx, nz, "x", "z"), (ny, nz, "y", "z"))[slice_plane]
         lines = ["{", '  "detector_type": "FieldMovie2D",', f'  "field_component": "{COMPONENT_NAMES[component]}",',
@@ -197,10 +212,9 @@ class FieldMovie2DWriter:
     def frame(self, fields: dict) -> str:
         """Writes the next frame from FP64 arrays fields["Ex"] ... ["Hz"]."""
         v = component_values(self.component, fields)
-        s = (v[:, :, self.index], v[:, self.index, :], v[self.index, :, :])[self.plane]
         path = os.path.join(self.dir, self.frame_pattern % self.frame_id)
         self.frame_id += 1
-        np.ascontiguousarray(s, dtype=np.float64 if self.float64 else np.float32).tofile(path)
+        _plane_of(v, self.plane, self.index).astype(np.float64 if self.float64 else np.float32).tofile(path)
         return path
 
 
@@ -290,3 +304,67 @@ def write_mesh_files(root: str, setup, spec, lambda0: float, mode: str = "UNIFOR
     with open(p_info, "w") as f:
         f.write(text)
     return [p_grid, p_coord, p_info]
+
+
+# ---- MeshDetector (src/detectors/mesh_detector.hpp:19-263) ----
+
+def write_mesh_detector(root: str, name: str, setup, slice_plane: int, slice_position: float, n_phys, L_phys,
+                        domain_min=(0.0, 0.0, 0.0), float64: bool = True, mesh_lines: bool = True,
+                        spacing_arrays: bool = True) -> str:
+    """<root>/<name>/refractive_index.raw (the slice of the cell-centre index
+    grid, Setup.array("n_grid"), as make_n_grid_from_scene builds it),
+    metadata.json, and optionally x/y/z_bounds.txt and dx/dy/dz_array.txt,
+    as the detector package writes them at set-up (src/params.hpp:695-739)."""
+    d = os.path.join(root, name)
+    os.makedirs(d, exist_ok=True)
+    nx, ny, nz = setup.shape
+    npml = setup.npml
+    idx, coord = _slice_index(setup, slice_plane, slice_position, domain_min)
+    n_grid = setup.array("n_grid")
+    _plane_of(n_grid, slice_plane, idx).astype(np.float64 if float64 else np.float32).tofile(
+        os.path.join(d, "refractive_index.raw"))
+    b = [setup.array(q, shaped=False) for q in ("x_bounds", "y_bounds", "z_bounds")]
+    g = lambda x: _g(x, 15)  # noqa: E731
+    dims = ((nx, ny, "x", "y"), (nx, nz, "x", "z"), (ny, nz, "y", "z"))[slice_plane]
+    lines = ["{", '  "detector_type": "MeshDetector",', f'  "slice_plane": "{("XY", "XZ", "YZ")[slice_plane]}",',
+             f'  "slice_index": {idx},', f'  "slice_physical_m": {g(coord)},', f'  "slice_physical_nm": {g(coord * 1e9)},',
+             f'  "NxT": {nx},', f'  "NyT": {ny},', f'  "NzT": {nz},', f'  "Nx_phys": {n_phys[0]},',
+             f'  "Ny_phys": {n_phys[1]},', f'  "Nz_phys": {n_phys[2]},', f'  "npml": {npml},']
+    lines += [f'  "pml_offset_{ax}_m": {g(_off(b[q], npml))},' for q, ax in enumerate("xyz")]
+    lines += [f'  "L{ax}_phys_m": {g(L_phys[q])},' for q, ax in enumerate("xyz")]
+    lines += [f'  "dtype": "{"float64" if float64 else "float32"}",', f'  "slice_dim1": {dims[0]},',
+              f'  "slice_dim2": {dims[1]},', f'  "dim1_label": "{dims[2]}",', f'  "dim2_label": "{dims[3]}",',
+              '  "note": "row-major storage, dim1 varies fastest"', "}"]
+    with open(os.path.join(d, "metadata.json"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+    if mesh_lines:
+        for q, ax in enumerate("xyz"):
+            with open(os.path.join(d, f"{ax}_bounds.txt"), "w") as f:
+                f.write(f"# {ax.upper()} cell boundaries (meters)\n" + "".join(g(v) + "\n" for v in b[q].tolist()))
+    if spacing_arrays:
+        for ax in "xyz":
+            arr = setup.array(f"d{ax}", shaped=False)
+            with open(os.path.join(d, f"d{ax}_array.txt"), "w") as f:
+                f.write(f"# d{ax} spacing array (meters)\n" + "".join(g(v) + "\n" for v in arr.tolist()))
+    return d
+
+
+# ---- 1-D mesh profiles (src/mesh_1d_profile_export.hpp:14-66) ----
+
+def write_mesh_1d_profiles(out_dir: str, setup) -> Sequence[str]:
+    """mesh_dx_profile.csv and mesh_dy_profile.csv: physical cell centres
+    and widths in nm over the physical domain, 10 significant digits."""
+    os.makedirs(out_dir, exist_ok=True)
+    npml = setup.npml
+    paths = []
+    for ax in "xy":
+        b = setup.array(f"{ax}_bounds", shaped=False)
+        d = setup.array(f"d{ax}", shaped=False)
+        start = float(b[npml])
+        rows = [f"{_g((0.5 * (float(b[i]) + float(b[i + 1])) - start) * 1e9, 10)},{_g(float(d[i]) * 1e9, 10)}\n"
+                for i in range(npml, len(d) - npml)]
+        p = os.path.join(out_dir, f"mesh_d{ax}_profile.csv")
+        with open(p, "w") as f:
+            f.write(f"{ax}_center_nm,d{ax}_nm\n" + "".join(rows))
+        paths.append(p)
+    return paths

@@ -61,6 +61,9 @@ def _lib(impl: str):
         sig.update({
             "add_point_detector": (i, [vp, C.c_char_p, C.c_char_p, d, d, d, i, vp, i, l, i, d, d, d]),
             "add_movie": (i, [vp, C.c_char_p, C.c_char_p, i, i, d, i, l, i, l, l, l, d, d, d, d, d, d]),
+            "add_mesh_detector": (i, [vp, C.c_char_p, C.c_char_p, i, d, i, i, i, l, l, l, d, d, d, d, d, d]),
+            "dump_mesh_profiles": (i, [vp, C.c_char_p]),
+            "n_grid": (l, [vp, vp]),
         })
     if impl == "oracle":
         sig.update({
@@ -115,6 +118,24 @@ class CpuSim:
         """The reference's FieldMovie2D writing its own files (ref only)."""
         self.f["add_movie"](self.h, out_root.encode(), name.encode(), component, plane, position, save_every, n_steps,
                             int(float64), *n_phys, *L_phys, *domain_min)
+
+    def add_mesh_detector(self, out_root, name, plane, position, float64, mesh_lines, spacing, n_phys, L_phys,
+                          domain_min=(0.0, 0.0, 0.0)):
+        """The reference's MeshDetector writing its files now (ref only)."""
+        self.f["add_mesh_detector"](self.h, out_root.encode(), name.encode(), plane, position, int(float64),
+                                    int(mesh_lines), int(spacing), *n_phys, *L_phys, *domain_min)
+
+    def dump_mesh_profiles(self, out_dir):
+        """The reference's mesh_export::dump_mesh_1d_profiles (ref only)."""
+        if self.f["dump_mesh_profiles"](self.h, out_dir.encode()) != 0:
+            raise RuntimeError("dump_mesh_1d_profiles failed")
+
+    def n_grid(self) -> np.ndarray:
+        """The reference's make_n_grid_from_scene (ref only)."""
+        n = self.f["n_grid"](self.h, None)
+        out = np.empty(n, dtype=np.float64)
+        self.f["n_grid"](self.h, out.ctypes.data_as(C.c_void_p))
+        return out.reshape(self.shape)
 
     def __del__(self):
         try:

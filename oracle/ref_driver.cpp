@@ -28,6 +28,8 @@
 #include "detectors/idetector.hpp"
 #include "detectors/point_field_detector.hpp"
 #include "detectors/field_movie_2d.hpp"
+#include "detectors/mesh_detector.hpp"
+#include "mesh_1d_profile_export.hpp"
 #include "stability_monitor.hpp"
 
 #include "../include/fdtd_scene.h"
@@ -323,6 +325,51 @@ int ref_add_movie(ref_sim* S, const char* out_root, const char* name, int comp, 
                                                      Lz, dmx, dmy, dmz));
     std::cout.rdbuf(old_out);
     return 0;
+}
+
+// MeshDetector (src/detectors/mesh_detector.hpp:265-280) as the detector
+// package runs it (src/params.hpp:695-739): the refractive-index slice of
+// make_n_grid_from_scene, then metadata, mesh lines and spacing arrays.
+int ref_add_mesh_detector(ref_sim* S, const char* out_root, const char* name, int plane, double pos, int float64,
+                          int lines, int spacing, long nx_phys, long ny_phys, long nz_phys, double Lx, double Ly,
+                          double Lz, double dmx, double dmy, double dmz) {
+    NullBuf nb;
+    std::streambuf* old_out = std::cout.rdbuf(&nb);
+    Detectors::MeshDetectorConfig c;
+    c.name = name;
+    c.slice_plane = plane;
+    c.slice_position = pos;
+    c.write_float64 = float64 != 0;
+    c.export_mesh_lines = lines != 0;
+    c.export_spacing_arrays = spacing != 0;
+    std::vector<real> n_grid;
+    make_n_grid_from_scene(S->scene, n_grid);
+    auto det = Detectors::make_mesh_detector(out_root, c, S->NxT, S->NyT, S->NzT, (std::size_t)nx_phys,
+                                             (std::size_t)ny_phys, (std::size_t)nz_phys, S->npml, S->gs, Lx, Ly, Lz,
+                                             dmx, dmy, dmz);
+    det->save_index_slice(n_grid);
+    det->save_mesh_info();
+    std::cout.rdbuf(old_out);
+    return 0;
+}
+
+// mesh_export::dump_mesh_1d_profiles (src/mesh_1d_profile_export.hpp:14-66):
+// <out_dir>/mesh_dx_profile.csv and mesh_dy_profile.csv.
+int ref_dump_mesh_profiles(ref_sim* S, const char* out_dir) {
+    NullBuf nb;
+    std::streambuf* old_out = std::cout.rdbuf(&nb);
+    const bool ok = mesh_export::dump_mesh_1d_profiles(S->gs, S->npml, out_dir);
+    std::cout.rdbuf(old_out);
+    return ok ? 0 : -1;
+}
+
+// make_n_grid_from_scene (src/structure_material.hpp:291-310): the
+// refractive index at every cell centre (out: NxT*NyT*NzT values).
+long ref_n_grid(ref_sim* S, double* out) {
+    std::vector<real> n_grid;
+    make_n_grid_from_scene(S->scene, n_grid);
+    if (out) std::memcpy(out, n_grid.data(), n_grid.size() * sizeof(double));
+    return (long)n_grid.size();
 }
 
 void ref_info(const ref_sim* S, long* out) {

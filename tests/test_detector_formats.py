@@ -269,3 +269,78 @@ def test_mesh_files_match_iostream(case):
         det.write_mesh_files(our_root, setup, sp, lambda0, mode)
         files = assert_trees_equal(ref_root, our_root)
         assert files == ["coordinate_mapping.txt", "grid_spacing.txt", "mesh_info.json"]
+
+
+def _mesh_scenes(pkg):
+    S = pkg.scene
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+    import cases
+
+    sp0, _, _, _ = detector_case(pkg, 1)
+    return {"graded_sphere": sp0,
+            "gain_cuboids": S.gain_slab_case(n_total=40, npml=6, steps=4, fine_len=100e-9, coarse=20e-9, fine=5e-9,
+                                             n_cuboids=4, seed=9),
+            "auto_mesh_cylinder": cases.auto_mesh_cylinder(),
+            "pec_cavity": cases.pec_cavity()}
+
+
+def test_index_grid_matches_reference(oracle_mod):
+    """Setup.array("n_grid") against the reference's make_n_grid_from_scene
+    (src/structure_material.hpp:291-310), bitwise, on a graded sphere, gain
+    cuboids, a z-cylinder on an auto-meshed grid and a PEC cavity."""
+    if not oracle_mod.available("ref"):
+        pytest.skip("compiled reference harness (oracle/_ref) not built")
+    pkg = importlib.import_module("3d_fdtd_version008_amd")
+    for name, sp in _mesh_scenes(pkg).items():
+        setup = pkg.Setup(sp)
+        ref = oracle_mod.CpuSim(sp, impl="ref")
+        a, b = setup.array("n_grid"), ref.n_grid()
+        assert a.shape == b.shape and len(np.unique(b)) > 1, name
+        assert np.array_equal(a, b), name
+        ref.close()
+
+
+@pytest.mark.parametrize("plane,float64,flags", [(0, True, (1, 1)), (1, False, (1, 0)), (2, True, (0, 1))])
+def test_mesh_detector_files_match_reference(oracle_mod, plane, float64, flags):
+    """MeshDetector (src/detectors/mesh_detector.hpp) run by the compiled
+    reference against write_mesh_detector: refractive_index.raw, metadata.json,
+    x/y/z_bounds.txt, dx/dy/dz_array.txt byte for byte, on an auto-meshed
+    grid with a shifted domain origin."""
+    if not oracle_mod.available("ref"):
+        pytest.skip("compiled reference harness (oracle/_ref) not built")
+    pkg = importlib.import_module("3d_fdtd_version008_amd")
+    det = importlib.import_module("3d_fdtd_version008_amd.detectors")
+    sp = _mesh_scenes(pkg)["auto_mesh_cylinder"]
+    setup = pkg.Setup(sp)
+    nphys = (sp.nx_core, sp.ny_core, sp.nz_core)
+    L = (320e-9, 320e-9, 320e-9)
+    dmin = (2e-8, -1e-8, 3e-8)
+    pos = (0.41 * L[2] + dmin[2], 0.55 * L[1] + dmin[1], 0.5 * L[0] + dmin[0])[plane]
+    with tempfile.TemporaryDirectory() as ref_root, tempfile.TemporaryDirectory() as our_root:
+        ref = oracle_mod.CpuSim(sp, impl="ref")
+        ref.add_mesh_detector(ref_root, "mesh_info", plane, pos, float64, flags[0], flags[1], nphys, L, dmin)
+        ref.close()
+        det.write_mesh_detector(our_root, "mesh_info", setup, plane, pos, nphys, L, dmin, float64, bool(flags[0]),
+                                bool(flags[1]))
+        files = assert_trees_equal(ref_root, our_root)
+        assert os.path.join("mesh_info", "refractive_index.raw") in files
+        raw = np.fromfile(os.path.join(our_root, "mesh_info", "refractive_index.raw"),
+                          dtype=np.float64 if float64 else np.float32)
+        assert len(np.unique(raw)) > 1  # the slice crosses the cylinder
+
+
+def test_mesh_profiles_match_reference(oracle_mod):
+    """mesh_export::dump_mesh_1d_profiles (src/mesh_1d_profile_export.hpp)
+    against write_mesh_1d_profiles, byte for byte, on graded meshes."""
+    if not oracle_mod.available("ref"):
+        pytest.skip("compiled reference harness (oracle/_ref) not built")
+    pkg = importlib.import_module("3d_fdtd_version008_amd")
+    det = importlib.import_module("3d_fdtd_version008_amd.detectors")
+    for name in ("graded_sphere", "auto_mesh_cylinder"):
+        sp = _mesh_scenes(pkg)[name]
+        with tempfile.TemporaryDirectory() as ref_root, tempfile.TemporaryDirectory() as our_root:
+            ref = oracle_mod.CpuSim(sp, impl="ref")
+            ref.dump_mesh_profiles(ref_root)
+            ref.close()
+            det.write_mesh_1d_profiles(our_root, pkg.Setup(sp))
+            assert assert_trees_equal(ref_root, our_root) == ["mesh_dx_profile.csv", "mesh_dy_profile.csv"], name
