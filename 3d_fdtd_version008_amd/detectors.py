@@ -368,3 +368,45 @@ def write_mesh_1d_profiles(out_dir: str, setup) -> Sequence[str]:
             f.write(f"{ax}_center_nm,d{ax}_nm\n" + "".join(rows))
         paths.append(p)
     return paths
+
+
+# ---- PopulationTimeSeriesDetector frames (src/two_level_detectors.hpp:17-109) ----
+
+class PopulationFrames:
+    """<root>/<name>/populations_Nu.bin and populations_Ng.bin: the full-grid
+    FP64 densities in idx3 order at the first recorded step and at the end of
+    the run (two frames each), and metadata_populations.json, as
+    PopulationTimeSeriesDetector writes them.  Its CSV is write_tls_csv's
+    populations/inversion_time.csv."""
+
+    def __init__(self, root: str, setup, save_every: int, n_steps: int, name: str = "populations"):
+        self.dir = os.path.join(root, name)
+        os.makedirs(self.dir, exist_ok=True)
+        self.shape, self.every, self.n_steps, self.dt = setup.shape, save_every, n_steps, setup.dt
+        self.first_done = False
+        self.paths = [os.path.join(self.dir, "populations_Nu.bin"), os.path.join(self.dir, "populations_Ng.bin")]
+        for p in self.paths:  # the detector opens (and truncates) both at construction
+            open(p, "wb").close()
+
+    def _append(self, nu, ng):
+        for p, a in zip(self.paths, (nu, ng)):
+            with open(p, "ab") as f:
+                np.ascontiguousarray(a, dtype=np.float64).tofile(f)
+
+    def record(self, n: int, nu, ng) -> None:
+        """After step n's population update; the first recorded step's frame."""
+        if n % self.every != 0 or self.first_done:
+            return
+        self._append(nu, ng)
+        self.first_done = True
+
+    def finalize(self, nu, ng) -> None:
+        self._append(nu, ng)
+        nx, ny, nz = self.shape
+        lines = ["{", f'  "NxT": {nx},', f'  "NyT": {ny},', f'  "NzT": {nz},', f'  "saveEvery": {self.every},',
+                 f'  "nSteps": {self.n_steps},', f'  "dt": {_g18(self.dt)},', '  "dtype": "float64",',
+                 '  "binary_frames": 2,', '  "binary_mode": "first_and_last",',
+                 '  "binary_data_units": "m^-3 (density)",', '  "csv_total_units": "atoms (integrated)",',
+                 '  "what": "population_densities_Nu_and_Ng_vs_time"', "}"]
+        with open(os.path.join(self.dir, "metadata_populations.json"), "w") as f:
+            f.write("\n".join(lines) + "\n")

@@ -64,6 +64,7 @@ def _lib(impl: str):
             "add_mesh_detector": (i, [vp, C.c_char_p, C.c_char_p, i, d, i, i, i, l, l, l, d, d, d, d, d, d]),
             "dump_mesh_profiles": (i, [vp, C.c_char_p]),
             "n_grid": (l, [vp, vp]),
+            "add_tls_detectors": (i, [vp, C.c_char_p, l, l, l, l, l]),
         })
     if impl == "oracle":
         sig.update({
@@ -129,6 +130,12 @@ class CpuSim:
         """The reference's mesh_export::dump_mesh_1d_profiles (ref only)."""
         if self.f["dump_mesh_profiles"](self.h, out_dir.encode()) != 0:
             raise RuntimeError("dump_mesh_1d_profiles failed")
+
+    def add_tls_detectors(self, out_root, ic, jc, kc, save_every, n_steps):
+        """main's population, polarization and TLS-history detectors (ref only;
+        files are complete after close())."""
+        if self.f["add_tls_detectors"](self.h, out_root.encode(), ic, jc, kc, save_every, n_steps) != 0:
+            raise RuntimeError("no TLS region")
 
     def n_grid(self) -> np.ndarray:
         """The reference's make_n_grid_from_scene (ref only)."""

@@ -30,6 +30,7 @@
 #include "detectors/field_movie_2d.hpp"
 #include "detectors/mesh_detector.hpp"
 #include "mesh_1d_profile_export.hpp"
+#include "two_level_detectors.hpp"
 #include "stability_monitor.hpp"
 
 #include "../include/fdtd_scene.h"
@@ -69,6 +70,11 @@ struct ref_sim {
     size_t n{0};
     // the reference's own file-writing detectors, recorded after the E update
     std::vector<std::unique_ptr<Detectors::IDetector>> dets;
+    // main's TLS detectors (src/3D_FDTD_version008.cpp:182-199), recorded
+    // after the population update (:330-337)
+    std::unique_ptr<PopulationTimeSeriesDetector> pop_det;
+    std::unique_ptr<PolarizationDiagnostics> pol_det;
+    std::unique_ptr<TLSGlobalHistory> tls_hist;
 };
 
 extern "C" {
@@ -205,6 +211,20 @@ ref_sim* ref_create(const fdtd_scene_spec* sp) {
         }
         S->tls_manager.initialize(S->tls_state, S->gs);
         S->tls_manager.finalize_with_dt(S->dt);
+        // the union box of the regions (src/3D_FDTD_version008.cpp:146-163),
+        // which TLSGlobalHistory integrates over
+        if (S->tls_manager.num_regions() > 0) {
+            auto& st = S->tls_state;
+            const auto& r0 = S->tls_manager.regions[0];
+            st.gain_i0 = r0.i0; st.gain_i1 = r0.i1; st.gain_j0 = r0.j0;
+            st.gain_j1 = r0.j1; st.gain_k0 = r0.k0; st.gain_k1 = r0.k1;
+            for (size_t r = 1; r < S->tls_manager.num_regions(); ++r) {
+                const auto& rr = S->tls_manager.regions[r];
+                st.gain_i0 = std::min(st.gain_i0, rr.i0); st.gain_i1 = std::max(st.gain_i1, rr.i1);
+                st.gain_j0 = std::min(st.gain_j0, rr.j0); st.gain_j1 = std::max(st.gain_j1, rr.j1);
+                st.gain_k0 = std::min(st.gain_k0, rr.k0); st.gain_k1 = std::max(st.gain_k1, rr.k1);
+            }
+        }
     }
 
     // Probe placement of PointFieldDetector::initialize, point_field_detector.hpp:57-70
@@ -267,6 +287,14 @@ int ref_step(ref_sim* S, int nsteps) {
         }
         for (auto& d : S->dets) d->record_after_E(n, S->dt, S->Ex, S->Ey, S->Ez, S->Hx, S->Hy, S->Hz);
         if (S->tls_on) update_populations_multi_region(NxT, NyT, NzT, S->dt, S->tls_manager, S->Ez, S->tls_state);
+        if (S->pop_det) {
+            NullBuf nb2;
+            std::streambuf* o2 = std::cout.rdbuf(&nb2);
+            S->pop_det->record(n, S->tls_state);
+            S->pol_det->record(n, S->tls_state, S->Ez, S->gs);
+            S->tls_hist->record(n, S->tls_state);
+            std::cout.rdbuf(o2);
+        }
         S->n++;
         if (n % S->stab_interval == 0) {
             real cons = S->mon.calculate_conservation_error(S->tls_state);
@@ -281,7 +309,30 @@ int ref_step(ref_sim* S, int nsteps) {
 
 void ref_destroy(ref_sim* S) {
     for (auto& d : S->dets) d->finalize();
+    if (S->pop_det) {  // main's end of run (src/3D_FDTD_version008.cpp:384-389)
+        NullBuf nb;
+        std::streambuf* o = std::cout.rdbuf(&nb);
+        S->pop_det->finalize(S->tls_state);
+        S->tls_hist->finalize();
+        std::cout.rdbuf(o);
+    }
     delete S;
+}
+
+// main's three TLS detectors (src/3D_FDTD_version008.cpp:182-199) under
+// out_root, with the polarization monitor at cell (ic, jc, kc).
+int ref_add_tls_detectors(ref_sim* S, const char* out_root, long ic, long jc, long kc, long save_every, long n_steps) {
+    if (!S->tls_on) return -1;
+    NullBuf nb;
+    std::streambuf* o = std::cout.rdbuf(&nb);
+    const fs::path root(out_root);
+    S->pop_det = std::make_unique<PopulationTimeSeriesDetector>(root, "populations", S->NxT, S->NyT, S->NzT,
+                                                                (size_t)save_every, (size_t)n_steps, S->dt, S->gs);
+    S->pol_det = std::make_unique<PolarizationDiagnostics>(root, "polarization", S->NxT, S->NyT, S->NzT, (size_t)ic,
+                                                           (size_t)jc, (size_t)kc, (size_t)save_every, S->dt);
+    S->tls_hist = std::make_unique<TLSGlobalHistory>(root, S->tls_state, S->gs, (size_t)save_every, S->dt, false);
+    std::cout.rdbuf(o);
+    return 0;
 }
 
 // PointFieldDetector (src/detectors/point_field_detector.hpp:183-196)

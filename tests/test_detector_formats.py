@@ -344,3 +344,36 @@ def test_mesh_profiles_match_reference(oracle_mod):
             ref.close()
             det.write_mesh_1d_profiles(our_root, pkg.Setup(sp))
             assert assert_trees_equal(ref_root, our_root) == ["mesh_dx_profile.csv", "mesh_dy_profile.csv"], name
+
+
+@pytest.mark.parametrize("save_every", [1, 5])
+def test_population_frames_match_reference(oracle_mod, save_every):
+    """PopulationTimeSeriesDetector's binary frames (first recorded step and
+    end of run) and metadata_populations.json, written by the compiled
+    reference's detector and by PopulationFrames fed with the same harness's
+    population arrays: byte for byte."""
+    if not oracle_mod.available("ref"):
+        pytest.skip("compiled reference harness (oracle/_ref) not built")
+    pkg = importlib.import_module("3d_fdtd_version008_amd")
+    det = importlib.import_module("3d_fdtd_version008_amd.detectors")
+    sp = pkg.scene.gain_slab_case(n_total=40, npml=6, steps=12, fine_len=100e-9, coarse=20e-9, fine=5e-9)
+    setup = pkg.Setup(sp)
+    with tempfile.TemporaryDirectory() as ref_root, tempfile.TemporaryDirectory() as our_root:
+        ref = oracle_mod.CpuSim(sp, impl="ref")
+        ref.add_tls_detectors(ref_root, 3, 3, 3, save_every, sp.n_steps)
+        ref.step(sp.n_steps)
+        ref.close()
+        sim = oracle_mod.CpuSim(sp, impl="ref")
+        frames = det.PopulationFrames(our_root, setup, save_every, sp.n_steps)
+        sim.step(1)
+        frames.record(0, sim.array("Nu"), sim.array("Ng"))
+        sim.step(sp.n_steps - 1)
+        frames.finalize(sim.array("Nu"), sim.array("Ng"))
+        sim.close()
+        for f in ("populations_Nu.bin", "populations_Ng.bin", "metadata_populations.json"):
+            a = open(os.path.join(ref_root, "populations", f), "rb").read()
+            b = open(os.path.join(our_root, "populations", f), "rb").read()
+            assert len(a) > 0 and a == b, f
+        n = setup.shape[0] * setup.shape[1] * setup.shape[2]
+        nu = np.fromfile(os.path.join(our_root, "populations", "populations_Nu.bin"))
+        assert nu.size == 2 * n and not np.array_equal(nu[:n], nu[n:])  # first and last frames differ
