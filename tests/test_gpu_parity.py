@@ -153,6 +153,31 @@ def test_step_kernels_agree_bitwise(pkg, precision):
         assert not diff, f"mode {mode} differs in {diff}"
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "fp16"])
+def test_step_kernels_agree_bitwise_ragged(pkg, precision):
+    """A grid whose rows span several 64-cell k-segments, the last one
+    partial (NzT = 150), with a partial last group of rows (NyT = 30) and a
+    partial last chunk of planes: the fused kernel carries state across
+    segments and planes, and agrees with the two-kernel step (every
+    wavefront k-width, whose wavefronts straddle the row end) bit for bit."""
+    sp = pkg.scene.gain_slab_case(shape=(20, 30, 150), npml=6, steps=60, fine_len=300e-9, coarse=20e-9, fine=5e-9,
+                                  n_cuboids=3, seed=9)
+    setup = pkg.Setup(sp)
+    runs = {}
+    for mode in [(0, 32), (0, 16), (0, 64), (1, 3), (1, 20)]:
+        _say(f"[case] ragged {precision} mode={mode}")
+        sc = pkg.Scene(setup, precision)
+        sc.set_kernel_mode(*mode)
+        sc.step(sp.n_steps)
+        runs[mode] = {n: sc.field(n) for n in FIELDS + ("Nu", "Pz") + tuple(pkg.runtime.PSI_NAMES)}
+        sc.close()
+    base = runs[(0, 32)]
+    assert np.abs(base["Ex"]).max() > 0
+    for mode, got in runs.items():
+        diff = [n for n in base if not np.array_equal(base[n], got[n])]
+        assert not diff, f"mode {mode} differs in {diff}"
+
+
 GOLDEN = __import__("os").path.join(__import__("os").path.dirname(__file__), "golden")
 
 
