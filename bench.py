@@ -78,20 +78,27 @@ def copy_calibration(nbytes: int) -> None:
 
 
 def cpu_baseline(spec, steps: int):
+    """The reference's own step (its headers compiled with OpenMP into
+    oracle/_ref/libref.so, kind "reference") timed on this host's cores, or
+    where that library is absent the oracle's C restatement (kind "port")."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle  # checker / CPU baseline only
 
-    if not oracle.available("oracle"):
+    impl = "ref" if oracle.available("ref") else "oracle"
+    if impl == "oracle" and not oracle.available("oracle"):
         oracle.build()
-    sim = oracle.CpuSim(spec, "oracle")
+    sim = oracle.CpuSim(spec, impl)
     sim.time_steps(2)
     secs = sim.time_steps(steps)
     n = spec.n_cells()
     rate = n * steps / secs / 1e9
     sim.close()
-    return {"value": rate, "unit": "Gcell-updates/s", "cores": oracle.threads(), "kind": "port",
-            "sample": f"{spec.name}: {steps} FP64 steps of the full grid after 2 warm-up steps "
-                      f"(oracle/fdtd_oracle.c, OpenMP over i, serial CPML, J zero-fill, monitor every 10)"}
+    what = ("oracle/_ref/libref.so: the reference's fdtd_update_H/E(_with_gain), CPMLBoundary, TLS and "
+            "StabilityMonitor code in main's step order, OpenMP as the reference builds it" if impl == "ref" else
+            "oracle/fdtd_oracle.c, OpenMP over i, serial CPML, J zero-fill, monitor every 10")
+    return {"value": rate, "unit": "Gcell-updates/s", "cores": oracle.threads(impl),
+            "kind": "reference" if impl == "ref" else "port",
+            "sample": f"{spec.name}: {steps} FP64 steps of the full grid after 2 warm-up steps ({what})"}
 
 
 def _kw_param(args) -> int:
@@ -343,7 +350,7 @@ def main():
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--workload", default="c2", choices=["c2", "c3"],
                     help="plain form: c2 = size^3 gain slab (default), c3 = 512x512x384 with 64 cuboids")
-    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--cpu-steps", type=int, default=80, help="CPU baseline: timed FP64 steps of the workload")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mode", type=int, default=0, help="0: separate H/E kernels, 1: fused step kernel")
     ap.add_argument("--ichunk", type=int, default=0, help="fused kernel x-planes per work-group; 0 = library default")

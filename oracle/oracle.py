@@ -65,6 +65,7 @@ def _lib(impl: str):
             "dump_mesh_profiles": (i, [vp, C.c_char_p]),
             "n_grid": (l, [vp, vp]),
             "add_tls_detectors": (i, [vp, C.c_char_p, l, l, l, l, l]),
+            "time_steps": (C.c_double, [vp, C.c_int]),
         })
     if impl == "oracle":
         sig.update({
@@ -80,9 +81,9 @@ def _lib(impl: str):
         f.restype = res
         f.argtypes = args
         fns[name] = f
-    if impl == "oracle":
-        lib.oracle_threads.restype = C.c_int
-        fns["threads"] = lib.oracle_threads
+    f = getattr(lib, p + "threads")
+    f.restype = C.c_int
+    fns["threads"] = f
     _libs[impl] = fns
     return fns
 
@@ -228,8 +229,8 @@ class CpuSim:
                 "max_E": self.f["last_max_E"](self.h)}
 
 
-def threads() -> int:
-    return _lib("oracle")["threads"]()
+def threads(impl: str = "oracle") -> int:
+    return _lib(impl)["threads"]()
 
 
 REF_MESH_SO = os.path.join(HERE, "_ref", "libref_mesh.so")

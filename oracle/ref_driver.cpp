@@ -17,6 +17,8 @@
 #include <memory>
 #include <cstring>
 #include <streambuf>
+#include <chrono>
+#include <omp.h>
 
 #include "global_function.hpp"
 #include "fdtd_stepper.hpp"
@@ -504,5 +506,15 @@ int ref_set_array(ref_sim* S, const char* name, const double* in) {
         if (nm == fn[q]) { std::memcpy(fv[q]->data(), in, S->N * sizeof(double)); return 0; }
     return -1;
 }
+
+// Wall-clock seconds of nsteps steps of the reference's own step functions
+// (their OpenMP loops over omp_get_max_threads() threads): the CPU baseline.
+double ref_time_steps(ref_sim* S, int nsteps) {
+    const auto t0 = std::chrono::steady_clock::now();
+    ref_step(S, nsteps);
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+int ref_threads(void) { return omp_get_max_threads(); }
 
 }  // extern "C"
