@@ -1178,7 +1178,11 @@ static int step_finish(fdtd_handle h) {
 
 // Enqueues nsteps steps on the handle's stream (and, with ms, the timing
 // events); the monitor records are read by step_finish.
-static int step_launch(fdtd_handle h, long n0, int nsteps, const double* src_values, std::vector<hipEvent_t>* ev) {
+// ev (optional) receives HIP events around the H and E launches of every
+// `every`-th step (steps 0, every, 2*every, ... of the call), 4 per timed step,
+// plus one at each end of the call.
+static int step_launch(fdtd_handle h, long n0, int nsteps, const double* src_values, std::vector<hipEvent_t>* ev,
+                       int every = 1) {
     if (!h || nsteps < 0 || n0 < 0) return fail(FDTD_EINVAL, "bad step arguments");
     if (h->batch) return fail(FDTD_EINVAL, "domain is a batch member: step it through its batch");
     if (int rc = step_finish(h)) return rc;  // the monitor buffer is reused
@@ -1209,13 +1213,13 @@ static int step_launch(fdtd_handle h, long n0, int nsteps, const double* src_val
     if (!mon_steps.empty()) HIPCHK(hipMemsetAsync(h->d_mon, 0, mon_steps.size() * sizeof(MonOut), h->stream));
     int slot = 0;
     if (ev) {
-        ev->resize((size_t)nsteps * 4 + 2);
+        ev->resize((size_t)((nsteps + every - 1) / every) * 4 + 2);
         for (auto& e : *ev) HIPCHK(hipEventCreate(&e));
         HIPCHK(hipEventRecord((*ev)[0], h->stream));
     }
     for (int s = 0; s < nsteps; ++s) {
         const long n = n0 + s;
-        dispatch_step(h, n, s, ev ? &(*ev)[1 + (size_t)s * 4] : nullptr);
+        dispatch_step(h, n, s, ev && s % every == 0 ? &(*ev)[1 + (size_t)(s / every) * 4] : nullptr);
         if (h->mon_interval > 0 && n % h->mon_interval == 0) dispatch_monitor(h, h->d_mon + slot++);
         if (h->prec == FDTD_FP16 && n % h->fp16_interval == 0) fp16_maintain(h, h->stream);
     }
@@ -1226,13 +1230,15 @@ static int step_launch(fdtd_handle h, long n0, int nsteps, const double* src_val
     return FDTD_OK;
 }
 
-static int step_impl(fdtd_handle h, long n0, int nsteps, const double* src_values, double* ms) {
+static int step_impl(fdtd_handle h, long n0, int nsteps, const double* src_values, double* ms, int every = 1,
+                     double* n_timed = nullptr) {
     std::vector<hipEvent_t> ev;
-    if (int rc = step_launch(h, n0, nsteps, src_values, ms ? &ev : nullptr)) return rc;
+    if (int rc = step_launch(h, n0, nsteps, src_values, ms ? &ev : nullptr, every)) return rc;
     if (ms && !ev.empty()) {
         HIPCHK(hipEventSynchronize(ev.back()));
         double th = 0, te = 0;
-        for (int s = 0; s < nsteps; ++s) {
+        const int nt = (nsteps + every - 1) / every;
+        for (int s = 0; s < nt; ++s) {
             float a = 0, b = 0;
             HIPCHK(hipEventElapsedTime(&a, ev[1 + (size_t)s * 4], ev[2 + (size_t)s * 4]));
             HIPCHK(hipEventElapsedTime(&b, ev[3 + (size_t)s * 4], ev[4 + (size_t)s * 4]));
@@ -1242,6 +1248,7 @@ static int step_impl(fdtd_handle h, long n0, int nsteps, const double* src_value
         float tot = 0;
         HIPCHK(hipEventElapsedTime(&tot, ev[0], ev.back()));
         ms[0] = th; ms[1] = te; ms[2] = tot;
+        if (n_timed) *n_timed = nt;
         for (auto& e : ev) (void)hipEventDestroy(e);
     }
     return step_finish(h);
@@ -1273,6 +1280,11 @@ int fdtd_step_wait(fdtd_handle h) {
 int fdtd_step_timed(fdtd_handle h, long n0, int nsteps, const double* src_values, double* ms) {
     if (!ms) return fail(FDTD_EINVAL, "null timing output");
     return step_impl(h, n0, nsteps, src_values, ms);
+}
+
+int fdtd_step_timed_every(fdtd_handle h, long n0, int nsteps, const double* src_values, int every, double* ms) {
+    if (!ms || every < 1) return fail(FDTD_EINVAL, "null timing output or stride below 1");
+    return step_impl(h, n0, nsteps, src_values, ms, every, ms + 3);
 }
 
 int fdtd_synchronize(fdtd_handle h) {

@@ -843,6 +843,16 @@ int fdtd_scene_step_timed(fdtd_scene* sc, int nsteps, double* ms) {
     return rc;
 }
 
+int fdtd_scene_step_timed_every(fdtd_scene* sc, int nsteps, int every, double* ms) {
+    if (!sc || nsteps < 0 || !ms || every < 1) return FDTD_EINVAL;
+    const size_t ns = sc->setup->src.size();
+    sc->vals.resize((size_t)nsteps * std::max<size_t>(ns, 1));
+    if (ns) fdtd_setup_source_values(sc->setup, sc->n, nsteps, sc->vals.data());
+    const int rc = fdtd_step_timed_every(sc->h, sc->n, nsteps, ns ? sc->vals.data() : nullptr, every, ms);
+    if (rc == FDTD_OK || rc == FDTD_EUNSTABLE) sc->n += nsteps;
+    return rc;
+}
+
 int fdtd_scene_step_async(fdtd_scene* sc, int nsteps) {
     if (!sc || nsteps < 0) return FDTD_EINVAL;
     const size_t ns = sc->setup->src.size();

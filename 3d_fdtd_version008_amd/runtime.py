@@ -32,7 +32,8 @@ EXPORTS = [
     "fdtd_add_tls_monitor", "fdtd_read_tls_monitor",
     "fdtd_set_sources", "fdtd_add_probe",
     "fdtd_add_dft", "fdtd_set_monitor", "fdtd_step", "fdtd_synchronize", "fdtd_download", "fdtd_upload",
-    "fdtd_read_probe", "fdtd_read_dft", "fdtd_read_check", "fdtd_info", "fdtd_step_timed",
+    "fdtd_read_probe", "fdtd_read_dft", "fdtd_read_check", "fdtd_info", "fdtd_step_timed", "fdtd_step_timed_every",
+    "fdtd_scene_step_timed_every",
     "fdtd_setup_build", "fdtd_setup_free", "fdtd_setup_info", "fdtd_setup_dt", "fdtd_setup_array",
     "fdtd_setup_source_values", "fdtd_setup_probe", "fdtd_scene_create", "fdtd_scene_destroy", "fdtd_scene_step",
     "fdtd_scene_steps_done", "fdtd_scene_handle", "fdtd_scene_step_timed", "fdtd_scene_step_async",
@@ -105,6 +106,8 @@ def load() -> C.CDLL:
         "fdtd_scene_destroy": (i, [vp]),
         "fdtd_scene_step": (i, [vp, i]),
         "fdtd_scene_step_timed": (i, [vp, i, vp]),
+        "fdtd_scene_step_timed_every": (i, [vp, i, i, vp]),
+        "fdtd_step_timed_every": (i, [vp, l, i, vp, i, vp]),
         "fdtd_scene_step_async": (i, [vp, i]),
         "fdtd_scene_step_wait": (i, [vp]),
         "fdtd_step_async": (i, [vp, l, i, vp]),
@@ -244,10 +247,12 @@ class Scene:
             _check(rc)
         return rc
 
-    def step_timed(self, nsteps: int):
-        """Steps and returns (H-kernel ms, E-kernel ms, call ms) from HIP events."""
-        ms = (C.c_double * 3)()
-        _check(self.lib.fdtd_scene_step_timed(self.sc, nsteps, ms))
+    def step_timed(self, nsteps: int, every: int = 1):
+        """Steps and returns (H-kernel ms, E-kernel ms, call ms, steps timed)
+        from HIP events around the kernels of every `every`-th step (the
+        kernel times are sums over the steps timed)."""
+        ms = (C.c_double * 4)()
+        _check(self.lib.fdtd_scene_step_timed_every(self.sc, nsteps, every, ms))
         return tuple(ms)
 
     def step_async(self, nsteps: int):

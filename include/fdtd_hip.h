@@ -233,6 +233,14 @@ int fdtd_synchronize(fdtd_handle h);
  * Synchronizes.  No counterpart in the reference (its timing is the
  * per-10-step wall clock of src/3D_FDTD_version008.cpp:351-353). */
 int fdtd_step_timed(fdtd_handle h, long n0, int nsteps, const double* src_values, double* ms);
+/* fdtd_step_timed with events around the H and E launches of every
+ * `every`-th step only (steps 0, every, 2*every, ... of the call): ms[0] and
+ * ms[1] are the sums over those steps, ms[2] the elapsed time of the whole
+ * call, ms[3] the number of steps timed.  An event pair costs the stream a
+ * few microseconds, so sampling keeps the call's elapsed time that of
+ * fdtd_step while the kernel averages stay live measurements.  No
+ * counterpart in the reference. */
+int fdtd_step_timed_every(fdtd_handle h, long n0, int nsteps, const double* src_values, int every, double* ms);
 
 /* ---- data movement (reference layout, FP64 on the host side) ----
  * which: 0..5 = Ex,Ey,Ez,Hx,Hy,Hz; 6 = Nu, 7 = Ng, 8 = Pz, 9 = Pz_prev

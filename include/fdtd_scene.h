@@ -170,6 +170,9 @@ int  fdtd_scene_destroy(fdtd_scene* sc);
 int  fdtd_scene_step(fdtd_scene* sc, int nsteps);
 /* fdtd_scene_step through fdtd_step_timed (per-kernel HIP-event times, ms[3]). */
 int  fdtd_scene_step_timed(fdtd_scene* sc, int nsteps, double* ms);
+/* fdtd_step_timed_every on the scene's handle (ms[0..3]).  No reference
+ * counterpart. */
+int  fdtd_scene_step_timed_every(fdtd_scene* sc, int nsteps, int every, double* ms);
 /* fdtd_scene_step through fdtd_step_async / fdtd_step_wait: enqueue, and
  * wait for (and check) what was enqueued.  steps_done counts enqueued steps. */
 int  fdtd_scene_step_async(fdtd_scene* sc, int nsteps);
