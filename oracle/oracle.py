@@ -267,21 +267,23 @@ def ref_auto_mesh(cfg, domain, npml, boxes=(), background_n=1.0, cap=20000):
     return [out[a * cap:a * cap + counts[a]].copy() for a in range(3)]
 
 
-def ref_dropin_compare(spec, nsteps: int, precision: int = 0):
+def ref_dropin_compare(spec, nsteps: int, precision: int = 0, device_bake: bool = False):
     """Runs the reference's own setup for `spec`, hands its arrays to the
     device library through the C ABI (fdtd_create, fdtd_set_sources,
     fdtd_add_probe, fdtd_step) and steps both for nsteps; returns (number
-    of probe channels, relative L2 of Ex Ey Ez Hx Hy Hz Nu Ng Pz Pz_prev and
-    the largest over the probe traces).  precision: 0 FP64, 1 FP32, 2 FP16."""
+    of probe channels, relative L2 of Ex Ey Ez Hx Hy Hz Nu Ng Pz Pz_prev, the
+    largest over the probe traces and the largest over the 12 material
+    arrays).  precision: 0 FP64, 1 FP32, 2 FP16; device_bake: the binding
+    passes the reference's structure list and the device bakes."""
     if not os.path.exists(REF_DROPIN_SO):
         raise FileNotFoundError(f"{REF_DROPIN_SO} is not built (run make -C oracle after the device library)")
     lib = C.CDLL(REF_DROPIN_SO)
     f = lib.ref_dropin_compare
     f.restype = C.c_int
-    f.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p]
+    f.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p]
     cs = spec.to_c()
-    rel = (C.c_double * 11)()
-    rc = f(C.addressof(cs), nsteps, precision, rel)
+    rel = (C.c_double * 12)()
+    rc = f(C.addressof(cs), nsteps, precision, int(device_bake), rel)
     if rc < 0:
         raise RuntimeError(f"drop-in run failed ({rc})")
     return rc, list(rel)

@@ -526,10 +526,14 @@ int ref_threads(void) { return omp_get_max_threads(); }
 // the reference's own state (its setup code above) is handed to
 // fdtd_create exactly as the maintainer's binding hands it over, the device
 // advances it with fdtd_step, and the reference advances the same state with
-// its own step on the CPU.  rel receives the relative L2 difference of
-// Ex Ey Ez Hx Hy Hz Nu Ng Pz Pz_prev (0..9) and the largest one over the
-// probe traces (10); returns the number of probe channels, or < 0 on error.
-int ref_dropin_compare(const fdtd_scene_spec* sp, int nsteps, int precision, double* rel) {
+// its own step on the CPU.  With device_bake the binding passes the
+// structure list (StructureScene::items and bg) instead of the baked
+// MaterialGrids, and the device bakes.  rel receives the relative L2
+// difference of Ex Ey Ez Hx Hy Hz Nu Ng Pz Pz_prev (0..9), the largest one
+// over the probe traces (10) and over the 12 material arrays the handle
+// holds against the reference's bake (11); returns the number of probe
+// channels, or < 0 on error.
+int ref_dropin_compare(const fdtd_scene_spec* sp, int nsteps, int precision, int device_bake, double* rel) {
     ref_sim* S = ref_create(sp);
     auto* cp = dynamic_cast<CPMLBoundary*>(S->bc.get());
     fdtd_grid_desc g{(int)S->NxT, (int)S->NyT, (int)S->NzT, S->gs.inv_dx.data(), S->gs.inv_dy.data(),
@@ -538,6 +542,32 @@ int ref_dropin_compare(const fdtd_scene_spec* sp, int nsteps, int precision, dou
     fdtd_material_desc m{mg.aEx.data(), mg.bEx.data(), mg.aEy.data(), mg.bEy.data(), mg.aEz.data(), mg.bEz.data(),
                          mg.aHx.data(), mg.bHx.data(), mg.aHy.data(), mg.bHy.data(), mg.aHz.data(), mg.bHz.data(),
                          nullptr};
+    // the structure list for a device bake (type + parameters of Box /
+    // Sphere / CylinderZ in insertion order, with their Material)
+    std::vector<fdtd_bake_shape> shapes;
+    for (const auto& it : S->scene.items) {
+        fdtd_bake_shape q{};
+        if (auto* bx = dynamic_cast<const Box*>(it.shape.get())) {
+            q.type = 0;
+            q.p[0] = bx->bb.x0; q.p[1] = bx->bb.x1; q.p[2] = bx->bb.y0;
+            q.p[3] = bx->bb.y1; q.p[4] = bx->bb.z0; q.p[5] = bx->bb.z1;
+        } else if (auto* sh = dynamic_cast<const Sphere*>(it.shape.get())) {
+            q.type = 1;
+            q.p[0] = sh->cx; q.p[1] = sh->cy; q.p[2] = sh->cz; q.p[3] = sh->r;
+        } else if (auto* cy = dynamic_cast<const CylinderZ*>(it.shape.get())) {
+            q.type = 2;
+            q.p[0] = cy->cx; q.p[1] = cy->cy; q.p[2] = cy->r; q.p[3] = cy->z0; q.p[4] = cy->z1;
+        }
+        q.eps_r = it.mat.eps_r; q.mu_r = it.mat.mu_r; q.sigma = it.mat.sigma; q.sigma_m = it.mat.sigma_m;
+        shapes.push_back(q);
+    }
+    const Material& bg = S->scene.bg;
+    fdtd_bake_desc bd{S->gs.x_bounds.data(), S->gs.y_bounds.data(), S->gs.z_bounds.data(), S->gs.dx.data(),
+                      S->gs.dy.data(), S->gs.dz.data(), bg.eps_r, bg.mu_r, bg.sigma, bg.sigma_m,
+                      (int)shapes.size(), shapes.data(), sp->subpixel_samples > 0 ? sp->subpixel_samples : 8,
+                      S->dt, PhysConst::EPS0, PhysConst::MU0};
+    if (device_bake) m = fdtd_material_desc{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                            nullptr, nullptr, nullptr, nullptr, &bd};
     fdtd_boundary_desc b{cp ? 1 : 0, (int)S->npml};
     if (cp) {
         b.kx = cp->kx_.data(); b.ky = cp->ky_.data(); b.kz = cp->kz_.data();
@@ -623,6 +653,13 @@ int ref_dropin_compare(const fdtd_scene_spec* sp, int nsteps, int precision, dou
         if (q >= 6 && !S->tls_on) continue;
         rc = fdtd_download(h, q, dev.data());
         rel[q] = rl2(dev, *ref[q]);
+    }
+    rel[11] = 0.0;
+    const std::vector<real>* mref[12] = {&mg.aEx, &mg.bEx, &mg.aEy, &mg.bEy, &mg.aEz, &mg.bEz,
+                                         &mg.aHx, &mg.bHx, &mg.aHy, &mg.bHy, &mg.aHz, &mg.bHz};
+    for (int q = 0; q < 12 && rc == 0; ++q) {
+        rc = fdtd_download_material(h, q, dev.data());
+        rel[11] = std::max(rel[11], rl2(dev, *mref[q]));
     }
     rel[10] = 0.0;
     std::vector<size_t> seen(S->probes.size(), 0);

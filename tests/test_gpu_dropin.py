@@ -5,7 +5,10 @@ profiles, the TLS regions and state, the sources and the probes; the binding
 hands those host arrays to fdtd_create, registers the source footprints and
 probes, and calls fdtd_step once per step with the value each source's own
 inject_half_step adds.  The reference's step advances the same state on the
-CPU in the same process.
+CPU in the same process.  In the device-bake form the binding passes the
+reference's structure list (StructureScene::items, bg) instead of its baked
+MaterialGrids, and the handle's material arrays are compared with the
+reference's bake too.
 
 Bar: FP64 fields, TLS arrays and probe traces within 1e-12 relative L2 of
 the reference's own step after the case's full step count."""
@@ -23,15 +26,17 @@ NAMES = ["uniform_sphere", "gain_slab", "gain_clamp_dipole", "plane_waves", "pec
          "auto_mesh_cylinder"]
 
 
+@pytest.mark.parametrize("device_bake", [False, True], ids=["host_bake", "device_bake"])
 @pytest.mark.parametrize("name", NAMES)
-def test_reference_main_with_device_step_fp64(pkg, oracle_mod, name):
+def test_reference_main_with_device_step_fp64(pkg, oracle_mod, name, device_bake):
     import cases
 
     if not os.path.exists(oracle_mod.REF_DROPIN_SO):
         pytest.skip("oracle/_ref/libref_dropin.so is not built (needs the reference tree at build time)")
     sp = cases.CASES[name]()
-    print(f"[dropin] {name} fp64 steps={sp.n_steps}", flush=True)
-    nch, rel = oracle_mod.ref_dropin_compare(sp, sp.n_steps, 0)
-    print(f"[dropin] {name}: worst relative L2 {max(rel):.3e} over fields, TLS arrays and {nch} probe channels")
+    print(f"[dropin] {name} fp64 steps={sp.n_steps} device_bake={device_bake}", flush=True)
+    nch, rel = oracle_mod.ref_dropin_compare(sp, sp.n_steps, 0, device_bake)
+    print(f"[dropin] {name}: worst relative L2 {max(rel):.3e} over fields, TLS arrays, {nch} probe channels"
+          f" and material arrays")
     assert nch == sum(len(p.components) for p in sp.probes)
-    assert max(rel) <= 1e-12, dict(zip("Ex Ey Ez Hx Hy Hz Nu Ng Pz Pz_prev probes".split(), rel))
+    assert max(rel) <= 1e-12, dict(zip("Ex Ey Ez Hx Hy Hz Nu Ng Pz Pz_prev probes materials".split(), rel))
