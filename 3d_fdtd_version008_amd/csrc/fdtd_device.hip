@@ -747,12 +747,15 @@ fdtdk::ArrSet fp16_arrays(const fdtd_sim* s) {
     return a;
 }
 
+constexpr int FP16_MAX_BLOCKS = 256;  // blocks per array of the maxima scan (one record each)
+
 // Keeps every FP16 array's largest stored magnitude inside [2^6, 2^13] by
 // power-of-two rescaling (exact), entirely on the device.
 void fp16_maintain(fdtd_sim* s, hipStream_t st) {
     const fdtdk::ArrSet a = fp16_arrays(s);
-    hipLaunchKernelGGL(fdtdk::k_fp16_maxima, dim3(256, 18), dim3(256), 0, st, a, s->d_maxbits);
-    hipLaunchKernelGGL(fdtdk::k_fp16_plan, dim3(1), dim3(64), 0, st, s->d_sc, s->d_factor, s->d_maxbits);
+    hipLaunchKernelGGL(fdtdk::k_fp16_maxima, dim3(FP16_MAX_BLOCKS, 18), dim3(256), 0, st, a, s->d_maxbits);
+    hipLaunchKernelGGL(fdtdk::k_fp16_plan, dim3(18), dim3(64), 0, st, s->d_sc, s->d_factor,
+                       (const unsigned*)s->d_maxbits, FP16_MAX_BLOCKS);
     hipLaunchKernelGGL(fdtdk::k_fp16_rescale, dim3(512), dim3(256), 0, st, a, (const float*)s->d_factor);
     fdtdk::ArrSet o{};  // the other buffer holds the never-updated cells too
     for (int q = 0; q < 6; ++q) { o.p[q] = (const uint16_t*)s->F2[s->cur ^ 1][q]; o.n[q] = s->N; }
@@ -921,7 +924,7 @@ int fdtd_create(const fdtd_grid_desc* g, const fdtd_material_desc* m, const fdtd
     if (!rc) rc = s->alloc(reinterpret_cast<void**>(&s->d_src), sizeof(SourceBox) * 64);
     if (!rc) rc = s->alloc(reinterpret_cast<void**>(&s->d_sc), 54 * sizeof(float));
     if (!rc) rc = s->alloc(reinterpret_cast<void**>(&s->d_factor), 18 * sizeof(float));
-    if (!rc) rc = s->alloc(reinterpret_cast<void**>(&s->d_maxbits), 18 * sizeof(unsigned));
+    if (!rc) rc = s->alloc(reinterpret_cast<void**>(&s->d_maxbits), 18 * FP16_MAX_BLOCKS * sizeof(unsigned));
     if (!rc) {
         // Initial FP16 scales: E ~ 1, H ~ E/eta0 (2^-8), psi_E ~ H/d, psi_H ~ E/d;
         // they adapt from the first monitor step on.
@@ -941,7 +944,8 @@ int fdtd_create(const fdtd_grid_desc* g, const fdtd_material_desc* m, const fdtd
         for (int q = 0; q < 18; ++q) sc[18 + q] = 1.0f / sc[q];
         for (int q = 0; q < 18; ++q) sc[36 + q] = std::ldexp(sc[q], -6);  // floors
         if (hipMemcpy(s->d_sc, sc, sizeof(sc), hipMemcpyHostToDevice) != hipSuccess) rc = fail(FDTD_EDEVICE, "scale init failed");
-        if (!rc && hipMemset(s->d_maxbits, 0, 18 * sizeof(unsigned)) != hipSuccess) rc = fail(FDTD_EDEVICE, "scale init failed");
+        if (!rc && hipMemset(s->d_maxbits, 0, 18 * FP16_MAX_BLOCKS * sizeof(unsigned)) != hipSuccess)
+            rc = fail(FDTD_EDEVICE, "scale init failed");
     }
     if (!rc) {
         // 32-bit byte offsets in the H/E kernels when every array they index

@@ -1816,13 +1816,23 @@ __global__ void __launch_bounds__(256) k_fp16_maxima(ArrSet s, unsigned* maxbits
     const uint4* v = reinterpret_cast<const uint4*>(p);
     unsigned m = 0;
     const long stride = (long)gridDim.x * blockDim.x;
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n8; i += stride) {
-        const uint4 w = v[i];
-        const unsigned x[4] = {w.x & 0x7fff7fffu, w.y & 0x7fff7fffu, w.z & 0x7fff7fffu, w.w & 0x7fff7fffu};
-        for (int q = 0; q < 4; ++q) {
-            const unsigned lo = x[q] & 0xffffu, hi = x[q] >> 16;
-            m = lo > m ? lo : m;
-            m = hi > m ? hi : m;
+    // four independent 16-byte loads per thread and iteration
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n8; i += 4 * stride) {
+        uint4 w[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const long iu = i + u * stride;
+            w[u] = iu < n8 ? v[iu] : make_uint4(0u, 0u, 0u, 0u);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const unsigned x[4] = {w[u].x & 0x7fff7fffu, w[u].y & 0x7fff7fffu, w[u].z & 0x7fff7fffu,
+                                   w[u].w & 0x7fff7fffu};
+            for (int q = 0; q < 4; ++q) {
+                const unsigned lo = x[q] & 0xffffu, hi = x[q] >> 16;
+                m = lo > m ? lo : m;
+                m = hi > m ? hi : m;
+            }
         }
     }
     for (long i = n8 * 8 + (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
@@ -1833,10 +1843,16 @@ __global__ void __launch_bounds__(256) k_fp16_maxima(ArrSet s, unsigned* maxbits
         const unsigned o = __shfl_down(m, off);
         m = o > m ? o : m;
     }
-    if ((threadIdx.x & 63) == 0 && m) {
+    // one record per (array, block), no atomics: same-address atomics from
+    // every wavefront serialized at the L2 and held the scan at half speed
+    __shared__ unsigned wm[4];
+    if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = wm[w] > m ? wm[w] : m;
         // as the float bit pattern of |value| (what the plan reads)
         const float f = __half2float(__ushort_as_half((unsigned short)m));
-        atomicMax(&maxbits[c], __float_as_uint(f));
+        maxbits[(long)c * gridDim.x + blockIdx.x] = __float_as_uint(f);
     }
 }
 // Decide new scales: keep the largest stored magnitude near 2^8 (2^8 x
@@ -1847,11 +1863,21 @@ __global__ void __launch_bounds__(256) k_fp16_maxima(ArrSet s, unsigned* maxbits
 // sc[36 + c] is the floor of scale c: a scale never drops below it, so a
 // field that is still growing from zero keeps the scale its physical
 // magnitude calls for instead of one fitted to its first tiny values.
-__global__ void k_fp16_plan(float* sc, float* factor, unsigned* maxbits) {
-    const int c = threadIdx.x;
-    if (c >= 18) return;
-    const float m = __uint_as_float(maxbits[c]);
-    maxbits[c] = 0u;
+// One 64-thread block per array c: the maximum over its `nblk` block records
+// from k_fp16_maxima, then the plan.
+__global__ void __launch_bounds__(64) k_fp16_plan(float* sc, float* factor, const unsigned* maxbits, int nblk) {
+    const int c = blockIdx.x;
+    unsigned mb = 0;
+    for (int b = threadIdx.x; b < nblk; b += 64) {
+        const unsigned x = maxbits[(long)c * nblk + b];
+        mb = x > mb ? x : mb;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned o = __shfl_down(mb, off);
+        mb = o > mb ? o : mb;
+    }
+    if (threadIdx.x != 0) return;
+    const float m = __uint_as_float(mb);
     float f = 1.0f;
     if (m > 2048.0f || (m > 0.0f && m < 4.0f)) {
         int e;
