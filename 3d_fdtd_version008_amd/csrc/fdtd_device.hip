@@ -101,6 +101,11 @@ struct fdtd_sim {
     int kwid_e = 0;     // the same for the E kernel
     bool off32 = false; // every array of the H/E kernels below 4 GiB: 32-bit byte offsets
     bool force64 = false;  // use the 64-bit offset kernels anyway
+    // The E kernel walks each XCD's range of tiles from the end, so it starts
+    // on the tiles the H kernel finished last and the next H kernel starts on
+    // the tiles E finished last: both begin on lines still held in the
+    // memory-side cache (C2 FP32 step 0.341 -> 0.331 ms).  Off: forward order.
+    bool erev = true;
     long psi_len[12] = {};
     void* inv[3] = {};
     void* cp[3] = {};
@@ -210,6 +215,7 @@ StepArgs<T, R> make_args(const fdtd_sim* s) {
     a.ntls = (int)s->h_tls.size();
     a.tk = fdtdk::make_fastdiv((unsigned)((s->nz + 63) / 64));
     a.tj = fdtdk::make_fastdiv((unsigned)((s->ny + fdtdk::TILE_ROWS - 1) / fdtdk::TILE_ROWS));
+    a.rev = 0;
     return a;
 }
 
@@ -638,6 +644,7 @@ int launch_step(fdtd_sim* s, long n, int sidx, hipEvent_t* ev) {
         }
         if (ev) (void)hipEventRecord(ev[1], s->stream);
         if (ev) (void)hipEventRecord(ev[2], s->stream);
+        a.rev = s->erev ? 1 : 0;  // E walks back over the tiles H has just walked
         switch (kwid_e * 2 + (o32 ? 1 : 0)) {
         case 129: hipLaunchKernelGGL((fdtdk::k_update_E<T, R, 64, unsigned>), grd, blk, 0, s->stream, a); break;
         case 128: hipLaunchKernelGGL((fdtdk::k_update_E<T, R, 64>), grd, blk, 0, s->stream, a); break;
@@ -646,6 +653,7 @@ int launch_step(fdtd_sim* s, long n, int sidx, hipEvent_t* ev) {
         case 33: hipLaunchKernelGGL((fdtdk::k_update_E<T, R, 16, unsigned>), grd, blk, 0, s->stream, a); break;
         default: hipLaunchKernelGGL((fdtdk::k_update_E<T, R, 16>), grd, blk, 0, s->stream, a); break;
         }
+        a.rev = 0;
         if (ev) (void)hipEventRecord(ev[3], s->stream);
     }
     // probes: PointFieldDetector records when n % save_every == 0
@@ -1117,11 +1125,10 @@ int fdtd_get_fp16_scales(fdtd_handle h, double* scales18) {
 int fdtd_set_kernel_mode(fdtd_handle h, int mode, int param) {
     if (!h || mode < 0 || mode > 1) return fail(FDTD_EINVAL, "mode must be 0 (H and E kernels) or 1 (fused step kernel)");
     auto width_ok = [](int w) { return w == 16 || w == 32 || w == 64; };
-    if (mode == 0 && (param & 0x20000)) {  // 64-bit element offsets even where 32-bit ones fit
-        h->force64 = true;
-        param &= ~0x20000;
-    } else if (mode == 0) {
-        h->force64 = false;
+    if (mode == 0) {
+        h->force64 = (param & 0x20000) != 0;  // 64-bit element offsets even where 32-bit ones fit
+        h->erev = (param & 0x100000) == 0;    // set: E tiles in forward order, as H walks them
+        param &= ~0x120000;
     }
     const int wh = param < 256 ? param : (param & 255), we = param < 256 ? param : (param >> 8);
     if (mode == 0 && param != 0 && !(param < 65536 && width_ok(wh) && width_ok(we)))

@@ -168,6 +168,9 @@ struct StepArgs {
     // tiles of the two-kernel grid: k-tiles of 64 cells and j-tiles of
     // TILE_ROWS rows, as divisors of the tile index
     FastDiv tk, tj;
+    // 1: each XCD walks its range of tiles from the end (the launch starts
+    // where the previous one, walking forward, finished)
+    int rev;
 };
 
 __device__ __forceinline__ bool in_pml(int i, int n, int np) {
@@ -575,8 +578,8 @@ __device__ __forceinline__ Tile tile_of(const StepArgs<T, R>& a) {
     constexpr int G = 64 / KW;  // k-groups per tile = rows per wavefront
     const unsigned nb = gridDim.x, b = blockIdx.x;
     const unsigned per = (nb + 7) / 8;
-    unsigned bb = (b % 8) * per + b / 8;
-    if (nb % 8 != 0) bb = b;  // the remap is a permutation only for multiples of 8
+    unsigned bb = (b % 8) * per + (a.rev ? per - 1 - b / 8 : b / 8);
+    if (nb % 8 != 0) bb = a.rev ? nb - 1 - b : b;  // the remap is a permutation only for multiples of 8
     const unsigned r = fdiv(bb, a.tk);
     const unsigned kt = bb - r * a.tk.d;
     const unsigned i = fdiv(r, a.tj);

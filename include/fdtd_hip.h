@@ -191,7 +191,9 @@ int fdtd_get_fp16_scales(fdtd_handle h, double* scales18);
  * value), for both kernels, or
  * width_H + 256 * width_E to set them apart (plus 131072: 64-bit element
  * offsets even where 32-bit byte offsets, the default with FP16 storage
- * when every array is below 4 GiB, would do);
+ * when every array is below 4 GiB, would do; plus 1048576: the E kernel
+ * walks its tiles in the H kernel's order instead of backwards, which by
+ * default lets each kernel start on the tiles the previous one finished);
  * mode 1 = one fused step kernel in which each wavefront owns one j-row and
  * marches along x over `param` planes (1..128; 0 keeps the current value),
  * double-buffering E, H and psi_H.  Grids the fused kernel does not take

@@ -133,14 +133,16 @@ def test_fp32_gain(pkg, oracle_mod, mode):
 @pytest.mark.parametrize("precision", ["fp32", "fp16"])
 def test_step_kernels_agree_bitwise(pkg, precision):
     """The two-kernel step (at every wavefront k-width, with 32-bit and 64-bit
-    element offsets: 0x20000 + width selects the latter) and the fused step
-    kernel run the same operations per value, so their results are identical
-    in every storage format."""
+    element offsets: 0x20000 + width selects the latter; with the E kernel
+    walking its tiles backwards, the default, or forwards: 0x100000) and the
+    fused step kernel run the same operations per value, so their results
+    are identical in every storage format."""
     sp = pkg.scene.gain_slab_case(n_total=44, npml=6, steps=120, fine_len=100e-9, coarse=20e-9, fine=5e-9,
                                   n_cuboids=4, seed=5)
     setup = pkg.Setup(sp)
     runs = {}
-    for mode in [(0, 16), (0, 32), (0, 64), (0, 0x20000 + 32), (0, 0x20000 + 16 + 256 * 64), (1, 16), (1, 5)]:
+    for mode in [(0, 16), (0, 32), (0, 64), (0, 0x20000 + 32), (0, 0x20000 + 16 + 256 * 64), (1, 16), (1, 5),
+                 (0, 0x100000 + 32), (0, 0x100000 + 0x20000 + 64)]:
         _say(f"[case] {sp.name} {precision} mode={mode}")
         sc = pkg.Scene(setup, precision)
         sc.set_kernel_mode(*mode)
@@ -164,7 +166,7 @@ def test_step_kernels_agree_bitwise_ragged(pkg, precision):
                                   n_cuboids=3, seed=9)
     setup = pkg.Setup(sp)
     runs = {}
-    for mode in [(0, 32), (0, 16), (0, 64), (1, 3), (1, 20)]:
+    for mode in [(0, 32), (0, 16), (0, 64), (0, 0x100000 + 32), (1, 3), (1, 20)]:
         _say(f"[case] ragged {precision} mode={mode}")
         sc = pkg.Scene(setup, precision)
         sc.set_kernel_mode(*mode)
