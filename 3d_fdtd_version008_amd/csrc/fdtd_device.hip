@@ -603,6 +603,25 @@ void set_cur(fdtd_sim* s, int c) {
     for (int q = 0; q < 12; ++q) s->psi[q] = (q < 6) ? s->psi2[0][q] : s->psi2[c][q];
 }
 
+// The second buffer of the fields and of psi_H, holding a copy of the
+// current contents (cells a step does not write keep their values in both).
+int ensure_second(fdtd_sim* s) {
+    const int o = s->cur ^ 1;
+    if (s->F2[o][0]) return FDTD_OK;
+    for (int q = 0; q < 6; ++q) {
+        if (int rc = s->alloc(&s->F2[o][q], (size_t)s->N * s->esz)) return rc;
+        HIPCHK(hipMemcpyAsync(s->F2[o][q], s->F2[s->cur][q], (size_t)s->N * s->esz, hipMemcpyDeviceToDevice, s->stream));
+    }
+    for (int q = 6; q < 12; ++q) {
+        if (!s->psi2[s->cur][q]) continue;
+        if (int rc = s->alloc(&s->psi2[o][q], (size_t)s->psi_len[q] * s->esz)) return rc;
+        HIPCHK(hipMemcpyAsync(s->psi2[o][q], s->psi2[s->cur][q], (size_t)s->psi_len[q] * s->esz,
+                              hipMemcpyDeviceToDevice, s->stream));
+    }
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return FDTD_OK;
+}
+
 constexpr int TLS_PARTS = 256;  // blocks of the TLS diagnostic reduction
 constexpr long MONPART_MAX = 2048;  // blocks (and partial records) of the monitor reduction
 
@@ -610,7 +629,7 @@ template <typename T, typename R>
 int launch_step(fdtd_sim* s, long n, int sidx, hipEvent_t* ev) {
     StepArgs<T, R> a = make_args<T, R>(s);
     a.srcv = s->d_srcv + (long)sidx * s->nsrc;
-    if (s->mode == 1 && rows_ok(s)) {
+    if (s->mode == 1 && rows_ok(s) && s->F2[s->cur ^ 1][0]) {
         const int o = s->cur ^ 1;
         a.Exw = (T*)s->F2[o][0]; a.Eyw = (T*)s->F2[o][1]; a.Ezw = (T*)s->F2[o][2];
         a.Hxw = (T*)s->F2[o][3]; a.Hyw = (T*)s->F2[o][4]; a.Hzw = (T*)s->F2[o][5];
@@ -625,7 +644,11 @@ int launch_step(fdtd_sim* s, long n, int sidx, hipEvent_t* ev) {
         a.srcv = s->d_srcv + (long)sidx * s->nsrc;
     } else {
         const dim3 blk(64 * fdtdk::TILE_ROWS);
-        const dim3 grd(a.tk.d * a.tj.d * (unsigned)s->nx);
+        // one work-group per tile; a 1-D launch holds at most 2^32 threads,
+        // so larger grids put the planes in gridDim.y (tile_of linearizes)
+        const unsigned long ntile = (unsigned long)a.tk.d * a.tj.d * (unsigned long)s->nx;
+        const dim3 grd = ntile * blk.x < (1ul << 32) ? dim3((unsigned)ntile)
+                                                     : dim3(a.tk.d * a.tj.d, (unsigned)s->nx);
         // narrow wavefronts pay off where instruction issue, not HBM, bounds
         // the kernels (FP16 storage)
         const int kw_def = 32;  // measured best or tied for every storage format
@@ -765,6 +788,7 @@ void fp16_maintain(fdtd_sim* s, hipStream_t st) {
     hipLaunchKernelGGL(fdtdk::k_fp16_plan, dim3(18), dim3(64), 0, st, s->d_sc, s->d_factor,
                        (const unsigned*)s->d_maxbits, FP16_MAX_BLOCKS);
     hipLaunchKernelGGL(fdtdk::k_fp16_rescale, dim3(512), dim3(256), 0, st, a, (const float*)s->d_factor);
+    if (!s->F2[s->cur ^ 1][0]) return;  // single-buffered
     fdtdk::ArrSet o{};  // the other buffer holds the never-updated cells too
     for (int q = 0; q < 6; ++q) { o.p[q] = (const uint16_t*)s->F2[s->cur ^ 1][q]; o.n[q] = s->N; }
     for (int q = 6; q < 12; ++q) {
@@ -909,29 +933,30 @@ int fdtd_create(const fdtd_grid_desc* g, const fdtd_material_desc* m, const fdtd
         delete s;
         return fail(FDTD_EDEVICE, "hipStreamCreate failed");
     }
-    for (int b = 0; b < 2; ++b)
-        for (int q = 0; q < 6 && !rc; ++q) rc = s->alloc(&s->F2[b][q], (size_t)s->N * s->esz);
+    // one buffer of fields and psi; the second (read by the fused step
+    // kernel while it writes the first) is allocated on the first switch to
+    // a schedule that needs it (ensure_second)
+    for (int q = 0; q < 6 && !rc; ++q) rc = s->alloc(&s->F2[0][q], (size_t)s->N * s->esz);
     if (!rc && s->bc_type == 1) {
         const long lx = 2L * s->np * s->ny * s->nz, ly = (long)s->nx * 2 * s->np * s->nz, lz = (long)s->nx * s->ny * 2 * s->np;
         static const int axis[12] = {1, 2, 2, 0, 0, 1, 1, 2, 2, 0, 0, 1};
         for (int q = 0; q < 12 && !rc; ++q) {
             s->psi_len[q] = axis[q] == 0 ? lx : (axis[q] == 1 ? ly : lz);
-            for (int b = 0; b < (q < 6 ? 1 : 2) && !rc; ++b) rc = s->alloc(&s->psi2[b][q], (size_t)s->psi_len[q] * s->esz);
+            rc = s->alloc(&s->psi2[0][q], (size_t)s->psi_len[q] * s->esz);
         }
     }
     if (!rc) {
-        for (int b = 0; b < 2; ++b) {
-            for (int q = 0; q < 6; ++q) hipMemsetAsync(s->F2[b][q], 0, (size_t)s->N * s->esz, s->stream);
-            for (int q = 0; q < 12; ++q)
-                if (s->psi2[b][q]) hipMemsetAsync(s->psi2[b][q], 0, (size_t)s->psi_len[q] * s->esz, s->stream);
-        }
+        for (int q = 0; q < 6; ++q) hipMemsetAsync(s->F2[0][q], 0, (size_t)s->N * s->esz, s->stream);
+        for (int q = 0; q < 12; ++q)
+            if (s->psi2[0][q]) hipMemsetAsync(s->psi2[0][q], 0, (size_t)s->psi_len[q] * s->esz, s->stream);
         set_cur(s, 0);
     }
     if (!rc) rc = precision == FDTD_FP64 ? setup_coeffs<double>(s, g, m, bc) : setup_coeffs<float>(s, g, m, bc);
     if (!rc) rc = setup_gain(s, tls);
     if (!rc) rc = s->alloc(reinterpret_cast<void**>(&s->d_src), sizeof(SourceBox) * 64);
     if (!rc) rc = s->alloc(reinterpret_cast<void**>(&s->d_sc), 54 * sizeof(float));
-    if (!rc) rc = s->alloc(reinterpret_cast<void**>(&s->d_factor), 18 * sizeof(float));
+    if (!rc) rc = s->alloc(reinterpret_cast<void**>(&s->d_factor), 20 * sizeof(float));
+    if (!rc && hipMemset(s->d_factor, 0, 20 * sizeof(float)) != hipSuccess) rc = fail(FDTD_EDEVICE, "scale init failed");
     if (!rc) rc = s->alloc(reinterpret_cast<void**>(&s->d_maxbits), 18 * FP16_MAX_BLOCKS * sizeof(unsigned));
     if (!rc) {
         // Initial FP16 scales: E ~ 1, H ~ E/eta0 (2^-8), psi_E ~ H/d, psi_H ~ E/d;
@@ -1135,6 +1160,8 @@ int fdtd_set_kernel_mode(fdtd_handle h, int mode, int param) {
         return fail(FDTD_EINVAL, "mode 0: the wavefront k-widths must be 16, 32 or 64 (0 keeps them)");
     if (mode == 1 && (param < 0 || param > fdtdk::ROW_MAX_CHUNK))
         return fail(FDTD_EINVAL, "mode 1: planes_per_chunk must be in [0, 128]");
+    if (mode == 1)
+        if (int rc = ensure_second(h)) return rc;
     h->mode = mode;
     if (param && mode == 0) {
         h->kwid_h = wh;
@@ -1166,6 +1193,11 @@ static int step_finish(fdtd_handle h) {
     }
     std::vector<MonOut> mo(mon_steps.size());
     HIPCHK(hipMemcpyAsync(mo.data(), h->d_mon, mo.size() * sizeof(MonOut), hipMemcpyDeviceToHost, h->stream));
+    // FP16 storage: the scale planner's sticky flags of a NaN/Inf stored in
+    // an E-class or H-class array (fields or psi)
+    unsigned nonfinite[2] = {0u, 0u};
+    if (h->prec == FDTD_FP16)
+        HIPCHK(hipMemcpyAsync(nonfinite, h->d_factor + 18, sizeof(nonfinite), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     bool ok = true;
     long fail_step = -1;
@@ -1180,6 +1212,10 @@ static int step_finish(fdtd_handle h) {
         r.max_P = bits_to_double(mo[q].maxP);
         r.pop_total = mo[q].pop_total;
         r.pop_initial = mo[q].pop_init;
+        if (q + 1 == mo.size()) {
+            r.nan_E |= nonfinite[0] != 0u;
+            r.nan_H |= nonfinite[1] != 0u;
+        }
         h->checks.push_back(r);
         if (ok && !apply_policy(h, r)) { ok = false; fail_step = r.step; }
     }
@@ -1347,11 +1383,13 @@ int fdtd_download(fdtd_handle h, int which, double* out) {
 int fdtd_upload(fdtd_handle h, int which, const double* in) {
     if (!h || !in) return fail(FDTD_EINVAL, "null argument");
     if (which >= 0 && which < 6) {
+        double scale = 1.0;
         if (h->prec == FDTD_FP16) {  // choose the component's scale from the data
             float sc[36];
             if (int rc = read_scales(h, sc)) return rc;
             double m = 0;
-            for (long i = 0; i < h->N; ++i) m = std::max(m, std::fabs(in[i]));
+            for (long i = 0; i < h->N; ++i)  // non-finite values are stored as they are
+                if (std::isfinite(in[i])) m = std::max(m, std::fabs(in[i]));
             if (m > 0) {
                 int e;
                 std::frexp(m, &e);
@@ -1359,11 +1397,13 @@ int fdtd_upload(fdtd_handle h, int which, const double* in) {
                 sc[18 + which] = std::ldexp(1.0f, 8 - e);
                 if (int rc = write_scales(h, sc)) return rc;
             }
-            int rc = upload_any(h, h->F2[0][which], in, (size_t)h->N, sc[which]);
-            return rc ? rc : upload_any(h, h->F2[1][which], in, (size_t)h->N, sc[which]);
+            scale = sc[which];
         }
-        int rc = upload_any(h, h->F2[0][which], in, (size_t)h->N, 1.0);
-        return rc ? rc : upload_any(h, h->F2[1][which], in, (size_t)h->N, 1.0);
+        for (int b = 0; b < 2; ++b) {  // both buffers where the second exists
+            if (!h->F2[b][which]) continue;
+            if (int rc = upload_any(h, h->F2[b][which], in, (size_t)h->N, scale)) return rc;
+        }
+        return FDTD_OK;
     }
     if (which >= 6 && which < 10) {
         if (!h->gain) return FDTD_OK;
@@ -1401,6 +1441,11 @@ int fdtd_read_dft(fdtd_handle h, int m, double* out) {
     HIPCHK(hipMemcpyAsync(out, d.d_acc, (size_t)d.ncell * d.nf * 2 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     return FDTD_OK;
+}
+
+int fdtd_is_stable(fdtd_handle h) {
+    if (!h) return fail(FDTD_EINVAL, "null handle");
+    return h->stable ? 1 : 0;
 }
 
 int fdtd_read_check(fdtd_handle h, fdtd_check_record* out, int max, int* count) {

@@ -28,6 +28,16 @@
 
 namespace fdtdk {
 
+// Saturation at the largest finite FP16 value: a finite value that outgrows
+// the array's scale between two scale checks is clipped (and the next check
+// raises the scale) instead of turning into Inf.  A NaN or an Inf is stored
+// as it is, so the stability monitor sees it as the reference's NaN/Inf scan
+// would (comparisons, not fminf/fmaxf, which return the non-NaN operand).
+__device__ __forceinline__ float fp16_sat(float x) {
+    constexpr float big = 3.402823466e38f;  // the largest finite float
+    return (x > 65504.0f && x <= big) ? 65504.0f : ((x < -65504.0f && x >= -big) ? -65504.0f : x);
+}
+
 // Storage <-> compute conversion.  FP16 storage carries a power-of-two scale
 // per component (value = stored * scale), so scaling is exact.
 template <typename T, typename R>
@@ -42,11 +52,7 @@ __device__ __forceinline__ R ld(const T* p, long i, R scale) {
 template <typename T, typename R>
 __device__ __forceinline__ void st(T* p, long i, R v, R inv_scale) {
     if constexpr (sizeof(T) == 2) {
-        // saturate at the largest finite FP16 value: an overflow between two
-        // scale checks is clipped (and the next check raises the scale)
-        // instead of turning into Inf/NaN
-        const float x = fminf(fmaxf((float)(v * inv_scale), -65504.0f), 65504.0f);
-        reinterpret_cast<__half*>(p)[i] = __float2half_rn(x);
+        reinterpret_cast<__half*>(p)[i] = __float2half_rn(fp16_sat((float)(v * inv_scale)));
     } else {
         (void)inv_scale;
         p[i] = (T)v;
@@ -112,8 +118,7 @@ __device__ __forceinline__ R cv(typename Raw<T>::type x, R scale) {
 template <typename T, typename R>
 __device__ __forceinline__ double stored_value(R v, R scale, R inv_scale) {
     if constexpr (sizeof(T) == 2) {
-        const float x = fminf(fmaxf((float)(v * inv_scale), -65504.0f), 65504.0f);
-        return (double)(__half2float(__float2half_rn(x)) * scale);
+        return (double)(__half2float(__float2half_rn(fp16_sat((float)(v * inv_scale)))) * scale);
     } else {
         (void)scale; (void)inv_scale;
         return (double)(T)v;
@@ -576,7 +581,9 @@ template <int KW, typename T, typename R>
 __device__ __forceinline__ Tile tile_of(const StepArgs<T, R>& a) {
     static_assert(KW == 16 || KW == 32 || KW == 64, "wavefront k-width");
     constexpr int G = 64 / KW;  // k-groups per tile = rows per wavefront
-    const unsigned nb = gridDim.x, b = blockIdx.x;
+    // linear work-group id; grids whose tile count times 256 threads
+    // exceeds 32 bits are launched two-dimensional (gridDim.y planes' worth)
+    const unsigned nb = gridDim.x * gridDim.y, b = blockIdx.y * gridDim.x + blockIdx.x;
     const unsigned per = (nb + 7) / 8;
     unsigned bb = (b % 8) * per + (a.rev ? per - 1 - b / 8 : b / 8);
     if (nb % 8 != 0) bb = a.rev ? nb - 1 - b : b;  // the remap is a permutation only for multiples of 8
@@ -1867,7 +1874,8 @@ __global__ void __launch_bounds__(256) k_fp16_maxima(ArrSet s, unsigned* maxbits
 // field that is still growing from zero keeps the scale its physical
 // magnitude calls for instead of one fitted to its first tiny values.
 // One 64-thread block per array c: the maximum over its `nblk` block records
-// from k_fp16_maxima, then the plan.
+// from k_fp16_maxima, then the plan.  factor[0..17] receives the rescale
+// factors; factor[18..19] are the non-finite flags (unsigned words).
 __global__ void __launch_bounds__(64) k_fp16_plan(float* sc, float* factor, const unsigned* maxbits, int nblk) {
     const int c = blockIdx.x;
     unsigned mb = 0;
@@ -1882,7 +1890,14 @@ __global__ void __launch_bounds__(64) k_fp16_plan(float* sc, float* factor, cons
     if (threadIdx.x != 0) return;
     const float m = __uint_as_float(mb);
     float f = 1.0f;
-    if (m > 2048.0f || (m > 0.0f && m < 4.0f)) {
+    if (mb >= 0x7f800000u) {
+        // a NaN or Inf is stored: keep the scale (frexpf of a non-finite
+        // value has no defined exponent) and raise the sticky flag of the
+        // array's class (word 18: E fields and psi_E, word 19: H fields and
+        // psi_H), which the host turns into a failed stability check
+        const int cls = (c < 3 || (c >= 6 && c < 12)) ? 18 : 19;
+        atomicOr(reinterpret_cast<unsigned*>(factor) + cls, 1u);
+    } else if (m > 2048.0f || (m > 0.0f && m < 4.0f)) {
         int e;
         frexpf(m, &e);      // m = fr * 2^e, fr in [0.5, 1)
         int shift = e - 8;  // new stored max in [2^7, 2^8)

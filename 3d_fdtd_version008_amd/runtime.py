@@ -18,6 +18,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib3dfdtd_hip.so")
 
 FP64, FP32, FP16 = 0, 1, 2
+# status codes of the C ABI (include/fdtd_hip.h)
+FDTD_OK, FDTD_EINVAL, FDTD_EDEVICE, FDTD_ENOMEM, FDTD_EUNSTABLE = 0, -1, -2, -3, -4
 PRECISION = {"fp64": FP64, "fp32": FP32, "fp16": FP16}
 FIELD_ID = {"Ex": 0, "Ey": 1, "Ez": 2, "Hx": 3, "Hy": 4, "Hz": 5, "Nu": 6, "Ng": 7, "Pz": 8, "Pz_prev": 9}
 PSI_NAMES = ["psi_Ex_y", "psi_Ex_z", "psi_Ey_z", "psi_Ey_x", "psi_Ez_x", "psi_Ez_y",
@@ -44,7 +46,7 @@ EXPORTS = [
     "fdtd_batch_set_tuning", "fdtd_batch_get_tuning", "fdtd_batch_set_graph", "fdtd_batch_step",
     "fdtd_scene_batch_create", "fdtd_scene_batch_destroy", "fdtd_scene_batch_add", "fdtd_scene_batch_remove",
     "fdtd_scene_batch_count", "fdtd_scene_batch_step", "fdtd_scene_batch_autotune", "fdtd_scene_batch_handle",
-    "fdtd_scene_set_stop", "fdtd_auto_mesh_axis", "fdtd_structure_mesh_bounds",
+    "fdtd_scene_set_stop", "fdtd_auto_mesh_axis", "fdtd_structure_mesh_bounds", "fdtd_is_stable",
 ]
 
 
@@ -132,6 +134,7 @@ def load() -> C.CDLL:
         "fdtd_scene_batch_autotune": (i, [vp, i, vp]),
         "fdtd_scene_batch_handle": (vp, [vp]),
         "fdtd_scene_set_stop": (i, [vp, l]),
+        "fdtd_is_stable": (i, [vp]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(lib, name)
@@ -317,6 +320,10 @@ class Scene:
         out = np.empty(n, dtype=np.float64)
         _check(self.lib.fdtd_read_dft(self.h, m, out.ctypes.data_as(C.c_void_p)))
         return out
+
+    def stable(self) -> bool:
+        """False once the stability policy has stopped this simulation."""
+        return _check(self.lib.fdtd_is_stable(self.h)) == 1
 
     def checks(self):
         n = C.c_int()

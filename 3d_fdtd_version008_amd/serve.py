@@ -63,6 +63,12 @@ class Result:
     admitted: float
     finished: float
     device_bytes: int
+    # "ok", or "unstable" (the stability monitor stopped the solve, as the
+    # reference's main returns 1) or "error" (the library refused it); a
+    # failed request is retired at once, its memory freed, and the others
+    # keep running
+    status: str = "ok"
+    error: str = ""
     probes: List[np.ndarray] = field(default_factory=list)
     dfts: List[np.ndarray] = field(default_factory=list)
     fields: Dict[str, np.ndarray] = field(default_factory=dict)
@@ -190,7 +196,11 @@ class Server:
                 for req, setup, est in ready:
                     if len(live) < self.max_active and used + est <= self.budget:
                         ta = now()
-                        sc = runtime.Scene(setup, req.precision, probe_capacity=req.steps + 16)
+                        try:
+                            sc = runtime.Scene(setup, req.precision, probe_capacity=req.steps + 16)
+                        except runtime.FdtdError as e:
+                            done.append(self._refused(req, setup, ta, e))
+                            continue
                         lv = _Live(req, setup, sc, req.steps, now(), sc.device_bytes())
                         live.append(lv)
                         used += lv.device_bytes
@@ -207,7 +217,14 @@ class Server:
                         keep.append(lv)
                         continue
                     progressed = True
-                    lv.scene.step_wait()  # the steps have run: applies the stability policy
+                    try:
+                        lv.scene.step_wait()  # the steps have run: applies the stability policy
+                    except runtime.FdtdError as e:
+                        tr = now()
+                        done.append(self._retire(lv, tr, e))
+                        used -= lv.device_bytes
+                        st["retire_s"] += now() - tr
+                        continue
                     if lv.scene.steps_done() >= lv.req.steps:
                         tr = now()
                         done.append(self._retire(lv, tr))
@@ -264,7 +281,11 @@ class Server:
                 for req, setup, est in ready:
                     if len(live) < self.max_active and used + est <= self.budget:
                         ta = now()
-                        sc = runtime.Scene(setup, req.precision, probe_capacity=req.steps + 16)
+                        try:
+                            sc = runtime.Scene(setup, req.precision, probe_capacity=req.steps + 16)
+                        except runtime.FdtdError as e:
+                            done.append(self._refused(req, setup, ta, e))
+                            continue
                         sc.set_stop(req.steps)
                         actual = sc.device_bytes()
                         cells = setup.shape[0] * setup.shape[1] * setup.shape[2]
@@ -293,14 +314,31 @@ class Server:
                 # their kernels start while the host enqueues the rest), run
                 # the batches meanwhile, then wait
                 solo = sorted((x for x in live if not x.batched), key=lambda x: -x.cells)
+                failed = {}  # id(_Live) -> FdtdError
+
+                def guarded(fn, x):
+                    if id(x) in failed:
+                        return
+                    try:
+                        fn(x)
+                    except runtime.FdtdError as e:
+                        failed[id(x)] = e
+
                 if solo:
-                    list(enq.map(lambda x: x.scene.step_async(min(self.tick, x.req.steps - x.scene.steps_done())),
-                                 solo))
-                for b in self.batches.values():
+                    list(enq.map(lambda x: guarded(
+                        lambda y: y.scene.step_async(min(self.tick, y.req.steps - y.scene.steps_done())), x), solo))
+                for prec, b in self.batches.items():
                     if len(b):
-                        st["device_ms"] += b.step(self.tick)
+                        try:
+                            st["device_ms"] += b.step(self.tick)
+                        except runtime.FdtdError as e:
+                            # the batch ran its steps; the members the policy
+                            # stopped are the unstable ones
+                            for x in live:
+                                if x.batched and x.req.precision == prec and not x.scene.stable():
+                                    failed[id(x)] = e
                 if solo:
-                    list(enq.map(lambda x: x.scene.step_wait(), solo))
+                    list(enq.map(lambda x: guarded(lambda y: y.scene.step_wait(), x), solo))
                 st["step_s"] += now() - ts
                 st["ticks"] += 1
                 for lv in live:
@@ -308,8 +346,8 @@ class Server:
                 t_end = now()
                 keep = []
                 for lv in live:
-                    if lv.scene.steps_done() >= lv.req.steps:
-                        done.append(self._retire(lv, t_end))
+                    if id(lv) in failed or lv.scene.steps_done() >= lv.req.steps:
+                        done.append(self._retire(lv, t_end, failed.get(id(lv))))
                         used -= lv.device_bytes
                     else:
                         keep.append(lv)
@@ -328,13 +366,26 @@ class Server:
         setup = runtime.Setup(req.spec)
         return req, setup, estimate_device_bytes(setup, req.precision)
 
-    def _retire(self, lv: _Live, t_end: float) -> Result:
+    @staticmethod
+    def _refused(req: Request, setup: runtime.Setup, t: float, err: Exception) -> Result:
+        """A request the library refused at admission (e.g. device memory)."""
+        return Result(rid=req.rid, precision=req.precision, steps=req.steps,
+                      cells=setup.shape[0] * setup.shape[1] * setup.shape[2], arrival=req.arrival, admitted=t,
+                      finished=t, device_bytes=0, status="error", error=str(err))
+
+    def _retire(self, lv: _Live, t_end: float, err: Optional[Exception] = None) -> Result:
+        """Reads back a finished (or failed) request and frees its device state."""
         if lv.batched:
             self.batches[lv.req.precision].remove(lv.scene)
         st = lv.setup
         res = Result(rid=lv.req.rid, precision=lv.req.precision, steps=lv.req.steps,
                      cells=st.shape[0] * st.shape[1] * st.shape[2], arrival=lv.req.arrival, admitted=lv.admitted,
                      finished=t_end, device_bytes=lv.device_bytes)
+        if err is not None:
+            unstable = isinstance(err, runtime.FdtdError) and err.code == runtime.FDTD_EUNSTABLE
+            res.status = "unstable" if unstable else "error"
+            res.error = str(err)
+            res.steps = lv.scene.steps_done()
         res.probes = [lv.scene.probe(c) for c in range(st.n_channels)]
         res.dfts = [lv.scene.dft(m) for m in range(st.n_dfts)]
         if lv.req.keep_fields:
@@ -393,6 +444,8 @@ def request_stream(n: int, rate: float, seed: int = 1, lo: int = 48, hi: int = 2
 
 
 def summarize(results: List[Result]) -> dict:
+    failed = sum(1 for r in results if r.status != "ok")
+    results = [r for r in results if r.status == "ok"] or results
     lat = np.array([r.latency for r in results])
     work = sum(r.cells * r.steps for r in results)
     t_first = min(r.arrival for r in results)
@@ -405,4 +458,5 @@ def summarize(results: List[Result]) -> dict:
         "latency_p50_s": float(np.percentile(lat, 50)),
         "latency_p99_s": float(np.percentile(lat, 99)),
         "latency_max_s": float(lat.max()),
+        "failed": failed,
     }

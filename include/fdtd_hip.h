@@ -276,6 +276,12 @@ int fdtd_add_tls_monitor(fdtd_handle h, const double* dx, const double* dy, cons
 int fdtd_read_tls_monitor(fdtd_handle h, double* out, long max, long* count);
 /* Monitor records of the last fdtd_step call (up to max); *count = number. */
 int fdtd_read_check(fdtd_handle h, fdtd_check_record* out, int max, int* count);
+/* 1 while the stability policy has passed every monitor step so far, 0 once
+ * it has failed (StabilityMonitor::simulation_stable,
+ * src/stability_monitor.hpp:28, after which the reference's main returns 1,
+ * src/3D_FDTD_version008.cpp:339-349).  Tells a batch's failed members from
+ * the rest after fdtd_batch_step returns FDTD_EUNSTABLE. */
+int fdtd_is_stable(fdtd_handle h);
 
 /* ---- introspection ---- */
 /* info[0..7] = nx, ny, nz, precision, n_materials (LUT entries), material id

@@ -83,3 +83,72 @@ def test_async_steps_equal_sync_steps(pkg):
         assert np.array_equal(a.probe(c), ref.probe(c))
     for sc in (ref, a, b, bref):
         sc.close()
+
+
+@pytest.mark.parametrize("engine", ["streams", "batch", "hybrid"])
+def test_unstable_request_is_isolated(pkg, engine):
+    """One request of the stream is unstable (Courant number 1.2: its fields
+    grow until the reference's policy stops it).  It is retired with status
+    "unstable" and its memory freed; every other request still equals its
+    solo run bit for bit."""
+    serve = importlib.import_module("3d_fdtd_version008_amd.serve")
+    reqs = serve.request_stream(6, rate=60.0, seed=4, lo=32, hi=44, steps=(80, 160))
+    bad = reqs[2]
+    bad.spec = bad.spec.copy(S=1.2)
+    bad.steps = bad.spec.n_steps = 400
+    _say(f"[case] serve {engine} with request {bad.rid} unstable")
+    small = int(np.median([pkg.Setup(r.spec).shape[0] ** 3 for r in reqs]))
+    srv = serve.Server(budget_bytes=50e9, max_active=4, tick=40, graph=True, setup_workers=4, engine=engine,
+                       small_cells=small)
+    served = {r.rid: r for r in srv.run(reqs)}
+    assert sorted(served) == [r.rid for r in reqs]
+    assert served[bad.rid].status == "unstable", served[bad.rid].error
+    assert served[bad.rid].steps < bad.steps
+    s = serve.summarize(list(served.values()))
+    assert s["failed"] == 1
+    for r in reqs:
+        if r.rid == bad.rid:
+            continue
+        got = served[r.rid]
+        assert got.status == "ok"
+        alone = serve.solve_alone(r, "streams" if engine == "hybrid" else engine)
+        for a, b in zip(alone.probes, got.probes):
+            assert np.array_equal(a, b), f"request {r.rid} probe differs"
+
+
+def test_served_requests_match_oracle(pkg, oracle_mod):
+    """Every request of a served mixed FP32/FP16 stream (the serving mix of
+    dielectric, gain and multi-block scenes, on reduced sizes and step
+    counts so the CPU oracle finishes in seconds) against the FP64 CPU oracle
+    on the same request: probe traces within 2e-4 (FP32) and 5e-2 (FP16)
+    relative L2, DFT spectra within 2e-4 (FP32) and 1e-2 (FP16).  FP16
+    storage rounds every stored value to 11 significant bits, and over a few
+    hundred steps these roundings leave errors of a few 1e-3 to 1e-2 at a
+    probe; the bar is the FP16 form's stated accuracy for these requests."""
+    serve = importlib.import_module("3d_fdtd_version008_amd.serve")
+    reqs = serve.request_stream(8, rate=50.0, seed=7, lo=26, hi=40, steps=(150, 300))
+    assert {r.precision for r in reqs} == {"fp32", "fp16"}
+    srv = serve.Server(budget_bytes=50e9, max_active=4, tick=50, graph=True, setup_workers=4, engine="streams")
+    served = {r.rid: r for r in srv.run(reqs)}
+    bars = {"fp32": (2e-4, 2e-4), "fp16": (5e-2, 1e-2)}
+    for r in reqs:
+        got = served[r.rid]
+        assert got.status == "ok"
+        ref = oracle_mod.CpuSim(r.spec, "oracle")
+        ref.step(r.steps)
+        errs, c = {}, 0
+        for p, pr in enumerate(r.spec.probes):
+            n = len(pr.components)
+            a = np.stack([got.probes[c + q] for q in range(n)])
+            b = np.stack([ref.probe(p, q) for q in range(n)])
+            assert a.shape == b.shape
+            errs[f"probe{p}"] = rel_l2(a, b)
+            c += n
+        for m in range(len(r.spec.dfts)):
+            errs[f"dft{m}"] = rel_l2(got.dfts[m], ref.dft(m))
+        _say(f"  request {r.rid} {r.precision} {r.spec.name} steps={r.steps}: "
+             + " ".join(f"{k}={v:.1e}" for k, v in errs.items()))
+        bp, bd = bars[r.precision]
+        bad = {k: v for k, v in errs.items() if not v <= (bp if k.startswith("probe") else bd)}
+        assert not bad, (r.rid, r.precision, bad)
+        ref.close()
