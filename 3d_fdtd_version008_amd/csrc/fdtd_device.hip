@@ -18,6 +18,7 @@
 
 #include "../../include/fdtd_hip.h"
 #include "fdtd_kernels.hpp"
+#include "fdtd_fused.hpp"
 #include "fdtd_bake.hpp"
 
 using fdtdk::MonOut;
@@ -96,10 +97,13 @@ struct fdtd_sim {
     void* psi[12] = {}; // = psi2[cur] (psi_E always psi2[0])
     int cur = 0;
     int mode = 0;       // 0: H and E kernels (default), 1: fused step kernel
-    int ichunk = 8;     // x-planes per work-group of the fused kernel
+    int ichunk = 0;     // x-planes per work-group of the fused kernel (0: by grid size)
+    int fz_cw = 0;      // fused kernel: cells per lane per row chunk (0: by grid size)
+    int fz_jw = 0;      // fused kernel: rows per work-group (0: by grid size)
     int kwid_h = 0;     // k-width of a wavefront's cells in the H kernel (16, 32, 64; 0: by precision)
     int kwid_e = 0;     // the same for the E kernel
-    bool off32 = false; // every array of the H/E kernels below 4 GiB: 32-bit byte offsets
+    bool fits32 = false; // every array a step kernel indexes by cell is below 4 GiB
+    bool off32 = false; // the H/E kernels use 32-bit byte offsets (fits32, FP16 storage)
     bool force64 = false;  // use the 64-bit offset kernels anyway
     // The E kernel walks each XCD's range of tiles from the end, so it starts
     // on the tiles the H kernel finished last and the next H kernel starts on
@@ -567,33 +571,65 @@ int setup_gain(fdtd_sim* s, const fdtd_tls_desc* t) {
     return s->upload(&s->d_tls, s->h_tls.data(), s->h_tls.size());
 }
 
-template <typename T, typename R>
-void launch_rows(fdtd_sim* s, StepArgs<T, R>& a) {
-    const int njt = (s->ny + fdtdk::ROW_WAVES - 1) / fdtdk::ROW_WAVES;
-    const int nic = (s->nx + s->ichunk - 1) / s->ichunk;
-    const int nwg = njt * nic;
+// Geometry of the fused step kernel for this grid: CW cells per lane and kc
+// k-chunk wavefronts per row (kc * CW * 64 >= NzT with the least unused
+// lanes, then the most rows per work-group), jw rows per work-group plus the
+// halo row within 16 wavefronts and 64 KiB of LDS, ic planes per
+// work-group.  Overrides (fdtd_set_kernel_mode): s->fz_cw, s->fz_jw.
+template <typename R>
+bool fused_geom(const fdtd_sim* s, fdtdk::FusedGeom& g, int& cw) {
     const int nseg = (s->nz + 63) / 64;
-    const unsigned lds = (unsigned)fdtdk::row_lds_bytes<T, R>(s->nz, nseg);
-    hipLaunchKernelGGL((fdtdk::k_step_rows<T, R>), dim3((unsigned)nwg), dim3(64 * fdtdk::ROW_WAVES), lds, s->stream, a,
-                       s->ichunk, njt, nwg, nseg);
+    int best = -1, best_cw = 0, best_kc = 0, best_jw = 0;
+    for (int c = 1; c <= fdtdk::FZ_MAX_CW; ++c) {
+        if (s->fz_cw && c != s->fz_cw) continue;
+        const int kc = (nseg + c - 1) / c;
+        int jw = fdtdk::FZ_MAX_WAVES / kc - 1;
+        while (jw > 0 && fdtdk::fused_lds_bytes<R>((jw + 1) * kc, c) > 64 * 1024) --jw;
+        jw = std::min(jw, s->fz_jw ? s->fz_jw : 8);
+        if (jw < 1) continue;
+        // unused lanes first, then taller blocks (less halo work), then
+        // fewer cells per lane (fewer registers)
+        const int waste = kc * c - nseg;
+        const int score = (100 - waste) * 1000 + std::min(jw, 7) * 10 + (fdtdk::FZ_MAX_CW - c);
+        if (score > best) { best = score; best_cw = c; best_kc = kc; best_jw = jw; }
+    }
+    if (best < 0) return false;
+    cw = best_cw;
+    g.kc = best_kc;
+    g.jw = best_jw;
+    g.njb = (s->ny + g.jw - 1) / g.jw;
+    int ic = s->ichunk;
+    if (ic <= 0) {
+        // enough work-groups to fill every CU several times over, with few
+        // halo planes (each chunk recomputes H of one extra plane)
+        const long per = std::max(1L, (long)s->nx * g.njb / 1024);
+        ic = (int)std::min<long>(64, std::max<long>(4, per));
+    }
+    g.ic = std::min(ic, s->nx);
+    g.nwg = g.njb * ((s->nx + g.ic - 1) / g.ic);
+    return true;
+}
+// The fused kernel addresses cells by 32-bit byte offsets, reads the
+// cell-invariant H coefficients, and keeps a row in at most 16 wavefronts.
+bool fused_ok(const fdtd_sim* s) {
+    if (!s->h_uniform || !s->fits32) return false;
+    fdtdk::FusedGeom g{};
+    int cw = 0;
+    return s->prec == FDTD_FP64 ? fused_geom<double>(s, g, cw) : fused_geom<float>(s, g, cw);
 }
 template <typename T, typename R>
-bool rows_fit(const fdtd_sim* s) {
-    return fdtdk::row_lds_bytes<T, R>(s->nz, (s->nz + 63) / 64) <= 48 * 1024 &&
-           (unsigned long)s->N * sizeof(T) < (1ul << 32);
-}
-// The row kernel keeps the z tables and its carried state in LDS and
-// addresses fields by 32-bit offsets.
-// It also stages its small tables (sources, TLS regions) in LDS and needs
-// cell-invariant H coefficients; other grids use the two-kernel schedule.
-bool rows_ok(const fdtd_sim* s) {
-    if (!s->h_uniform || s->nsrc > fdtdk::STAGE_MAX_SRC || (int)s->h_tls.size() > fdtdk::STAGE_MAX_TLS ||
-        s->ichunk > fdtdk::ROW_MAX_CHUNK || s->nz > fdtdk::ROW_MAX_NZ)
-        return false;
-    switch (s->prec) {
-    case FDTD_FP64: return rows_fit<double, double>(s);
-    case FDTD_FP32: return rows_fit<float, float>(s);
-    default: return rows_fit<uint16_t, float>(s);
+void launch_fused(fdtd_sim* s, StepArgs<T, R>& a) {
+    fdtdk::FusedGeom g{};
+    int cw = 0;
+    fused_geom<R>(s, g, cw);
+    const int waves = (g.jw + 1) * g.kc;
+    const unsigned lds = (unsigned)fdtdk::fused_lds_bytes<R>(waves, cw);
+    const dim3 grd((unsigned)g.nwg), blk((unsigned)(64 * waves));
+    switch (cw) {
+    case 1: hipLaunchKernelGGL((fdtdk::k_step_fused<T, R, 1>), grd, blk, lds, s->stream, a, g); break;
+    case 2: hipLaunchKernelGGL((fdtdk::k_step_fused<T, R, 2>), grd, blk, lds, s->stream, a, g); break;
+    case 3: hipLaunchKernelGGL((fdtdk::k_step_fused<T, R, 3>), grd, blk, lds, s->stream, a, g); break;
+    default: hipLaunchKernelGGL((fdtdk::k_step_fused<T, R, 4>), grd, blk, lds, s->stream, a, g); break;
     }
 }
 
@@ -629,13 +665,13 @@ template <typename T, typename R>
 int launch_step(fdtd_sim* s, long n, int sidx, hipEvent_t* ev) {
     StepArgs<T, R> a = make_args<T, R>(s);
     a.srcv = s->d_srcv + (long)sidx * s->nsrc;
-    if (s->mode == 1 && rows_ok(s) && s->F2[s->cur ^ 1][0]) {
+    if (s->mode == 1 && fused_ok(s) && s->F2[s->cur ^ 1][0]) {
         const int o = s->cur ^ 1;
         a.Exw = (T*)s->F2[o][0]; a.Eyw = (T*)s->F2[o][1]; a.Ezw = (T*)s->F2[o][2];
         a.Hxw = (T*)s->F2[o][3]; a.Hyw = (T*)s->F2[o][4]; a.Hzw = (T*)s->F2[o][5];
         for (int q = 6; q < 12; ++q) a.psiw[q] = (T*)s->psi2[o][q];
         if (ev) (void)hipEventRecord(ev[0], s->stream);
-        launch_rows<T, R>(s, a);
+        launch_fused<T, R>(s, a);
         if (ev) (void)hipEventRecord(ev[1], s->stream);
         if (ev) (void)hipEventRecord(ev[2], s->stream);
         if (ev) (void)hipEventRecord(ev[3], s->stream);
@@ -644,39 +680,42 @@ int launch_step(fdtd_sim* s, long n, int sidx, hipEvent_t* ev) {
         a.srcv = s->d_srcv + (long)sidx * s->nsrc;
     } else {
         const dim3 blk(64 * fdtdk::TILE_ROWS);
-        // one work-group per tile; a 1-D launch holds at most 2^32 threads,
-        // so larger grids put the planes in gridDim.y (tile_of linearizes)
-        const unsigned long ntile = (unsigned long)a.tk.d * a.tj.d * (unsigned long)s->nx;
-        const dim3 grd = ntile * blk.x < (1ul << 32) ? dim3((unsigned)ntile)
-                                                     : dim3(a.tk.d * a.tj.d, (unsigned)s->nx);
         // narrow wavefronts pay off where instruction issue, not HBM, bounds
         // the kernels (FP16 storage)
         const int kw_def = 32;  // measured best or tied for every storage format
         const int kwid = s->kwid_h ? s->kwid_h : kw_def;
         const int kwid_e = s->kwid_e ? s->kwid_e : kw_def;
-        if (ev) (void)hipEventRecord(ev[0], s->stream);
         // 32-bit byte offsets where every array the kernels touch is below 4 GiB
         const bool o32 = s->off32 && !s->force64;
-        switch (kwid * 2 + (o32 ? 1 : 0)) {
-        case 129: hipLaunchKernelGGL((fdtdk::k_update_H<T, R, 64, unsigned>), grd, blk, 0, s->stream, a); break;
-        case 128: hipLaunchKernelGGL((fdtdk::k_update_H<T, R, 64>), grd, blk, 0, s->stream, a); break;
-        case 65: hipLaunchKernelGGL((fdtdk::k_update_H<T, R, 32, unsigned>), grd, blk, 0, s->stream, a); break;
-        case 64: hipLaunchKernelGGL((fdtdk::k_update_H<T, R, 32>), grd, blk, 0, s->stream, a); break;
-        case 33: hipLaunchKernelGGL((fdtdk::k_update_H<T, R, 16, unsigned>), grd, blk, 0, s->stream, a); break;
-        default: hipLaunchKernelGGL((fdtdk::k_update_H<T, R, 16>), grd, blk, 0, s->stream, a); break;
+        if (ev) (void)hipEventRecord(ev[0], s->stream);
+        {
+            const int n = s->nx;
+            // one work-group per tile; a 1-D launch holds at most 2^32 threads,
+            // so larger grids put the planes in gridDim.y (tile_of linearizes)
+            const unsigned long ntile = (unsigned long)a.tk.d * a.tj.d * (unsigned long)n;
+            const dim3 grd = ntile * blk.x < (1ul << 32) ? dim3((unsigned)ntile) : dim3(a.tk.d * a.tj.d, (unsigned)n);
+            a.rev = 0;
+            switch (kwid * 2 + (o32 ? 1 : 0)) {
+            case 129: hipLaunchKernelGGL((fdtdk::k_update_H<T, R, 64, unsigned>), grd, blk, 0, s->stream, a); break;
+            case 128: hipLaunchKernelGGL((fdtdk::k_update_H<T, R, 64>), grd, blk, 0, s->stream, a); break;
+            case 65: hipLaunchKernelGGL((fdtdk::k_update_H<T, R, 32, unsigned>), grd, blk, 0, s->stream, a); break;
+            case 64: hipLaunchKernelGGL((fdtdk::k_update_H<T, R, 32>), grd, blk, 0, s->stream, a); break;
+            case 33: hipLaunchKernelGGL((fdtdk::k_update_H<T, R, 16, unsigned>), grd, blk, 0, s->stream, a); break;
+            default: hipLaunchKernelGGL((fdtdk::k_update_H<T, R, 16>), grd, blk, 0, s->stream, a); break;
+            }
+            if (ev) (void)hipEventRecord(ev[1], s->stream);
+            if (ev) (void)hipEventRecord(ev[2], s->stream);
+            a.rev = s->erev ? 1 : 0;  // E walks back over the tiles H has just walked
+            switch (kwid_e * 2 + (o32 ? 1 : 0)) {
+            case 129: hipLaunchKernelGGL((fdtdk::k_update_E<T, R, 64, unsigned>), grd, blk, 0, s->stream, a); break;
+            case 128: hipLaunchKernelGGL((fdtdk::k_update_E<T, R, 64>), grd, blk, 0, s->stream, a); break;
+            case 65: hipLaunchKernelGGL((fdtdk::k_update_E<T, R, 32, unsigned>), grd, blk, 0, s->stream, a); break;
+            case 64: hipLaunchKernelGGL((fdtdk::k_update_E<T, R, 32>), grd, blk, 0, s->stream, a); break;
+            case 33: hipLaunchKernelGGL((fdtdk::k_update_E<T, R, 16, unsigned>), grd, blk, 0, s->stream, a); break;
+            default: hipLaunchKernelGGL((fdtdk::k_update_E<T, R, 16>), grd, blk, 0, s->stream, a); break;
+            }
+            a.rev = 0;
         }
-        if (ev) (void)hipEventRecord(ev[1], s->stream);
-        if (ev) (void)hipEventRecord(ev[2], s->stream);
-        a.rev = s->erev ? 1 : 0;  // E walks back over the tiles H has just walked
-        switch (kwid_e * 2 + (o32 ? 1 : 0)) {
-        case 129: hipLaunchKernelGGL((fdtdk::k_update_E<T, R, 64, unsigned>), grd, blk, 0, s->stream, a); break;
-        case 128: hipLaunchKernelGGL((fdtdk::k_update_E<T, R, 64>), grd, blk, 0, s->stream, a); break;
-        case 65: hipLaunchKernelGGL((fdtdk::k_update_E<T, R, 32, unsigned>), grd, blk, 0, s->stream, a); break;
-        case 64: hipLaunchKernelGGL((fdtdk::k_update_E<T, R, 32>), grd, blk, 0, s->stream, a); break;
-        case 33: hipLaunchKernelGGL((fdtdk::k_update_E<T, R, 16, unsigned>), grd, blk, 0, s->stream, a); break;
-        default: hipLaunchKernelGGL((fdtdk::k_update_E<T, R, 16>), grd, blk, 0, s->stream, a); break;
-        }
-        a.rev = 0;
         if (ev) (void)hipEventRecord(ev[3], s->stream);
     }
     // probes: PointFieldDetector records when n % save_every == 0
@@ -988,7 +1027,8 @@ int fdtd_create(const fdtd_grid_desc* g, const fdtd_material_desc* m, const fdtd
         // bound by memory, not issue), so FP16 storage only.
         const double esz = precision == FDTD_FP64 ? 8.0 : precision == FDTD_FP32 ? 4.0 : 2.0;
         const double lim = 4294967295.0;
-        s->off32 = precision == FDTD_FP16 && (double)s->N * std::max(esz, 2.0) < lim && (double)s->gbox * 8.0 < lim;
+        s->fits32 = (double)s->N * std::max(esz, 2.0) < lim && (double)s->gbox * 8.0 < lim;
+        s->off32 = precision == FDTD_FP16 && s->fits32;
     }
     if (!rc) {  // monitor block partials (allocated here so the footprint is known at creation)
         void* p = nullptr;
@@ -1158,16 +1198,24 @@ int fdtd_set_kernel_mode(fdtd_handle h, int mode, int param) {
     const int wh = param < 256 ? param : (param & 255), we = param < 256 ? param : (param >> 8);
     if (mode == 0 && param != 0 && !(param < 65536 && width_ok(wh) && width_ok(we)))
         return fail(FDTD_EINVAL, "mode 0: the wavefront k-widths must be 16, 32 or 64 (0 keeps them)");
-    if (mode == 1 && (param < 0 || param > fdtdk::ROW_MAX_CHUNK))
-        return fail(FDTD_EINVAL, "mode 1: planes_per_chunk must be in [0, 128]");
-    if (mode == 1)
+    const int m1_ic = param & 0xfff, m1_cw = (param >> 12) & 0xf, m1_jw = (param >> 16) & 0xf;
+    if (mode == 1 && (param < 0 || param >= (1 << 20) || m1_cw > fdtdk::FZ_MAX_CW))
+        return fail(FDTD_EINVAL, "mode 1: planes_per_chunk + 4096 * cells_per_lane + 65536 * rows, "
+                                 "planes_per_chunk in [0, 4095], cells_per_lane in [0, 4], rows in [0, 15]");
+    if (mode == 1) {
+        const int keep[3] = {h->ichunk, h->fz_cw, h->fz_jw};
+        if (param) { h->ichunk = m1_ic; h->fz_cw = m1_cw; h->fz_jw = m1_jw; }
+        if (!fused_ok(h)) {
+            h->ichunk = keep[0]; h->fz_cw = keep[1]; h->fz_jw = keep[2];
+            return fail(FDTD_EINVAL, "mode 1: the fused step kernel does not take this grid (it needs cell-invariant H "
+                                     "coefficients, arrays below 4 GiB and a row of at most 1024 cells in 16 wavefronts)");
+        }
         if (int rc = ensure_second(h)) return rc;
+    }
     h->mode = mode;
     if (param && mode == 0) {
         h->kwid_h = wh;
         h->kwid_e = we;
-    } else if (param) {
-        h->ichunk = param;
     }
     return FDTD_OK;
 }

@@ -228,9 +228,7 @@ __device__ __forceinline__ Axes<R> global_axes(const StepArgs<T, R>& a) {
 
 // Small read-only tables the compute phase reads: axis tables, the
 // cell-invariant H coefficients, source footprints and values, TLS
-// coefficients.  The fused kernel points them at LDS copies so that the
-// compute phase issues no vector-memory loads (the in-order load counter
-// would otherwise make it wait for every prefetch in flight).
+// coefficients.
 template <typename R>
 struct Tabs {
     Axes<R> ax;
@@ -238,48 +236,31 @@ struct Tabs {
     const SourceBox* src;
     const double* srcv;
     const TLSCoef* tls;
-    // When set, the psi and gain arrays are reached through `far`, an LDS
-    // table of byte offsets from `far_base` read at each use (psi 0..11,
-    // psi_H write 12..17, gain Nu Ng Pz Pp 18..21, region ids 22), so that
-    // their addresses do not occupy scalar registers for the whole kernel.
-    bool use_far;
-    char* far_base;
-    const long* far;
 };
 template <typename T, typename R>
 __device__ __forceinline__ Tabs<R> global_tabs(const StepArgs<T, R>& a) {
-    return Tabs<R>{global_axes(a), a.lut + 6 * a.nmat, a.src, a.srcv, a.tls, false, nullptr, nullptr};
-}
-constexpr int FAR_SLOTS = 23;
-// Offsets from a kernel-argument pointer keep the global address space
-// known to the compiler (global, not flat, loads and stores).
-// The slot index goes through a register-only asm (no instruction) so that
-// the compiler cannot hoist the table read out of the step loop.
-template <typename P, typename R>
-__device__ __forceinline__ P far_ptr(const Tabs<R>& tb, int q) {
-    int slot = q;
-    asm volatile("" : "+s"(slot));
-    return reinterpret_cast<P>(tb.far_base + tb.far[slot]);
+    return Tabs<R>{global_axes(a), a.lut + 6 * a.nmat, a.src, a.srcv, a.tls};
 }
 // psi array q as read (0..11; psi_E is updated in place, so also its write array)
 template <typename T, typename R>
 __device__ __forceinline__ T* psi_rd(const StepArgs<T, R>& a, const Tabs<R>& tb, int q) {
-    return tb.use_far ? far_ptr<T*>(tb, q) : a.psi[q];
+    (void)tb;
+    return a.psi[q];
 }
 // psi_H write array (q = 6..11)
 template <typename T, typename R>
 __device__ __forceinline__ T* psi_wr(const StepArgs<T, R>& a, const Tabs<R>& tb, int q) {
-    return tb.use_far ? far_ptr<T*>(tb, q + 6) : a.psiw[q];
+    (void)tb;
+    return a.psiw[q];
 }
 // gain state array: 0 Nu, 1 Ng, 2 Pz, 3 Pz_prev
 template <typename T, typename R>
 __device__ __forceinline__ double* gain_arr(const StepArgs<T, R>& a, const Tabs<R>& tb, int q) {
-    if (tb.use_far) return far_ptr<double*>(tb, 18 + q);
+    (void)tb;
     return q == 0 ? a.gNu : q == 1 ? a.gNg : q == 2 ? a.gPz : a.gPp;
 }
 
-constexpr int STAGE_MAX_TLS = 16;  // TLS regions staged in LDS
-constexpr int STAGE_MAX_SRC = 16;  // sources staged in LDS by the row kernel
+constexpr int STAGE_MAX_TLS = 16;  // TLS regions of one simulation
 
 // Arguments kept in a device-side table (batches) store every pointer as its
 // byte offset from an anchor address; the kernel adds the anchor (a kernel
@@ -353,78 +334,6 @@ struct HIn {
     R idx, idy, idz;
     R kx[4], ky[4], kz[4];
 };
-
-// Axis-table values and H coefficients of a cell, read from `tb` (for callers
-// that hold the tables in LDS; h_gather reads them with the other inputs).
-// With cell-invariant H coefficients cH comes from tb.hco, else v.cH must
-// already hold the cell's row.
-template <typename T, typename R>
-__device__ __forceinline__ void h_tabs(const StepArgs<T, R>& a, const Tabs<R>& tb, int i, int j, int k, HIn<R>& v) {
-    const Axes<R>& ax = tb.ax;
-    v.idx = ax.X(0, i);
-    v.idy = ax.Y(0, j);
-    v.idz = ax.Z(0, k);
-    const bool c = a.cpml != 0;
-    const bool px = c && in_pml(i, a.nx, a.np), py = c && in_pml(j, a.ny, a.np), pz = c && in_pml(k, a.nz, a.np);
-    for (int q = 0; q < 4; ++q) v.kx[q] = v.ky[q] = v.kz[q] = 0;
-    if (px) { v.kx[0] = ax.X(1, i); v.kx[1] = ax.X(6, i); v.kx[2] = ax.X(4, i); v.kx[3] = ax.X(5, i); }
-    if (py) { v.ky[0] = ax.Y(1, j); v.ky[1] = ax.Y(6, j); v.ky[2] = ax.Y(4, j); v.ky[3] = ax.Y(5, j); }
-    if (pz) { v.kz[0] = ax.Z(1, k); v.kz[1] = ax.Z(6, k); v.kz[2] = ax.Z(4, k); v.kz[3] = ax.Z(5, k); }
-    if (a.h_uniform) {
-        const int nm = tb.hco == a.lut + 6 * a.nmat ? a.nmat : 1;
-        for (int q = 0; q < 6; ++q) v.cH[q] = tb.hco[q * nm];
-    }
-}
-
-// Per-cell H inputs other than the fields: H coefficients (when they vary
-// per cell) and psi_H on the slabs.
-template <typename T, typename R>
-__device__ __forceinline__ void h_load_rest(const StepArgs<T, R>& a, int i, int j, int k, long id, HIn<R>& v) {
-    if (!a.h_uniform) {
-        const int m = mat_of<R>(a.matid, a.mid16, id);
-        for (int q = 0; q < 6; ++q) v.cH[q] = a.lut[(6 + q) * a.nmat + m];
-    }
-    if (a.cpml) {
-        const int np = a.np, nx = a.nx, ny = a.ny, nz = a.nz;
-        const bool px = in_pml(i, nx, np), py = in_pml(j, ny, np), pz = in_pml(k, nz, np);
-        if (py) {
-            const long q = psi_y(i, j, k, nx, ny, nz, np);
-            v.p[0] = ld<T, R>(a.psi[6], q, a.sc[12]);
-            v.p[5] = ld<T, R>(a.psi[11], q, a.sc[17]);
-        }
-        if (pz) {
-            const long q = psi_z(i, j, k, nx, ny, nz, np);
-            v.p[1] = ld<T, R>(a.psi[7], q, a.sc[13]);
-            v.p[2] = ld<T, R>(a.psi[8], q, a.sc[14]);
-        }
-        if (px) {
-            const long q = psi_x(i, j, k, nx, ny, nz, np);
-            v.p[3] = ld<T, R>(a.psi[9], q, a.sc[15]);
-            v.p[4] = ld<T, R>(a.psi[10], q, a.sc[16]);
-        }
-    }
-}
-
-template <typename T, typename R>
-__device__ __forceinline__ HIn<R> h_load(const StepArgs<T, R>& a, const Tabs<R>& tb, int i, int j, int k) {
-    HIn<R> v;
-    const long id = ((long)i * a.ny + j) * a.nz + k;
-    v.hx = ld<T, R>(a.Hx, id, a.sc[3]);
-    v.hy = ld<T, R>(a.Hy, id, a.sc[4]);
-    v.hz = ld<T, R>(a.Hz, id, a.sc[5]);
-    v.upd = (i < a.nx - 1) & (j < a.ny - 1) & (k < a.nz - 1);
-    for (int q = 0; q < 6; ++q) v.p[q] = 0;
-    for (int q = 0; q < 6; ++q) v.cH[q] = 0;
-    v.ex = v.ex_j = v.ex_k = v.ey = v.ey_i = v.ey_k = v.ez = v.ez_i = v.ez_j = 0;
-    if (!v.upd) return v;
-    const long sI = a.sI, sJ = a.sJ;
-    v.ex = ld<T, R>(a.Ex, id, a.sc[0]); v.ex_j = ld<T, R>(a.Ex, id + sJ, a.sc[0]); v.ex_k = ld<T, R>(a.Ex, id + 1, a.sc[0]);
-    v.ey = ld<T, R>(a.Ey, id, a.sc[1]); v.ey_i = ld<T, R>(a.Ey, id + sI, a.sc[1]); v.ey_k = ld<T, R>(a.Ey, id + 1, a.sc[1]);
-    v.ez = ld<T, R>(a.Ez, id, a.sc[2]); v.ez_i = ld<T, R>(a.Ez, id + sI, a.sc[2]); v.ez_j = ld<T, R>(a.Ez, id + sJ, a.sc[2]);
-    h_load_rest<T, R>(a, i, j, k, id, v);
-    h_tabs<T, R>(a, tb, i, j, k, v);
-    return v;
-}
 
 // Inputs of the H update of cell (i, j, k), one the reference updates
 // (i < NxT-1, j < NyT-1, k < NzT-1), for a wavefront that covers KW
@@ -548,11 +457,6 @@ __device__ __forceinline__ H3<R> h_compute(const StepArgs<T, R>& a, const Tabs<R
     }
     return H3<R>{(R)stored_value<T, R>(hx, a.sc[3], a.sc[21]), (R)stored_value<T, R>(hy, a.sc[4], a.sc[22]),
                  (R)stored_value<T, R>(hz, a.sc[5], a.sc[23])};
-}
-
-template <typename T, typename R>
-__device__ __forceinline__ H3<R> h_new(const StepArgs<T, R>& a, const Tabs<R>& tb, int i, int j, int k, bool own) {
-    return h_compute<T, R>(a, tb, i, j, k, h_load<T, R>(a, tb, i, j, k), own);
 }
 
 // The FP16 scales copied into registers at kernel start, before any store:
@@ -689,18 +593,6 @@ __device__ __forceinline__ void e_sources(const Tabs<R>& tb, int nsrc, int i, in
         jz += b.comp == 2 ? add : (R)0;
     }
 }
-// Axis-table values of an E cell (px, py, pz: the slabs whose rows are read).
-template <typename R>
-__device__ __forceinline__ void e_axes(const Axes<R>& ax, int i, int j, int k, bool px, bool py, bool pz, EIn<R>& v) {
-    v.idx = ax.X(0, i);
-    v.idy = ax.Y(0, j);
-    v.idz = ax.Z(0, k);
-    for (int q = 0; q < 4; ++q) v.kx[q] = v.ky[q] = v.kz[q] = 0;
-    if (px) { v.kx[0] = ax.X(1, i); v.kx[1] = ax.X(6, i); v.kx[2] = ax.X(2, i); v.kx[3] = ax.X(3, i); }
-    if (py) { v.ky[0] = ax.Y(1, j); v.ky[1] = ax.Y(6, j); v.ky[2] = ax.Y(2, j); v.ky[3] = ax.Y(3, j); }
-    if (pz) { v.kz[0] = ax.Z(1, k); v.kz[1] = ax.Z(6, k); v.kz[2] = ax.Z(2, k); v.kz[3] = ax.Z(3, k); }
-}
-
 // Inputs of the E update of cell (i, j, k) for a wavefront that covers KW
 // consecutive k of 64/KW consecutive rows of plane i: i, kw and jw (the k
 // and j of the wavefront's first lane) are uniform across it.  Every input is requested
@@ -940,325 +832,6 @@ __global__ void __launch_bounds__(256) k_update_E(StepArgs<T, R> a) {
     e_cell<T, R, KW, true, true, IX>(a, tb, i, j, k, t.kw, t.jw);
 }
 
-// Row-marching step kernel: one launch advances H and E of the whole grid,
-// with no barrier and no data exchange between wavefronts.  A wavefront owns
-// one j-row over a chunk of x-planes [i0, i1) and, per plane, walks the row's
-// NCH 64-cell k-segments.  Per (plane, segment) item it
-//   - computes H^{n+1/2} of its own cells and stores it,
-//   - recomputes Hx, Hz of the row below (j-1), which its E update needs,
-//     without storing them (the owner of that row stores the same values),
-//   - takes H(k-1) from the neighbouring lane (lane 0: the previous segment's
-//     lane 63), H(i-1) from registers carried from the previous plane,
-//   - updates E^{n+1} with J, CPML and gain.
-// E^n(i+1) loaded for the H update is carried to the next plane as E^n(i).
-// Latency is hidden by occupancy (many independent wavefronts per CU), not by
-// prefetching into registers: with the in-order load counter, a prefetch
-// that is issued under a branch forces the compiler to wait for it as well.
-// The compute phase reads only registers and LDS.  Reads come from buffer
-// A (E^n, H^{n-1/2}, psi_H), H/E/psi_H writes go to buffer B; psi_E and the
-// gain state are touched only by their owner and stay in place.
-constexpr int ROW_WAVES = 8;           // wavefronts (rows) per work-group
-constexpr int ROW_MAX_CHUNK = 128;     // planes per chunk
-constexpr int ROW_MAX_NZ = 1024;       // segments per row <= 16
-
-// Loads by 32-bit element offset (the byte offset fits 32 bits: checked on
-// the host), so that an address is one VGPR added to the array base.
-template <typename T, typename R>
-__device__ __forceinline__ R ldu(const T* p, unsigned i, R scale) {
-    return ld<T, R>(reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + i * (unsigned)sizeof(T)), 0, scale);
-}
-__device__ __forceinline__ int matu(const void* m, int mid16, unsigned i) {
-    const char* b = reinterpret_cast<const char*>(m);
-    return mid16 ? (int)*reinterpret_cast<const uint16_t*>(b + 2u * i) : (int)*reinterpret_cast<const uint8_t*>(b + i);
-}
-
-template <typename R>
-struct RowIn {
-    R ex, hx, hy, hz, eyn, ezn, exk, eyk, exj, ezj;  // own row: E, H at i; Ey, Ez at i+1; Ex, Ey at k+1; Ex, Ez at j+1
-    R gex, gez, ghx, ghz, geyn, geyk;                // row j-1: Ex, Ez, Hx, Hz at i; Ey at i+1, at k+1
-    R ph[6];  // psi_H of the own cell (Hx_y, Hx_z, Hy_z, Hy_x, Hz_x, Hz_y)
-    R qh[4];  // psi_H of the cell at j-1 (Hx_y, Hx_z, Hz_x, Hz_y)
-    R pe[6];  // psi_E of the own cell
-    R cE[6];
-    int reg;  // TLS region of the cell (-1: none)
-    double Nu, Ng, Pz, Pp;
-};
-
-// Material id of a row-kernel item, loaded one item ahead.  Every lane reads
-// a valid cell (its value is discarded where the cell is not an E cell).
-template <typename T, typename R>
-__device__ __forceinline__ int row_mat(const StepArgs<T, R>& a, int i, int j, int k) {
-    const unsigned id = ((unsigned)min(i, a.nx - 1) * a.ny + j) * a.nz + (k < a.nz ? k : 0);
-    return matu(a.matid, a.mid16, id);
-}
-
-// Gathers the inputs of one item in one round trip: the coefficient gathers
-// use the material id loaded one item earlier (`m`), then the id of the next
-// item is issued (`mnext`), then the field, psi and gain loads.  The field
-// loads are unconditional (a lane that does not need a value reads its own
-// cell instead and discards it).
-template <typename T, typename R>
-__device__ __forceinline__ void row_load(const StepArgs<T, R>& a, const Tabs<R>& tb, int i, int j, int k, int m, int in,
-                                         int kn, int& mnext, RowIn<R>& v) {
-    const int nx = a.nx, ny = a.ny, nz = a.nz, np = a.np;
-    const unsigned sI = (unsigned)a.sI, sJ = (unsigned)a.sJ;
-    const bool kin = k < nz;
-    const unsigned id = ((unsigned)i * ny + j) * nz + (kin ? k : 0);  // a valid cell for every lane
-    const bool hup = kin & (i < nx - 1) & (j < ny - 1) & (k < nz - 1);
-    const bool halo = kin & (j > 0);
-    const bool gup = halo & (i < nx - 1) & (k < nz - 1);
-    const bool nxt = kin & (i + 1 < nx);
-    const bool ein = kin & (i > 0) & (j > 0) & (k > 0);
-    const int mm = ein ? m : 0;
-    for (int q = 0; q < 6; ++q) v.cE[q] = a.lut[q * a.nmat + mm];
-    mnext = row_mat(a, in, j, kn);
-    const R ex = ldu<T, R>(a.Ex, id, a.sc[0]);
-    const R hx = ldu<T, R>(a.Hx, id, a.sc[3]);
-    const R hy = ldu<T, R>(a.Hy, id, a.sc[4]);
-    const R hz = ldu<T, R>(a.Hz, id, a.sc[5]);
-    const R eyn = ldu<T, R>(a.Ey, nxt ? id + sI : id, a.sc[1]);
-    const R ezn = ldu<T, R>(a.Ez, nxt ? id + sI : id, a.sc[2]);
-    const R exk = ldu<T, R>(a.Ex, hup ? id + 1 : id, a.sc[0]);
-    const R eyk = ldu<T, R>(a.Ey, hup ? id + 1 : id, a.sc[1]);
-    const R exj = ldu<T, R>(a.Ex, hup ? id + sJ : id, a.sc[0]);
-    const R ezj = ldu<T, R>(a.Ez, hup ? id + sJ : id, a.sc[2]);
-    const R ghx = ldu<T, R>(a.Hx, halo ? id - sJ : id, a.sc[3]);
-    const R ghz = ldu<T, R>(a.Hz, halo ? id - sJ : id, a.sc[5]);
-    const R geyn = ldu<T, R>(a.Ey, (halo & nxt) ? id - sJ + sI : id, a.sc[1]);
-    const R gex = ldu<T, R>(a.Ex, gup ? id - sJ : id, a.sc[0]);
-    const R gez = ldu<T, R>(a.Ez, gup ? id - sJ : id, a.sc[2]);
-    const R geyk = ldu<T, R>(a.Ey, gup ? id - sJ + 1 : id, a.sc[1]);
-    // Values a lane does not need are left as loaded: the compute reads a
-    // value only where its condition holds (a select here would wait for
-    // each load as it is issued).
-    v.ex = ex; v.hx = hx; v.hy = hy; v.hz = hz; v.eyn = eyn; v.ezn = ezn;
-    v.exk = exk; v.eyk = eyk; v.exj = exj; v.ezj = ezj;
-    v.ghx = ghx; v.ghz = ghz; v.geyn = geyn; v.gex = gex; v.gez = gez; v.geyk = geyk;
-    for (int q = 0; q < 6; ++q) v.ph[q] = v.pe[q] = 0;
-    for (int q = 0; q < 4; ++q) v.qh[q] = 0;
-    if (a.cpml) {
-        const bool px = in_pml(i, nx, np), py = in_pml(j, ny, np), pz = in_pml(k, nz, np);
-        const bool pyg = (j > 0) && in_pml(j - 1, ny, np);
-        if (hup) {
-            if (py) {
-                const unsigned q = (unsigned)psi_y(i, j, k, nx, ny, nz, np);
-                v.ph[0] = ldu<T, R>(psi_rd(a, tb, 6), q, a.sc[12]);
-                v.ph[5] = ldu<T, R>(psi_rd(a, tb, 11), q, a.sc[17]);
-            }
-            if (pz) {
-                const unsigned q = (unsigned)psi_z(i, j, k, nx, ny, nz, np);
-                v.ph[1] = ldu<T, R>(psi_rd(a, tb, 7), q, a.sc[13]);
-                v.ph[2] = ldu<T, R>(psi_rd(a, tb, 8), q, a.sc[14]);
-            }
-            if (px) {
-                const unsigned q = (unsigned)psi_x(i, j, k, nx, ny, nz, np);
-                v.ph[3] = ldu<T, R>(psi_rd(a, tb, 9), q, a.sc[15]);
-                v.ph[4] = ldu<T, R>(psi_rd(a, tb, 10), q, a.sc[16]);
-            }
-        }
-        if (gup) {
-            if (pyg) {
-                const unsigned q = (unsigned)psi_y(i, j - 1, k, nx, ny, nz, np);
-                v.qh[0] = ldu<T, R>(psi_rd(a, tb, 6), q, a.sc[12]);
-                v.qh[3] = ldu<T, R>(psi_rd(a, tb, 11), q, a.sc[17]);
-            }
-            if (pz) v.qh[1] = ldu<T, R>(psi_rd(a, tb, 7), (unsigned)psi_z(i, j - 1, k, nx, ny, nz, np), a.sc[13]);
-            if (px) v.qh[2] = ldu<T, R>(psi_rd(a, tb, 10), (unsigned)psi_x(i, j - 1, k, nx, ny, nz, np), a.sc[16]);
-        }
-        if (ein) {
-            if (py) {
-                const unsigned q = (unsigned)psi_y(i, j, k, nx, ny, nz, np);
-                v.pe[0] = ldu<T, R>(psi_rd(a, tb, 0), q, a.sc[6]);
-                v.pe[5] = ldu<T, R>(psi_rd(a, tb, 5), q, a.sc[11]);
-            }
-            if (pz) {
-                const unsigned q = (unsigned)psi_z(i, j, k, nx, ny, nz, np);
-                v.pe[1] = ldu<T, R>(psi_rd(a, tb, 1), q, a.sc[7]);
-                v.pe[2] = ldu<T, R>(psi_rd(a, tb, 2), q, a.sc[8]);
-            }
-            if (px) {
-                const unsigned q = (unsigned)psi_x(i, j, k, nx, ny, nz, np);
-                v.pe[3] = ldu<T, R>(psi_rd(a, tb, 3), q, a.sc[9]);
-                v.pe[4] = ldu<T, R>(psi_rd(a, tb, 4), q, a.sc[10]);
-            }
-        }
-    }
-    v.reg = -1;
-    v.Nu = v.Ng = v.Pz = v.Pp = 0;
-    if (a.gain && kin) {
-        const int gi = i - a.g0x, gj = j - a.g0y, gk = k - a.g0z;
-        if ((unsigned)gi < (unsigned)a.gnx && (unsigned)gj < (unsigned)a.gny && (unsigned)gk < (unsigned)a.gnz) {
-            const unsigned g = ((unsigned)gi * a.gny + gj) * a.gnz + gk;
-            v.reg = far_ptr<const int8_t*>(tb, 22)[g];
-            v.Nu = gain_arr(a, tb, 0)[g]; v.Ng = gain_arr(a, tb, 1)[g];
-            v.Pz = gain_arr(a, tb, 2)[g]; v.Pp = gain_arr(a, tb, 3)[g];
-        }
-    }
-}
-
-// Dynamic LDS of the row kernel: the z axis tables [7][nz] and, per
-// wavefront, the state carried from plane to plane [5][nseg][64] in the
-// storage format (exact: every carried value is a stored value).
-template <typename T, typename R>
-__host__ __device__ constexpr unsigned long row_lds_bytes(int nz, int nseg) {
-    return (unsigned long)7 * nz * sizeof(R) + (unsigned long)ROW_WAVES * 5 * nseg * 64 * sizeof(T);
-}
-
-template <typename T, typename R>
-__global__ void __launch_bounds__(64 * ROW_WAVES) k_step_rows(StepArgs<T, R> a, int ichunk, int njt, int nwg, int nseg) {
-    extern __shared__ __align__(16) unsigned char row_lds[];
-    __shared__ R ty[7][ROW_WAVES + 1];
-    __shared__ R tx[7][ROW_MAX_CHUNK + 1];
-    __shared__ R th[6];
-    __shared__ TLSCoef tl[STAGE_MAX_TLS];
-    __shared__ SourceBox tsb[STAGE_MAX_SRC];
-    __shared__ double tsv[STAGE_MAX_SRC];
-    __shared__ float tsc[54];
-    __shared__ long tfar[FAR_SLOTS];
-    R* tz = reinterpret_cast<R*>(row_lds);
-    const int b = blockIdx.x;
-    const int bb = (nwg % 8 == 0) ? (b % 8) * (nwg / 8) + b / 8 : b;
-    const int jt = bb % njt;
-    const int ic = bb / njt;
-    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    const int j0 = jt * ROW_WAVES;
-    const int j = j0 + w;
-    const int i0 = ic * ichunk;
-    const int i1 = min(i0 + ichunk, a.nx);
-    for (int t = threadIdx.x; t < 7 * a.nz; t += blockDim.x) tz[t] = a.cpz[t];
-    for (int t = threadIdx.x; t < 7 * (ROW_WAVES + 1); t += blockDim.x) {
-        const int r = t / (ROW_WAVES + 1), n = j0 - 1 + t % (ROW_WAVES + 1);
-        ty[r][t % (ROW_WAVES + 1)] = (n >= 0 && n < a.ny) ? a.cpy[r * a.ny + n] : (R)0;
-    }
-    for (int t = threadIdx.x; t < 7 * (ROW_MAX_CHUNK + 1); t += blockDim.x) {
-        const int r = t / (ROW_MAX_CHUNK + 1), n = i0 - 1 + t % (ROW_MAX_CHUNK + 1);
-        tx[r][t % (ROW_MAX_CHUNK + 1)] = (n >= 0 && n < a.nx) ? a.cpx[r * a.nx + n] : (R)0;
-    }
-    if (threadIdx.x < 6) th[threadIdx.x] = a.lut[(6 + threadIdx.x) * a.nmat];
-    if (a.gain && threadIdx.x < a.ntls) tl[threadIdx.x] = a.tls[threadIdx.x];
-    if (threadIdx.x < a.nsrc) {
-        tsb[threadIdx.x] = a.src[threadIdx.x];
-        tsv[threadIdx.x] = a.srcv[threadIdx.x];
-    }
-    if (sizeof(T) == 2 && threadIdx.x < 54) tsc[threadIdx.x] = a.sc[threadIdx.x];
-    if (threadIdx.x < FAR_SLOTS) {
-        const int q = threadIdx.x;
-        const char* p = q < 12 ? (const char*)a.psi[q] : q < 18 ? (const char*)a.psiw[q - 6]
-                      : q == 18 ? (const char*)a.gNu : q == 19 ? (const char*)a.gNg : q == 20 ? (const char*)a.gPz
-                      : q == 21 ? (const char*)a.gPp : (const char*)a.greg;
-        tfar[q] = p ? (long)(p - (const char*)a.lut) : 0;
-    }
-    __syncthreads();
-    if (j >= a.ny) return;  // no barrier follows
-    StepArgs<T, R> s = a;
-    if (sizeof(T) == 2) s.sc = tsc;
-    const Tabs<R> tb{Axes<R>{&tx[0][0], ROW_MAX_CHUNK + 1, i0 - 1, &ty[0][0], ROW_WAVES + 1, j0 - 1, tz, s.nz, 0},
-                     &th[0], &tsb[0], &tsv[0], &tl[0], true, (char*)a.lut, &tfar[0]};
-    // this wavefront's carried state: E^n(i) of the own row (Ey, Ez) and of
-    // row j-1 (Ey), H^{n+1/2}(i-1) of the own row (Hy, Hz)
-    T* cw = reinterpret_cast<T*>(row_lds + 7 * s.nz * sizeof(R)) + (long)w * 5 * nseg * 64 + lane;
-    auto carry = [&](int q, int c) -> T& { return cw[(q * nseg + c) * 64]; };
-    auto put = [&](int q, int c, R v, int f) {
-        if constexpr (sizeof(T) == 2) carry(q, c) = (T)__half_as_ushort(__float2half_rn((float)(v * s.sc[18 + f])));
-        else carry(q, c) = (T)v;
-    };
-    auto get = [&](int q, int c, int f) -> R { return ld<T, R>(&carry(q, c), 0, s.sc[f]); };
-#pragma unroll 1
-    for (int c = 0; c < nseg; ++c) {
-        const int k = c * 64 + lane;
-        R ey = 0, ez = 0, eyg = 0, hyp = 0, hzp = 0;
-        if (k < s.nz) {
-            const long id = ((long)i0 * s.ny + j) * s.nz + k;
-            ey = ld<T, R>(s.Ey, id, s.sc[1]);
-            ez = ld<T, R>(s.Ez, id, s.sc[2]);
-            if (j > 0) eyg = ld<T, R>(s.Ey, id - s.sJ, s.sc[1]);
-            if (i0 > 0) {
-                const H3<R> h = h_new<T, R>(s, tb, i0 - 1, j, k, false);
-                hyp = h.y;
-                hzp = h.z;
-            }
-        }
-        put(0, c, ey, 1);
-        put(1, c, ez, 2);
-        put(2, c, eyg, 1);
-        put(3, c, hyp, 4);
-        put(4, c, hzp, 5);
-    }
-    int mcur = row_mat(s, i0, j, lane);
-    asm volatile("" : "+v"(mcur));  // arrived before the loop (see below)
-    for (int i = i0; i < i1; ++i) {
-        R xprev = 0, yprev = 0;  // H of the previous segment's last lane
-#pragma unroll 1
-        for (int c = 0; c < nseg; ++c) {
-            const int k = c * 64 + lane;
-            const bool last = c + 1 == nseg;
-            RowIn<R> cur;
-            int mnext;
-            row_load<T, R>(s, tb, i, j, k, mcur, last ? i + 1 : i, last ? lane : k + 64, mnext, cur);
-            // The id was issued before this item's loads, so it has arrived by
-            // the time they have.  Taking it here (a register-only asm, no
-            // instruction) ends its load's tracking, so that the gathers of the
-            // next item do not wait for this item's stores.
-            asm volatile("" : "+v"(mnext));
-            mcur = mnext;
-            const R eyc = get(0, c, 1), ezc = get(1, c, 2), eyg = get(2, c, 1);
-            const R hyp = get(3, c, 4), hzp = get(4, c, 5);
-            const bool kin = k < s.nz;
-            const bool hup = kin & (i < s.nx - 1) & (j < s.ny - 1) & (k < s.nz - 1);
-            // H of the own cell
-            HIn<R> hv;
-            hv.ex = cur.ex; hv.ex_j = cur.exj; hv.ex_k = cur.exk;
-            hv.ey = eyc; hv.ey_i = cur.eyn; hv.ey_k = cur.eyk;
-            hv.ez = ezc; hv.ez_i = cur.ezn; hv.ez_j = cur.ezj;
-            hv.hx = cur.hx; hv.hy = cur.hy; hv.hz = cur.hz;
-            for (int q = 0; q < 6; ++q) { hv.p[q] = cur.ph[q]; hv.cH[q] = 0; }
-            hv.upd = hup;
-            h_tabs<T, R>(s, tb, i, j, k, hv);
-            const H3<R> h = h_compute<T, R>(s, tb, i, j, k, hv, kin);
-            // Hx, Hz of the cell at j-1
-            R gx = 0, gz = 0;
-            if (j > 0) {
-                HIn<R> gv;
-                gv.ex = cur.gex; gv.ex_j = cur.ex; gv.ex_k = 0;
-                gv.ey = eyg; gv.ey_i = cur.geyn; gv.ey_k = cur.geyk;
-                gv.ez = cur.gez; gv.ez_i = 0; gv.ez_j = ezc;
-                gv.hx = cur.ghx; gv.hy = 0; gv.hz = cur.ghz;
-                gv.p[0] = cur.qh[0]; gv.p[1] = cur.qh[1]; gv.p[2] = 0;
-                gv.p[3] = 0; gv.p[4] = cur.qh[2]; gv.p[5] = cur.qh[3];
-                for (int q = 0; q < 6; ++q) gv.cH[q] = 0;
-                gv.upd = kin & (i < s.nx - 1) & (k < s.nz - 1);
-                h_tabs<T, R>(s, tb, i, j - 1, k, gv);
-                const H3<R> g = h_compute<T, R>(s, tb, i, j - 1, k, gv, false);
-                gx = g.x;
-                gz = g.z;
-            }
-            // H(k-1): the neighbouring lane, lane 0 from the previous segment
-            R hxk = __shfl_up(h.x, 1), hyk = __shfl_up(h.y, 1);
-            if (lane == 0) { hxk = xprev; hyk = yprev; }
-            xprev = __shfl(h.x, 63);
-            yprev = __shfl(h.y, 63);
-            if (kin) {
-                EIn<R> ev;
-                ev.ex0 = cur.ex; ev.ey0 = eyc; ev.ez0 = ezc;
-                for (int q = 0; q < 6; ++q) { ev.p[q] = cur.pe[q]; ev.cE[q] = cur.cE[q]; }
-                ev.reg = cur.reg;
-                ev.g = ev.reg >= 0 ? ((long)(i - s.g0x) * s.gny + (j - s.g0y)) * s.gnz + (k - s.g0z) : 0;
-                ev.Nu = cur.Nu; ev.Ng = cur.Ng; ev.Pz = cur.Pz; ev.Pp = cur.Pp;
-                const bool cp = s.cpml != 0;
-                e_axes(tb.ax, i, j, k, cp && in_pml(i, s.nx, s.np), cp && in_pml(j, s.ny, s.np),
-                       cp && in_pml(k, s.nz, s.np), ev);
-                e_sources(tb, s.nsrc, i, j, k, ev.jx, ev.jy, ev.jz);
-                if (ev.reg >= 0) ev.tc = tb.tls[ev.reg];
-                e_compute<T, R>(s, tb, i, j, k, ev, h.x, h.y, h.z, gx, gz, hxk, hyk, hyp, hzp);
-            }
-            put(0, c, cur.eyn, 1);
-            put(1, c, cur.ezn, 2);
-            put(2, c, cur.geyn, 1);
-            put(3, c, h.y, 4);
-            put(4, c, h.z, 5);
-        }
-    }
-}
 
 struct ProbeSlot {
     long id;

@@ -194,13 +194,17 @@ int fdtd_get_fp16_scales(fdtd_handle h, double* scales18);
  * when every array is below 4 GiB, would do; plus 1048576: the E kernel
  * walks its tiles in the H kernel's order instead of backwards, which by
  * default lets each kernel start on the tiles the previous one finished);
- * mode 1 = one fused step kernel in which each wavefront owns one j-row and
- * marches along x over `param` planes (1..128; 0 keeps the current value),
- * double-buffering E, H and psi_H.  Grids the fused kernel does not take
- * (per-cell H coefficients, more than 16 sources or TLS regions, NzT > 1024,
- * field arrays of 4 GiB or more) run mode 0.  Results are identical; only
- * the instruction count and memory traffic differ.  No reference
- * counterpart. */
+ * mode 1 = one fused step kernel: a work-group owns a block of rows over
+ * the whole k extent (plus one halo row whose H it recomputes) and marches
+ * along x, reading E^n and H^{n-1/2} once and writing E^{n+1} and
+ * H^{n+1/2} once (the second field and psi_H buffer is allocated on the
+ * first switch); `param` = planes per work-group (0..4095) + 4096 * cells
+ * per lane per row chunk (1..4) + 65536 * rows per block (1..15), each 0 for
+ * the library's choice.  Returns FDTD_EINVAL (and keeps the schedule) for a
+ * grid the fused kernel does not take: per-cell H coefficients, an array of
+ * 4 GiB or more, or a row that does not fit 16 wavefronts.  Results are
+ * identical in every schedule; only instruction count and memory traffic
+ * differ.  No reference counterpart. */
 int fdtd_set_kernel_mode(fdtd_handle h, int mode, int param);
 /* Stability monitor cadence (STABILITY_MONITOR_INTERVAL, default 10; 0 = off). */
 int fdtd_set_monitor(fdtd_handle h, int interval);

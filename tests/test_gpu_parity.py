@@ -142,7 +142,8 @@ def test_step_kernels_agree_bitwise(pkg, precision):
     setup = pkg.Setup(sp)
     runs = {}
     for mode in [(0, 16), (0, 32), (0, 64), (0, 0x20000 + 32), (0, 0x20000 + 16 + 256 * 64), (1, 16), (1, 5),
-                 (0, 0x100000 + 32), (0, 0x100000 + 0x20000 + 64)]:
+                 (0, 0x100000 + 32), (0, 0x100000 + 0x20000 + 64), (1, 4096 * 1 + 65536 * 2 + 3),
+                 (1, 4096 * 3 + 65536 * 1 + 9), (1, 4096 * 4 + 7)]:
         _say(f"[case] {sp.name} {precision} mode={mode}")
         sc = pkg.Scene(setup, precision)
         sc.set_kernel_mode(*mode)
@@ -166,7 +167,11 @@ def test_step_kernels_agree_bitwise_ragged(pkg, precision):
                                   n_cuboids=3, seed=9)
     setup = pkg.Setup(sp)
     runs = {}
-    for mode in [(0, 32), (0, 16), (0, 64), (0, 0x100000 + 32), (1, 3), (1, 20)]:
+    # (1, planes + 4096 * cells_per_lane + 65536 * rows): the fused kernel
+    # with 1..4 cells per lane, i.e. 3, 2 or 1 k-chunk wavefronts per row
+    # (dead lanes past NzT in the last chunk), and 1..7 rows per block
+    for mode in [(0, 32), (0, 16), (0, 64), (0, 0x100000 + 32), (1, 3), (1, 20), (1, 4096 * 1 + 65536 * 2 + 4),
+                 (1, 4096 * 2 + 65536 * 7 + 6), (1, 4096 * 3 + 65536 * 1 + 20), (1, 4096 * 4 + 65536 * 3 + 1)]:
         _say(f"[case] ragged {precision} mode={mode}")
         sc = pkg.Scene(setup, precision)
         sc.set_kernel_mode(*mode)
