@@ -176,6 +176,18 @@ struct StepArgs {
     // 1: each XCD walks its range of tiles from the end (the launch starts
     // where the previous one, walking forward, finished)
     int rev;
+    // Tile order: bands of `bj` j-tiles, each band swept plane by plane
+    // (order band, plane, j-tile in band, k-tile).  A tile reads the next
+    // plane's E (or the previous plane's H) that the same band of the next
+    // (previous) plane loads again one band-plane later, while it is still in
+    // the XCD's L2; with whole planes in the sweep (bj = tj) a plane's six
+    // arrays outgrow the L2 once a plane is larger than about 4 MB.
+    // tbp: tiles per band-plane, tband: tiles per band (full bands), tbpl:
+    // tiles per plane of the last, partial band; full_tiles: tiles in full
+    // bands.
+    int bj;
+    unsigned full_tiles;
+    FastDiv tbp, tband, tbpl;
 };
 
 __device__ __forceinline__ bool in_pml(int i, int n, int np) {
@@ -491,10 +503,21 @@ __device__ __forceinline__ Tile tile_of(const StepArgs<T, R>& a) {
     const unsigned per = (nb + 7) / 8;
     unsigned bb = (b % 8) * per + (a.rev ? per - 1 - b / 8 : b / 8);
     if (nb % 8 != 0) bb = a.rev ? nb - 1 - b : b;  // the remap is a permutation only for multiples of 8
-    const unsigned r = fdiv(bb, a.tk);
-    const unsigned kt = bb - r * a.tk.d;
-    const unsigned i = fdiv(r, a.tj);
-    const unsigned jt = r - i * a.tj.d;
+    unsigned band, i, r2;
+    if (bb < a.full_tiles) {
+        band = fdiv(bb, a.tband);
+        const unsigned rem = bb - band * a.tband.d;
+        i = fdiv(rem, a.tbp);
+        r2 = rem - i * a.tbp.d;
+    } else {
+        band = a.full_tiles / a.tband.d;
+        const unsigned rem = bb - a.full_tiles;
+        i = fdiv(rem, a.tbpl);
+        r2 = rem - i * a.tbpl.d;
+    }
+    const unsigned jr = fdiv(r2, a.tk);
+    const unsigned kt = r2 - jr * a.tk.d;
+    const unsigned jt = band * (unsigned)a.bj + jr;
     const int w = (int)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wavefront in the work-group
     const int l = (int)(threadIdx.x & 63);
     Tile t;

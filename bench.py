@@ -1,10 +1,12 @@
 """Benchmark of the per-timestep Yee update on one MI355X.
 
-Workload (value): the graded-mesh gain case, 256x256x256 total cells (12-cell
-CPML, 10 nm cells in a central 400 nm z slab graded at 1.2 per cell to 40 nm,
-20 nm lateral), eps_r 3.5 substrate, 200 nm two-level gain slab, Ex current
-sheet source, probes and a 3-frequency DFT plane, stability monitor every 10
-steps, FP32 storage/arithmetic with FP64 populations.  A "step" is one full
+Workload (value): the ~100M-cell precision geometry, 512x512x384 total cells
+(12-cell CPML, 10 nm cells in a central 400 nm z slab graded at 1.2 per cell
+to 40 nm, 20 nm lateral), eps_r 3.5 substrate, 200 nm two-level gain slab, 64
+seeded dielectric cuboids (eps_r U[1,12], edges 8-40 cells), Ex current sheet
+source, probes and a 3-frequency DFT plane, stability monitor every 10 steps,
+FP32 storage/arithmetic with FP64 populations (--workload c2: the 256^3 gain
+slab without cuboids).  A "step" is one full
 Yee step of the whole grid (H, CPML-H, E with J, CPML-E, gain, probes, DFT,
 monitor).  Inputs are resident in HBM before the timed region.
 
@@ -77,37 +79,81 @@ def copy_calibration(nbytes: int) -> None:
     hip.hipFree(b)
 
 
-def cpu_baseline(spec, steps: int):
+def host_info():
+    """nproc, CPU model and NUMA nodes of the host the baseline runs on."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    nodes = 0
+    try:
+        nodes = sum(1 for d in os.listdir("/sys/devices/system/node") if d.startswith("node") and d[4:].isdigit())
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model, "numa_nodes": nodes}
+
+
+def cpu_baseline(pkg, sample, steps: int, steps_1t: int):
     """The reference's own step (its headers compiled with OpenMP into
-    oracle/_ref/libref.so, kind "reference") timed on this host's cores, or
-    where that library is absent the oracle's C restatement (kind "port")."""
+    oracle/_ref/libref.so, kind "reference") timed on this host's cores, on
+    `sample`, a grid of the workload's structure: all threads, then one
+    thread.  The reference's serial subpixel bake would take hours on a grid
+    with dozens of structures, so the harness takes the MaterialGrids arrays
+    of this library's bake (bitwise the reference's, tests/test_setup_parity.py)
+    and runs the rest of the reference's setup itself.  Where the harness is
+    absent the oracle's C restatement is timed instead (kind "port")."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle  # checker / CPU baseline only
 
-    impl = "ref" if oracle.available("ref") else "oracle"
-    if impl == "oracle" and not oracle.available("oracle"):
-        oracle.build()
-    sim = oracle.CpuSim(spec, impl)
+    n = sample.n_cells()
+    if oracle.available("ref"):
+        setup = pkg.Setup(sample)
+        mats = [setup.array(q) for q in pkg.runtime.MATERIAL_NAMES]
+        sim = oracle.CpuSim(sample, "ref", materials=mats)
+        del mats
+        setup.close()
+        kind, impl = "reference", "ref"
+    else:
+        if not oracle.available("oracle"):
+            oracle.build()
+        sim = oracle.CpuSim(sample, "oracle")
+        kind, impl = "port", "oracle"
+    threads = oracle.threads(impl)
     sim.time_steps(2)
-    secs = sim.time_steps(steps)
-    n = spec.n_cells()
-    rate = n * steps / secs / 1e9
+    rate = n * steps / sim.time_steps(steps) / 1e9
+    rate1 = None
+    if impl == "ref" and steps_1t > 0:
+        oracle.set_threads(1)
+        sim.time_steps(1)
+        rate1 = n * steps_1t / sim.time_steps(steps_1t) / 1e9
+        oracle.set_threads(threads)
     sim.close()
+    nx, ny, nz = sample.total_dims()
     what = ("oracle/_ref/libref.so: the reference's fdtd_update_H/E(_with_gain), CPMLBoundary, TLS and "
             "StabilityMonitor code in main's step order, OpenMP as the reference builds it" if impl == "ref" else
             "oracle/fdtd_oracle.c, OpenMP over i, serial CPML, J zero-fill, monitor every 10")
-    return {"value": rate, "unit": "Gcell-updates/s", "cores": oracle.threads(impl),
-            "kind": "reference" if impl == "ref" else "port",
-            "sample": f"{spec.name}: {steps} FP64 steps of the full grid after 2 warm-up steps ({what})"}
+    out = {"value": rate, "unit": "Gcell-updates/s", "cores": threads, "kind": kind,
+           "sample": f"{sample.name} ({nx}x{ny}x{nz} = {n} cells, the workload's z profile, CPML, gain slab and "
+                     f"cuboids on a smaller x-y extent): {steps} FP64 steps after 2 warm-up steps ({what})"}
+    if rate1 is not None:
+        out["value_1_thread"] = rate1
+        out["steps_1_thread"] = steps_1t
+    out.update(host_info())
+    return out
 
 
 def _kw_param(args) -> int:
     """fdtd_set_kernel_mode's mode-0 parameter: one width, or H + 256 * E,
     plus 131072 for 64-bit element offsets."""
     o64 = 0x20000 if getattr(args, "offsets64", False) else 0
+    band = (getattr(args, "band", 0) or 0) << 21
     if args.kwidth_e and args.kwidth_e != args.kwidth:
-        return (args.kwidth or 32) + 256 * args.kwidth_e + o64
-    return (args.kwidth or (32 if o64 else 0)) + o64
+        return (args.kwidth or 32) + 256 * args.kwidth_e + o64 + band
+    return (args.kwidth or (32 if o64 else 0)) + o64 + band
 
 
 def plain(args):
@@ -140,7 +186,7 @@ def plain(args):
     t_h = ms_h / n_timed / 1e3
     t_e = ms_e / n_timed / 1e3
     if args.mode == 1:
-        kname, kbytes, kt = "k_step_rows", step_bytes, t_h
+        kname, kbytes, kt = "k_step_fused", step_bytes, t_h
         kernels = {kname: round(ms_h / n_timed, 4)}
     else:
         kernels = {"k_update_H": round(ms_h / n_timed, 4), "k_update_E": round(ms_e / n_timed, 4)}
@@ -172,7 +218,7 @@ def plain(args):
         "config": {"workload": spec.name, "grid": list(setup.shape), "cells": N, "npml": setup.npml,
                    "precision": args.precision, "gain_cells": gain_cells, "materials": info["n_materials"],
                    "mat_id_bytes": info["mat_id_bytes"], "setup_s": round(t_setup, 2),
-                   "kernel_mode": ["h_e_kernels", "fused_rows"][args.mode], "planes_per_chunk": args.ichunk,
+                   "kernel_mode": ["h_e_kernels", "fused_step"][args.mode], "fused_param": args.ichunk,
                    "wavefront_kwidth": {"H": args.kwidth or 32, "E": args.kwidth_e or args.kwidth or 32}
                    if args.mode == 0 else None},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
@@ -180,12 +226,22 @@ def plain(args):
                      "algorithmic_bytes_per_launch": kbytes, "avg_launch_us": round(kt * 1e6, 2),
                      "launches_timed": int(n_timed)},
         "kernels_ms_per_step": kernels,
-        "step_alg_bytes_per_cell": round(step_bytes / N, 2),
-        "step_roofline_frac": round(step_bytes / (wall / args.steps) / 1e9 / HBM_PEAK_GBS, 4),
+        # the whole step against the fused minimum: step_bytes (SURVEY 8(d)'s
+        # B_alg per cell x cells) per measured step
+        "step_roofline": {"bound": "hbm", "alg_bytes_per_cell": round(step_bytes / N, 2),
+                          "alg_bytes_per_step": step_bytes,
+                          "achieved": round(step_bytes / (wall / args.steps) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                          "unit": "GB/s", "frac": round(step_bytes / (wall / args.steps) / 1e9 / HBM_PEAK_GBS, 4)},
     }
+    if traffic:
+        line["roofline"]["traffic_over_algorithmic"] = round(traffic / kbytes, 3)
     sc.close()
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(spec, args.cpu_steps)
+        if args.workload == "c3":
+            sample = pkg.scene.precision_case(steps=5000, shape=(128, 128, 384))
+        else:
+            sample = spec
+        line["cpu_baseline"] = cpu_baseline(pkg, sample, args.cpu_steps, args.cpu_steps_1t)
     print(json.dumps(line), flush=True)
 
 
@@ -337,8 +393,8 @@ def serve_form(args):
                                  "fp16_share": 0.5, "budget_gb": 200, "tick_steps": 50, "engine": args.engine,
                                  "makespan_s": round(summ["makespan_s"], 2), "cell_updates": summ["cell_updates"]},
                       "one_at_a_time": {"value": round(solo, 3), "sample_requests": len(sample),
-                                        "precision": "fp32", "kernel_mode": ["h_e_kernels", "fused_rows"][best],
-                                        "by_mode": {["h_e_kernels", "fused_rows"][m]: round(v, 3)
+                                        "precision": "fp32", "kernel_mode": ["h_e_kernels", "fused_step"][best],
+                                        "by_mode": {["h_e_kernels", "fused_step"][m]: round(v, 3)
                                                     for m, v in solo_by_mode.items()}},
                       "speedup_vs_one_at_a_time": round(summ["gcells_per_s"] / solo, 2),
                       "server_time_s": {k: round(v, 3) for k, v in srv.stats.items()}}), flush=True)
@@ -352,17 +408,23 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--precision", default="fp32", choices=["fp64", "fp32", "fp16"])
     ap.add_argument("--size", type=int, default=256)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3"],
-                    help="plain form: c2 = size^3 gain slab (default), c3 = 512x512x384 with 64 cuboids")
+    ap.add_argument("--workload", default="c3", choices=["c2", "c3"],
+                    help="plain form: c3 = 512x512x384 graded grid with gain slab and 64 cuboids (default), "
+                         "c2 = size^3 graded gain slab")
     ap.add_argument("--event-every", type=int, default=10,
                     help="plain form: HIP events around the kernels of every n-th timed step")
-    ap.add_argument("--cpu-steps", type=int, default=80, help="CPU baseline: timed FP64 steps of the workload")
+    ap.add_argument("--cpu-steps", type=int, default=80, help="CPU baseline: timed FP64 steps, all threads")
+    ap.add_argument("--cpu-steps-1t", type=int, default=12, help="CPU baseline: timed FP64 steps on one thread")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mode", type=int, default=0, help="0: separate H/E kernels, 1: fused step kernel")
-    ap.add_argument("--ichunk", type=int, default=0, help="fused kernel x-planes per work-group; 0 = library default")
+    ap.add_argument("--ichunk", type=int, default=0,
+                    help="fused kernel parameter (planes per work-group + 4096 * cells per lane + 65536 * rows); "
+                         "0 = library default")
     ap.add_argument("--kwidth", type=int, default=0, help="H/E kernels: wavefront k-width (16, 32, 64); 0 = library default")
     ap.add_argument("--kwidth-e", type=int, default=0, help="E kernel's own wavefront k-width; 0 = same as --kwidth")
     ap.add_argument("--offsets64", action="store_true", help="H/E kernels with 64-bit element offsets")
+    ap.add_argument("--band", type=int, default=0,
+                    help="H/E kernels: j-tiles (of 4 rows) per band of the tile order; 0 = library default")
     ap.add_argument("--rate-steps", type=int, default=200, help="precision form: timed steps per format")
     ap.add_argument("--domains", type=int, default=256, help="batch form: domains")
     ap.add_argument("--requests", type=int, default=64, help="serve form: requests")

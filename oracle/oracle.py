@@ -68,6 +68,8 @@ def _lib(impl: str):
             "n_grid": (l, [vp, vp]),
             "add_tls_detectors": (i, [vp, C.c_char_p, l, l, l, l, l]),
             "time_steps": (C.c_double, [vp, C.c_int]),
+            "create_prebaked": (vp, [vp, vp]),
+            "set_threads": (None, [C.c_int]),
         })
     if impl == "oracle":
         sig.update({
@@ -94,11 +96,22 @@ class CpuSim:
     """One CPU simulation (impl 'oracle' = the C restatement, 'ref' = the
     compiled reference harness)."""
 
-    def __init__(self, spec, impl: str = "oracle"):
+    def __init__(self, spec, impl: str = "oracle", materials=None):
+        """materials (reference harness only): the 12 MaterialGrids arrays
+        (aEx ... bHz, FP64, idx3 order) used in place of the reference's
+        serial bake, for timing large grids."""
         self.impl = impl
         self.f = _lib(impl)
         self._cspec = spec.to_c()
-        self.h = self.f["create"](C.addressof(self._cspec))
+        if materials is not None:
+            if impl != "ref":
+                raise ValueError("materials are taken by the reference harness only")
+            self._mats = [np.ascontiguousarray(m, dtype=np.float64).ravel() for m in materials]
+            ptrs = (C.c_void_p * 12)(*[m.ctypes.data for m in self._mats])
+            self.h = self.f["create_prebaked"](C.addressof(self._cspec), ptrs)
+            self._mats = None
+        else:
+            self.h = self.f["create"](C.addressof(self._cspec))
         if not self.h:
             raise RuntimeError("create failed")
         info = self.info()
@@ -233,6 +246,11 @@ class CpuSim:
 
 def threads(impl: str = "oracle") -> int:
     return _lib(impl)["threads"]()
+
+
+def set_threads(n: int, impl: str = "ref") -> None:
+    """OpenMP threads of the reference harness's parallel loops."""
+    _lib(impl)["set_threads"](n)
 
 
 REF_MESH_SO = os.path.join(HERE, "_ref", "libref_mesh.so")

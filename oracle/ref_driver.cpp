@@ -85,7 +85,24 @@ struct ref_sim {
 
 extern "C" {
 
-ref_sim* ref_create(const fdtd_scene_spec* sp) {
+static ref_sim* create_impl(const fdtd_scene_spec* sp, const double* const* mats12);
+
+ref_sim* ref_create(const fdtd_scene_spec* sp) { return create_impl(sp, nullptr); }
+
+// ref_create with the 12 MaterialGrids arrays given (idx3 order, NxT*NyT*NzT
+// each: aEx bEx aEy bEy aEz bEz aHx bHx aHy bHy aHz bHz) in place of
+// StructureScene::bake, whose serial subpixel loop over every structure takes
+// hours on 1e8-cell grids with many structures.  The arrays a bake would
+// produce are the input; everything else is the reference's own setup.  Used
+// to time the reference's step on such grids (the CPU baseline).
+ref_sim* ref_create_prebaked(const fdtd_scene_spec* sp, const double* const* mats12) {
+    return mats12 ? create_impl(sp, mats12) : nullptr;
+}
+
+// OpenMP threads of the reference's parallel loops (omp_set_num_threads).
+void ref_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+
+static ref_sim* create_impl(const fdtd_scene_spec* sp, const double* const* mats12) {
     NullBuf nb;
     std::streambuf* old_out = std::cout.rdbuf(&nb);
     std::streambuf* old_err = std::cerr.rdbuf(&nb);
@@ -163,7 +180,14 @@ ref_sim* ref_create(const fdtd_scene_spec* sp) {
             sc.add_cylinder_z(pml_x + s.params[0], pml_y + s.params[1], s.params[2], pml_z + s.params[3],
                               pml_z + s.params[4], mat, tls_cfg);
     }
-    sc.bake(S->mats, S->dt, PhysConst::EPS0, PhysConst::MU0, sp->subpixel_samples > 0 ? sp->subpixel_samples : 8);
+    if (mats12) {
+        S->mats.allocate(S->NxT, S->NyT, S->NzT);
+        std::vector<real>* mv[12] = {&S->mats.aEx, &S->mats.bEx, &S->mats.aEy, &S->mats.bEy, &S->mats.aEz, &S->mats.bEz,
+                                     &S->mats.aHx, &S->mats.bHx, &S->mats.aHy, &S->mats.bHy, &S->mats.aHz, &S->mats.bHz};
+        for (int q = 0; q < 12; ++q) mv[q]->assign(mats12[q], mats12[q] + S->N);
+    } else {
+        sc.bake(S->mats, S->dt, PhysConst::EPS0, PhysConst::MU0, sp->subpixel_samples > 0 ? sp->subpixel_samples : 8);
+    }
     S->bc->bind_material_coeffs(&S->mats.bEx, &S->mats.bEy, &S->mats.bEz, &S->mats.bHx, &S->mats.bHy, &S->mats.bHz);
 
     for (int d = 0; d < sp->n_dipoles; ++d) {

@@ -105,3 +105,27 @@ def test_workload_generators_are_seeded(pkg):
     assert all(s1[q].arrival <= s1[q + 1].arrival for q in range(5))
     st = pkg.Setup(s1[0].spec)
     assert serve.estimate_device_bytes(st, "fp32") > serve.estimate_device_bytes(st, "fp16")
+
+
+def test_reference_harness_with_prebaked_materials(pkg, oracle_mod):
+    """The CPU baseline of a many-structure grid times the reference's step
+    with the MaterialGrids arrays this library's bake produces, in place of
+    the reference's serial bake.  On a graded grid with cuboids and gain the
+    two harness states step to bitwise equal fields, TLS state and probes."""
+    import oracle
+
+    if not oracle.available("ref"):
+        pytest.skip("oracle/_ref/libref.so not built")
+    sp = pkg.scene.gain_slab_case(n_total=40, npml=6, steps=40, fine=5e-9, coarse=20e-9, fine_len=100e-9,
+                                  n_cuboids=6, seed=4)
+    setup = pkg.Setup(sp)
+    mats = [setup.array(n) for n in ("aEx", "bEx", "aEy", "bEy", "aEz", "bEz", "aHx", "bHx", "aHy", "bHy", "aHz",
+                                     "bHz")]
+    a = oracle.CpuSim(sp, "ref")
+    b = oracle.CpuSim(sp, "ref", materials=mats)
+    assert a.step(sp.n_steps) == 0 and b.step(sp.n_steps) == 0
+    for f in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz", "Nu", "Ng", "Pz"):
+        assert np.array_equal(a.array(f), b.array(f)), f
+    assert np.abs(a.array("Ex")).max() > 0
+    for p in range(len(sp.probes)):
+        assert np.array_equal(a.probe(p, 0), b.probe(p, 0))
