@@ -676,6 +676,40 @@ struct fdtd_scene {
 
 extern "C" {
 
+// Initial FP16 scales from the field the sources drive: a current sheet J
+// radiates |E| ~ J eta0 d / 2 (a dipole: d the square root of its cell
+// area); H ~ E / eta0; psi ~ field / d.  The device adapts them from there
+// (never below 2^-6 of these).  Returns 1 with the 18 scales (powers of two)
+// in out18, 0 when the scene has no source (the library's defaults stay).
+int fdtd_setup_fp16_scales(const fdtd_setup* S, double* out18) {
+    if (!S || !out18) return FDTD_EINVAL;
+    if (S->src.empty()) return 0;
+    double e_est = 0.0;
+    const int nprobe = 4096;
+    std::vector<double> v((size_t)nprobe * S->src.size());
+    for (long n0 = 0; n0 < 40000; n0 += nprobe) {
+        fdtd_setup_source_values(S, n0, nprobe, v.data());
+        for (int t = 0; t < nprobe; ++t)
+            for (size_t q = 0; q < S->src.size(); ++q) {
+                const Source& src = S->src[q];
+                const double d = src.kind == 1 ? src.plane_d : std::sqrt(src.cell_area);
+                e_est = std::max(e_est, std::fabs(v[(size_t)t * S->src.size() + q]) * ETA0 * d / 2.0);
+            }
+    }
+    if (!(e_est > 0)) return 0;
+    double imax = 0;
+    for (double x : S->gs.inv_dx) imax = std::max(imax, x);
+    for (double x : S->gs.inv_dy) imax = std::max(imax, x);
+    for (double x : S->gs.inv_dz) imax = std::max(imax, x);
+    auto p2 = [](double x) { int e; std::frexp(x, &e); return std::ldexp(1.0, e - 8); };
+    const double h_est = e_est / ETA0;
+    for (int q = 0; q < 3; ++q) out18[q] = p2(e_est);
+    for (int q = 3; q < 6; ++q) out18[q] = p2(h_est);
+    for (int q = 6; q < 12; ++q) out18[q] = p2(h_est * imax);   // psi_E ~ dH/dx
+    for (int q = 12; q < 18; ++q) out18[q] = p2(e_est * imax);  // psi_H ~ dE/dx
+    return 1;
+}
+
 int fdtd_scene_create(const fdtd_setup* S, int precision, long probe_capacity, fdtd_scene** out) {
     if (!S || !out) return FDTD_EINVAL;
     *out = nullptr;
@@ -773,36 +807,9 @@ int fdtd_scene_create(const fdtd_setup* S, int precision, long probe_capacity, f
         if (id < 0) rc = id;
     }
     if (rc >= 0) rc = fdtd_set_monitor(h, S->stab_interval);
-    if (rc >= 0 && precision == FDTD_FP16 && !S->src.empty()) {
-        // Initial FP16 scales from the field the sources drive: a current
-        // sheet J radiates |E| ~ J eta0 d / 2; H ~ E / eta0; psi ~ field / d.
-        // The device adapts them from there (never below 2^-6 of these).
-        double e_est = 0.0;
-        const int nprobe = 4096;
-        std::vector<double> v((size_t)nprobe * S->src.size());
-        for (long n0 = 0; n0 < 40000; n0 += nprobe) {
-            fdtd_setup_source_values(S, n0, nprobe, v.data());
-            for (int t = 0; t < nprobe; ++t)
-                for (size_t q = 0; q < S->src.size(); ++q) {
-                    const Source& src = S->src[q];
-                    const double d = src.kind == 1 ? src.plane_d : std::sqrt(src.cell_area);
-                    e_est = std::max(e_est, std::fabs(v[(size_t)t * S->src.size() + q]) * ETA0 * d / 2.0);
-                }
-        }
-        if (e_est > 0) {
-            double imax = 0;
-            for (double x : S->gs.inv_dx) imax = std::max(imax, x);
-            for (double x : S->gs.inv_dy) imax = std::max(imax, x);
-            for (double x : S->gs.inv_dz) imax = std::max(imax, x);
-            auto p2 = [](double x) { int e; std::frexp(x, &e); return std::ldexp(1.0, e - 8); };
-            double sc[18];
-            const double h_est = e_est / ETA0;
-            for (int q = 0; q < 3; ++q) sc[q] = p2(e_est);
-            for (int q = 3; q < 6; ++q) sc[q] = p2(h_est);
-            for (int q = 6; q < 12; ++q) sc[q] = p2(h_est * imax);   // psi_E ~ dH/dx
-            for (int q = 12; q < 18; ++q) sc[q] = p2(e_est * imax);  // psi_H ~ dE/dx
-            rc = fdtd_set_fp16_scales(h, sc);
-        }
+    if (rc >= 0 && precision == FDTD_FP16) {
+        double sc[18];
+        if (fdtd_setup_fp16_scales(S, sc) == 1) rc = fdtd_set_fp16_scales(h, sc);
     }
     if (rc < 0) {
         fdtd_destroy(h);

@@ -47,6 +47,7 @@ EXPORTS = [
     "fdtd_scene_batch_create", "fdtd_scene_batch_destroy", "fdtd_scene_batch_add", "fdtd_scene_batch_remove",
     "fdtd_scene_batch_count", "fdtd_scene_batch_step", "fdtd_scene_batch_autotune", "fdtd_scene_batch_handle",
     "fdtd_scene_set_stop", "fdtd_auto_mesh_axis", "fdtd_structure_mesh_bounds", "fdtd_is_stable",
+    "fdtd_setup_fp16_scales",
 ]
 
 
@@ -135,6 +136,7 @@ def load() -> C.CDLL:
         "fdtd_scene_batch_handle": (vp, [vp]),
         "fdtd_scene_set_stop": (i, [vp, l]),
         "fdtd_is_stable": (i, [vp]),
+        "fdtd_setup_fp16_scales": (i, [vp, vp]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(lib, name)
@@ -198,6 +200,12 @@ class Setup:
         v = (C.c_long * 4)()
         _check(self.lib.fdtd_setup_probe(self.h, c, v))
         return tuple(v[:3]), v[3]
+
+    def fp16_scales(self):
+        """The 18 initial FP16 storage scales a scene of this setup starts
+        from, or None when the library keeps its defaults (no source)."""
+        out = np.empty(18, dtype=np.float64)
+        return out if _check(self.lib.fdtd_setup_fp16_scales(self.h, out.ctypes.data_as(C.c_void_p))) == 1 else None
 
     def close(self):
         if getattr(self, "h", None):

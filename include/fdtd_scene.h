@@ -161,6 +161,11 @@ long   fdtd_setup_array(const fdtd_setup* s, const char* name, double* out);
 int    fdtd_setup_source_values(const fdtd_setup* s, long n0, int nsteps, double* out);
 /* Probe channel c (one per probe component): cell and component. */
 int    fdtd_setup_probe(const fdtd_setup* s, int c, long* ijk_comp);
+/* The initial FP16 storage scales a scene of this setup starts from (fields
+ * Ex..Hz, then psi Ex_y..Hz_y; powers of two estimated from the source
+ * amplitudes, see fdtd_set_fp16_scales).  Returns 1 and fills out18, or 0
+ * when the setup has no source.  No reference counterpart. */
+int    fdtd_setup_fp16_scales(const fdtd_setup* s, double* out18);
 
 /* ---- scene: setup + one device simulation ---- */
 typedef struct fdtd_scene fdtd_scene;

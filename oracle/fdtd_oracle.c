@@ -126,6 +126,10 @@ struct oracle_sim {
     int st_stable;
     long st_fail_step;
     size_t n;          /* steps done */
+    /* storage-format emulation (oracle_set_storage): 0 = FP64 (none) */
+    int emu, emu_bits, emu_psi_bits, emu_stoch;
+    double emu_sc[18], emu_floor[18];
+    uint64_t emu_rng;
 };
 
 static inline size_t idx3(size_t i, size_t j, size_t k, size_t Ny, size_t Nz) {
@@ -1083,6 +1087,131 @@ oracle_sim* oracle_create(const fdtd_scene_spec* sp) {
 }
 
 /* One serial-mode step, src/3D_FDTD_version008.cpp:269-349 */
+/* ---- Storage-format emulation (test infrastructure, no reference
+ * counterpart).  The device's FP16 form stores every field and psi value as
+ * an IEEE binary16 number times a per-array power-of-two scale and computes
+ * in FP32; its populations and polarization stay FP64.  This emulates that
+ * storage on the FP64 oracle: after the H update with its CPML correction,
+ * and after the E update with its CPML correction (where the device stores
+ * each array once per step), every H/psi_H (then E/psi_E) value is rounded
+ * as the store would round it.  The compute stays FP64 (FP32 rounding,
+ * 2^-24, is far below the 2^-12 of the store).  Formats:
+ *   1: binary16 with scale (11 significant bits, round to nearest even,
+ *      subnormal below 2^-14 x scale, saturation at 65504 x scale);
+ *   2: the same with stochastic rounding (seeded xorshift);
+ *   3: binary16 plus a residual byte (19 significant bits, same range);
+ *   4: binary16 fields, psi rounded to binary32.
+ * Scales start from `scales18` (fields Ex..Hz, psi Ex_y..Hz_y, the values the
+ * device starts from) and follow the device's plan every 10 steps: when the
+ * largest stored magnitude leaves [2^2, 2^11] the scale moves so that it
+ * lands in [2^7, 2^8), never below 2^-6 of its start. */
+static inline uint64_t dbits(double x) { uint64_t b; memcpy(&b, &x, 8); return b; }
+static inline double bitsd(uint64_t b) { double x; memcpy(&x, &b, 8); return x; }
+static inline uint64_t xorshift64(uint64_t* s) {
+    uint64_t x = *s;
+    x ^= x << 13; x ^= x >> 7; x ^= x << 17;
+    *s = x;
+    return x;
+}
+/* y rounded to `bits` significant bits (normal range of binary16 and below:
+ * a fixed quantum 2^-24 x 2^-(bits-11)), saturated at the binary16 maximum. */
+static double round_store(double y, int bits, int stoch, uint64_t* rng) {
+    if (!(y == y) || y == 0.0) return y;
+    const double ay = fabs(y);
+    if (ay >= 65504.0 * 2.0) return copysign(65504.0, y); /* far beyond: saturate */
+    if (ay < 6.103515625e-05) { /* below 2^-14: binary16 subnormal spacing */
+        const double q = ldexp(1.0, -24 - (bits - 11));
+        double t = y / q, r;
+        if (stoch) {
+            const double fl = floor(t);
+            r = ((double)(xorshift64(rng) >> 11) * 0x1.0p-53 < t - fl) ? fl + 1.0 : fl;
+        } else {
+            r = nearbyint(t);
+        }
+        return r * q;
+    }
+    const int drop = 52 - (bits - 1);
+    uint64_t b = dbits(y);
+    const uint64_t mask = ((uint64_t)1 << drop) - 1;
+    if (stoch) {
+        b += xorshift64(rng) & mask;
+    } else {
+        const uint64_t lsb = (b >> drop) & 1;
+        b += ((uint64_t)1 << (drop - 1)) - 1 + lsb;
+    }
+    b &= ~mask;
+    double r = bitsd(b);
+    if (fabs(r) > 65504.0) r = copysign(65504.0, r);
+    return r;
+}
+static void emu_round(oracle_sim* S, double* a, int arr) {
+    const double sc = S->emu_sc[arr];
+    const int bits = arr >= 6 ? S->emu_psi_bits : S->emu_bits;
+    const long n = (long)S->N;
+    if (bits >= 24) { /* binary32 */
+#pragma omp parallel for
+        for (long i = 0; i < n; ++i) a[i] = (double)(float)a[i];
+        return;
+    }
+    if (S->emu_stoch) { /* one random stream, in cell order */
+        for (long i = 0; i < n; ++i)
+            if (a[i] != 0.0) a[i] = round_store(a[i] / sc, bits, 1, &S->emu_rng) * sc;
+        return;
+    }
+#pragma omp parallel for
+    for (long i = 0; i < n; ++i)
+        if (a[i] != 0.0) a[i] = round_store(a[i] / sc, bits, 0, NULL) * sc;
+}
+/* psi index (0..11) of the device's scale table for oracle psi array q */
+static void emu_store_H(oracle_sim* S) {
+    emu_round(S, S->Hx, 3); emu_round(S, S->Hy, 4); emu_round(S, S->Hz, 5);
+    if (S->bc_type == FDTD_BC_CPML)
+        for (int q = 6; q < 12; ++q) emu_round(S, S->psi[q], 6 + q);
+}
+static void emu_store_E(oracle_sim* S) {
+    emu_round(S, S->Ex, 0); emu_round(S, S->Ey, 1); emu_round(S, S->Ez, 2);
+    if (S->bc_type == FDTD_BC_CPML)
+        for (int q = 0; q < 6; ++q) emu_round(S, S->psi[q], 6 + q);
+}
+static void emu_plan(oracle_sim* S) {
+    double* arr[18] = {S->Ex, S->Ey, S->Ez, S->Hx, S->Hy, S->Hz};
+    for (int q = 0; q < 12; ++q) arr[6 + q] = S->psi[q];
+    for (int c = 0; c < 18; ++c) {
+        if (!arr[c] || (c >= 6 && S->bc_type != FDTD_BC_CPML)) continue;
+        double m = 0.0;
+        for (size_t i = 0; i < S->N; ++i) m = fmax(m, fabs(arr[c][i]));
+        m /= S->emu_sc[c];
+        if (m > 2048.0 || (m > 0.0 && m < 4.0)) {
+            int e;
+            frexp(m, &e);
+            int shift = e - 8;
+            while (shift < 0 && ldexp(S->emu_sc[c], shift) < S->emu_floor[c]) ++shift;
+            S->emu_sc[c] = ldexp(S->emu_sc[c], shift);
+        }
+    }
+}
+
+/* One emulated store of a value already divided by its scale (for tests). */
+double oracle_round_store(double y, int bits, int stochastic) {
+    uint64_t rng = 0x9e3779b97f4a7c15ull;
+    return round_store(y, bits, stochastic, &rng);
+}
+
+/* fmt 0 turns the emulation off. */
+int oracle_set_storage(oracle_sim* S, int fmt, const double* scales18, unsigned long long seed) {
+    if (fmt < 0 || fmt > 4 || (fmt && !scales18)) return -1;
+    S->emu = fmt;
+    S->emu_bits = fmt == 3 ? 19 : 11;
+    S->emu_psi_bits = fmt == 3 ? 19 : fmt == 4 ? 24 : 11;
+    S->emu_stoch = fmt == 2;
+    S->emu_rng = seed ? seed : 0x9e3779b97f4a7c15ull;
+    for (int q = 0; fmt && q < 18; ++q) {
+        S->emu_sc[q] = scales18[q];
+        S->emu_floor[q] = ldexp(scales18[q], -6);
+    }
+    return 0;
+}
+
 static int one_step(oracle_sim* S) {
     const size_t n = S->n;
     memset(S->Jx, 0, S->N * sizeof(double));
@@ -1092,12 +1221,15 @@ static int one_step(oracle_sim* S) {
     if (S->tls_on) update_polarization(S);
     update_H(S);
     if (S->bc_type == FDTD_BC_CPML) cpml_after_H(S);
+    if (S->emu) emu_store_H(S);
     update_E(S, S->tls_on);
     if (S->bc_type == FDTD_BC_CPML) cpml_after_E(S); else pec_after_E(S);
+    if (S->emu) emu_store_E(S);
     record_probes(S, n);
     accumulate_dft(S, n);
     if (S->tls_on) update_populations(S);
     S->n++;
+    if (S->emu && n % 10 == 0) emu_plan(S);
     if (n % S->stab_interval == 0) {
         double cons = conservation_error(S);
         S->st_last_cons = cons;

@@ -76,6 +76,8 @@ def _lib(impl: str):
             "get_region": (C.c_int, [vp, C.c_int, vp]),
             "get_source": (C.c_int, [vp, C.c_int, C.c_long, vp]),
             "get_dft": (C.c_long, [vp, C.c_int, vp]),
+            "set_storage": (C.c_int, [vp, C.c_int, vp, C.c_ulonglong]),
+            "round_store": (C.c_double, [C.c_double, C.c_int, C.c_int]),
             "time_steps": (C.c_double, [vp, C.c_int]),
             "last_conservation": (C.c_double, [vp]),
         })
@@ -237,6 +239,16 @@ class CpuSim:
         out = np.empty(n, dtype=np.float64)
         self.f["get_dft"](self.h, m, out.ctypes.data_as(C.c_void_p))
         return out
+
+    STORAGE = {"fp64": 0, "fp16": 1, "fp16_stochastic": 2, "fp16_residual8": 3, "fp16_psi32": 4}
+
+    def set_storage(self, fmt: str, scales18=None, seed: int = 1) -> None:
+        """Emulated storage rounding of fields and psi (oracle only): see
+        oracle_set_storage in fdtd_oracle.c."""
+        sc = None if scales18 is None else np.ascontiguousarray(scales18, dtype=np.float64)
+        ptr = None if sc is None else sc.ctypes.data_as(C.c_void_p)
+        if self.f["set_storage"](self.h, self.STORAGE[fmt], ptr, seed) != 0:
+            raise ValueError(fmt)
 
     def stability(self):
         info = self.info()

@@ -546,161 +546,96 @@ double ref_time_steps(ref_sim* S, int nsteps) {
 int ref_threads(void) { return omp_get_max_threads(); }
 
 #ifdef REF_DROPIN
+}  // extern "C"
+#include "../include/fdtd_gpu_stepper.hpp"
+extern "C" {
 // The binding of the device library into the reference's main(), compiled:
-// the reference's own state (its setup code above) is handed to
-// fdtd_create exactly as the maintainer's binding hands it over, the device
-// advances it with fdtd_step, and the reference advances the same state with
-// its own step on the CPU.  With device_bake the binding passes the
-// structure list (StructureScene::items and bg) instead of the baked
-// MaterialGrids, and the device bakes.  rel receives the relative L2
-// difference of Ex Ey Ez Hx Hy Hz Nu Ng Pz Pz_prev (0..9), the largest one
-// over the probe traces (10) and over the 12 material arrays the handle
-// holds against the reference's bake (11); returns the number of probe
-// channels, or < 0 on error.
+// the reference's own state (its setup code above) goes to the adaptor of
+// include/fdtd_gpu_stepper.hpp exactly as a maintainer's main() hands it
+// over (GpuStepper from GridSpacing, MaterialGrids or the structure list,
+// the boundary and the TLS objects; its sources; its probes), the device
+// advances it in blocks of 10 steps (the stability monitor's cadence), and
+// the reference advances the same state with its own step on the CPU.  rel
+// receives the relative L2 difference of Ex Ey Ez Hx Hy Hz Nu Ng Pz Pz_prev
+// (0..9), the largest one over the probe traces (10) and over the 12
+// material arrays the handle holds against the reference's bake (11);
+// returns the number of probe channels, or < 0 on error.
 int ref_dropin_compare(const fdtd_scene_spec* sp, int nsteps, int precision, int device_bake, double* rel) {
     ref_sim* S = ref_create(sp);
-    auto* cp = dynamic_cast<CPMLBoundary*>(S->bc.get());
-    fdtd_grid_desc g{(int)S->NxT, (int)S->NyT, (int)S->NzT, S->gs.inv_dx.data(), S->gs.inv_dy.data(),
-                     S->gs.inv_dz.data(), S->dt};
     MaterialGrids& mg = S->mats;
-    fdtd_material_desc m{mg.aEx.data(), mg.bEx.data(), mg.aEy.data(), mg.bEy.data(), mg.aEz.data(), mg.bEz.data(),
-                         mg.aHx.data(), mg.bHx.data(), mg.aHy.data(), mg.bHy.data(), mg.aHz.data(), mg.bHz.data(),
-                         nullptr};
-    // the structure list for a device bake (type + parameters of Box /
-    // Sphere / CylinderZ in insertion order, with their Material)
-    std::vector<fdtd_bake_shape> shapes;
-    for (const auto& it : S->scene.items) {
-        fdtd_bake_shape q{};
-        if (auto* bx = dynamic_cast<const Box*>(it.shape.get())) {
-            q.type = 0;
-            q.p[0] = bx->bb.x0; q.p[1] = bx->bb.x1; q.p[2] = bx->bb.y0;
-            q.p[3] = bx->bb.y1; q.p[4] = bx->bb.z0; q.p[5] = bx->bb.z1;
-        } else if (auto* sh = dynamic_cast<const Sphere*>(it.shape.get())) {
-            q.type = 1;
-            q.p[0] = sh->cx; q.p[1] = sh->cy; q.p[2] = sh->cz; q.p[3] = sh->r;
-        } else if (auto* cy = dynamic_cast<const CylinderZ*>(it.shape.get())) {
-            q.type = 2;
-            q.p[0] = cy->cx; q.p[1] = cy->cy; q.p[2] = cy->r; q.p[3] = cy->z0; q.p[4] = cy->z1;
-        }
-        q.eps_r = it.mat.eps_r; q.mu_r = it.mat.mu_r; q.sigma = it.mat.sigma; q.sigma_m = it.mat.sigma_m;
-        shapes.push_back(q);
-    }
-    const Material& bg = S->scene.bg;
-    fdtd_bake_desc bd{S->gs.x_bounds.data(), S->gs.y_bounds.data(), S->gs.z_bounds.data(), S->gs.dx.data(),
-                      S->gs.dy.data(), S->gs.dz.data(), bg.eps_r, bg.mu_r, bg.sigma, bg.sigma_m,
-                      (int)shapes.size(), shapes.data(), sp->subpixel_samples > 0 ? sp->subpixel_samples : 8,
-                      S->dt, PhysConst::EPS0, PhysConst::MU0};
-    if (device_bake) m = fdtd_material_desc{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
-                                            nullptr, nullptr, nullptr, nullptr, &bd};
-    fdtd_boundary_desc b{cp ? 1 : 0, (int)S->npml};
-    if (cp) {
-        b.kx = cp->kx_.data(); b.ky = cp->ky_.data(); b.kz = cp->kz_.data();
-        b.bEx = cp->bEx_.data(); b.cEx = cp->cEx_.data(); b.bEy = cp->bEy_.data(); b.cEy = cp->cEy_.data();
-        b.bEz = cp->bEz_.data(); b.cEz = cp->cEz_.data(); b.bHx = cp->bHx_.data(); b.cHx = cp->cHx_.data();
-        b.bHy = cp->bHy_.data(); b.cHy = cp->cHy_.data(); b.bHz = cp->bHz_.data(); b.cHz = cp->cHz_.data();
-    }
-    std::vector<fdtd_tls_region> regs;
-    for (auto& r : S->tls_manager.regions)
-        regs.push_back({r.params.kapa_coeff, r.params.pa2, r.params.pa3, r.params.inv_dt, r.params.tau, r.params.hbar,
-                        r.params.omega_a, r.params.N0_total, S->dt, r.params.enable_population_clamp ? 1 : 0});
     TwoLevelState& ts = S->tls_state;
-    fdtd_tls_desc t{(int)regs.size(), regs.data(), S->tls_manager.cell_region_map.data(), ts.Ndip.data(),
-                    ts.Ng0.data(), ts.Nu.data(), ts.Ng.data(), ts.Pz.data(), ts.Pz_prev.data()};
-    fdtd_handle h = nullptr;
-    if (fdtd_create(&g, &m, &b, S->tls_on ? &t : nullptr, precision, &h) != 0) {
-        ref_destroy(S);
-        return -1;
-    }
-    // source footprints (DipoleSource / PlaneWaveSource members); a plane
-    // whose polarization injects nothing keeps a footprint with value 0
-    auto comp_of = [](Sources::Polarization p) {
-        using P = Sources::Polarization;
-        return (p == P::Ex || p == P::Hx) ? 0 : (p == P::Ey || p == P::Hy) ? 1 : 2;
-    };
-    std::vector<fdtd_source_desc> fp;
-    for (auto& src : S->sources) {
-        if (auto* d = dynamic_cast<Sources::DipoleSource*>(src.get())) {
-            fp.push_back({comp_of(d->polarization), (int)d->i, (int)d->i, (int)d->j, (int)d->j, (int)d->k, (int)d->k});
-        } else if (auto* pw = dynamic_cast<Sources::PlaneWaveSource*>(src.get())) {
-            using D = Sources::PlaneWaveDirection;
-            fdtd_source_desc f{comp_of(pw->polarization), (int)pw->i_min, (int)pw->i_max, (int)pw->j_min,
-                               (int)pw->j_max, (int)pw->k_min, (int)pw->k_max};
-            if (pw->direction == D::PlusZ || pw->direction == D::MinusZ) f.k0 = f.k1 = (int)pw->plane_index;
-            else if (pw->direction == D::PlusX || pw->direction == D::MinusX) f.i0 = f.i1 = (int)pw->plane_index;
-            else f.j0 = f.j1 = (int)pw->plane_index;
-            fp.push_back(f);
+    const TwoLevelState* tls = S->tls_on ? &ts : nullptr;
+    const TLSRegionManager* regs = S->tls_on ? &S->tls_manager : nullptr;
+    int rc = 0;
+    try {
+        std::unique_ptr<fdtd_gpu::GpuStepper> gpu;
+        if (device_bake)
+            gpu.reset(new fdtd_gpu::GpuStepper(S->gs, S->dt, S->scene, sp->subpixel_samples, PhysConst::EPS0,
+                                               PhysConst::MU0, *S->bc, tls, regs, precision));
+        else
+            gpu.reset(new fdtd_gpu::GpuStepper(S->gs, S->dt, mg, *S->bc, tls, regs, precision));
+        gpu->bind_sources(S->sources);
+        // the harness's probes are PointFieldDetectors placed and clamped by
+        // the reference's own code
+        std::vector<std::vector<int>> chans;
+        for (const Probe& pr : S->probes) {
+            Detectors::PointFieldDetector d;
+            d.i0 = pr.i0; d.j0 = pr.j0; d.k0 = pr.k0; d.save_every = S->save_every;
+            for (int c : pr.comps) d.components.push_back(static_cast<Detectors::FieldComponent>(c));
+            chans.push_back(gpu->bind_point_detector(d, (size_t)nsteps));
         }
-    }
-    int rc = fdtd_set_sources(h, (int)fp.size(), fp.data());
-    std::vector<std::pair<int, size_t>> chans;  // (device probe id, reference record stride position)
-    for (size_t p = 0; p < S->probes.size() && rc == 0; ++p)
-        for (int c : S->probes[p].comps) {
-            const int id = fdtd_add_probe(h, (int)S->probes[p].i0, (int)S->probes[p].j0, (int)S->probes[p].k0, c,
-                                          (int)S->save_every, nsteps);
-            if (id < 0) rc = id;
-            chans.push_back({id, p});
+        for (int s = 0; s < nsteps; s += 10) {
+            const int k = std::min(10, nsteps - s);
+            gpu->step(S->n, k, S->dt);  // an unstable verdict does not stop the comparison
+            ref_step(S, k);
         }
-    // per-step source values: what the source's own inject_half_step adds at
-    // its footprint (read from scratch J arrays, which are cleared again)
-    std::vector<real> Jx(S->N, 0.0), Jy(S->N, 0.0), Jz(S->N, 0.0);
-    std::vector<double> vals(fp.size());
-    for (int s = 0; s < nsteps && rc == 0; ++s) {
-        const size_t n = S->n;
-        for (size_t q = 0; q < fp.size(); ++q) {
-            S->sources[q]->inject_half_step(n, S->dt, Jx, Jy, Jz);
-            std::vector<real>& J = fp[q].component == 0 ? Jx : fp[q].component == 1 ? Jy : Jz;
-            vals[q] = J[idx3((size_t)fp[q].i0, (size_t)fp[q].j0, (size_t)fp[q].k0, S->NyT, S->NzT)];
-            for (int i = fp[q].i0; i <= fp[q].i1; ++i)
-                for (int j = fp[q].j0; j <= fp[q].j1; ++j)
-                    for (int k = fp[q].k0; k <= fp[q].k1; ++k) {
-                        const size_t id = idx3((size_t)i, (size_t)j, (size_t)k, S->NyT, S->NzT);
-                        Jx[id] = Jy[id] = Jz[id] = 0.0;
-                    }
+        auto rl2 = [](const std::vector<double>& a, const std::vector<double>& r) {
+            double num = 0, den = 0;
+            for (size_t q = 0; q < a.size(); ++q) {
+                num += (a[q] - r[q]) * (a[q] - r[q]);
+                den += r[q] * r[q];
+            }
+            return den > 0 ? std::sqrt(num / den) : std::sqrt(num);
+        };
+        std::vector<real> f[6];
+        for (auto& v : f) v.resize(S->N);
+        gpu->download_fields(f[0], f[1], f[2], f[3], f[4], f[5]);
+        const std::vector<real>* ref[6] = {&S->Ex, &S->Ey, &S->Ez, &S->Hx, &S->Hy, &S->Hz};
+        for (int q = 0; q < 6; ++q) rel[q] = rl2(f[q], *ref[q]);
+        for (int q = 6; q < 10; ++q) rel[q] = 0.0;
+        if (S->tls_on) {
+            TwoLevelState dev = ts;
+            gpu->download_tls(dev);
+            rel[6] = rl2(dev.Nu, ts.Nu); rel[7] = rl2(dev.Ng, ts.Ng);
+            rel[8] = rl2(dev.Pz, ts.Pz); rel[9] = rl2(dev.Pz_prev, ts.Pz_prev);
         }
-        rc = fdtd_step(h, (long)n, 1, vals.data());
-        if (rc == FDTD_EUNSTABLE) rc = 0;  // the comparison below still holds
-        ref_step(S, 1);
-    }
-    auto rl2 = [](const std::vector<double>& a, const std::vector<double>& r) {
-        double num = 0, den = 0;
-        for (size_t q = 0; q < a.size(); ++q) {
-            num += (a[q] - r[q]) * (a[q] - r[q]);
-            den += r[q] * r[q];
+        rel[11] = 0.0;
+        std::vector<double> dev(S->N);
+        const std::vector<real>* mref[12] = {&mg.aEx, &mg.bEx, &mg.aEy, &mg.bEy, &mg.aEz, &mg.bEz,
+                                             &mg.aHx, &mg.bHx, &mg.aHy, &mg.bHy, &mg.aHz, &mg.bHz};
+        for (int q = 0; q < 12 && rc == 0; ++q) {
+            rc = fdtd_download_material(gpu->handle(), q, dev.data());
+            rel[11] = std::max(rel[11], rl2(dev, *mref[q]));
         }
-        return den > 0 ? std::sqrt(num / den) : std::sqrt(num);
-    };
-    std::vector<double> dev(S->N);
-    const std::vector<real>* ref[10] = {&S->Ex, &S->Ey, &S->Ez, &S->Hx, &S->Hy, &S->Hz,
-                                        &ts.Nu, &ts.Ng, &ts.Pz, &ts.Pz_prev};
-    for (int q = 0; q < 10 && rc == 0; ++q) {
-        rel[q] = 0.0;
-        if (q >= 6 && !S->tls_on) continue;
-        rc = fdtd_download(h, q, dev.data());
-        rel[q] = rl2(dev, *ref[q]);
+        rel[10] = 0.0;
+        size_t nch = 0;
+        for (size_t p = 0; p < S->probes.size(); ++p) {
+            const Probe& pr = S->probes[p];
+            const size_t nc = pr.comps.size();
+            for (size_t c = 0; c < nc; ++c, ++nch) {
+                std::vector<double> r;
+                for (size_t t = c; t < pr.rec.size(); t += nc) r.push_back(pr.rec[t]);
+                const std::vector<double> d = gpu->probe_trace(chans[p][c]);
+                if (d.size() != r.size()) rc = -2;
+                else rel[10] = std::max(rel[10], rl2(d, r));
+            }
+        }
+        if (rc == 0) rc = (int)nch;
+    } catch (const std::exception&) {
+        rc = -1;
     }
-    rel[11] = 0.0;
-    const std::vector<real>* mref[12] = {&mg.aEx, &mg.bEx, &mg.aEy, &mg.bEy, &mg.aEz, &mg.bEz,
-                                         &mg.aHx, &mg.bHx, &mg.aHy, &mg.bHy, &mg.aHz, &mg.bHz};
-    for (int q = 0; q < 12 && rc == 0; ++q) {
-        rc = fdtd_download_material(h, q, dev.data());
-        rel[11] = std::max(rel[11], rl2(dev, *mref[q]));
-    }
-    rel[10] = 0.0;
-    std::vector<size_t> seen(S->probes.size(), 0);
-    for (size_t c = 0; c < chans.size() && rc == 0; ++c) {
-        const Probe& pr = S->probes[chans[c].second];
-        const size_t nc = pr.comps.size(), q = seen[chans[c].second]++;
-        std::vector<double> r;
-        for (size_t t = q; t < pr.rec.size(); t += nc) r.push_back(pr.rec[t]);
-        std::vector<double> d(r.size());
-        long cnt = 0;
-        rc = fdtd_read_probe(h, chans[c].first, d.data(), (long)d.size(), &cnt);
-        if (rc == 0 && (size_t)cnt != r.size()) rc = -2;
-        if (rc == 0) rel[10] = std::max(rel[10], rl2(d, r));
-    }
-    fdtd_destroy(h);
     ref_destroy(S);
-    return rc == 0 ? (int)chans.size() : rc;
+    return rc;
 }
 #endif
 

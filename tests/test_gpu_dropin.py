@@ -1,11 +1,13 @@
 """The drop-in boundary exercised the way the reference's main() would use
 it: the reference's own setup code (its headers compiled into
 oracle/_ref/libref_dropin.so) builds GridSpacing, MaterialGrids, the CPML
-profiles, the TLS regions and state, the sources and the probes; the binding
-hands those host arrays to fdtd_create, registers the source footprints and
-probes, and calls fdtd_step once per step with the value each source's own
-inject_half_step adds.  The reference's step advances the same state on the
-CPU in the same process.  In the device-bake form the binding passes the
+profiles, the TLS regions and state, the sources and the probes; the adaptor
+of include/fdtd_gpu_stepper.hpp (fdtd_gpu::GpuStepper, compiled with the
+reference's headers) takes those objects, binds the sources (footprints, and
+per-step values from each source's own waveform, with no N-sized J array)
+and the point detectors, and advances the state in blocks of 10 steps (one
+fdtd_step call each).  The reference's step advances the same state on the
+CPU in the same process.  In the device-bake form the adaptor passes the
 reference's structure list (StructureScene::items, bg) instead of its baked
 MaterialGrids, and the handle's material arrays are compared with the
 reference's bake too.

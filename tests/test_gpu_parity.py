@@ -242,3 +242,23 @@ def test_fp16_storage_adaptive_scales(pkg, oracle_mod):
     assert max(errs.values()) < 3e-3
     for f in FIELDS:
         assert np.isfinite(sc.field(f)).all()
+
+
+def test_fp16_error_is_the_store_rounding(pkg, oracle_mod):
+    """The device's FP16 probe error against the FP64 oracle equals, within a
+    factor of 2.5, the error of the FP64 oracle with every store rounded to
+    binary16 at the device's scales (oracle_set_storage): FP16 storage
+    accuracy is a property of the format (tests/test_fp16_emulation.py)."""
+    sp = pkg.scene.gain_slab_case(n_total=40, npml=6, steps=300, fine_len=100e-9, coarse=20e-9, fine=5e-9)
+    _say(f"[case] fp16 vs emulated binary16 stores, {sp.name}")
+    setup, sc, ref = _run_pair(pkg, oracle_mod, sp, "fp16")
+    emu = oracle_mod.CpuSim(sp, "oracle")
+    emu.set_storage("fp16", setup.fp16_scales())
+    emu.step(sp.n_steps)
+    for p, pr in enumerate(sp.probes):
+        chans = [c for c, (pp, _) in enumerate(_channels(sp)) if pp == p]
+        d = rel_l2(np.stack([sc.probe(c) for c in chans]), np.stack([ref.probe(p, q) for q in range(len(chans))]))
+        e = rel_l2(np.stack([emu.probe(p, q) for q in range(len(chans))]),
+                   np.stack([ref.probe(p, q) for q in range(len(chans))]))
+        _say(f"  probe {p}: device {d:.2e}, emulated stores {e:.2e}")
+        assert 0.4 < d / e < 2.5, (p, d, e)

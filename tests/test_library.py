@@ -31,11 +31,17 @@ def test_library_exports_every_declared_symbol(pkg):
     assert set(pkg.runtime.EXPORTS) <= exported
 
 
-def test_library_is_gfx950(pkg):
-    out = subprocess.run(["/opt/rocm/bin/roc-obj-ls", pkg.runtime.library_path()], capture_output=True, text=True)
-    if out.returncode != 0:
-        pytest.skip("roc-obj-ls unavailable")
-    assert "gfx950" in out.stdout
+def test_library_is_gfx950(pkg, tmp_path):
+    """The library's device code is one gfx950 code object: the offload
+    bundle in its .hip_fatbin section lists the gfx950 target and the host
+    entry only."""
+    fatbin = tmp_path / "fatbin.bin"
+    subprocess.run(["objcopy", "-O", "binary", "--only-section=.hip_fatbin", pkg.runtime.library_path(), str(fatbin)],
+                   check=True)
+    out = subprocess.run(["/opt/rocm/llvm/bin/clang-offload-bundler", "--list", "--type=o", f"--input={fatbin}"],
+                         capture_output=True, text=True, check=True).stdout.split()
+    device = [t for t in out if not t.startswith("host-")]
+    assert device == ["hipv4-amdgcn-amd-amdhsa--gfx950"], out
 
 
 def test_no_cpu_fallback_without_device(pkg):
