@@ -245,20 +245,30 @@ def test_fp16_storage_adaptive_scales(pkg, oracle_mod):
 
 
 def test_fp16_error_is_the_store_rounding(pkg, oracle_mod):
-    """The device's FP16 probe error against the FP64 oracle equals, within a
-    factor of 2.5, the error of the FP64 oracle with every store rounded to
+    """The device's FP16 error against the FP64 oracle equals, within a
+    factor of 2, the error of the FP64 oracle with every store rounded to
     binary16 at the device's scales (oracle_set_storage): FP16 storage
-    accuracy is a property of the format (tests/test_fp16_emulation.py)."""
+    accuracy is a property of the format (tests/test_fp16_emulation.py).
+
+    Rounding errors are pseudo-random, and the two runs round different
+    values, so single probe traces (a few independent contributions each)
+    scatter by a factor of 3 either way; the error of each whole field (64000
+    cells) and of all probe channels together is the stable statistic."""
     sp = pkg.scene.gain_slab_case(n_total=40, npml=6, steps=300, fine_len=100e-9, coarse=20e-9, fine=5e-9)
     _say(f"[case] fp16 vs emulated binary16 stores, {sp.name}")
     setup, sc, ref = _run_pair(pkg, oracle_mod, sp, "fp16")
     emu = oracle_mod.CpuSim(sp, "oracle")
     emu.set_storage("fp16", setup.fp16_scales())
     emu.step(sp.n_steps)
-    for p, pr in enumerate(sp.probes):
-        chans = [c for c, (pp, _) in enumerate(_channels(sp)) if pp == p]
-        d = rel_l2(np.stack([sc.probe(c) for c in chans]), np.stack([ref.probe(p, q) for q in range(len(chans))]))
-        e = rel_l2(np.stack([emu.probe(p, q) for q in range(len(chans))]),
-                   np.stack([ref.probe(p, q) for q in range(len(chans))]))
-        _say(f"  probe {p}: device {d:.2e}, emulated stores {e:.2e}")
-        assert 0.4 < d / e < 2.5, (p, d, e)
+    ratios = {}
+    for f in FIELDS:
+        r = ref.array(f)
+        ratios[f] = rel_l2(sc.field(f), r) / rel_l2(emu.array(f), r)
+    chans = _channels(sp)
+    dev = np.concatenate([sc.probe(c) for c in range(len(chans))])
+    want = np.concatenate([ref.probe(p, q) for p, q in chans])
+    got = np.concatenate([emu.probe(p, q) for p, q in chans])
+    ratios["probes"] = rel_l2(dev, want) / rel_l2(got, want)
+    _say("  device error / emulated-store error: " + " ".join(f"{k}={v:.2f}" for k, v in ratios.items()))
+    bad = {k: v for k, v in ratios.items() if not 0.5 < v < 2.0}
+    assert not bad, bad
