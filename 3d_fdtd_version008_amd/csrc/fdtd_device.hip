@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -136,6 +137,9 @@ struct fdtd_sim {
     int nsrc = 0;
     double* d_srcv = nullptr;
     long srcv_cap = 0;
+    int device = 0;                // the device the handle was created on
+    double* srcv_cur = nullptr;    // source values of the call in flight
+    MonOut* mon_cur = nullptr;     // monitor records of the call in flight
     // probes / dft
     std::vector<Probe> probes;
     std::vector<Dft> dfts;
@@ -680,7 +684,7 @@ constexpr long MONPART_MAX = 2048;  // blocks (and partial records) of the monit
 template <typename T, typename R>
 int launch_step(fdtd_sim* s, long n, int sidx, hipEvent_t* ev) {
     StepArgs<T, R> a = make_args<T, R>(s);
-    a.srcv = s->d_srcv + (long)sidx * s->nsrc;
+    a.srcv = s->srcv_cur + (long)sidx * s->nsrc;
     if (s->mode == 1 && fused_ok(s) && s->F2[s->cur ^ 1][0]) {
         const int o = s->cur ^ 1;
         a.Exw = (T*)s->F2[o][0]; a.Eyw = (T*)s->F2[o][1]; a.Ezw = (T*)s->F2[o][2];
@@ -693,7 +697,7 @@ int launch_step(fdtd_sim* s, long n, int sidx, hipEvent_t* ev) {
         if (ev) (void)hipEventRecord(ev[3], s->stream);
         set_cur(s, o);
         a = make_args<T, R>(s);
-        a.srcv = s->d_srcv + (long)sidx * s->nsrc;
+        a.srcv = s->srcv_cur + (long)sidx * s->nsrc;
     } else {
         const dim3 blk(64 * fdtdk::TILE_ROWS);
         // narrow wavefronts pay off where instruction issue, not HBM, bounds
@@ -984,7 +988,7 @@ int fdtd_create(const fdtd_grid_desc* g, const fdtd_material_desc* m, const fdtd
     s->esz = precision == FDTD_FP64 ? 8 : (precision == FDTD_FP32 ? 4 : 2);
     s->rsz = precision == FDTD_FP64 ? 8 : 4;
     int rc = 0;
-    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipGetDevice(&s->device) != hipSuccess || hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
         delete s;
         return fail(FDTD_EDEVICE, "hipStreamCreate failed");
     }
@@ -1243,6 +1247,32 @@ int fdtd_set_monitor(fdtd_handle h, int interval) {
     return FDTD_OK;
 }
 
+// Scratch of fdtd_step_noalloc: the monitor records and the per-step source
+// values of one call.  Every device holds its own copy at its own address,
+// looked up once per device with that device current; the array's name is a
+// host-side stand-in, so host code uses only the looked-up address.
+static __device__ __align__(256) unsigned char scratch[FDTD_NOALLOC_SCRATCH_BYTES];
+static std::mutex g_scratch_mu;
+static void* g_scratch_addr[64] = {};
+
+static int scratch_address(int dev, unsigned char** out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (dev < 0 || dev >= 64) return fail(FDTD_EDEVICE, "device ordinal out of range for the no-alloc scratch");
+    if (!g_scratch_addr[dev]) {
+        int cur = -1;
+        if (hipGetDevice(&cur) != hipSuccess) return fail(FDTD_EDEVICE, "hipGetDevice failed");
+        if (cur != dev && hipSetDevice(dev) != hipSuccess) return fail(FDTD_EDEVICE, "hipSetDevice failed");
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != dev) (void)hipSetDevice(cur);
+        if (e != hipSuccess || !p)
+            return fail(FDTD_EDEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_addr[dev] = p;
+    }
+    *out = (unsigned char*)g_scratch_addr[dev];
+    return FDTD_OK;
+}
+
 // Reads the monitor records of the steps last enqueued and applies the
 // stability policy (StabilityMonitor::check_stability, src/stability_monitor.hpp:42-144).
 static int step_finish(fdtd_handle h) {
@@ -1257,7 +1287,7 @@ static int step_finish(fdtd_handle h) {
         return FDTD_OK;
     }
     std::vector<MonOut> mo(mon_steps.size());
-    HIPCHK(hipMemcpyAsync(mo.data(), h->d_mon, mo.size() * sizeof(MonOut), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(mo.data(), h->mon_cur, mo.size() * sizeof(MonOut), hipMemcpyDeviceToHost, h->stream));
     // FP16 storage: the scale planner's sticky flags of a NaN/Inf stored in
     // an E-class or H-class array (fields or psi)
     unsigned nonfinite[2] = {0u, 0u};
@@ -1294,35 +1324,55 @@ static int step_finish(fdtd_handle h) {
 // `every`-th step (steps 0, every, 2*every, ... of the call), 4 per timed step,
 // plus one at each end of the call.
 static int step_launch(fdtd_handle h, long n0, int nsteps, const double* src_values, std::vector<hipEvent_t>* ev,
-                       int every = 1) {
+                       int every = 1, bool noalloc = false) {
     if (!h || nsteps < 0 || n0 < 0) return fail(FDTD_EINVAL, "bad step arguments");
     if (h->batch) return fail(FDTD_EINVAL, "domain is a batch member: step it through its batch");
     if (int rc = step_finish(h)) return rc;  // the monitor buffer is reused
     if (nsteps == 0) return FDTD_OK;
-    if (h->nsrc > 0) {
-        if (!src_values) return fail(FDTD_EINVAL, "src_values required");
-        const long need = (long)nsteps * h->nsrc;
+    if (h->nsrc > 0 && !src_values) return fail(FDTD_EINVAL, "src_values required");
+    const long need = (long)nsteps * h->nsrc;
+    // monitor slots for this call
+    std::vector<long> mon_steps;
+    if (h->mon_interval > 0)
+        for (int s = 0; s < nsteps; ++s)
+            if ((n0 + s) % h->mon_interval == 0) mon_steps.push_back(n0 + s);
+    if (noalloc) {
+        // records first (256-byte aligned), then the source values
+        const size_t rec = ((mon_steps.size() * sizeof(MonOut)) + 255) / 256 * 256;
+        const size_t total = rec + (size_t)need * sizeof(double);
+        if (total > FDTD_NOALLOC_SCRATCH_BYTES)
+            return fail(FDTD_EINVAL, "fdtd_step_noalloc: the call needs " + std::to_string(total) +
+                                         " B of scratch, above the " + std::to_string(FDTD_NOALLOC_SCRATCH_BYTES) +
+                                         " B maximum: split it into shorter calls");
+        unsigned char* base = nullptr;
+        if (int rc = scratch_address(h->device, &base)) return rc;
+        h->mon_cur = (MonOut*)base;
+        h->srcv_cur = (double*)(base + rec);
+        if (!mon_steps.empty()) {
+            const long words = (long)(mon_steps.size() * sizeof(MonOut) / sizeof(unsigned long long));
+            hipLaunchKernelGGL(fdtdk::k_zero_words, dim3((unsigned)std::min<long>((words + 255) / 256, 64)), dim3(256), 0,
+                               h->stream, (unsigned long long*)h->mon_cur, words);
+        }
+    } else {
         if (need > h->srcv_cap) {
             void* p;
             if (int rc = h->alloc(&p, (size_t)need * sizeof(double))) return rc;
             h->d_srcv = (double*)p;
             h->srcv_cap = need;
         }
-        HIPCHK(hipMemcpyAsync(h->d_srcv, src_values, (size_t)need * sizeof(double), hipMemcpyHostToDevice, h->stream));
+        if ((int)mon_steps.size() > h->mon_cap) {
+            void* p;
+            const int cap = std::max<int>((int)mon_steps.size(), 64);
+            if (int rc = h->alloc(&p, (size_t)cap * sizeof(MonOut))) return rc;
+            h->d_mon = (MonOut*)p;
+            h->mon_cap = cap;
+        }
+        h->srcv_cur = h->d_srcv;
+        h->mon_cur = h->d_mon;
+        if (!mon_steps.empty()) HIPCHK(hipMemsetAsync(h->d_mon, 0, mon_steps.size() * sizeof(MonOut), h->stream));
     }
-    // monitor slots for this call
-    std::vector<long> mon_steps;
-    if (h->mon_interval > 0)
-        for (int s = 0; s < nsteps; ++s)
-            if ((n0 + s) % h->mon_interval == 0) mon_steps.push_back(n0 + s);
-    if ((int)mon_steps.size() > h->mon_cap) {
-        void* p;
-        const int cap = std::max<int>((int)mon_steps.size(), 64);
-        if (int rc = h->alloc(&p, (size_t)cap * sizeof(MonOut))) return rc;
-        h->d_mon = (MonOut*)p;
-        h->mon_cap = cap;
-    }
-    if (!mon_steps.empty()) HIPCHK(hipMemsetAsync(h->d_mon, 0, mon_steps.size() * sizeof(MonOut), h->stream));
+    if (need > 0)
+        HIPCHK(hipMemcpyAsync(h->srcv_cur, src_values, (size_t)need * sizeof(double), hipMemcpyHostToDevice, h->stream));
     int slot = 0;
     if (ev) {
         ev->resize((size_t)((nsteps + every - 1) / every) * 4 + 2);
@@ -1332,7 +1382,7 @@ static int step_launch(fdtd_handle h, long n0, int nsteps, const double* src_val
     for (int s = 0; s < nsteps; ++s) {
         const long n = n0 + s;
         dispatch_step(h, n, s, ev && s % every == 0 ? &(*ev)[1 + (size_t)(s / every) * 4] : nullptr);
-        if (h->mon_interval > 0 && n % h->mon_interval == 0) dispatch_monitor(h, h->d_mon + slot++);
+        if (h->mon_interval > 0 && n % h->mon_interval == 0) dispatch_monitor(h, h->mon_cur + slot++);
         if (h->prec == FDTD_FP16 && n % h->fp16_interval == 0) fp16_maintain(h, h->stream);
     }
     HIPCHK(hipGetLastError());
@@ -1368,6 +1418,11 @@ static int step_impl(fdtd_handle h, long n0, int nsteps, const double* src_value
 
 int fdtd_step(fdtd_handle h, long n0, int nsteps, const double* src_values) {
     return step_impl(h, n0, nsteps, src_values, nullptr);
+}
+
+int fdtd_step_noalloc(fdtd_handle h, long n0, int nsteps, const double* src_values) {
+    if (int rc = step_launch(h, n0, nsteps, src_values, nullptr, 1, true)) return rc;
+    return step_finish(h);
 }
 
 int fdtd_step_async(fdtd_handle h, long n0, int nsteps, const double* src_values) {

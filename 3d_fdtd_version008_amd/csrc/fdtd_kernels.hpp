@@ -1057,6 +1057,7 @@ struct MonOut {
     unsigned long long maxE, maxH, maxP, nanN;
     double pop_total, pop_init;
 };
+static_assert(sizeof(MonOut) == 48, "monitor record layout (48 B, six 64-bit words)");
 
 // One monitor record per work-group: wavefront reduction (64 lanes), then
 // across the work-group's wavefronts in LDS in wavefront order.  Every thread
@@ -1253,6 +1254,12 @@ __global__ void __launch_bounds__(64) k_monitor_final(const MonOut* __restrict__
         pi += __shfl_down(pi, off);
     }
     if (threadIdx.x == 0) *slot = MonOut{mE, mH, mP, nN, pt, pi};
+}
+
+// Zeroes n 64-bit words: a kernel rather than a memset node, so that the
+// clear also holds inside a captured graph.
+__global__ void __launch_bounds__(256) k_zero_words(unsigned long long* __restrict__ p, long n) {
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256L) p[i] = 0ull;
 }
 
 // ---- Multi-domain (batch) step ----

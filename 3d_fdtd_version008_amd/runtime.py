@@ -35,6 +35,7 @@ EXPORTS = [
     "fdtd_set_sources", "fdtd_add_probe",
     "fdtd_add_dft", "fdtd_set_monitor", "fdtd_step", "fdtd_synchronize", "fdtd_download", "fdtd_upload",
     "fdtd_read_probe", "fdtd_read_dft", "fdtd_read_check", "fdtd_info", "fdtd_step_timed", "fdtd_step_timed_every",
+    "fdtd_step_noalloc",
     "fdtd_scene_step_timed_every",
     "fdtd_setup_build", "fdtd_setup_free", "fdtd_setup_info", "fdtd_setup_dt", "fdtd_setup_array",
     "fdtd_setup_source_values", "fdtd_setup_probe", "fdtd_scene_create", "fdtd_scene_destroy", "fdtd_scene_step",
@@ -111,6 +112,7 @@ def load() -> C.CDLL:
         "fdtd_scene_step_timed": (i, [vp, i, vp]),
         "fdtd_scene_step_timed_every": (i, [vp, i, i, vp]),
         "fdtd_step_timed_every": (i, [vp, l, i, vp, i, vp]),
+        "fdtd_step_noalloc": (i, [vp, l, i, vp]),
         "fdtd_scene_step_async": (i, [vp, i]),
         "fdtd_scene_step_wait": (i, [vp]),
         "fdtd_step_async": (i, [vp, l, i, vp]),

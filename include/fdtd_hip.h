@@ -247,6 +247,18 @@ int fdtd_step_timed(fdtd_handle h, long n0, int nsteps, const double* src_values
  * fdtd_step while the kernel averages stay live measurements.  No
  * counterpart in the reference. */
 int fdtd_step_timed_every(fdtd_handle h, long n0, int nsteps, const double* src_values, int every, double* ms);
+/* fdtd_step that allocates no device memory, for callers that cannot call
+ * hipMalloc (code under graph capture, tight serving loops, harnesses that
+ * forbid allocation).  No counterpart in the reference.  fdtd_step grows
+ * two per-handle buffers on demand (the call's source values and its
+ * monitor records); here both live in one static device array per GPU of
+ * FDTD_NOALLOC_SCRATCH_BYTES, records first (48 B per monitor step, rounded
+ * up to 256 B) then nsteps * n_sources doubles.  A call needing more returns
+ * FDTD_EINVAL without stepping.  Results and return codes are those of
+ * fdtd_step; synchronizes.  Every call on one GPU shares that GPU's array,
+ * so calls that overlap on one GPU (from several threads) race. */
+#define FDTD_NOALLOC_SCRATCH_BYTES 1048576
+int fdtd_step_noalloc(fdtd_handle h, long n0, int nsteps, const double* src_values);
 
 /* ---- data movement (reference layout, FP64 on the host side) ----
  * which: 0..5 = Ex,Ey,Ez,Hx,Hy,Hz; 6 = Nu, 7 = Ng, 8 = Pz, 9 = Pz_prev
