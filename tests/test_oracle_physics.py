@@ -1,5 +1,5 @@
 """Checks of the oracle against what the mathematics fixes, independent of the
-reference: closed-form CPML profiles and source waveforms, the explicit-Euler
+reference: the Courant time step, closed-form CPML profiles and source waveforms, the explicit-Euler
 free decay of the upper population, discrete div H = 0 of the Yee update,
 and the DFT monitor (which has no reference counterpart) against a direct
 sum over a probe trace and against the closed-form DFT of a sampled
@@ -151,3 +151,35 @@ def test_dft_of_sampled_sinusoid_closed_form():
 
     closed = 0.5 * (geo(2 * math.pi * (f0 - f)) + geo(-2 * math.pi * (f0 + f)))
     assert abs(acc - closed) <= 1e-9 * abs(closed)
+
+
+def test_dt_closed_form(pkg, oracle_mod):
+    """make_context's time step (src/params.hpp:264-265) is the 3-D Courant
+    limit scaled by S over the smallest spacing of each axis: on a uniform
+    cube dt = S d / (c0 sqrt 3); on an anisotropic uniform grid and on the
+    graded z axis it is S / (c0 sqrt(sum 1/d_min^2)), with d_min taken from
+    the scene's parameters (not from the spacing arrays the setup builds).
+    The product's setup, the oracle and the compiled reference harness must
+    all give it; a wrong axis, a max in place of the min or a dropped square
+    moves dt by far more than the 4-ulp bar."""
+    import dataclasses
+
+    d = 20e-9
+    cube = pkg.scene.uniform_sphere_case(core=16, npml=4, d=d, steps=2)
+    aniso = dataclasses.replace(cube, dx_uniform=10e-9, dy_uniform=20e-9, dz_uniform=35e-9)
+    graded = pkg.scene.gain_slab_case(n_total=64, npml=8, steps=2, lateral=20e-9, fine=5e-9, coarse=20e-9,
+                                      fine_len=100e-9)
+    cases = [
+        (cube, cube.S * d / (C0 * math.sqrt(3.0))),
+        (aniso, aniso.S / (C0 * math.sqrt(1 / 10e-9 ** 2 + 1 / 20e-9 ** 2 + 1 / 35e-9 ** 2))),
+        (graded, graded.S / (C0 * math.sqrt(2 / 20e-9 ** 2 + 1 / 5e-9 ** 2))),
+    ]
+    impls = ["oracle"] + (["ref"] if oracle_mod.available("ref") else [])
+    for sp, want in cases:
+        got = {"setup": pkg.Setup(sp).dt}
+        for impl in impls:
+            got[impl] = oracle_mod.CpuSim(sp, impl).dt
+        for who, v in got.items():
+            assert abs(v - want) <= 4 * math.ulp(want), (sp.name, who, v, want)
+        # and below the 1-D limit of the finest axis
+        assert max(got.values()) < min(sp.dx_uniform, sp.dy_uniform, sp.dz_uniform, 5e-9 if sp is graded else 1.0) / C0
