@@ -263,11 +263,14 @@ def _probe_errors(a, b, spec):
 
 
 def precision_form(args):
-    """FP32 and FP16-storage throughput on the ~100M-cell precision geometry,
-    and their error against the FP64 device path (the oracle at this size,
+    """FP32 and FP16-storage throughput on the ~100M-cell precision geometry
+    (or, with --workload c2, the 256^3 graded gain slab), and their error against the FP64 device path (the oracle at this size,
     itself pinned to the CPU oracle at smaller sizes) after args.steps steps."""
     pkg = importlib.import_module("3d_fdtd_version008_amd")
-    spec = pkg.scene.precision_case(steps=args.steps)
+    if args.workload == "c3":
+        spec = pkg.scene.precision_case(steps=args.steps)
+    else:
+        spec = pkg.scene.gain_slab_case(n_total=args.size, npml=12, steps=args.steps)
     t0 = time.time()
     setup = pkg.Setup(spec)
     t_setup = time.time() - t0
