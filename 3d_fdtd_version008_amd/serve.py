@@ -33,6 +33,8 @@ No reference counterpart (the reference runs one simulation per process).
 from __future__ import annotations
 
 import concurrent.futures as cf
+import dataclasses
+import threading
 import time
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional
@@ -76,6 +78,21 @@ class Result:
     @property
     def latency(self) -> float:
         return self.finished - self.arrival
+
+
+class _Inline:
+    """Executor stand-in that runs each task at once on the calling thread."""
+
+    def submit(self, fn, *a):
+        f = cf.Future()
+        try:
+            f.set_result(fn(*a))
+        except BaseException as e:  # handed to the caller through the future
+            f.set_exception(e)
+        return f
+
+    def shutdown(self, wait=True):
+        pass
 
 
 @dataclass
@@ -125,12 +142,23 @@ class Server:
 
     def __init__(self, budget_bytes: float = 200e9, max_active: int = 256, tick: int = 50, graph: bool = True,
                  setup_workers: int = 8, tuning: Optional[tuple] = None, engine: str = "streams",
-                 small_cells: int = 2_000_000, enqueue_workers: int = 4):
+                 small_cells: int = 2_000_000, enqueue_workers: int = 4, stream_workers: int = 0,
+                 defer_free: bool = False, host_bake: bool = False):
         if engine not in ("streams", "batch", "hybrid"):
             raise ValueError("engine must be 'streams', 'batch' or 'hybrid'")
         self.engine = engine
         self.small_cells = small_cells
         self.enqueue_workers = enqueue_workers
+        # streams engine: threads that make the per-request device calls
+        # (0: the loop thread makes them itself)
+        self.stream_workers = stream_workers
+        # streams engine: a finished request is read back at once but its
+        # device memory is released only when admission needs the room (or
+        # at the end), because a device free waits for the whole device
+        self.defer_free = defer_free
+        # bake materials on the host in the setup pool instead of on the
+        # device at admission (the device bake frees its temporaries)
+        self.host_bake = host_bake
         self.budget = int(budget_bytes)
         self.max_active = max_active
         self.tick = tick
@@ -159,28 +187,80 @@ class Server:
         return self._run_ticks(requests)
 
     def _run_streams(self, requests: List[Request]) -> List[Result]:
+        """Streams engine.  The loop polls the resident requests' streams
+        (fdtd_step_query never blocks); a request whose steps have run gets
+        the stability check of those steps and the enqueue of its next tick,
+        or, once finished, its read-back and release.  With `stream_workers`
+        > 0 these calls (and scene creation at admission) go to a pool of
+        that many threads: the library's calls release the interpreter lock
+        and touch only their own handle, so the host cost of kernel launches,
+        uploads and frees is spread over the workers; with 0 the loop thread
+        makes them in turn."""
         reqs = sorted(requests, key=lambda r: r.arrival)
         pool = cf.ThreadPoolExecutor(max_workers=self.setup_workers)
+        work = cf.ThreadPoolExecutor(max_workers=self.stream_workers) if self.stream_workers > 0 else _Inline()
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0  # noqa: E731
         nxt = 0
         setting_up: List[cf.Future] = []
         ready: List[tuple] = []
+        admitting: List[tuple] = []    # (future, estimate)
         live: List[_Live] = []
+        busy: Dict[int, cf.Future] = {}  # id(_Live) -> its tick hand-off in flight
+        retiring: List[tuple] = []     # (future, _Live)
+        released: List[_Live] = []     # read back, device memory not yet freed
+        freeing: List[tuple] = []      # (future, bytes) of frees in flight
         used = 0
         done: List[Result] = []
-        st = {"enqueue_s": 0.0, "admit_s": 0.0, "retire_s": 0.0, "idle_s": 0.0, "ticks": 0, "polls": 0}
+        st = {"enqueue_s": 0.0, "admit_s": 0.0, "retire_s": 0.0, "free_s": 0.0, "idle_s": 0.0, "ticks": 0,
+              "polls": 0}
         self.stats = st
+        lock = threading.Lock()
 
-        def enqueue(lv: _Live):
-            te = now()
-            lv.scene.step_async(min(self.tick, lv.req.steps - lv.scene.steps_done()))
-            st["enqueue_s"] += now() - te
-            st["ticks"] += 1
+        def count(key, v):
+            with lock:
+                st[key] += v
+
+        def timed(key, fn, *a):
+            ts = time.perf_counter()
+            try:
+                return fn(*a)
+            finally:
+                count(key, time.perf_counter() - ts)
+
+        def admit(req, setup):
+            ta = now()
+            try:
+                sc = runtime.Scene(setup, req.precision, probe_capacity=req.steps + 16)
+            except runtime.FdtdError as e:
+                return self._refused(req, setup, ta, e)
+            lv = _Live(req, setup, sc, req.steps, now(), sc.device_bytes())
+            try:
+                sc.step_async(min(self.tick, req.steps))
+                count("ticks", 1)
+            except runtime.FdtdError as e:
+                return self._retire(lv, now(), e)
+            return lv
+
+        def advance(lv):
+            """The steps enqueued last have run: apply the stability policy,
+            then enqueue the next tick; returns None, or the error that
+            stopped the request, or True when the request has finished."""
+            try:
+                lv.scene.step_wait()
+                left = lv.req.steps - lv.scene.steps_done()
+                if left <= 0:
+                    return True
+                lv.scene.step_async(min(self.tick, left))
+                count("ticks", 1)
+                return None
+            except runtime.FdtdError as e:
+                return e
 
         try:
-            while nxt < len(reqs) or setting_up or ready or live:
+            while nxt < len(reqs) or setting_up or ready or admitting or live or retiring or freeing:
                 t = now()
+                progressed = False
                 while nxt < len(reqs) and reqs[nxt].arrival <= t:
                     setting_up.append(pool.submit(self._prepare, reqs[nxt]))
                     nxt += 1
@@ -194,57 +274,91 @@ class Server:
                 ready.sort(key=lambda x: x[0].arrival)
                 waiting = []
                 for req, setup, est in ready:
-                    if len(live) < self.max_active and used + est <= self.budget:
-                        ta = now()
-                        try:
-                            sc = runtime.Scene(setup, req.precision, probe_capacity=req.steps + 16)
-                        except runtime.FdtdError as e:
-                            done.append(self._refused(req, setup, ta, e))
-                            continue
-                        lv = _Live(req, setup, sc, req.steps, now(), sc.device_bytes())
-                        live.append(lv)
-                        used += lv.device_bytes
-                        st["admit_s"] += now() - ta
-                        enqueue(lv)
+                    if len(live) + len(admitting) < self.max_active and used + est <= self.budget:
+                        used += est  # charged at the estimate until the scene exists
+                        admitting.append((work.submit(timed, "admit_s", admit, req, setup), est))
                     else:
                         waiting.append((req, setup, est))
                 ready = waiting
-                progressed = False
+                if released and any(used + est > self.budget for _, _, est in ready):
+                    # a request waits for memory: free what finished requests hold
+                    nbytes = sum(x.device_bytes for x in released)
+                    freeing.append((work.submit(timed, "free_s", lambda xs: [x.scene.close() for x in xs],
+                                                released), nbytes))
+                    released = []
+                still = []
+                for f, nbytes in freeing:
+                    if f.done():
+                        f.result()
+                        used -= nbytes
+                        progressed = True
+                    else:
+                        still.append((f, nbytes))
+                freeing = still
+                still = []
+                for f, est in admitting:
+                    if not f.done():
+                        still.append((f, est))
+                        continue
+                    progressed = True
+                    used -= est
+                    r = f.result()
+                    if isinstance(r, Result):
+                        done.append(r)
+                    else:
+                        live.append(r)
+                        used += r.device_bytes
+                admitting = still
                 keep = []
                 for lv in live:
-                    st["polls"] += 1
-                    if not lv.scene.step_ready():
+                    f = busy.get(id(lv))
+                    if f is None:
+                        st["polls"] += 1
+                        if lv.scene.step_ready():
+                            busy[id(lv)] = work.submit(timed, "enqueue_s", advance, lv)
+                            progressed = True
+                        keep.append(lv)
+                        continue
+                    if not f.done():
                         keep.append(lv)
                         continue
                     progressed = True
-                    try:
-                        lv.scene.step_wait()  # the steps have run: applies the stability policy
-                    except runtime.FdtdError as e:
-                        tr = now()
-                        done.append(self._retire(lv, tr, e))
-                        used -= lv.device_bytes
-                        st["retire_s"] += now() - tr
-                        continue
-                    if lv.scene.steps_done() >= lv.req.steps:
-                        tr = now()
-                        done.append(self._retire(lv, tr))
-                        used -= lv.device_bytes
-                        st["retire_s"] += now() - tr
-                    else:
-                        enqueue(lv)
+                    del busy[id(lv)]
+                    r = f.result()
+                    if r is None:
                         keep.append(lv)
+                        continue
+                    err = None if r is True else r
+                    retiring.append((work.submit(timed, "retire_s", lambda x, e: self._retire(
+                        x, now(), e, close=not self.defer_free), lv, err), lv))
                 live = keep
+                still = []
+                for f, lv in retiring:
+                    if f.done():
+                        done.append(f.result())
+                        if self.defer_free:
+                            released.append(lv)
+                        else:
+                            used -= lv.device_bytes
+                        progressed = True
+                    else:
+                        still.append((f, lv))
+                retiring = still
                 if not progressed:
                     ti = now()
-                    if not live and setting_up:
+                    quiet = not live and not admitting and not retiring and not freeing
+                    if quiet and setting_up:
                         cf.wait(setting_up, timeout=0.05, return_when=cf.FIRST_COMPLETED)
-                    elif not live and nxt < len(reqs):
+                    elif quiet and nxt < len(reqs):
                         time.sleep(max(0.0, min(0.05, reqs[nxt].arrival - now())))
                     else:
                         time.sleep(0.0002)
                     st["idle_s"] += now() - ti
         finally:
+            work.shutdown(wait=True)
             pool.shutdown(wait=True)
+            for x in released:
+                x.scene.close()
         return done
 
     def _run_ticks(self, requests: List[Request]) -> List[Result]:
@@ -361,9 +475,9 @@ class Server:
             self.batches = {}
         return done
 
-    @staticmethod
-    def _prepare(req: Request):
-        setup = runtime.Setup(req.spec)
+    def _prepare(self, req: Request):
+        spec = dataclasses.replace(req.spec, device_bake=0) if self.host_bake else req.spec
+        setup = runtime.Setup(spec)
         return req, setup, estimate_device_bytes(setup, req.precision)
 
     @staticmethod
@@ -373,8 +487,9 @@ class Server:
                       cells=setup.shape[0] * setup.shape[1] * setup.shape[2], arrival=req.arrival, admitted=t,
                       finished=t, device_bytes=0, status="error", error=str(err))
 
-    def _retire(self, lv: _Live, t_end: float, err: Optional[Exception] = None) -> Result:
-        """Reads back a finished (or failed) request and frees its device state."""
+    def _retire(self, lv: _Live, t_end: float, err: Optional[Exception] = None, close: bool = True) -> Result:
+        """Reads back a finished (or failed) request and (with `close`) frees
+        its device state."""
         if lv.batched:
             self.batches[lv.req.precision].remove(lv.scene)
         st = lv.setup
@@ -390,7 +505,8 @@ class Server:
         res.dfts = [lv.scene.dft(m) for m in range(st.n_dfts)]
         if lv.req.keep_fields:
             res.fields = {f: lv.scene.field(f) for f in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")}
-        lv.scene.close()
+        if close:
+            lv.scene.close()
         return res
 
 
