@@ -1273,10 +1273,15 @@ __global__ void __launch_bounds__(256) k_zero_words(unsigned long long* __restri
 // of steps can be captured once in a HIP graph and replayed.  Each cell goes
 // through the same operations as in the single-domain step, so a domain's
 // results do not depend on the batch it runs in.
+// The tile list of a batch pass is cut into BATCH_SEGS contiguous segments,
+// one per XCD: a work-group takes tiles from its own XCD's segment first, so
+// neighbouring tiles (whose rows and planes overlap as halos) run under one
+// L2, and moves on to the other segments when its own is done.
+constexpr int BATCH_SEGS = 8;
 struct BatchCtl {
-    int ctr_h, ctr_e;  // work counters of the persistent H and E launches
-    int lstep;         // step index within the current block
-    int pad;
+    int ctr[2][BATCH_SEGS][16];  // tiles taken per pass (H, E) and segment, one 64-byte line each
+    int lstep;                   // step index within the current block
+    int pad[15];
 };
 struct BatchDom {
     long n0;         // global step index of the block's first step
@@ -1299,10 +1304,12 @@ struct BatchDft {
 };
 
 __global__ void k_batch_reset(BatchCtl* c) {
-    if (threadIdx.x == 0) { c->ctr_h = 0; c->ctr_e = 0; c->lstep = -1; }
+    for (int q = threadIdx.x; q < 2 * BATCH_SEGS * 16; q += blockDim.x) (&c->ctr[0][0][0])[q] = 0;
+    if (threadIdx.x == 0) c->lstep = -1;
 }
 __global__ void k_batch_begin(BatchCtl* c) {
-    if (threadIdx.x == 0) { c->ctr_h = 0; c->ctr_e = 0; c->lstep += 1; }
+    for (int q = threadIdx.x; q < 2 * BATCH_SEGS * 16; q += blockDim.x) (&c->ctr[0][0][0])[q] = 0;
+    if (threadIdx.x == 0) c->lstep += 1;
 }
 
 // Domain of tile t: cum[d] <= t < cum[d+1].
@@ -1324,13 +1331,23 @@ __global__ void __launch_bounds__(256) k_batch_step(const StepArgs<T, R>* __rest
     const int ty = __builtin_amdgcn_readfirstlane(threadIdx.y);
     const int total = cum[ndom];
     const int lstep = ctl->lstep;
+    // work-groups are dealt to the XCDs round-robin by id
+    const int home = (int)(blockIdx.x % BATCH_SEGS);
+    int seg = home, lo = (int)((long)total * seg / BATCH_SEGS), hi = (int)((long)total * (seg + 1) / BATCH_SEGS);
     for (;;) {
-        if (threadIdx.x == 0 && threadIdx.y == 0) first_s = atomicAdd(EPASS ? &ctl->ctr_e : &ctl->ctr_h, chunk);
+        if (threadIdx.x == 0 && threadIdx.y == 0) first_s = lo + atomicAdd(&ctl->ctr[EPASS ? 1 : 0][seg][0], chunk);
         __syncthreads();
         const int first = __builtin_amdgcn_readfirstlane(first_s);  // one value for the work-group
         __syncthreads();
-        if (first >= total) return;
-        const int last = min(first + chunk, total);
+        if (first >= hi) {
+            // this segment is done: the next one, until all are
+            seg = (seg + 1) % BATCH_SEGS;
+            if (seg == home) return;
+            lo = (int)((long)total * seg / BATCH_SEGS);
+            hi = (int)((long)total * (seg + 1) / BATCH_SEGS);
+            continue;
+        }
+        const int last = min(first + chunk, hi);
         int d = batch_domain(cum, ndom, first);
         for (int t = first; t < last; ++t) {
             while (t >= cum[d + 1]) ++d;
