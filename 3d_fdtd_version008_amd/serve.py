@@ -112,9 +112,11 @@ class _Live:
 
 def estimate_device_bytes(setup: runtime.Setup, precision: str) -> int:
     """Upper estimate of a scene's device footprint before it is created:
-    double-buffered fields, psi slabs (6 arrays per axis: E in place, H
-    double-buffered), material ids and axis tables, gain state in its
-    bounding box, probe and DFT buffers, and 1 MiB for small tables."""
+    the fields (one buffer: the two-kernel step updates in place; the fused
+    schedule's second buffer is allocated only when that schedule is
+    chosen), psi slabs (4 arrays per axis), material ids and axis tables,
+    gain state in its bounding box, probe and DFT buffers, and 1 MiB for
+    small tables."""
     s = {"fp64": 8, "fp32": 4, "fp16": 2}[precision]
     nx, ny, nz = setup.shape
     n = nx * ny * nz
@@ -130,7 +132,7 @@ def estimate_device_bytes(setup: runtime.Setup, precision: str) -> int:
     probes = setup.n_channels * 8 * (setup.spec.n_steps + 16)
     dfts = sum(len(d.freqs) * 16 * (d.box[1] - d.box[0]) * (d.box[3] - d.box[2]) * (d.box[5] - d.box[4])
                for d in setup.spec.dfts)
-    return int(12 * s * n + 2 * n + 6 * s * slab + gain + probes + dfts + (1 << 20))
+    return int(6 * s * n + 2 * n + 4 * s * slab + gain + probes + dfts + (1 << 20))
 
 
 class Server:

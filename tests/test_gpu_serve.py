@@ -41,6 +41,9 @@ def test_served_results_equal_alone(pkg, engine):
     spans = sorted((x.admitted, x.finished) for x in served.values())
     overlap = any(spans[q + 1][0] < spans[q][1] for q in range(len(spans) - 1))
     assert overlap
+    # the admission estimate is an upper bound of the footprint it charges
+    for r, e in zip(reqs, est):
+        assert served[r.rid].device_bytes <= e, (r.rid, served[r.rid].device_bytes, e)
     # the budget bounded the resident footprint
     for x in served.values():
         live = sum(y.device_bytes for y in served.values() if y.admitted <= x.admitted < y.finished)
