@@ -10,7 +10,13 @@ centre: same cell spacing, dt and source values, and zeros elsewhere.  The
 probe traces (6 cells x 6 components), the DFT accumulators on an 8^3 box
 round the dipole and the monitor maxima (over all 2.3e9 cells) of the two
 grids must be bitwise equal.  The dipole sits at cell (1290, 1290, 1290),
-whose offset 2.25e9 exceeds 2^31."""
+whose offset 2.25e9 exceeds 2^31.
+
+A 1640^3 grid (4.41e9 cells) also takes more than 2^32 threads per H or E
+launch (17.5M tiles of 256), so its planes go into the second grid
+dimension (every work-group but the first plane's 10,660 has blockIdx.y >
+0); its dipole at (1610, 1610, 1610) has the offset 4.33e9, beyond 2^32.
+The same bitwise bar holds."""
 import sys
 
 import numpy as np
@@ -61,6 +67,21 @@ def test_grid_beyond_2_31_cells_matches_small_grid(pkg, precision):
     n, src = 1320, (1290, 1290, 1290)
     assert n ** 3 > 2 ** 31 and (src[0] * n + src[1]) * n + src[2] > 2 ** 31
     assert src[0] + 17 < n - NPML and src[0] - 17 > NPML  # the cone stays clear of the CPML
+    small = _run(pkg, 64, (32, 32, 32), precision)
+    big = _run(pkg, n, src, precision)
+    assert len(small["checks"]) == 2 and small["checks"] == big["checks"], (small["checks"], big["checks"])
+    assert small["checks"][-1][1] > 0
+    for c, (a, b) in enumerate(zip(small["probes"], big["probes"])):
+        assert len(a) == STEPS and np.array_equal(a, b), f"probe channel {c}"
+    assert np.abs(small["dft"]).max() > 0 and np.array_equal(small["dft"], big["dft"])
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp16"])
+def test_launch_beyond_2_32_threads_matches_small_grid(pkg, precision):
+    n, src = 1640, (1610, 1610, 1610)
+    tiles = ((n + 63) // 64) * ((n + 3) // 4) * n
+    assert tiles * 256 >= 2 ** 32 and (src[0] * n + src[1]) * n + src[2] > 2 ** 32
+    assert src[0] + 17 < n - NPML and src[0] - 17 > NPML
     small = _run(pkg, 64, (32, 32, 32), precision)
     big = _run(pkg, n, src, precision)
     assert len(small["checks"]) == 2 and small["checks"] == big["checks"], (small["checks"], big["checks"])
