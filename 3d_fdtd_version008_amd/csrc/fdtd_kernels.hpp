@@ -32,10 +32,11 @@ namespace fdtdk {
 // the array's scale between two scale checks is clipped (and the next check
 // raises the scale) instead of turning into Inf.  A NaN or an Inf is stored
 // as it is, so the stability monitor sees it as the reference's NaN/Inf scan
-// would (comparisons, not fminf/fmaxf, which return the non-NaN operand).
+// would: a finite value goes through the median-of-three clamp, anything
+// else (one class test) passes unchanged (fminf/fmaxf alone would turn a NaN
+// into a bound).
 __device__ __forceinline__ float fp16_sat(float x) {
-    constexpr float big = 3.402823466e38f;  // the largest finite float
-    return (x > 65504.0f && x <= big) ? 65504.0f : ((x < -65504.0f && x >= -big) ? -65504.0f : x);
+    return __builtin_isfinite(x) ? __builtin_amdgcn_fmed3f(x, -65504.0f, 65504.0f) : x;
 }
 
 // Storage <-> compute conversion.  FP16 storage carries a power-of-two scale
