@@ -775,7 +775,8 @@ int launch_step(fdtd_sim* s, long n, int sidx, hipEvent_t* ev) {
         L.fz = fdtdk::make_fastdiv((unsigned)da.nz_b);
         L.fy = fdtdk::make_fastdiv((unsigned)da.ny_b);
         L.bpf = (unsigned)((d.ncell + 255) / 256);
-        const unsigned nb = L.bpf * (unsigned)d.nf + (pa.n ? 1u : 0u);
+        L.per_cell = da.nz_b == 1 && d.nf > 1;
+        const unsigned nb = L.bpf * (L.per_cell ? 1u : (unsigned)d.nf) + (pa.n ? 1u : 0u);
         hipLaunchKernelGGL((fdtdk::k_observe<T, R>), dim3(nb), dim3(256), 0, s->stream, a, L, pa);
         pa.n = 0;
     }

@@ -1017,21 +1017,41 @@ __device__ __forceinline__ void dft_cell(const StepArgs<T, R>& a, const DftArgs&
 }
 
 // One step's observers on one domain in one launch: DftMonitor accumulation
-// (one thread per cell and frequency: blocks [f * bpf, (f + 1) * bpf) hold
-// frequency f) and, in the last block when pa.n > 0, the point probes.  Each
-// accumulator gets the same += F * w as in dft_cell.
+// and, in the last block when pa.n > 0, the point probes.  Each accumulator
+// gets the same += F * w as in dft_cell.  A box with a k extent reads its
+// field along k, coalesced: one thread per cell and frequency (blocks
+// [f * bpf, (f + 1) * bpf) hold frequency f).  A box one cell thick in k (a
+// z-normal plane) reads one value per nz-strided line: one thread per cell
+// then reads it once for every frequency (`per_cell`).
 struct DftLaunch {
     DftArgs d;
     FastDiv fz, fy;  // divisions by nz_b and ny_b of the box
-    unsigned bpf;    // blocks per frequency
+    unsigned bpf;    // blocks per frequency (per_cell: blocks of the box)
+    unsigned per_cell;
 };
 
 template <typename T, typename R>
 __global__ void __launch_bounds__(256) k_observe(StepArgs<T, R> a, DftLaunch L, ProbeArgs pa) {
-    const unsigned nb_dft = L.bpf * (unsigned)L.d.nf;
+    const unsigned nb_dft = L.per_cell ? L.bpf : L.bpf * (unsigned)L.d.nf;
     if (blockIdx.x >= nb_dft) {
         const int t = threadIdx.x;
         if (t < pa.n) *pa.p[t].out = field_value<T, R>(a, pa.p[t].comp, pa.p[t].id);
+        return;
+    }
+    if (L.per_cell) {
+        const unsigned c = blockIdx.x * blockDim.x + threadIdx.x;
+        if (c >= (unsigned)L.d.ncell) return;
+        const unsigned r = fdiv(c, L.fz);
+        const unsigned ii = fdiv(r, L.fy);
+        const int kk = (int)(c - r * (unsigned)L.d.nz_b), jj = (int)(r - ii * (unsigned)L.d.ny_b);
+        const long id = ((long)(L.d.i0 + (int)ii) * a.ny + (L.d.j0 + jj)) * a.nz + (L.d.k0 + kk);
+        const double F = field_value<T, R>(a, L.d.comp, id);
+        for (int f = 0; f < L.d.nf; ++f) {
+            double* p = L.d.acc + ((long)f * L.d.ncell + c) * 2;
+            const double re = p[0], im = p[1];
+            p[0] = re + F * L.d.wr[f];
+            p[1] = im + F * L.d.wi[f];
+        }
         return;
     }
     const unsigned f = blockIdx.x / L.bpf;
