@@ -138,6 +138,7 @@ struct fdtd_sim {
     double* d_srcv = nullptr;
     long srcv_cap = 0;
     int device = 0;                // the device the handle was created on
+    bool mon_fused_h = true;       // monitor field maxima in the next H launch (else a separate pass)
     double* srcv_cur = nullptr;    // source values of the call in flight
     MonOut* mon_cur = nullptr;     // monitor records of the call in flight
     // probes / dft
@@ -148,6 +149,12 @@ struct fdtd_sim {
     MonOut* d_mon = nullptr;
     int mon_cap = 0;
     MonOut* d_monpart = nullptr;  // per-block partial records of the monitor reduction
+    // Field maxima of a monitor step taken by the next step's H launch
+    // (k_update_H_mon): their slots, the record they complete, and the
+    // number of gain partial records in d_monpart for it.
+    unsigned long long* d_fmax = nullptr;
+    MonOut* mon_defer = nullptr;
+    int mon_defer_nb = 0;
     std::vector<fdtd_check_record> checks;
     double prev_max_E = 0.0;
     size_t inst_count = 0;
@@ -681,6 +688,8 @@ int ensure_second(fdtd_sim* s) {
 constexpr int TLS_PARTS = 256;  // blocks of the TLS diagnostic reduction
 constexpr long MONPART_MAX = 2048;  // blocks (and partial records) of the monitor reduction
 
+void dispatch_monitor(fdtd_sim* s, MonOut* slot);
+
 template <typename T, typename R>
 int launch_step(fdtd_sim* s, long n, int sidx, hipEvent_t* ev) {
     StepArgs<T, R> a = make_args<T, R>(s);
@@ -690,6 +699,10 @@ int launch_step(fdtd_sim* s, long n, int sidx, hipEvent_t* ev) {
         a.Exw = (T*)s->F2[o][0]; a.Eyw = (T*)s->F2[o][1]; a.Ezw = (T*)s->F2[o][2];
         a.Hxw = (T*)s->F2[o][3]; a.Hyw = (T*)s->F2[o][4]; a.Hzw = (T*)s->F2[o][5];
         for (int q = 6; q < 12; ++q) a.psiw[q] = (T*)s->psi2[o][q];
+        if (s->mon_defer) {  // the fused step takes no monitor maxima: the full pass, before it
+            dispatch_monitor(s, s->mon_defer);
+            s->mon_defer = nullptr;
+        }
         if (ev) (void)hipEventRecord(ev[0], s->stream);
         launch_fused<T, R>(s, a);
         if (ev) (void)hipEventRecord(ev[1], s->stream);
@@ -715,7 +728,18 @@ int launch_step(fdtd_sim* s, long n, int sidx, hipEvent_t* ev) {
             const unsigned long ntile = (unsigned long)a.tk.d * a.tj.d * (unsigned long)n;
             const dim3 grd = ntile * blk.x < (1ul << 32) ? dim3((unsigned)ntile) : dim3(a.tk.d * a.tj.d, (unsigned)n);
             a.rev = 0;
-            switch (kwid * 2 + (o32 ? 1 : 0)) {
+            // after a monitor step: the H launch also takes its field maxima
+            unsigned long long* fm = s->mon_defer ? s->d_fmax : nullptr;
+            if (fm) {
+                switch (kwid * 2 + (o32 ? 1 : 0)) {
+                case 129: hipLaunchKernelGGL((fdtdk::k_update_H_mon<T, R, 64, unsigned>), grd, blk, 0, s->stream, a, fm); break;
+                case 128: hipLaunchKernelGGL((fdtdk::k_update_H_mon<T, R, 64>), grd, blk, 0, s->stream, a, fm); break;
+                case 65: hipLaunchKernelGGL((fdtdk::k_update_H_mon<T, R, 32, unsigned>), grd, blk, 0, s->stream, a, fm); break;
+                case 64: hipLaunchKernelGGL((fdtdk::k_update_H_mon<T, R, 32>), grd, blk, 0, s->stream, a, fm); break;
+                case 33: hipLaunchKernelGGL((fdtdk::k_update_H_mon<T, R, 16, unsigned>), grd, blk, 0, s->stream, a, fm); break;
+                default: hipLaunchKernelGGL((fdtdk::k_update_H_mon<T, R, 16>), grd, blk, 0, s->stream, a, fm); break;
+                }
+            } else switch (kwid * 2 + (o32 ? 1 : 0)) {
             case 129: hipLaunchKernelGGL((fdtdk::k_update_H<T, R, 64, unsigned>), grd, blk, 0, s->stream, a); break;
             case 128: hipLaunchKernelGGL((fdtdk::k_update_H<T, R, 64>), grd, blk, 0, s->stream, a); break;
             case 65: hipLaunchKernelGGL((fdtdk::k_update_H<T, R, 32, unsigned>), grd, blk, 0, s->stream, a); break;
@@ -790,6 +814,11 @@ int launch_step(fdtd_sim* s, long n, int sidx, hipEvent_t* ev) {
                            (const fdtdk::TLSCoef*)s->d_tls, s->tls_gmon, s->d_tlsrec + 5 * s->tls_count);
         s->tls_count++;
     }
+    if (s->mon_defer) {  // the previous step's monitor record: gain partials and field-maxima slots
+        hipLaunchKernelGGL(fdtdk::k_monitor_final, dim3(1), dim3(64), 0, s->stream, (const MonOut*)s->d_monpart,
+                           s->mon_defer_nb, s->mon_defer, (const unsigned long long*)s->d_fmax, fdtdk::MON_SLOTS);
+        s->mon_defer = nullptr;
+    }
     return 0;
 }
 
@@ -809,11 +838,38 @@ void launch_monitor(fdtd_sim* s, MonOut* slot) {
         hipLaunchKernelGGL((fdtdk::k_monitor<T, R, true>), dim3((unsigned)nb), dim3(256), 0, s->stream, a, s->N, ng,
                            s->d_monpart);
         hipLaunchKernelGGL(fdtdk::k_monitor_final, dim3(1), dim3(64), 0, s->stream, (const MonOut*)s->d_monpart,
-                           (int)nb, slot);
+                           (int)nb, slot, (const unsigned long long*)nullptr, 0);
     } else {
         const long nb = std::min<long>(std::max<long>((s->N + 2047) / 2048, 1), 2048);
         hipLaunchKernelGGL((fdtdk::k_monitor<T, R, false>), dim3((unsigned)nb), dim3(256), 0, s->stream, a, s->N, ng, slot);
     }
+}
+
+// The monitor of a step whose next step runs in the same call on the
+// two-kernel path: the gain part now (k_monitor over no field cells, with the
+// full pass's block decomposition, so its partial records are the same), the
+// field maxima in the next H launch (k_update_H_mon), merged after it.
+// false where that does not apply (then the full pass runs).
+template <typename T, typename R>
+bool launch_monitor_begin(fdtd_sim* s, MonOut* slot) {
+    if (s->mode == 1 && fused_ok(s) && s->F2[s->cur ^ 1][0]) return false;
+    StepArgs<T, R> a = make_args<T, R>(s);
+    const void* f[6] = {a.Ex, a.Ey, a.Ez, a.Hx, a.Hy, a.Hz};
+    bool aligned = true;
+    for (const void* p : f) aligned = aligned && ((uintptr_t)p % 16 == 0);
+    if (!aligned || !s->d_monpart || !s->d_fmax || s->mon_defer) return false;
+    const long ng = s->gain ? s->gbox : 0L;
+    const long nv = s->N / (16 / (long)sizeof(T));
+    const long nb = std::min<long>(std::max<long>((nv + 1023) / 1024, 1), MONPART_MAX);
+    const long words = (long)fdtdk::MON_SLOTS * fdtdk::MON_SLOT_WORDS;
+    hipLaunchKernelGGL(fdtdk::k_zero_words, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, s->stream, s->d_fmax,
+                       words);
+    if (ng > 0)
+        hipLaunchKernelGGL((fdtdk::k_monitor<T, R, true>), dim3((unsigned)nb), dim3(256), 0, s->stream, a, 0L, ng,
+                           s->d_monpart);
+    s->mon_defer = slot;
+    s->mon_defer_nb = ng > 0 ? (int)nb : 0;
+    return true;
 }
 
 int dispatch_step(fdtd_sim* s, long n, int sidx, hipEvent_t* ev) {
@@ -828,6 +884,13 @@ void dispatch_monitor(fdtd_sim* s, MonOut* slot) {
     case FDTD_FP64: launch_monitor<double, double>(s, slot); break;
     case FDTD_FP32: launch_monitor<float, float>(s, slot); break;
     default: launch_monitor<uint16_t, float>(s, slot); break;
+    }
+}
+bool dispatch_monitor_begin(fdtd_sim* s, MonOut* slot) {
+    switch (s->prec) {
+    case FDTD_FP64: return launch_monitor_begin<double, double>(s, slot);
+    case FDTD_FP32: return launch_monitor_begin<float, float>(s, slot);
+    default: return launch_monitor_begin<uint16_t, float>(s, slot);
     }
 }
 
@@ -1056,6 +1119,11 @@ int fdtd_create(const fdtd_grid_desc* g, const fdtd_material_desc* m, const fdtd
         rc = s->alloc(&p, (size_t)MONPART_MAX * sizeof(MonOut));
         s->d_monpart = (MonOut*)p;
     }
+    if (!rc) {
+        void* p = nullptr;
+        rc = s->alloc(&p, (size_t)fdtdk::MON_SLOTS * fdtdk::MON_SLOT_WORDS * sizeof(unsigned long long));
+        s->d_fmax = (unsigned long long*)p;
+    }
     if (!rc && hipStreamSynchronize(s->stream) != hipSuccess) rc = fail(FDTD_EDEVICE, "initialisation failed");
     if (rc) {
         delete s;
@@ -1214,6 +1282,7 @@ int fdtd_set_kernel_mode(fdtd_handle h, int mode, int param) {
     if (mode == 0) {
         h->force64 = (param & 0x20000) != 0;  // 64-bit element offsets even where 32-bit ones fit
         h->erev = (param & 0x100000) == 0;    // set: E tiles in forward order, as H walks them
+        h->mon_fused_h = (param & 0x40000) == 0;  // set: the monitor's field maxima in a pass of their own
         if (param >> 21) h->band_tiles = (unsigned)((param >> 21) & 0x3ff);  // j-tiles per band of the tile order
         param &= 0x1ffff;
     }
@@ -1383,7 +1452,11 @@ static int step_launch(fdtd_handle h, long n0, int nsteps, const double* src_val
     for (int s = 0; s < nsteps; ++s) {
         const long n = n0 + s;
         dispatch_step(h, n, s, ev && s % every == 0 ? &(*ev)[1 + (size_t)(s / every) * 4] : nullptr);
-        if (h->mon_interval > 0 && n % h->mon_interval == 0) dispatch_monitor(h, h->mon_cur + slot++);
+        if (h->mon_interval > 0 && n % h->mon_interval == 0) {
+            // the field maxima ride in the next step's H launch when it is in this call
+            MonOut* rec = h->mon_cur + slot++;
+            if (!(s + 1 < nsteps && h->mon_fused_h && dispatch_monitor_begin(h, rec))) dispatch_monitor(h, rec);
+        }
         if (h->prec == FDTD_FP16 && n % h->fp16_interval == 0) fp16_maintain(h, h->stream);
     }
     HIPCHK(hipGetLastError());

@@ -542,6 +542,54 @@ __global__ void __launch_bounds__(256) k_update_H(StepArgs<T, R> a) {
     (void)h_compute<T, R, true, IX>(a, tb, i, j, k, h_gather<T, R, KW, true, IX>(a, tb, i, j, k, t.kw, t.jw), true);
 }
 
+// The H update of the step after a monitor step, which also takes the
+// monitor's field maxima (StabilityMonitor::check_stability's max |E|,
+// max |H| and NaN/Inf scan, src/stability_monitor.hpp:51-121) of E^{n+1}
+// and H^{n+1/2} of the monitor step: exactly the E and H values this launch
+// reads at each cell before it writes H, so the separate pass over the six
+// fields is saved.  Every cell in the grid contributes its own E and H
+// (cells the reference does not H-update load them directly).  Each
+// wavefront reduces its lanes and takes atomicMax on one of MON_SLOTS slots
+// (the maximum of bit patterns is order-free, so the result is exact);
+// k_monitor_final merges the slots.
+constexpr int MON_SLOTS = 1024;
+constexpr int MON_SLOT_WORDS = 8;  // one 64-byte line per slot
+template <typename T, typename R>
+__device__ __forceinline__ unsigned long long absbits_v(R v) {
+    return (unsigned long long)__double_as_longlong(fabs((double)v));
+}
+template <typename T, typename R, int KW, typename IX = long>
+__global__ void __launch_bounds__(256) k_update_H_mon(StepArgs<T, R> a, unsigned long long* fmax) {
+    FDTD_SCALES_IN_REGISTERS(a)
+    const Tabs<R> tb = global_tabs(a);
+    const Tile t = tile_of<KW>(a);
+    const int i = t.i, j = t.j, k = t.k;
+    unsigned long long mE = 0, mH = 0;
+    if (k < a.nz - 1 && j < a.ny - 1 && i < a.nx - 1) {
+        const HIn<R> v = h_gather<T, R, KW, true, IX>(a, tb, i, j, k, t.kw, t.jw);
+        mE = max(max(absbits_v<T, R>(v.ex), absbits_v<T, R>(v.ey)), absbits_v<T, R>(v.ez));
+        mH = max(max(absbits_v<T, R>(v.hx), absbits_v<T, R>(v.hy)), absbits_v<T, R>(v.hz));
+        (void)h_compute<T, R, true, IX>(a, tb, i, j, k, v, true);
+    } else if (k < a.nz && j < a.ny && i < a.nx) {
+        const IX id = ((IX)i * a.ny + j) * a.nz + k;
+        mE = max(max(absbits_v<T, R>(ldx<T, R>(a.Ex, id, a.sc[0])), absbits_v<T, R>(ldx<T, R>(a.Ey, id, a.sc[1]))),
+                 absbits_v<T, R>(ldx<T, R>(a.Ez, id, a.sc[2])));
+        mH = max(max(absbits_v<T, R>(ldx<T, R>(a.Hx, id, a.sc[3])), absbits_v<T, R>(ldx<T, R>(a.Hy, id, a.sc[4]))),
+                 absbits_v<T, R>(ldx<T, R>(a.Hz, id, a.sc[5])));
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long e = __shfl_xor(mE, off), h = __shfl_xor(mH, off);
+        mE = e > mE ? e : mE;
+        mH = h > mH ? h : mH;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        const unsigned b = (blockIdx.y * gridDim.x + blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+        unsigned long long* q = fmax + (size_t)(b % MON_SLOTS) * MON_SLOT_WORDS;
+        if (mE) atomicMax(q, mE);
+        if (mH) atomicMax(q + 1, mH);
+    }
+}
+
 // Two-level polarization step (update_polarization_multi_region,
 // src/two_level_system.hpp:595-605).  Ndip = Ng0 = N0 on region cells.  The
 // FP64 form divides by N0 as the reference does; the FP32/FP16 forms (whose
@@ -1253,9 +1301,17 @@ __global__ void __launch_bounds__(256) k_monitor(StepArgs<T, R> a, long N, long 
 
 // Merges the per-block monitor records in a fixed order (so the population
 // sums are the same on every run) into the step's record.
-__global__ void __launch_bounds__(64) k_monitor_final(const MonOut* __restrict__ part, int n, MonOut* slot) {
+// fmax (optional): nf field-maxima slots of MON_SLOT_WORDS words each,
+// (max |E|, max |H|) first, filled by k_update_H_mon.
+__global__ void __launch_bounds__(64) k_monitor_final(const MonOut* __restrict__ part, int n, MonOut* slot,
+                                                      const unsigned long long* __restrict__ fmax, int nf) {
     unsigned long long mE = 0, mH = 0, mP = 0, nN = 0;
     double pt = 0.0, pi = 0.0;
+    for (int q = threadIdx.x; q < nf; q += 64) {
+        const unsigned long long e = fmax[q * MON_SLOT_WORDS], h = fmax[q * MON_SLOT_WORDS + 1];
+        mE = e > mE ? e : mE;
+        mH = h > mH ? h : mH;
+    }
     for (int q = threadIdx.x; q < n; q += 64) {
         const MonOut r = part[q];
         mE = r.maxE > mE ? r.maxE : mE;

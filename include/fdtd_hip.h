@@ -193,7 +193,10 @@ int fdtd_get_fp16_scales(fdtd_handle h, double* scales18);
  * offsets even where 32-bit byte offsets, the default with FP16 storage
  * when every array is below 4 GiB, would do; plus 1048576: the E kernel
  * walks its tiles in the H kernel's order instead of backwards, which by
- * default lets each kernel start on the tiles the previous one finished);
+ * default lets each kernel start on the tiles the previous one finished;
+ * plus 262144: the stability monitor reads the six fields in a pass of its
+ * own instead of taking their maxima in the next step's H launch; plus
+ * N << 21: bands of N j-tiles in the tile order);
  * mode 1 = one fused step kernel: a work-group owns a block of rows over
  * the whole k extent (plus one halo row whose H it recomputes) and marches
  * along x, reading E^n and H^{n-1/2} once and writing E^{n+1} and

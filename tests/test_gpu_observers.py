@@ -101,3 +101,33 @@ def test_monitor_records_are_field_maxima(pkg, precision):
         seen += 1
     assert seen == 4
     sc.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "fp16"])
+def test_monitor_maxima_in_next_h_launch_equal_separate_pass(pkg, precision):
+    """Within a multi-step call the monitor's field maxima of a monitor step
+    are taken by the next step's H launch; the records (every field,
+    including the population sums) equal bit for bit those of the separate
+    pass over the six fields (kernel-mode bit 262144), with a monitor step
+    also as the last step of a call (where the separate pass always runs) and
+    with the monitor every step."""
+    sp = _case(pkg, 1)
+    setup = pkg.Setup(sp)
+    _say(f"[case] {sp.name} {precision} monitor maxima in the H launch")
+    recs = {}
+    for tag, bit in (("h_launch", 0), ("separate", 262144)):
+        for interval in (10, 1):
+            sc = pkg.Scene(setup, precision)
+            sc.set_kernel_mode(0, bit)
+            pkg.load().fdtd_set_monitor(sc.h, interval)
+            out = []
+            for n in (21, 10, 9):  # calls ending on and between monitor steps
+                sc.step(n)
+                out += [(r.step, r.max_E, r.max_H, r.max_P, r.nan_E, r.nan_H, r.nan_P, r.nan_N, r.pop_total,
+                         r.pop_initial) for r in sc.checks()]
+            recs[(tag, interval)] = out
+            sc.close()
+    for interval in (10, 1):
+        a, b = recs[("h_launch", interval)], recs[("separate", interval)]
+        assert len(a) == (4 if interval == 10 else 40) and a == b, (interval, a[:3], b[:3])
+        assert a[-1][1] > 0 and a[-1][2] > 0 and a[-1][8] > 0
