@@ -1,5 +1,5 @@
-"""A grid of more than 2^31 cells (1320^3 = 2.3e9, 110 GB of FP32 fields in
-both buffers): every cell offset beyond 2^31 must be formed in 64 bits, in
+"""A grid of more than 2^31 cells (1320^3 = 2.3e9, 55 GB of FP32 fields):
+every cell offset beyond 2^31 must be formed in 64 bits, in
 the H and E kernels, the probes, the DFT monitor and the stability monitor.
 
 Bar: the Yee update spreads a point source by one cell per step in each
