@@ -226,8 +226,9 @@ def plain(args):
                      "algorithmic_bytes_per_launch": kbytes, "avg_launch_us": round(kt * 1e6, 2),
                      "launches_timed": int(n_timed)},
         "kernels_ms_per_step": kernels,
-        # the whole step against the fused minimum: step_bytes (SURVEY 8(d)'s
-        # B_alg per cell x cells) per measured step
+        # the whole step against the fused minimum: step_bytes (the bytes a
+        # step must move per cell, fields read and written once, times the
+        # cells) per measured step
         "step_roofline": {"bound": "hbm", "alg_bytes_per_cell": round(step_bytes / N, 2),
                           "alg_bytes_per_step": step_bytes,
                           "achieved": round(step_bytes / (wall / args.steps) / 1e9, 1), "peak": HBM_PEAK_GBS,
