@@ -433,8 +433,9 @@ def main():
     ap.add_argument("--workload", default="c3", choices=["c2", "c3"],
                     help="plain form: c3 = 512x512x384 graded grid with gain slab and 64 cuboids (default), "
                          "c2 = size^3 graded gain slab")
-    ap.add_argument("--event-every", type=int, default=10,
-                    help="plain form: HIP events around the kernels of every n-th timed step")
+    ap.add_argument("--event-every", type=int, default=0,
+                    help="plain form: HIP events around the kernels of every n-th timed step (0: 10, or fewer "
+                         "for short runs so that at least 20 launches of each kernel are timed)")
     ap.add_argument("--cpu-steps", type=int, default=80, help="CPU baseline: timed FP64 steps, all threads")
     ap.add_argument("--cpu-steps-1t", type=int, default=12, help="CPU baseline: timed FP64 steps on one thread")
     ap.add_argument("--no-cpu", action="store_true")
@@ -470,6 +471,8 @@ def main():
         sys.exit(2)
     if args.steps is None:
         args.steps = {"plain": 1000, "precision": 4000, "batch": 2000, "serve": 0}[args.form]
+    if args.event_every <= 0:
+        args.event_every = max(1, min(10, args.steps // 20))
     {"plain": plain, "precision": precision_form, "batch": batch_form, "serve": serve_form}[args.form](args)
 
 
