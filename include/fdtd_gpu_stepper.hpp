@@ -31,6 +31,7 @@
  */
 #pragma once
 
+#include <algorithm>
 #include <cmath>
 #include <cstddef>
 #include <memory>
@@ -180,6 +181,27 @@ public:
         const int rc = fdtd_step(h_, (long)n0, nsteps, srcs_.empty() ? nullptr : vals_.data());
         if (rc == FDTD_EUNSTABLE) return false;
         check(rc);
+        return true;
+    }
+
+    // Steps [n0, n0 + nsteps) for a main() that also runs host detectors
+    // (FieldMovie2D, MeshDetector: they record from whole fields, as
+    // rt.after_E hands them, src/params.hpp:793-810).  Blocks end at every
+    // 10th step (the stability cadence) and after each step n with
+    // wants(n); after such a step, after(n) runs with the device state at
+    // the end of step n (download_fields gives the fields the reference's
+    // record_after_E would see).  false where the reference's policy stops.
+    template <typename Wants, typename After>
+    bool run(size_t n0, size_t nsteps, real dt, Wants wants, After after) {
+        const size_t end = n0 + nsteps;
+        for (size_t n = n0; n < end;) {
+            size_t stop = std::min(end, (n / 10 + 1) * 10);
+            for (size_t s = n; s < stop; ++s)
+                if (wants(s)) { stop = s + 1; break; }
+            if (!step(n, (int)(stop - n), dt)) return false;
+            if (wants(stop - 1)) after(stop - 1);
+            n = stop;
+        }
         return true;
     }
 

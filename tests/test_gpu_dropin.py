@@ -5,15 +5,17 @@ profiles, the TLS regions and state, the sources and the probes; the adaptor
 of include/fdtd_gpu_stepper.hpp (fdtd_gpu::GpuStepper, compiled with the
 reference's headers) takes those objects, binds the sources (footprints, and
 per-step values from each source's own waveform, with no N-sized J array)
-and the point detectors, and advances the state in blocks of 10 steps (one
-fdtd_step call each).  The reference's step advances the same state on the
-CPU in the same process.  In the device-bake form the adaptor passes the
+and the point detectors, and advances the state with GpuStepper::run: blocks
+of at most 10 steps (one fdtd_step call each), cut after every step a host
+detector records (here every 7th step, offset 3, as a field movie would).
+At each such step the downloaded fields are compared with the reference's
+CPU step brought to the same step in the same process.  In the device-bake form the adaptor passes the
 reference's structure list (StructureScene::items, bg) instead of its baked
 MaterialGrids, and the handle's material arrays are compared with the
 reference's bake too.
 
-Bar: FP64 fields, TLS arrays and probe traces within 1e-12 relative L2 of
-the reference's own step after the case's full step count."""
+Bar: FP64 fields (at every host-detector step and at the end), TLS arrays
+and probe traces within 1e-12 relative L2 of the reference's own step."""
 import os
 import sys
 
