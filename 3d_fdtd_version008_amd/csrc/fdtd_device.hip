@@ -599,67 +599,61 @@ int setup_gain(fdtd_sim* s, const fdtd_tls_desc* t) {
 }
 
 // Geometry of the fused step kernel for this grid: CW cells per lane and kc
-// k-chunk wavefronts per row (kc * CW * 64 >= NzT with the least unused
-// lanes, then the most rows per work-group), jw rows per work-group plus the
-// halo row within 16 wavefronts and 64 KiB of LDS, ic planes per
-// work-group.  Overrides (fdtd_set_kernel_mode): s->fz_cw, s->fz_jw.
-template <typename R>
-bool fused_geom(const fdtd_sim* s, fdtdk::FusedGeom& g, int& cw) {
+// k-chunk wavefronts per work-group (kc * CW * 64 >= NzT with the least
+// unused lanes, then 2, 3, 4, 1 cells per lane in that order: 2 was fastest
+// at 512x512x384), jw rows per work-group (2), ic planes per work-group (8).
+// Overrides (fdtd_set_kernel_mode): s->fz_cw, s->fz_jw, s->ichunk.
+constexpr int MARCH_MAX_JW = 4;
+bool march_geom(const fdtd_sim* s, fdtdk::FusedGeom& g, int& cw) {
     const int nseg = (s->nz + 63) / 64;
-    int best = -1, best_cw = 0, best_kc = 0, best_jw = 0;
-    for (int c = 1; c <= fdtdk::FZ_MAX_CW; ++c) {
+    static const int pref[4] = {2, 3, 4, 1};
+    int best = -1;
+    for (int t = 0; t < 4; ++t) {
+        const int c = pref[t];
         if (s->fz_cw && c != s->fz_cw) continue;
         const int kc = (nseg + c - 1) / c;
-        int jw = fdtdk::FZ_MAX_WAVES / kc - 1;
-        while (jw > 0 && fdtdk::fused_lds_bytes<R>((jw + 1) * kc, c) > 64 * 1024) --jw;
-        jw = std::min(jw, s->fz_jw ? s->fz_jw : 8);
-        if (jw < 1) continue;
-        // unused lanes first, then taller blocks (less halo work), then
-        // fewer cells per lane (fewer registers)
-        const int waste = kc * c - nseg;
-        const int score = (100 - waste) * 1000 + std::min(jw, 7) * 10 + (fdtdk::FZ_MAX_CW - c);
-        if (score > best) { best = score; best_cw = c; best_kc = kc; best_jw = jw; }
+        if (kc > fdtdk::FZ_MARCH_MAX_KC) continue;
+        const int score = (100 - (kc * c - nseg)) * 10 + (4 - t);
+        if (score > best) { best = score; cw = c; g.kc = kc; }
     }
     if (best < 0) return false;
-    cw = best_cw;
-    g.kc = best_kc;
-    g.jw = best_jw;
+    g.jw = s->fz_jw ? std::min(s->fz_jw, MARCH_MAX_JW) : 2;
     g.njb = (s->ny + g.jw - 1) / g.jw;
-    int ic = s->ichunk;
-    if (ic <= 0) {
-        // enough work-groups to fill every CU several times over, with few
-        // halo planes (each chunk recomputes H of one extra plane)
-        const long per = std::max(1L, (long)s->nx * g.njb / 1024);
-        ic = (int)std::min<long>(64, std::max<long>(4, per));
-    }
-    g.ic = std::min(ic, s->nx);
+    g.ic = std::min(s->ichunk > 0 ? s->ichunk : 8, s->nx);
     g.nwg = g.njb * ((s->nx + g.ic - 1) / g.ic);
     return true;
 }
 // The fused kernel addresses cells by 32-bit byte offsets, reads the
-// cell-invariant H coefficients, and keeps a row in at most 16 wavefronts.
+// cell-invariant H coefficients, and covers a row with at most
+// FZ_MARCH_MAX_KC wavefronts.
 bool fused_ok(const fdtd_sim* s) {
     if (!s->h_uniform || !s->fits32) return false;
     fdtdk::FusedGeom g{};
     int cw = 0;
-    return s->prec == FDTD_FP64 ? fused_geom<double>(s, g, cw) : fused_geom<float>(s, g, cw);
+    return march_geom(s, g, cw);
+}
+template <typename T, typename R, int CW>
+void launch_march_cw(fdtd_sim* s, StepArgs<T, R>& a, const fdtdk::FusedGeom& g) {
+    const dim3 grd((unsigned)g.nwg), blk((unsigned)(64 * g.kc));
+    switch (g.jw) {
+    case 1: hipLaunchKernelGGL((fdtdk::k_step_march<T, R, CW, 1>), grd, blk, 0, s->stream, a, g); break;
+    case 2: hipLaunchKernelGGL((fdtdk::k_step_march<T, R, CW, 2>), grd, blk, 0, s->stream, a, g); break;
+    case 3: hipLaunchKernelGGL((fdtdk::k_step_march<T, R, CW, 3>), grd, blk, 0, s->stream, a, g); break;
+    default: hipLaunchKernelGGL((fdtdk::k_step_march<T, R, CW, 4>), grd, blk, 0, s->stream, a, g); break;
+    }
 }
 template <typename T, typename R>
 void launch_fused(fdtd_sim* s, StepArgs<T, R>& a) {
     fdtdk::FusedGeom g{};
     int cw = 0;
-    fused_geom<R>(s, g, cw);
-    const int waves = (g.jw + 1) * g.kc;
-    const unsigned lds = (unsigned)fdtdk::fused_lds_bytes<R>(waves, cw);
-    const dim3 grd((unsigned)g.nwg), blk((unsigned)(64 * waves));
+    march_geom(s, g, cw);
     switch (cw) {
-    case 1: hipLaunchKernelGGL((fdtdk::k_step_fused<T, R, 1>), grd, blk, lds, s->stream, a, g); break;
-    case 2: hipLaunchKernelGGL((fdtdk::k_step_fused<T, R, 2>), grd, blk, lds, s->stream, a, g); break;
-    case 3: hipLaunchKernelGGL((fdtdk::k_step_fused<T, R, 3>), grd, blk, lds, s->stream, a, g); break;
-    default: hipLaunchKernelGGL((fdtdk::k_step_fused<T, R, 4>), grd, blk, lds, s->stream, a, g); break;
+    case 1: launch_march_cw<T, R, 1>(s, a, g); break;
+    case 2: launch_march_cw<T, R, 2>(s, a, g); break;
+    case 3: launch_march_cw<T, R, 3>(s, a, g); break;
+    default: launch_march_cw<T, R, 4>(s, a, g); break;
     }
 }
-
 void set_cur(fdtd_sim* s, int c) {
     s->cur = c;
     for (int q = 0; q < 6; ++q) s->F[q] = s->F2[c][q];
@@ -1292,14 +1286,14 @@ int fdtd_set_kernel_mode(fdtd_handle h, int mode, int param) {
     const int m1_ic = param & 0xfff, m1_cw = (param >> 12) & 0xf, m1_jw = (param >> 16) & 0xf;
     if (mode == 1 && (param < 0 || param >= (1 << 20) || m1_cw > fdtdk::FZ_MAX_CW))
         return fail(FDTD_EINVAL, "mode 1: planes_per_chunk + 4096 * cells_per_lane + 65536 * rows, "
-                                 "planes_per_chunk in [0, 4095], cells_per_lane in [0, 4], rows in [0, 15]");
+                                 "planes_per_chunk in [0, 4095], cells_per_lane in [0, 4], rows in [0, 15] (above 4: 4)");
     if (mode == 1) {
         const int keep[3] = {h->ichunk, h->fz_cw, h->fz_jw};
-        if (param) { h->ichunk = m1_ic; h->fz_cw = m1_cw; h->fz_jw = m1_jw; }
+        h->ichunk = m1_ic; h->fz_cw = m1_cw; h->fz_jw = m1_jw;
         if (!fused_ok(h)) {
             h->ichunk = keep[0]; h->fz_cw = keep[1]; h->fz_jw = keep[2];
             return fail(FDTD_EINVAL, "mode 1: the fused step kernel does not take this grid (it needs cell-invariant H "
-                                     "coefficients, arrays below 4 GiB and a row of at most 1024 cells in 16 wavefronts)");
+                                     "coefficients, arrays below 4 GiB and a row in at most 8 wavefronts)");
         }
         if (int rc = ensure_second(h)) return rc;
     }

@@ -5,31 +5,12 @@
 // (18 s per cell plus psi, ids and gain state); this kernel reads E^n and
 // H^{n-1/2} once and writes H^{n+1/2} and E^{n+1} once (12 s).
 //
-// Shape.  A work-group owns a block of `jw` consecutive rows j of a chunk of
-// `ic` consecutive x-planes, over the whole k extent of the row, and marches
-// along x.  Its wavefronts are (jw + 1) row slots times `kc` k-chunks; a
-// wavefront holds CW cells per lane of one row (k = kbase + 64 q + lane,
-// q < CW), so every load of a cell block is one coalesced row segment.
-// Row slot 0 is the halo row j0 - 1: it computes the Hx and Hz that the
-// block's first row needs for its E update and stores nothing.
-//
-// Per plane i every wavefront
-//   1. loads E^n(i+1) and H^{n-1/2}(i) of its cells, Ex and Ez of the row
-//      above at plane i, and psi_H;
-//   2. computes H^{n+1/2}(i) from E^n(i) (carried from the previous plane),
-//      E^n(i+1), the row above, and E(k+1) from the next lane (DPP wave
-//      rotation; the chunk's last lane takes a broadcast load of the next
-//      chunk's first cell); stores it (buffer B) and puts Hx, Hz into LDS
-//      for the row above and the chunk's last Hx, Hy for the next chunk;
-//   3. waits at the work-group barrier (LDS only: no global-memory fence);
-//   4. loads the E-side inputs (material coefficients from the id loaded
-//      one plane earlier, psi_E, gain state) and computes E^{n+1}(i) from
-//      H(i) (registers), H(i-1) (carried), H(j-1) (LDS) and H(k-1) (DPP
-//      rotation; the chunk's first lane from LDS); stores E, psi_E and the
-//      gain state.
-// The LDS slots alternate with the plane parity, so a wavefront that runs
-// ahead never overwrites a slot a slower one still reads (a slot is written
-// again only after the next barrier, which every reader has passed).
+// Shape (k_step_march below).  A work-group is the kc k-chunk wavefronts of
+// one block of JW consecutive rows j and a chunk of consecutive x-planes;
+// a wavefront holds CW cells per lane of a row (k = kbase + 64 q + lane,
+// q < CW), so every load of a cell block is one coalesced row segment.  The
+// wavefront marches along x and, inside each plane, walks the halo row
+// j0 - 1 (H only, nothing stored) and then the block's rows in order.
 //
 // All reads come from buffer A (E^n, H^{n-1/2}, psi_H^{n-1/2}) and all field
 // writes go to buffer B, so the halo row and the row above are read while
@@ -46,12 +27,10 @@
 
 namespace fdtdk {
 
-constexpr int FZ_MAX_CW = 4;        // cells per lane per row chunk
-constexpr int FZ_MAX_WAVES = 16;    // 1024 threads
-constexpr int FZ_MAX_CHUNK = 4096;  // planes per work-group
+constexpr int FZ_MAX_CW = 4;  // cells per lane per row chunk
 
 struct FusedGeom {
-    int kc;   // k-chunk wavefronts per row
+    int kc;   // k-chunk wavefronts per work-group
     int jw;   // rows per work-group (plus the halo row)
     int ic;   // x-planes per work-group
     int njb;  // j-blocks
@@ -80,13 +59,6 @@ __device__ __forceinline__ R lane_shift(R x) {
 // (and scalar) operations, then meets the others.  Global loads and stores
 // in flight stay in flight (a __syncthreads fence would wait for them).
 __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
-
-// LDS bytes of a work-group: per parity and wavefront, Hx and Hz of its
-// CW x 64 cells, then per parity and wavefront the chunk's last Hx, Hy.
-template <typename R>
-__host__ __device__ constexpr unsigned long fused_lds_bytes(int waves, int cw) {
-    return (unsigned long)2 * waves * 2 * cw * 64 * sizeof(R) + (unsigned long)2 * waves * 2 * sizeof(R);
-}
 
 // The E-side inputs of one cell other than the fields and the z inverse
 // spacing: material coefficients (id m), psi_E, gain state, axis values,
@@ -225,179 +197,261 @@ __device__ __forceinline__ StepArgs<T, R> phase_args(KArgs<T, R> ka) {
     return a;
 }
 
-// The fused step.  Reads buffer A (a.Ex.., a.psi[6..11]), writes buffer B
-// (a.Exw.., a.psiw[6..11]).  Every wavefront reaches every barrier: the
-// per-wavefront work is under wave-uniform conditions only.
-template <typename T, typename R, int CW>
-__global__ void __launch_bounds__(64 * FZ_MAX_WAVES) k_step_fused(StepArgs<T, R> args, FusedGeom gm) {
+// ---- The fused step kernel (k_step_march) ----
+//
+// Work-groups do not wait on each other: a work-group is only the kc
+// k-chunk wavefronts of one row block (kc = 1 to 3 on most grids), and each
+// wavefront walks the block's JW rows in order inside every plane:
+//   halo row j0 - 1: H^{n+1/2} (Hx, Hz only used, nothing stored);
+//   row j0 + r:      H^{n+1/2}(i) -> store; E^{n+1}(i) with H(j-1) from the
+//                    row before it (registers), H(i-1) carried from the
+//                    previous plane, H(k-1) by DPP rotation (the chunk's
+//                    first lane from the previous chunk's wavefront through
+//                    LDS: the only barrier, and only between kc wavefronts).
+// Per plane a wavefront keeps E^n(i) and H^{n+1/2}(i-1) of its JW rows in
+// registers (carried to the next plane), so the block's own data is read
+// once; the halo row is the only recomputation (Hx, Hz of one row in
+// JW + 1), and the halo row and the row above the block are read again by
+// the neighbouring blocks.  The first plane of each chunk also recomputes
+// H of the plane before it (not stored) for the carried H(i0-1).
+constexpr int FZ_MARCH_MAX_KC = 8;  // k-chunk wavefronts per work-group
+
+template <typename T, typename R, int CW, int JW>
+__global__ void __launch_bounds__(64 * FZ_MARCH_MAX_KC) k_step_march(StepArgs<T, R> args, FusedGeom gm) {
     (void)args;
     using IX = unsigned;
     using SA = StepArgs<T, R>;
     const KArgs<T, R> ka = (KArgs<T, R>)__builtin_amdgcn_kernarg_segment_ptr();
-    extern __shared__ __align__(16) unsigned char fz_lds[];
-    const int waves = (gm.jw + 1) * gm.kc;
-    const int w = (int)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    // the last Hx, Hy of each wavefront's chunk, two alternating slots
+    __shared__ R kslot[2][FZ_MARCH_MAX_KC][2];
+    const int w = (int)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // k-chunk
     const int lane = (int)(threadIdx.x & 63);
-    const int rs = w / gm.kc;            // row slot: 0 = halo row
-    const int kc = w - rs * gm.kc;       // k-chunk
-    // work-groups are dealt round-robin to the 8 XCDs: give each XCD a
-    // contiguous run of (i-chunk, j-block) tiles, so a block's halo rows were
-    // read recently by the same XCD's L2
+    // work-groups are dealt round-robin to the 8 XCDs: each XCD gets a
+    // contiguous run of (j-block, i-chunk) tiles, so neighbouring row blocks
+    // (whose rows are each other's halo and row above) share an L2
     const unsigned nb = gridDim.x, b = blockIdx.x;
     const unsigned per = (nb + 7) / 8;
     const unsigned t = (nb % 8 == 0) ? (b % 8) * per + b / 8 : b;
     const int jb = (int)(t % (unsigned)gm.njb), ich = (int)(t / (unsigned)gm.njb);
-    const int j = jb * gm.jw - 1 + rs;
     int nx, ny, nz;
     nx = ka->nx; ny = ka->ny; nz = ka->nz;
+    const int j0 = jb * JW;
+    const int nrow = min(JW, ny - j0);  // own rows of the block
+    const bool hal = j0 > 0;            // the halo row j0 - 1 exists
+    const int jh = hal ? j0 - 1 : 0;
     const int i0 = ich * gm.ic, i1 = min(i0 + gm.ic, nx);
-    const bool live = (j >= 0) & (j < ny);
-    const bool halo = rs == 0;
-    const int kbase = kc * CW * 64;
-    const int jc = live ? j : 0;
-    const int jn = min(jc + 1, ny - 1);
+    const int kbase = w * CW * 64;
     int kq[CW], kk[CW];
     for (int q = 0; q < CW; ++q) {
         kq[q] = kbase + 64 * q + lane;
         kk[q] = kq[q] < nz ? kq[q] : nz - 1;
     }
-    // first k of the next chunk (its E values feed this chunk's last H)
-    const int knext = min(kbase + 64 * CW, nz - 1);
-    R* slot = reinterpret_cast<R*>(fz_lds);                 // [2][waves][2][CW][64]
-    R* kslot = slot + (long)2 * waves * 2 * CW * 64;        // [2][waves][2]
-    auto hs = [&](int par, int wv, int comp, int q) -> R* {
-        return slot + (((long)(par * waves + wv) * 2 + comp) * CW + q) * 64 + lane;
-    };
-    auto ks = [&](int par, int wv, int comp) -> R* { return kslot + ((long)(par * waves + wv) * 2 + comp); };
+    const int knext = min(kbase + 64 * CW, nz - 1);  // first k of the next chunk
 
-    // carried state: E^n(i) and H^{n+1/2}(i-1) (Hy, Hz) of the own cells,
-    // the material id of plane i, the z inverse spacing
-    R ex[CW], ey[CW], ez[CW], hpy[CW], hpz[CW], idz[CW];
-    int mid[CW];
-    for (int q = 0; q < CW; ++q) { ex[q] = ey[q] = ez[q] = hpy[q] = hpz[q] = idz[q] = 0; mid[q] = 0; }
+    // carried: E^n(i) and H^{n+1/2}(i-1) (Hy, Hz) of the own rows, E^n(i)
+    // of the halo row, the z inverse spacing
+    R ex[JW][CW], ey[JW][CW], ez[JW][CW], hpy[JW][CW], hpz[JW][CW];
+    R gx[CW], gy[CW], gz[CW], idz[CW];
+    for (int r = 0; r < JW; ++r)
+        for (int q = 0; q < CW; ++q) ex[r][q] = ey[r][q] = ez[r][q] = hpy[r][q] = hpz[r][q] = 0;
+    for (int q = 0; q < CW; ++q) gx[q] = gy[q] = gz[q] = 0;
 
-    // first plane of the march: i0 - 1 (H only, to carry H(i0-1)) or i0
+    // first plane of the march: i0 - 1 (H only, for the carried H(i0-1)) or i0
     const int ps = i0 > 0 ? i0 - 1 : i0;
-    if (live) {
+    {
         const SA a = phase_args<T, R>(ka);
-        for (int q = 0; q < CW; ++q) {
-            const IX id = ((IX)ps * ny + jc) * nz + kk[q];
-            ex[q] = ldx<T, R>(a.Ex, id, (R)a.sc[0]);
-            ey[q] = ldx<T, R>(a.Ey, id, (R)a.sc[1]);
-            ez[q] = ldx<T, R>(a.Ez, id, (R)a.sc[2]);
-            mid[q] = mat_of<R, IX>(a.matid, a.mid16, ((IX)i0 * ny + jc) * nz + kk[q]);
-            idz[q] = a.cpz[kk[q]];
+#pragma unroll
+        for (int r = 0; r < JW; ++r) {
+            if (r >= nrow) continue;
+            for (int q = 0; q < CW; ++q) {
+                const IX id = ((IX)ps * ny + (j0 + r)) * nz + kk[q];
+                ex[r][q] = ldx<T, R>(a.Ex, id, (R)a.sc[0]);
+                ey[r][q] = ldx<T, R>(a.Ey, id, (R)a.sc[1]);
+                ez[r][q] = ldx<T, R>(a.Ez, id, (R)a.sc[2]);
+            }
         }
+        if (hal) {
+            for (int q = 0; q < CW; ++q) {
+                const IX id = ((IX)ps * ny + jh) * nz + kk[q];
+                gx[q] = ldx<T, R>(a.Ex, id, (R)a.sc[0]);
+                gy[q] = ldx<T, R>(a.Ey, id, (R)a.sc[1]);
+                gz[q] = ldx<T, R>(a.Ez, id, (R)a.sc[2]);
+            }
+        }
+        for (int q = 0; q < CW; ++q) idz[q] = a.cpz[kk[q]];
     }
 
+    int slot = 0;
     for (int i = ps; i < i1; ++i) {
-        const bool hphase_only = i < i0;  // the carried H(i0-1)
-        const bool edo = !hphase_only && !halo;
-        const int par = i & 1;
+        const bool honly = i < i0;  // the plane before the chunk: H only
         const int ip = min(i + 1, nx - 1);
-        R hx[CW], hy[CW], hz[CW];
-        for (int q = 0; q < CW; ++q) hx[q] = hy[q] = hz[q] = 0;
-        R enx[CW], eny[CW], enz[CW];
-        for (int q = 0; q < CW; ++q) { enx[q] = ex[q]; eny[q] = ey[q]; enz[q] = ez[q]; }
-        if (live) {
-            // ---- H^{n+1/2}(i): loads, then the update ----
+        R hjx[CW], hjz[CW];  // H^{n+1/2}(i) Hx, Hz of the row below the current one
+        for (int q = 0; q < CW; ++q) hjx[q] = hjz[q] = 0;
+        R gnx[CW], gny[CW], gnz[CW];  // E^n(i+1) of the halo row
+        for (int q = 0; q < CW; ++q) { gnx[q] = gx[q]; gny[q] = gy[q]; gnz[q] = gz[q]; }
+        if (hal) {
+            // ---- halo row: E^n(i+1) for the carry, Hx and Hz of H^{n+1/2}(i) ----
             const SA a = phase_args<T, R>(ka);
             const Tabs<R> tb = global_tabs(a);
-            R hox[CW], hoy[CW], hoz[CW], ejx[CW], ejz[CW];
             for (int q = 0; q < CW; ++q) {
-                const IX id = ((IX)i * ny + jc) * nz + kk[q];
-                const IX idn = ((IX)ip * ny + jc) * nz + kk[q];
-                const IX idj = ((IX)i * ny + jn) * nz + kk[q];
-                enx[q] = ldx<T, R>(a.Ex, idn, (R)a.sc[0]);
-                eny[q] = ldx<T, R>(a.Ey, idn, (R)a.sc[1]);
-                enz[q] = ldx<T, R>(a.Ez, idn, (R)a.sc[2]);
-                hox[q] = ldx<T, R>(a.Hx, id, (R)a.sc[3]);
-                hoy[q] = ldx<T, R>(a.Hy, id, (R)a.sc[4]);
-                hoz[q] = ldx<T, R>(a.Hz, id, (R)a.sc[5]);
-                ejx[q] = ldx<T, R>(a.Ex, idj, (R)a.sc[0]);
-                ejz[q] = ldx<T, R>(a.Ez, idj, (R)a.sc[2]);
+                const IX idn = ((IX)ip * ny + jh) * nz + kk[q];
+                gnx[q] = ldx<T, R>(a.Ex, idn, (R)a.sc[0]);
+                gny[q] = ldx<T, R>(a.Ey, idn, (R)a.sc[1]);
+                gnz[q] = ldx<T, R>(a.Ez, idn, (R)a.sc[2]);
             }
-            const IX idk = ((IX)i * ny + jc) * nz + knext;
-            const R ekx = ldx<T, R>(a.Ex, idk, (R)a.sc[0]);
-            const R eky = ldx<T, R>(a.Ey, idk, (R)a.sc[1]);
-            R ph[CW][6], kzh[CW][4];
-            for (int q = 0; q < CW; ++q) fz_psi_h<T, R, IX>(a, i, jc, kk[q], kbase + 64 * q, ph[q], kzh[q], tb);
-            R rx[CW], ry[CW];
-            for (int q = 0; q < CW; ++q) {
-                rx[q] = lane_shift<true>(ex[q]);
-                ry[q] = lane_shift<true>(ey[q]);
-            }
-            const R idx = tb.ax.X(0, i), idy = tb.ax.Y(0, jc);
-            R kxr[4] = {0, 0, 0, 0}, kyr[4] = {0, 0, 0, 0};
-            if (a.cpml && in_pml(i, nx, a.np)) {
-                kxr[0] = tb.ax.X(1, i); kxr[1] = tb.ax.X(6, i); kxr[2] = tb.ax.X(4, i); kxr[3] = tb.ax.X(5, i);
-            }
-            if (a.cpml && in_pml(jc, ny, a.np)) {
-                kyr[0] = tb.ax.Y(1, jc); kyr[1] = tb.ax.Y(6, jc); kyr[2] = tb.ax.Y(4, jc); kyr[3] = tb.ax.Y(5, jc);
-            }
-            const bool own = !halo && !hphase_only;
-            for (int q = 0; q < CW; ++q) {
-                const bool last = lane == 63;
-                const R exk = last ? (q + 1 < CW ? rx[q + 1] : ekx) : rx[q];
-                const R eyk = last ? (q + 1 < CW ? ry[q + 1] : eky) : ry[q];
-                HIn<R> v;
-                v.ex = ex[q]; v.ey = ey[q]; v.ez = ez[q];
-                v.ex_j = ejx[q]; v.ez_j = ejz[q];
-                v.ey_i = eny[q]; v.ez_i = enz[q];
-                v.ex_k = exk;
-                v.ey_k = eyk;
-                v.hx = hox[q]; v.hy = hoy[q]; v.hz = hoz[q];
-                for (int u = 0; u < 6; ++u) { v.p[u] = ph[q][u]; v.cH[u] = tb.hco[u * a.nmat]; }
-                v.upd = (i < nx - 1) & (j < ny - 1) & (kq[q] < nz - 1);
-                v.idx = idx; v.idy = idy; v.idz = idz[q];
-                for (int u = 0; u < 4; ++u) { v.kx[u] = kxr[u]; v.ky[u] = kyr[u]; v.kz[u] = kzh[q][u]; }
-                const H3<R> h = h_compute<T, R, true, IX>(a, tb, i, jc, kk[q], v, own && kq[q] < nz);
-                hx[q] = h.x; hy[q] = h.y; hz[q] = h.z;
-            }
-            for (int q = 0; q < CW; ++q) {
-                *hs(par, w, 0, q) = hx[q];
-                *hs(par, w, 1, q) = hz[q];
-            }
-            if (lane == 63) {
-                *ks(par, w, 0) = hx[CW - 1];
-                *ks(par, w, 1) = hy[CW - 1];
+            if (!honly) {
+                R hox[CW], hoy[CW], hoz[CW];
+                for (int q = 0; q < CW; ++q) {
+                    const IX id = ((IX)i * ny + jh) * nz + kk[q];
+                    hox[q] = ldx<T, R>(a.Hx, id, (R)a.sc[3]);
+                    hoy[q] = ldx<T, R>(a.Hy, id, (R)a.sc[4]);
+                    hoz[q] = ldx<T, R>(a.Hz, id, (R)a.sc[5]);
+                }
+                const IX idk = ((IX)i * ny + jh) * nz + knext;
+                const R ekx = ldx<T, R>(a.Ex, idk, (R)a.sc[0]);
+                const R eky = ldx<T, R>(a.Ey, idk, (R)a.sc[1]);
+                R ph[CW][6], kzh[CW][4];
+                for (int q = 0; q < CW; ++q) fz_psi_h<T, R, IX>(a, i, jh, kk[q], kbase + 64 * q, ph[q], kzh[q], tb);
+                R rx[CW], ry[CW];
+                for (int q = 0; q < CW; ++q) {
+                    rx[q] = lane_shift<true>(gx[q]);
+                    ry[q] = lane_shift<true>(gy[q]);
+                }
+                const R idx = tb.ax.X(0, i), idy = tb.ax.Y(0, jh);
+                R kxr[4] = {0, 0, 0, 0}, kyr[4] = {0, 0, 0, 0};
+                if (a.cpml && in_pml(i, nx, a.np)) {
+                    kxr[0] = tb.ax.X(1, i); kxr[1] = tb.ax.X(6, i); kxr[2] = tb.ax.X(4, i); kxr[3] = tb.ax.X(5, i);
+                }
+                if (a.cpml && in_pml(jh, ny, a.np)) {
+                    kyr[0] = tb.ax.Y(1, jh); kyr[1] = tb.ax.Y(6, jh); kyr[2] = tb.ax.Y(4, jh); kyr[3] = tb.ax.Y(5, jh);
+                }
+                for (int q = 0; q < CW; ++q) {
+                    const bool last = lane == 63;
+                    HIn<R> v;
+                    v.ex = gx[q]; v.ey = gy[q]; v.ez = gz[q];
+                    v.ex_j = ex[0][q]; v.ez_j = ez[0][q];
+                    v.ey_i = gny[q]; v.ez_i = gnz[q];
+                    v.ex_k = last ? (q + 1 < CW ? rx[q + 1] : ekx) : rx[q];
+                    v.ey_k = last ? (q + 1 < CW ? ry[q + 1] : eky) : ry[q];
+                    v.hx = hox[q]; v.hy = hoy[q]; v.hz = hoz[q];
+                    for (int u = 0; u < 6; ++u) { v.p[u] = ph[q][u]; v.cH[u] = tb.hco[u * a.nmat]; }
+                    v.upd = (i < nx - 1) & (jh < ny - 1) & (kq[q] < nz - 1);
+                    v.idx = idx; v.idy = idy; v.idz = idz[q];
+                    for (int u = 0; u < 4; ++u) { v.kx[u] = kxr[u]; v.ky[u] = kyr[u]; v.kz[u] = kzh[q][u]; }
+                    const H3<R> h = h_compute<T, R, true, IX>(a, tb, i, jh, kk[q], v, false);
+                    hjx[q] = h.x; hjz[q] = h.z;
+                }
             }
         }
-        lds_barrier();
-        if (live && edo) {
-            // ---- E^{n+1}(i): inputs, the neighbours' H, then the update ----
-            const SA a = phase_args<T, R>(ka);
-            const Tabs<R> tb = global_tabs(a);
-            R px_[CW], py_[CW];
-            for (int q = 0; q < CW; ++q) {
-                px_[q] = lane_shift<false>(hx[q]);
-                py_[q] = lane_shift<false>(hy[q]);
+        // ---- the block's rows, in order ----
+#pragma unroll
+        for (int r = 0; r < JW; ++r) {
+            if (r >= nrow) continue;
+            const int j = j0 + r;
+            R hx[CW], hy[CW], hz[CW], enx[CW], eny[CW], enz[CW];
+            int mid[CW];
+            {
+                const SA a = phase_args<T, R>(ka);
+                const Tabs<R> tb = global_tabs(a);
+                R hox[CW], hoy[CW], hoz[CW], ejx[CW], ejz[CW];
+                const int jn = min(j + 1, ny - 1);
+                for (int q = 0; q < CW; ++q) {
+                    const IX id = ((IX)i * ny + j) * nz + kk[q];
+                    const IX idn = ((IX)ip * ny + j) * nz + kk[q];
+                    enx[q] = ldx<T, R>(a.Ex, idn, (R)a.sc[0]);
+                    eny[q] = ldx<T, R>(a.Ey, idn, (R)a.sc[1]);
+                    enz[q] = ldx<T, R>(a.Ez, idn, (R)a.sc[2]);
+                    hox[q] = ldx<T, R>(a.Hx, id, (R)a.sc[3]);
+                    hoy[q] = ldx<T, R>(a.Hy, id, (R)a.sc[4]);
+                    hoz[q] = ldx<T, R>(a.Hz, id, (R)a.sc[5]);
+                    mid[q] = 0;
+                    if (!honly) mid[q] = mat_of<R, IX>(a.matid, a.mid16, id);
+                }
+                if (r + 1 < nrow) {  // the row above is the block's next row: its carried E^n(i)
+                    for (int q = 0; q < CW; ++q) {
+                        ejx[q] = ex[r + 1 < JW ? r + 1 : r][q];
+                        ejz[q] = ez[r + 1 < JW ? r + 1 : r][q];
+                    }
+                } else {
+                    for (int q = 0; q < CW; ++q) {
+                        const IX idj = ((IX)i * ny + jn) * nz + kk[q];
+                        ejx[q] = ldx<T, R>(a.Ex, idj, (R)a.sc[0]);
+                        ejz[q] = ldx<T, R>(a.Ez, idj, (R)a.sc[2]);
+                    }
+                }
+                const IX idk = ((IX)i * ny + j) * nz + knext;
+                const R ekx = ldx<T, R>(a.Ex, idk, (R)a.sc[0]);
+                const R eky = ldx<T, R>(a.Ey, idk, (R)a.sc[1]);
+                R ph[CW][6], kzh[CW][4];
+                for (int q = 0; q < CW; ++q) fz_psi_h<T, R, IX>(a, i, j, kk[q], kbase + 64 * q, ph[q], kzh[q], tb);
+                R rx[CW], ry[CW];
+                for (int q = 0; q < CW; ++q) {
+                    rx[q] = lane_shift<true>(ex[r][q]);
+                    ry[q] = lane_shift<true>(ey[r][q]);
+                }
+                const R idx = tb.ax.X(0, i), idy = tb.ax.Y(0, j);
+                R kxr[4] = {0, 0, 0, 0}, kyr[4] = {0, 0, 0, 0};
+                if (a.cpml && in_pml(i, nx, a.np)) {
+                    kxr[0] = tb.ax.X(1, i); kxr[1] = tb.ax.X(6, i); kxr[2] = tb.ax.X(4, i); kxr[3] = tb.ax.X(5, i);
+                }
+                if (a.cpml && in_pml(j, ny, a.np)) {
+                    kyr[0] = tb.ax.Y(1, j); kyr[1] = tb.ax.Y(6, j); kyr[2] = tb.ax.Y(4, j); kyr[3] = tb.ax.Y(5, j);
+                }
+                for (int q = 0; q < CW; ++q) {
+                    const bool last = lane == 63;
+                    HIn<R> v;
+                    v.ex = ex[r][q]; v.ey = ey[r][q]; v.ez = ez[r][q];
+                    v.ex_j = ejx[q]; v.ez_j = ejz[q];
+                    v.ey_i = eny[q]; v.ez_i = enz[q];
+                    v.ex_k = last ? (q + 1 < CW ? rx[q + 1] : ekx) : rx[q];
+                    v.ey_k = last ? (q + 1 < CW ? ry[q + 1] : eky) : ry[q];
+                    v.hx = hox[q]; v.hy = hoy[q]; v.hz = hoz[q];
+                    for (int u = 0; u < 6; ++u) { v.p[u] = ph[q][u]; v.cH[u] = tb.hco[u * a.nmat]; }
+                    v.upd = (i < nx - 1) & (j < ny - 1) & (kq[q] < nz - 1);
+                    v.idx = idx; v.idy = idy; v.idz = idz[q];
+                    for (int u = 0; u < 4; ++u) { v.kx[u] = kxr[u]; v.ky[u] = kyr[u]; v.kz[u] = kzh[q][u]; }
+                    const H3<R> h = h_compute<T, R, true, IX>(a, tb, i, j, kk[q], v, !honly && kq[q] < nz);
+                    hx[q] = h.x; hy[q] = h.y; hz[q] = h.z;
+                }
             }
-            const R kpx = kc > 0 ? *ks(par, w - 1, 0) : (R)0;
-            const R kpy = kc > 0 ? *ks(par, w - 1, 1) : (R)0;
+            if (!honly) {
+                // ---- E^{n+1}(i) of the row ----
+                R kpx = 0, kpy = 0;
+                if (gm.kc > 1) {
+                    if (lane == 63) { kslot[slot][w][0] = hx[CW - 1]; kslot[slot][w][1] = hy[CW - 1]; }
+                    lds_barrier();
+                    if (w > 0) { kpx = kslot[slot][w - 1][0]; kpy = kslot[slot][w - 1][1]; }
+                    slot ^= 1;
+                }
+                const SA a = phase_args<T, R>(ka);
+                const Tabs<R> tb = global_tabs(a);
+                R px_[CW], py_[CW];
+                for (int q = 0; q < CW; ++q) {
+                    px_[q] = lane_shift<false>(hx[q]);
+                    py_[q] = lane_shift<false>(hy[q]);
+                }
+                for (int q = 0; q < CW; ++q) {
+                    const bool first = lane == 0;
+                    const R hxk = first ? (q > 0 ? px_[q - 1] : kpx) : px_[q];
+                    const R hyk = first ? (q > 0 ? py_[q - 1] : kpy) : py_[q];
+                    EIn<R> v;
+                    fz_e_inputs<T, R, IX>(a, tb, i, j, kq[q], kbase + 64 * q, mid[q], v);
+                    v.ex0 = ex[r][q]; v.ey0 = ey[r][q]; v.ez0 = ez[r][q];
+                    v.idz = idz[q];
+                    if (kq[q] < nz)
+                        e_compute<T, R, true, IX>(a, tb, i, j, kq[q], v, hx[q], hy[q], hz[q], hjx[q], hjz[q], hxk, hyk,
+                                                  hpy[r][q], hpz[r][q]);
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            }
+            // carry to the next row and plane
             for (int q = 0; q < CW; ++q) {
-                const int m = mid[q];
-                mid[q] = mat_of<R, IX>(a.matid, a.mid16, ((IX)ip * ny + jc) * nz + kk[q]);
-                const bool first = lane == 0;
-                const R hxk = first ? (q > 0 ? px_[q - 1] : kpx) : px_[q];
-                const R hyk = first ? (q > 0 ? py_[q - 1] : kpy) : py_[q];
-                const R hxj = *hs(par, w - gm.kc, 0, q);
-                const R hzj = *hs(par, w - gm.kc, 1, q);
-                EIn<R> v;
-                fz_e_inputs<T, R, IX>(a, tb, i, jc, kq[q], kbase + 64 * q, m, v);
-                v.ex0 = ex[q]; v.ey0 = ey[q]; v.ez0 = ez[q];
-                v.idz = idz[q];
-                if (kq[q] < nz)
-                    e_compute<T, R, true, IX>(a, tb, i, jc, kq[q], v, hx[q], hy[q], hz[q], hxj, hzj, hxk, hyk,
-                                              hpy[q], hpz[q]);
-                __builtin_amdgcn_sched_barrier(0);
+                hjx[q] = hx[q]; hjz[q] = hz[q];
+                hpy[r][q] = hy[q]; hpz[r][q] = hz[q];
+                ex[r][q] = enx[q]; ey[r][q] = eny[q]; ez[r][q] = enz[q];
             }
         }
-        // carry to plane i + 1
-        for (int q = 0; q < CW; ++q) {
-            ex[q] = enx[q]; ey[q] = eny[q]; ez[q] = enz[q];
-            hpy[q] = hy[q]; hpz[q] = hz[q];
-        }
+        for (int q = 0; q < CW; ++q) { gx[q] = gnx[q]; gy[q] = gny[q]; gz[q] = gnz[q]; }
     }
 }
 

@@ -186,7 +186,7 @@ def plain(args):
     t_h = ms_h / n_timed / 1e3
     t_e = ms_e / n_timed / 1e3
     if args.mode == 1:
-        kname, kbytes, kt = "k_step_fused", step_bytes, t_h
+        kname, kbytes, kt = "k_step_march", step_bytes, t_h
         kernels = {kname: round(ms_h / n_timed, 4)}
     else:
         kernels = {"k_update_H": round(ms_h / n_timed, 4), "k_update_E": round(ms_e / n_timed, 4)}

@@ -202,10 +202,11 @@ int fdtd_get_fp16_scales(fdtd_handle h, double* scales18);
  * along x, reading E^n and H^{n-1/2} once and writing E^{n+1} and
  * H^{n+1/2} once (the second field and psi_H buffer is allocated on the
  * first switch); `param` = planes per work-group (0..4095) + 4096 * cells
- * per lane per row chunk (1..4) + 65536 * rows per block (1..15), each 0 for
- * the library's choice.  Returns FDTD_EINVAL (and keeps the schedule) for a
- * grid the fused kernel does not take: per-cell H coefficients, an array of
- * 4 GiB or more, or a row that does not fit 16 wavefronts.  Results are
+ * per lane per row chunk (1..4) + 65536 * rows per block (1..15, above 4
+ * taken as 4), each 0 for the library's choice (8 planes, 2 cells, 2 rows).
+ * Returns FDTD_EINVAL (and keeps the schedule) for a grid the fused kernel
+ * does not take: per-cell H coefficients, an array of 4 GiB or more, or a
+ * row that does not fit 8 wavefronts of that many cells.  Results are
  * identical in every schedule; only instruction count and memory traffic
  * differ.  No reference counterpart. */
 int fdtd_set_kernel_mode(fdtd_handle h, int mode, int param);

@@ -169,7 +169,8 @@ def test_step_kernels_agree_bitwise_ragged(pkg, precision):
     runs = {}
     # (1, planes + 4096 * cells_per_lane + 65536 * rows): the fused kernel
     # with 1..4 cells per lane, i.e. 3, 2 or 1 k-chunk wavefronts per row
-    # (dead lanes past NzT in the last chunk), and 1..7 rows per block
+    # (dead lanes past NzT in the last chunk), and 1..4 rows per block (7 is
+    # taken as 4)
     for mode in [(0, 32), (0, 16), (0, 64), (0, 0x100000 + 32), (1, 3), (1, 20), (1, 4096 * 1 + 65536 * 2 + 4),
                  (1, 4096 * 2 + 65536 * 7 + 6), (1, 4096 * 3 + 65536 * 1 + 20), (1, 4096 * 4 + 65536 * 3 + 1)]:
         _say(f"[case] ragged {precision} mode={mode}")
