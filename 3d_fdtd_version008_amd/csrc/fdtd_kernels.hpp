@@ -1399,12 +1399,19 @@ __device__ __forceinline__ int batch_domain(const int* __restrict__ cum, int ndo
     return lo;
 }
 
-template <typename T, typename R, bool EPASS>
+// Persistent H (EPASS false) or E pass over every member of a batch.  A tile
+// of a domain is KW consecutive k of blockDim.y * (64 / KW) rows over
+// `planes` consecutive planes: each wavefront (one threadIdx.y) covers KW k
+// of 64 / KW rows, as in the plain kernels.  The host stores the tile
+// divisors of each domain's tiling in its table entry (tk: k-tiles per row
+// group, tj: row groups per plane group).  IX is the cell-offset type
+// (unsigned where every member's arrays are below 4 GiB).
+template <typename T, typename R, bool EPASS, int KW, typename IX>
 __global__ void __launch_bounds__(256) k_batch_step(const StepArgs<T, R>* __restrict__ doms, const char* anchor,
                                                     const int* __restrict__ cum, const BatchDom* __restrict__ bd,
                                                     int ndom, int chunk, int planes, BatchCtl* ctl) {
+    constexpr int G = 64 / KW;  // rows per wavefront
     __shared__ int first_s;
-    const int tj = blockDim.y;
     const int ty = __builtin_amdgcn_readfirstlane(threadIdx.y);
     const int total = cum[ndom];
     const int lstep = ctl->lstep;
@@ -1434,18 +1441,19 @@ __global__ void __launch_bounds__(256) k_batch_step(const StepArgs<T, R>* __rest
             a.srcv += (long)lstep * a.nsrc;  // this step's source values
             FDTD_SCALES_IN_REGISTERS(a)
             const Tabs<R> tb = global_tabs(a);
-            // tile = 64 k-values x blockDim.y rows x `planes` consecutive planes
-            const int lt = t - cum[d];
-            const int nkt = (a.nz + 63) >> 6, njt = (a.ny + tj - 1) / tj;
-            const int kt = lt % nkt, r = lt / nkt, jt = r % njt, i0 = (r / njt) * planes;
-            const int k = kt * 64 + threadIdx.x, j = jt * tj + ty;
-            const int i1 = min(i0 + planes, a.nx);
+            const unsigned lt = (unsigned)(t - cum[d]);
+            const unsigned r = fdiv(lt, a.tk), ig = fdiv(r, a.tj);
+            const int kt = (int)(lt - r * a.tk.d), jt = (int)(r - ig * a.tj.d);
+            const int kw = kt * KW, jw = (jt * (int)blockDim.y + ty) * G;
+            const int k = kw + (int)threadIdx.x % KW, j = jw + (int)threadIdx.x / KW;
+            const int i0 = (int)ig * planes, i1 = min(i0 + planes, a.nx);
             for (int i = i0; i < i1; ++i) {
                 if (!EPASS) {
                     if (k < a.nz - 1 && j < a.ny - 1 && i < a.nx - 1)
-                        (void)h_compute<T, R>(a, tb, i, j, k, h_gather<T, R>(a, tb, i, j, k, kt * 64, j), true);
+                        (void)h_compute<T, R, true, IX>(a, tb, i, j, k, h_gather<T, R, KW, true, IX>(a, tb, i, j, k, kw, jw),
+                                                        true);
                 } else if (k < a.nz && j < a.ny) {
-                    e_cell<T, R>(a, tb, i, j, k, kt * 64, j);
+                    e_cell<T, R, KW, true, true, IX>(a, tb, i, j, k, kw, jw);
                 }
             }
         }
