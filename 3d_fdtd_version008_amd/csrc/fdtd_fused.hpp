@@ -126,6 +126,7 @@ __device__ __forceinline__ void fz_e_inputs(const StepArgs<T, R>& a, const Tabs<
     v.g = (long)g;
     v.Nu = gs[0]; v.Ng = gs[1]; v.Pz = gs[2]; v.Pp = gs[3];
     for (int q = 0; q < 6; ++q) v.p[q] = cv<T, R>(praw[q], a.sc[6 + q]);
+    v.hns[0] = v.hns[1] = v.hns[2] = (R)1;  // the caller's H neighbours are values
 }
 
 // psi_H of one cell (zeros outside the slabs), raw loads with clamped
@@ -334,6 +335,7 @@ __global__ void __launch_bounds__(64 * FZ_MARCH_MAX_KC) k_step_march(StepArgs<T,
                     v.ex_k = last ? (q + 1 < CW ? rx[q + 1] : ekx) : rx[q];
                     v.ey_k = last ? (q + 1 < CW ? ry[q + 1] : eky) : ry[q];
                     v.hx = hox[q]; v.hy = hoy[q]; v.hz = hoz[q];
+                    v.ns[0] = v.ns[1] = v.ns[2] = (R)1;  // neighbours are values
                     for (int u = 0; u < 6; ++u) { v.p[u] = ph[q][u]; v.cH[u] = tb.hco[u * a.nmat]; }
                     v.upd = (i < nx - 1) & (jh < ny - 1) & (kq[q] < nz - 1);
                     v.idx = idx; v.idy = idy; v.idz = idz[q];
@@ -406,6 +408,7 @@ __global__ void __launch_bounds__(64 * FZ_MARCH_MAX_KC) k_step_march(StepArgs<T,
                     v.ex_k = last ? (q + 1 < CW ? rx[q + 1] : ekx) : rx[q];
                     v.ey_k = last ? (q + 1 < CW ? ry[q + 1] : eky) : ry[q];
                     v.hx = hox[q]; v.hy = hoy[q]; v.hz = hoz[q];
+                    v.ns[0] = v.ns[1] = v.ns[2] = (R)1;  // neighbours are values
                     for (int u = 0; u < 6; ++u) { v.p[u] = ph[q][u]; v.cH[u] = tb.hco[u * a.nmat]; }
                     v.upd = (i < nx - 1) & (j < ny - 1) & (kq[q] < nz - 1);
                     v.idx = idx; v.idy = idy; v.idz = idz[q];

@@ -333,12 +333,33 @@ struct H3 {
     R x, y, z;
 };
 
+// b * s - a for a neighbour b held unscaled with scale s (a power of two, or
+// 1): b * s is exact, so the single rounding of the FMA gives the bits of
+// the difference of the scaled values (signed zeros included).  With s = 1
+// it is the subtraction b - a.
+template <typename R>
+__device__ __forceinline__ R nb_minus(R b, R s, R a) {
+    if constexpr (sizeof(R) == 8) return __builtin_fma(b, s, -a);
+    else return __builtin_fmaf(b, s, -a);
+}
+// a - b * s, the same way (with s = 1 the subtraction a - b).
+template <typename R>
+__device__ __forceinline__ R own_minus(R a, R b, R s) {
+    if constexpr (sizeof(R) == 8) return __builtin_fma(b, -s, a);
+    else return __builtin_fmaf(b, -s, a);
+}
+
 // Inputs of one cell's H update, gathered by h_load so that a caller can
 // issue them ahead of use (the fused kernel loads plane i+1 while it
 // computes plane i).
 template <typename R>
 struct HIn {
     R ex, ex_j, ex_k, ey, ey_i, ey_k, ez, ez_i, ez_j, hx, hy, hz;
+    // scale of the neighbour values ex_j/ex_k, ey_i/ey_k, ez_i/ez_j: the FP16
+    // gathers hold them unscaled (the scale is applied inside the difference,
+    // one mixed-precision FMA instead of a conversion, a multiply and a
+    // subtraction); 1 where they are values already
+    R ns[3];
     R p[6];  // psi_H: Hx_y, Hx_z, Hy_z, Hy_x, Hz_x, Hz_y (where in a slab)
     R cH[6]; // aHx bHx aHy bHy aHz bHz
     bool upd;
@@ -370,9 +391,12 @@ __device__ __forceinline__ HIn<R> h_gather(const StepArgs<T, R>& a, const Tabs<R
     v.hx = ldx<T, R>(a.Hx, id, a.sc[3]);
     v.hy = ldx<T, R>(a.Hy, id, a.sc[4]);
     v.hz = ldx<T, R>(a.Hz, id, a.sc[5]);
-    v.ex = ldx<T, R>(a.Ex, id, a.sc[0]); v.ex_j = ldx<T, R>(a.Ex, id + sJ, a.sc[0]); v.ex_k = ldx<T, R>(a.Ex, id + 1, a.sc[0]);
-    v.ey = ldx<T, R>(a.Ey, id, a.sc[1]); v.ey_i = ldx<T, R>(a.Ey, id + sI, a.sc[1]); v.ey_k = ldx<T, R>(a.Ey, id + 1, a.sc[1]);
-    v.ez = ldx<T, R>(a.Ez, id, a.sc[2]); v.ez_i = ldx<T, R>(a.Ez, id + sI, a.sc[2]); v.ez_j = ldx<T, R>(a.Ez, id + sJ, a.sc[2]);
+    // neighbours unscaled (FP16; scale 1 otherwise), applied in h_compute
+    const R n1 = 1;
+    v.ex = ldx<T, R>(a.Ex, id, a.sc[0]); v.ex_j = ldx<T, R>(a.Ex, id + sJ, n1); v.ex_k = ldx<T, R>(a.Ex, id + 1, n1);
+    v.ey = ldx<T, R>(a.Ey, id, a.sc[1]); v.ey_i = ldx<T, R>(a.Ey, id + sI, n1); v.ey_k = ldx<T, R>(a.Ey, id + 1, n1);
+    v.ez = ldx<T, R>(a.Ez, id, a.sc[2]); v.ez_i = ldx<T, R>(a.Ez, id + sI, n1); v.ez_j = ldx<T, R>(a.Ez, id + sJ, n1);
+    for (int q = 0; q < 3; ++q) v.ns[q] = sizeof(T) == 2 ? (R)a.sc[q] : (R)1;
     using RawT = typename Raw<T>::type;
     RawT praw[6];
     for (int q = 0; q < 6; ++q) praw[q] = opaque((RawT)0);
@@ -425,9 +449,10 @@ __device__ __forceinline__ H3<R> h_compute(const StepArgs<T, R>& a, const Tabs<R
     if (!v.upd) return H3<R>{v.hx, v.hy, v.hz};
     const IX id = ((IX)i * a.ny + j) * a.nz + k;
     const R idx = v.idx, idy = v.idy, idz = v.idz;
-    const R dEz_y = (v.ez_j - v.ez) * idy, dEy_z = (v.ey_k - v.ey) * idz;
-    const R dEx_z = (v.ex_k - v.ex) * idz, dEz_x = (v.ez_i - v.ez) * idx;
-    const R dEy_x = (v.ey_i - v.ey) * idx, dEx_y = (v.ex_j - v.ex) * idy;
+    // (neighbour - own) * inverse spacing, the neighbour's scale inside the FMA
+    const R dEz_y = nb_minus(v.ez_j, v.ns[2], v.ez) * idy, dEy_z = nb_minus(v.ey_k, v.ns[1], v.ey) * idz;
+    const R dEx_z = nb_minus(v.ex_k, v.ns[0], v.ex) * idz, dEz_x = nb_minus(v.ez_i, v.ns[2], v.ez) * idx;
+    const R dEy_x = nb_minus(v.ey_i, v.ns[1], v.ey) * idx, dEx_y = nb_minus(v.ex_j, v.ns[0], v.ex) * idy;
     const R cx = dEz_y - dEy_z, cy = dEx_z - dEz_x, cz = dEy_x - dEx_y;
     const R aHx = v.cH[0], bHx = v.cH[1], aHy = v.cH[2], bHy = v.cH[3], aHz = v.cH[4], bHz = v.cH[5];
     R hx = aHx * v.hx - bHx * cx;
@@ -646,6 +671,9 @@ struct EIn {
     R kx[4], ky[4], kz[4];
     R jx, jy, jz;  // source current J at the cell
     TLSCoef tc;    // TLS coefficients of region `reg` (when reg >= 0)
+    // scale of the H neighbour values e_compute takes (hx_j, hx_k: Hx;
+    // hy_k, hy_i: Hy; hz_j, hz_i: Hz), as HIn::ns
+    R hns[3];
 };
 
 // Source current at cell (i, j, k): the sum over the footprints that hold it.
@@ -707,9 +735,12 @@ __device__ __forceinline__ EIn<R> e_gather(const StepArgs<T, R>& a, const Tabs<R
     const IX sI = i > 0 ? (IX)a.sI : 0, sJ = j > 0 ? (IX)a.sJ : 0;
     const IX km = id > 0 ? 1 : 0;
     v.ez0 = ldx<T, R>(a.Ez, id, a.sc[2]);
-    h[0] = ldx<T, R>(a.Hx, id, a.sc[3]); h[3] = ldx<T, R>(a.Hx, id - sJ, a.sc[3]); h[5] = ldx<T, R>(a.Hx, id - km, a.sc[3]);
-    h[1] = ldx<T, R>(a.Hy, id, a.sc[4]); h[7] = ldx<T, R>(a.Hy, id - sI, a.sc[4]); h[6] = ldx<T, R>(a.Hy, id - km, a.sc[4]);
-    h[2] = ldx<T, R>(a.Hz, id, a.sc[5]); h[8] = ldx<T, R>(a.Hz, id - sI, a.sc[5]); h[4] = ldx<T, R>(a.Hz, id - sJ, a.sc[5]);
+    // neighbours unscaled (FP16; scale 1 otherwise), applied in e_compute
+    const R n1 = 1;
+    h[0] = ldx<T, R>(a.Hx, id, a.sc[3]); h[3] = ldx<T, R>(a.Hx, id - sJ, n1); h[5] = ldx<T, R>(a.Hx, id - km, n1);
+    h[1] = ldx<T, R>(a.Hy, id, a.sc[4]); h[7] = ldx<T, R>(a.Hy, id - sI, n1); h[6] = ldx<T, R>(a.Hy, id - km, n1);
+    h[2] = ldx<T, R>(a.Hz, id, a.sc[5]); h[8] = ldx<T, R>(a.Hz, id - sI, n1); h[4] = ldx<T, R>(a.Hz, id - sJ, n1);
+    for (int q = 0; q < 3; ++q) v.hns[q] = sizeof(T) == 2 ? (R)a.sc[3 + q] : (R)1;
     v.ex0 = ldx<T, R>(a.Ex, id, a.sc[0]);
     v.ey0 = ldx<T, R>(a.Ey, id, a.sc[1]);
     // psi_E on the slabs
@@ -815,9 +846,10 @@ __device__ __forceinline__ H3<R> e_compute(const StepArgs<T, R>& a, const Tabs<R
         return H3<R>{v.ex0, v.ey0, v.ez0};
     }
     const R idx = v.idx, idy = v.idy, idz = v.idz;
-    const R dHz_y = (hz - hz_j) * idy, dHy_z = (hy - hy_k) * idz;
-    const R dHx_z = (hx - hx_k) * idz, dHz_x = (hz - hz_i) * idx;
-    const R dHy_x = (hy - hy_i) * idx, dHx_y = (hx - hx_j) * idy;
+    // (own - neighbour) * inverse spacing, the neighbour's scale inside the FMA
+    const R dHz_y = own_minus(hz, hz_j, v.hns[2]) * idy, dHy_z = own_minus(hy, hy_k, v.hns[1]) * idz;
+    const R dHx_z = own_minus(hx, hx_k, v.hns[0]) * idz, dHz_x = own_minus(hz, hz_i, v.hns[2]) * idx;
+    const R dHy_x = own_minus(hy, hy_i, v.hns[1]) * idx, dHx_y = own_minus(hx, hx_j, v.hns[0]) * idy;
     const R cx = dHz_y - dHy_z, cy = dHx_z - dHz_x, cz = dHy_x - dHx_y;
     const R jx = v.jx, jy = v.jy, jz = v.jz;
     const R bEx = v.cE[1], bEy = v.cE[3], bEz = v.cE[5];
