@@ -1810,9 +1810,10 @@ void batch_launch_step(fdtd_batch* b, int s, const long* n0, const int* msteps) 
     const int nwg = std::max(1, std::min(b->n_cu * b->wg_per_cu * (4 / b->rows), (total + b->chunk - 1) / b->chunk));
     const char* anchor = (const char*)b->d_ctl;
     // 32-bit byte offsets where every member's arrays are below 4 GiB
-    // (256-domain workload: 4.13 -> 4.01-4.06 ms per step)
+    // (256-domain workload: 4.13 -> 4.01-4.06 ms per step), unless a member
+    // asks for 64-bit offsets (fdtd_set_kernel_mode, mode 0, + 131072)
     bool o32 = true;
-    for (int d = 0; d < nd && o32; ++d) o32 = b->doms[d]->fits32;
+    for (int d = 0; d < nd && o32; ++d) o32 = b->doms[d]->fits32 && !b->doms[d]->force64;
     if (o32) {
         hipLaunchKernelGGL((fdtdk::k_batch_step<T, R, false, BATCH_KW, unsigned>), dim3((unsigned)nwg), blk, 0,
                            b->stream, args, anchor, b->d_cum, b->d_dom, nd, b->chunk, b->planes, b->d_ctl);

@@ -60,6 +60,30 @@ def _check_same(ref, got, tag):
             assert rel_l2(y, x) <= 1e-12, f"{tag}: DFT {rel_l2(y, x)}"
 
 
+@pytest.mark.parametrize("precision", ["fp32", "fp16"])
+def test_batch_offset_widths_agree(pkg, precision):
+    """The batch passes address cells by 32-bit byte offsets when every
+    member fits, by 64-bit element offsets when a member asks for them
+    (mode 0, + 131072): the same bits either way."""
+    specs = _small_specs(pkg, 4, steps=60, seed=11)
+    setups = [pkg.Setup(sp) for sp in specs]
+    runs = []
+    for force64 in (False, True):
+        _say(f"[batch offsets] {precision} 64-bit={force64}")
+        b = pkg.Batch(precision)
+        scs = [pkg.Scene(st, precision) for st in setups]
+        if force64:
+            scs[0].set_kernel_mode(0, 0x20000 + 32)
+        for sc in scs:
+            b.add(sc)
+        b.step(60)
+        runs.append([(_state(sc, st), _dfts(sc, st)) for sc, st in zip(scs, setups)])
+        b.close()
+        for sc in scs:
+            sc.close()
+    _check_same(runs[0], runs[1], f"{precision} offsets")
+
+
 @pytest.mark.parametrize("precision", ["fp32", "fp64", "fp16"])
 def test_batch_equals_alone(pkg, precision):
     specs = _small_specs(pkg, 6, steps=80)
