@@ -127,7 +127,7 @@ struct oracle_sim {
     long st_fail_step;
     size_t n;          /* steps done */
     /* storage-format emulation (oracle_set_storage): 0 = FP64 (none) */
-    int emu, emu_bits, emu_psi_bits, emu_stoch;
+    int emu, emu_bits, emu_psi_bits, emu_stoch, emu_block;
     double emu_sc[18], emu_floor[18];
     uint64_t emu_rng;
 };
@@ -1100,7 +1100,13 @@ oracle_sim* oracle_create(const fdtd_scene_spec* sp) {
  *      subnormal below 2^-14 x scale, saturation at 65504 x scale);
  *   2: the same with stochastic rounding (seeded xorshift);
  *   3: binary16 plus a residual byte (19 significant bits, same range);
- *   4: binary16 fields, psi rounded to binary32.
+ *   4: binary16 fields, psi rounded to binary32;
+ *   5, 6, 7: block floating point: a 16-bit signed mantissa per value and
+ *      one power-of-two exponent per block of 32, 64 or 8 consecutive k
+ *      cells of a row (the exponent puts the block's largest magnitude in
+ *      [2^14, 2^15); round to nearest even; no per-array scale).  A value
+ *      a factor f below its block's largest keeps about 15 - log2(f)
+ *      significant bits.
  * Scales start from `scales18` (fields Ex..Hz, psi Ex_y..Hz_y, the values the
  * device starts from) and follow the device's plan every 10 steps: when the
  * largest stored magnitude leaves [2^2, 2^11] the scale moves so that it
@@ -1144,7 +1150,33 @@ static double round_store(double y, int bits, int stoch, uint64_t* rng) {
     if (fabs(r) > 65504.0) r = copysign(65504.0, r);
     return r;
 }
+/* Block floating point (formats 5-7): each block of B consecutive k of a
+ * row shares the exponent that puts its largest magnitude in [2^14, 2^15);
+ * values become round-to-nearest-even multiples of 2^e (16-bit mantissas). */
+static void emu_round_blocks(oracle_sim* S, double* a) {
+    const int B = S->emu_block;
+    const long rows = (long)(S->N / S->NzT);
+    const int nz = (int)S->NzT;
+#pragma omp parallel for
+    for (long r = 0; r < rows; ++r) {
+        double* row = a + r * nz;
+        for (int k0 = 0; k0 < nz; k0 += B) {
+            const int k1 = k0 + B < nz ? k0 + B : nz;
+            double m = 0.0;
+            for (int k = k0; k < k1; ++k) m = fmax(m, fabs(row[k]));
+            if (!(m > 0.0) || !(m == m) || m > 1e300) continue;  /* zeros, NaN/Inf: unchanged */
+            int e;
+            frexp(m, &e);             /* m in [2^(e-1), 2^e) */
+            const double q = ldexp(1.0, e - 15); /* m / q in [2^14, 2^15) */
+            for (int k = k0; k < k1; ++k) row[k] = nearbyint(row[k] / q) * q;
+        }
+    }
+}
 static void emu_round(oracle_sim* S, double* a, int arr) {
+    if (S->emu_block) {
+        emu_round_blocks(S, a);
+        return;
+    }
     const double sc = S->emu_sc[arr];
     const int bits = arr >= 6 ? S->emu_psi_bits : S->emu_bits;
     const long n = (long)S->N;
@@ -1199,8 +1231,9 @@ double oracle_round_store(double y, int bits, int stochastic) {
 
 /* fmt 0 turns the emulation off. */
 int oracle_set_storage(oracle_sim* S, int fmt, const double* scales18, unsigned long long seed) {
-    if (fmt < 0 || fmt > 4 || (fmt && !scales18)) return -1;
+    if (fmt < 0 || fmt > 7 || (fmt && !scales18)) return -1;
     S->emu = fmt;
+    S->emu_block = fmt == 5 ? 32 : fmt == 6 ? 64 : fmt == 7 ? 8 : 0;
     S->emu_bits = fmt == 3 ? 19 : 11;
     S->emu_psi_bits = fmt == 3 ? 19 : fmt == 4 ? 24 : 11;
     S->emu_stoch = fmt == 2;

@@ -240,7 +240,8 @@ class CpuSim:
         self.f["get_dft"](self.h, m, out.ctypes.data_as(C.c_void_p))
         return out
 
-    STORAGE = {"fp64": 0, "fp16": 1, "fp16_stochastic": 2, "fp16_residual8": 3, "fp16_psi32": 4}
+    STORAGE = {"fp64": 0, "fp16": 1, "fp16_stochastic": 2, "fp16_residual8": 3, "fp16_psi32": 4,
+               "bfp16_32": 5, "bfp16_64": 6, "bfp16_8": 7}
 
     def set_storage(self, fmt: str, scales18=None, seed: int = 1) -> None:
         """Emulated storage rounding of fields and psi (oracle only): see
