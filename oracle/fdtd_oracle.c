@@ -1151,12 +1151,11 @@ static double round_store(double y, int bits, int stoch, uint64_t* rng) {
     return r;
 }
 /* Block floating point (formats 5-7): each block of B consecutive k of a
- * row shares the exponent that puts its largest magnitude in [2^14, 2^15);
- * values become round-to-nearest-even multiples of 2^e (16-bit mantissas). */
-static void emu_round_blocks(oracle_sim* S, double* a) {
-    const int B = S->emu_block;
-    const long rows = (long)(S->N / S->NzT);
-    const int nz = (int)S->NzT;
+ * row shares the exponent that puts its largest magnitude in [2^14, 2^15)
+ * (the next one up where that largest value would round to 2^15, so every
+ * mantissa fits a signed 16-bit integer); values become round-to-nearest-
+ * even multiples of the block's quantum.  rows x nz values, row-major. */
+static void round_blocks(double* a, long rows, int nz, int B) {
 #pragma omp parallel for
     for (long r = 0; r < rows; ++r) {
         double* row = a + r * nz;
@@ -1166,12 +1165,18 @@ static void emu_round_blocks(oracle_sim* S, double* a) {
             for (int k = k0; k < k1; ++k) m = fmax(m, fabs(row[k]));
             if (!(m > 0.0) || !(m == m) || m > 1e300) continue;  /* zeros, NaN/Inf: unchanged */
             int e;
-            frexp(m, &e);             /* m in [2^(e-1), 2^e) */
-            const double q = ldexp(1.0, e - 15); /* m / q in [2^14, 2^15) */
+            frexp(m, &e);                        /* m in [2^(e-1), 2^e) */
+            double q = ldexp(1.0, e - 15);       /* m / q in [2^14, 2^15) */
+            if (nearbyint(m / q) > 32767.0) q *= 2.0;
             for (int k = k0; k < k1; ++k) row[k] = nearbyint(row[k] / q) * q;
         }
     }
 }
+static void emu_round_blocks(oracle_sim* S, double* a) {
+    round_blocks(a, (long)(S->N / S->NzT), (int)S->NzT, S->emu_block);
+}
+/* The block rounding on its own (for tests). */
+void oracle_round_blocks(double* a, long rows, int nz, int B) { round_blocks(a, rows, nz, B); }
 static void emu_round(oracle_sim* S, double* a, int arr) {
     if (S->emu_block) {
         emu_round_blocks(S, a);

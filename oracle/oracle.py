@@ -78,6 +78,7 @@ def _lib(impl: str):
             "get_dft": (C.c_long, [vp, C.c_int, vp]),
             "set_storage": (C.c_int, [vp, C.c_int, vp, C.c_ulonglong]),
             "round_store": (C.c_double, [C.c_double, C.c_int, C.c_int]),
+            "round_blocks": (None, [vp, C.c_long, C.c_int, C.c_int]),
             "time_steps": (C.c_double, [vp, C.c_int]),
             "last_conservation": (C.c_double, [vp]),
         })

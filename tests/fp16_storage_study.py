@@ -6,8 +6,9 @@ oracle, for the emulated storage formats of oracle_set_storage
     python tests/fp16_storage_study.py [shape] [steps] [checkpoints]
 
 Defaults: the precision geometry (graded z profile, gain slab, 64 cuboids)
-on a 96x96x80 grid, 4000 steps, reported every 1000 steps.  Prints one JSON
-line per (format, checkpoint)."""
+on a 96x96x80 grid, 4000 steps, reported every 1000 steps, formats fp16,
+fp16_residual8, fp16_psi32, fp16_stochastic (bfp16_8, bfp16_32, bfp16_64:
+block floating point).  Prints one JSON line per (format, checkpoint)."""
 import importlib
 import json
 import os
