@@ -139,6 +139,34 @@ struct fdtd_sim {
     long srcv_cap = 0;
     int device = 0;                // the device the handle was created on
     bool mon_fused_h = true;       // monitor field maxima in the next H launch (else a separate pass)
+    // Step graphs (fdtd_set_step_graphs): a call's launches recorded once in
+    // a HIP graph and replayed for later calls of the same shape.  The
+    // per-step values that differ between calls (DFT phase factors, probe
+    // record slots) then come from a device table written before each
+    // launch; tab_step is the current step's row of it while launching.
+    bool graph_ticks = false;
+    const double* tab_step = nullptr;
+    std::vector<int> tab_dft_off;  // offsets of each DFT's 2*nf phases in a row
+    int tab_probe_off = 0, tab_stride = 0;
+    double* d_tab = nullptr;
+    long tab_cap = 0;
+    // pinned host staging of a call's uploads (source values, then the step
+    // table): a copy from pinned memory is queued without the host thread
+    // waiting on the copy engine (a pageable copy is staged synchronously)
+    double* h_pin = nullptr;
+    size_t pin_words = 0;
+    // pinned, coherent host copy of a call's monitor records and the FP16
+    // non-finite flags, written by a copy kernel at the end of the call
+    // (h_rec: host address, d_rec: its device address)
+    unsigned long long* h_rec = nullptr;
+    unsigned long long* d_rec = nullptr;
+    size_t rec_words = 0;
+    bool rec_host = false;  // the call in flight copies its records to h_rec
+    struct StepGraph {
+        std::vector<long> key;
+        hipGraphExec_t exec;
+    };
+    std::vector<StepGraph> graphs;
     double* srcv_cur = nullptr;    // source values of the call in flight
     MonOut* mon_cur = nullptr;     // monitor records of the call in flight
     // probes / dft
@@ -169,6 +197,9 @@ struct fdtd_sim {
 
     ~fdtd_sim() {
         if (stream) hipStreamSynchronize(stream);
+        for (auto& g : graphs) hipGraphExecDestroy(g.exec);
+        if (h_pin) hipHostFree(h_pin);
+        if (h_rec) hipHostFree(h_rec);
         for (void* p : allocs) hipFree(p);
         if (stream) hipStreamDestroy(stream);
     }
@@ -684,6 +715,17 @@ constexpr long MONPART_MAX = 2048;  // blocks (and partial records) of the monit
 
 void dispatch_monitor(fdtd_sim* s, MonOut* slot);
 
+// DFT phase factors of step n (the step's field time, times dt), as the
+// DFT monitor accumulates them.
+void dft_phases(const fdtd_sim* s, const Dft& d, long n, double* wr, double* wi) {
+    const double t = (d.comp < 3) ? ((double)n + 1.0) * s->dt : ((double)n + 0.5) * s->dt;
+    for (int f = 0; f < d.nf; ++f) {
+        const double w = 2.0 * 3.14159265358979323846 * d.freqs[f];
+        wr[f] = std::cos(w * t) * s->dt;
+        wi[f] = -std::sin(w * t) * s->dt;
+    }
+}
+
 template <typename T, typename R>
 int launch_step(fdtd_sim* s, long n, int sidx, hipEvent_t* ev) {
     StepArgs<T, R> a = make_args<T, R>(s);
@@ -756,14 +798,24 @@ int launch_step(fdtd_sim* s, long n, int sidx, hipEvent_t* ev) {
         }
         if (ev) (void)hipEventRecord(ev[3], s->stream);
     }
-    // probes: PointFieldDetector records when n % save_every == 0
+    // probes: PointFieldDetector records when n % save_every == 0 (with a
+    // step table, every probe rides in every step and the table holds this
+    // step's record slot, or null)
     fdtdk::ProbeArgs pa{};
+    const bool tab = s->tab_step != nullptr;
+    pa.ind = tab ? 1 : 0;
+    int pidx = 0;
     for (auto& p : s->probes) {
-        if (n % p.save_every != 0 || p.count >= p.cap) continue;
+        const int q = pidx++;
+        if (tab) {
+            pa.p[pa.n].out = (double*)(s->tab_step + s->tab_probe_off + q);
+        } else {
+            if (n % p.save_every != 0 || p.count >= p.cap) continue;
+            pa.p[pa.n].out = p.d_buf + p.count;
+            p.count++;
+        }
         pa.p[pa.n].id = p.id;
         pa.p[pa.n].comp = p.comp;
-        pa.p[pa.n].out = p.d_buf + p.count;
-        p.count++;
         if (++pa.n == 16) {
             hipLaunchKernelGGL((fdtdk::k_probes<T, R>), dim3(1), dim3(64), 0, s->stream, a, pa);
             pa.n = 0;
@@ -774,7 +826,8 @@ int launch_step(fdtd_sim* s, long n, int sidx, hipEvent_t* ev) {
         hipLaunchKernelGGL((fdtdk::k_probes<T, R>), dim3(1), dim3(64), 0, s->stream, a, pa);
         pa.n = 0;
     }
-    for (auto& d : s->dfts) {
+    for (size_t m = 0; m < s->dfts.size(); ++m) {
+        const Dft& d = s->dfts[m];
         fdtdk::DftLaunch L{};
         fdtdk::DftArgs& da = L.d;
         da.comp = d.comp;
@@ -783,12 +836,8 @@ int launch_step(fdtd_sim* s, long n, int sidx, hipEvent_t* ev) {
         da.nz_b = d.box[5] - d.box[4];
         da.ncell = d.ncell;
         da.nf = d.nf;
-        const double t = (d.comp < 3) ? ((double)n + 1.0) * s->dt : ((double)n + 0.5) * s->dt;
-        for (int f = 0; f < d.nf; ++f) {
-            const double w = 2.0 * 3.14159265358979323846 * d.freqs[f];
-            da.wr[f] = std::cos(w * t) * s->dt;
-            da.wi[f] = -std::sin(w * t) * s->dt;
-        }
+        if (tab) da.wtab = s->tab_step + s->tab_dft_off[m];
+        else dft_phases(s, d, n, da.wr, da.wi);
         da.acc = d.d_acc;
         L.fz = fdtdk::make_fastdiv((unsigned)da.nz_b);
         L.fy = fdtdk::make_fastdiv((unsigned)da.ny_b);
@@ -1305,6 +1354,17 @@ int fdtd_set_kernel_mode(fdtd_handle h, int mode, int param) {
     return FDTD_OK;
 }
 
+int fdtd_set_step_graphs(fdtd_handle h, int on) {
+    if (!h) return fail(FDTD_EINVAL, "null handle");
+    if (int rc = step_finish(h)) return rc;
+    h->graph_ticks = on != 0;
+    if (!h->graph_ticks) {
+        for (auto& g : h->graphs) (void)hipGraphExecDestroy(g.exec);
+        h->graphs.clear();
+    }
+    return FDTD_OK;
+}
+
 int fdtd_set_monitor(fdtd_handle h, int interval) {
     if (!h || interval < 0) return fail(FDTD_EINVAL, "bad monitor interval");
     h->mon_interval = interval;
@@ -1351,13 +1411,23 @@ static int step_finish(fdtd_handle h) {
         return FDTD_OK;
     }
     std::vector<MonOut> mo(mon_steps.size());
-    HIPCHK(hipMemcpyAsync(mo.data(), h->mon_cur, mo.size() * sizeof(MonOut), hipMemcpyDeviceToHost, h->stream));
     // FP16 storage: the scale planner's sticky flags of a NaN/Inf stored in
     // an E-class or H-class array (fields or psi)
     unsigned nonfinite[2] = {0u, 0u};
-    if (h->prec == FDTD_FP16)
-        HIPCHK(hipMemcpyAsync(nonfinite, h->d_factor + 18, sizeof(nonfinite), hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
+    if (h->rec_host) {
+        // the call's last launches copied both into pinned host memory: no
+        // copy to queue now behind other streams' work on the hardware queue
+        HIPCHK(hipStreamSynchronize(h->stream));
+        std::memcpy(mo.data(), h->h_rec, mo.size() * sizeof(MonOut));
+        if (h->prec == FDTD_FP16)
+            std::memcpy(nonfinite, h->h_rec + mo.size() * (sizeof(MonOut) / sizeof(unsigned long long)),
+                        sizeof(nonfinite));
+    } else {
+        HIPCHK(hipMemcpyAsync(mo.data(), h->mon_cur, mo.size() * sizeof(MonOut), hipMemcpyDeviceToHost, h->stream));
+        if (h->prec == FDTD_FP16)
+            HIPCHK(hipMemcpyAsync(nonfinite, h->d_factor + 18, sizeof(nonfinite), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+    }
     bool ok = true;
     long fail_step = -1;
     for (size_t q = 0; q < mo.size(); ++q) {
@@ -1382,6 +1452,127 @@ static int step_finish(fdtd_handle h) {
     return FDTD_OK;
 }
 
+constexpr int REC_FLAG_WORDS = 1;  // the FP16 non-finite flags (two 32-bit words)
+// Host copy of a call's monitor records (and FP16 flags): grown before the
+// call's launches, never while a call is in flight.
+static int ensure_host_records(fdtd_handle h, size_t nrec) {
+    const size_t words = nrec * (sizeof(MonOut) / sizeof(unsigned long long)) + REC_FLAG_WORDS;
+    if (words <= h->rec_words) return FDTD_OK;
+    if (h->h_rec) HIPCHK(hipHostFree(h->h_rec));
+    h->h_rec = h->d_rec = nullptr;
+    h->rec_words = 0;
+    const size_t cap = std::max<size_t>(words, 64 * (sizeof(MonOut) / sizeof(unsigned long long)) + REC_FLAG_WORDS);
+    if (hipHostMalloc((void**)&h->h_rec, cap * sizeof(unsigned long long), hipHostMallocMapped | hipHostMallocCoherent) !=
+        hipSuccess)
+        return fail(FDTD_ENOMEM, "hipHostMalloc of the monitor record copy failed");
+    HIPCHK(hipHostGetDevicePointer((void**)&h->d_rec, h->h_rec, 0));
+    h->rec_words = cap;
+    return FDTD_OK;
+}
+// The copy kernels at the end of a call: records, then the FP16 flags.
+static void enqueue_record_copy(fdtd_handle h, size_t nrec) {
+    const long rw = (long)(nrec * (sizeof(MonOut) / sizeof(unsigned long long)));
+    if (rw > 0)
+        hipLaunchKernelGGL(fdtdk::k_copy_words, dim3(1), dim3(64), 0, h->stream, (const unsigned long long*)h->mon_cur,
+                           h->d_rec, rw);
+    if (h->prec == FDTD_FP16)
+        hipLaunchKernelGGL(fdtdk::k_copy_words, dim3(1), dim3(64), 0, h->stream,
+                           (const unsigned long long*)(h->d_factor + 18), h->d_rec + rw, (long)REC_FLAG_WORDS);
+}
+
+// The launches of one call (steps n0 .. n0 + nsteps - 1) as a replayed HIP
+// graph.  The call's table (each step's DFT phase factors and probe record
+// slots) is uploaded first; the graph for this call's shape is recorded on
+// the first call of that shape: the launches are the direct path's, with
+// the monitor records cleared by a fill kernel (no memset node in a graph)
+// and the per-step values read from the table.  The shape is everything the
+// launch sequence and its arguments depend on: the step count, the step's
+// phase in the monitor and FP16 cadences, the kernel schedule and the device
+// buffers the launches name.
+static int step_graph_launch(fdtd_handle h, long n0, int nsteps, const std::vector<long>& mon_steps,
+                             double* host_tab) {
+    // the table: per step, each DFT's nf cos and nf sin factors, then one
+    // word per probe (its record slot's address, or null)
+    h->tab_dft_off.assign(h->dfts.size(), 0);
+    int stride = 0;
+    for (size_t m = 0; m < h->dfts.size(); ++m) {
+        h->tab_dft_off[m] = stride;
+        stride += 2 * h->dfts[m].nf;
+    }
+    h->tab_probe_off = stride;
+    stride += (int)h->probes.size();
+    h->tab_stride = stride;
+    const long words = (long)stride * nsteps;
+    if (words > h->tab_cap) {
+        void* p;
+        if (int rc = h->alloc(&p, (size_t)words * sizeof(double))) return rc;
+        h->d_tab = (double*)p;
+        h->tab_cap = words;
+    }
+    for (int st = 0; st < nsteps; ++st) {
+        const long n = n0 + st;
+        double* row = host_tab + (size_t)st * stride;
+        for (size_t m = 0; m < h->dfts.size(); ++m) {
+            const Dft& d = h->dfts[m];
+            dft_phases(h, d, n, row + h->tab_dft_off[m], row + h->tab_dft_off[m] + d.nf);
+        }
+        for (size_t q = 0; q < h->probes.size(); ++q) {
+            Probe& p = h->probes[q];
+            double* slot = nullptr;
+            if (n % p.save_every == 0 && p.count < p.cap) slot = p.d_buf + p.count++;
+            std::memcpy(row + h->tab_probe_off + q, &slot, sizeof(slot));
+        }
+    }
+    if (words > 0)
+        HIPCHK(hipMemcpyAsync(h->d_tab, host_tab, (size_t)words * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    std::vector<long> key = {nsteps,
+                             h->mon_interval > 0 ? n0 % h->mon_interval : -1,
+                             h->prec == FDTD_FP16 ? n0 % h->fp16_interval : -1,
+                             h->mon_interval,
+                             h->prec == FDTD_FP16 ? h->fp16_interval : 0,
+                             h->kwid_h, h->kwid_e, h->force64, h->erev, h->mon_fused_h, (long)h->band_tiles, h->cur,
+                             (long)h->probes.size(), (long)h->dfts.size(), h->nsrc,
+                             (long)(uintptr_t)h->srcv_cur, (long)(uintptr_t)h->mon_cur, (long)(uintptr_t)h->d_tab,
+                             (long)(uintptr_t)h->F2[h->cur ^ 1][0], (long)(uintptr_t)h->d_rec, h->rec_host};
+    hipGraphExec_t exec = nullptr;
+    for (auto& g : h->graphs)
+        if (g.key == key) exec = g.exec;
+    if (!exec) {
+        hipGraph_t graph = nullptr;
+        HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed));
+        if (!mon_steps.empty()) {
+            const long mw = (long)(mon_steps.size() * sizeof(MonOut) / sizeof(unsigned long long));
+            hipLaunchKernelGGL(fdtdk::k_zero_words, dim3((unsigned)std::min<long>((mw + 255) / 256, 64)), dim3(256), 0,
+                               h->stream, (unsigned long long*)h->mon_cur, mw);
+        }
+        int slot = 0;
+        for (int st = 0; st < nsteps; ++st) {
+            const long n = n0 + st;
+            h->tab_step = h->d_tab + (size_t)st * stride;
+            dispatch_step(h, n, st, nullptr);
+            if (h->mon_interval > 0 && n % h->mon_interval == 0) {
+                MonOut* rec = h->mon_cur + slot++;
+                if (!(st + 1 < nsteps && h->mon_fused_h && dispatch_monitor_begin(h, rec))) dispatch_monitor(h, rec);
+            }
+            if (h->prec == FDTD_FP16 && n % h->fp16_interval == 0) fp16_maintain(h, h->stream);
+        }
+        h->tab_step = nullptr;
+        if (h->rec_host) enqueue_record_copy(h, mon_steps.size());
+        const hipError_t e = hipStreamEndCapture(h->stream, &graph);
+        if (e != hipSuccess) return fail(FDTD_EDEVICE, std::string("step graph capture: ") + hipGetErrorString(e));
+        const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess) return fail(FDTD_EDEVICE, std::string("step graph: ") + hipGetErrorString(ei));
+        if (h->graphs.size() >= 8) {  // keep the most recent shapes
+            (void)hipGraphExecDestroy(h->graphs.front().exec);
+            h->graphs.erase(h->graphs.begin());
+        }
+        h->graphs.push_back(fdtd_sim::StepGraph{key, exec});
+    }
+    HIPCHK(hipGraphLaunch(exec, h->stream));
+    return FDTD_OK;
+}
+
 // Enqueues nsteps steps on the handle's stream (and, with ms, the timing
 // events); the monitor records are read by step_finish.
 // ev (optional) receives HIP events around the H and E launches of every
@@ -1395,6 +1586,9 @@ static int step_launch(fdtd_handle h, long n0, int nsteps, const double* src_val
     if (nsteps == 0) return FDTD_OK;
     if (h->nsrc > 0 && !src_values) return fail(FDTD_EINVAL, "src_values required");
     const long need = (long)nsteps * h->nsrc;
+    // a recorded call: the two-kernel step without timing events or TLS
+    // diagnostics (whose launches depend on more than the step's phase)
+    const bool use_graph = h->graph_ticks && !noalloc && !ev && h->mode == 0 && h->tls_every == 0;
     // monitor slots for this call
     std::vector<long> mon_steps;
     if (h->mon_interval > 0)
@@ -1433,10 +1627,42 @@ static int step_launch(fdtd_handle h, long n0, int nsteps, const double* src_val
         }
         h->srcv_cur = h->d_srcv;
         h->mon_cur = h->d_mon;
-        if (!mon_steps.empty()) HIPCHK(hipMemsetAsync(h->d_mon, 0, mon_steps.size() * sizeof(MonOut), h->stream));
+        if (!mon_steps.empty() && !use_graph)
+            HIPCHK(hipMemsetAsync(h->d_mon, 0, mon_steps.size() * sizeof(MonOut), h->stream));
     }
+    // the table's words (graph calls): per step, each DFT's 2 nf phase
+    // factors and one word per probe
+    long tab_words = 0;
+    if (use_graph) {
+        long row = (long)h->probes.size();
+        for (const Dft& d : h->dfts) row += 2 * d.nf;
+        tab_words = row * nsteps;
+    }
+    const double* up = src_values;
+    if (!noalloc && need + tab_words > 0) {
+        if ((size_t)(need + tab_words) > h->pin_words) {
+            if (h->h_pin) HIPCHK(hipHostFree(h->h_pin));
+            h->h_pin = nullptr;
+            h->pin_words = 0;
+            const size_t words = std::max<size_t>((size_t)(need + tab_words), 4096);
+            if (hipHostMalloc((void**)&h->h_pin, words * sizeof(double), hipHostMallocDefault) != hipSuccess)
+                return fail(FDTD_ENOMEM, "hipHostMalloc of the upload staging failed");
+            h->pin_words = words;
+        }
+        if (need > 0) std::memcpy(h->h_pin, src_values, (size_t)need * sizeof(double));
+        up = h->h_pin;
+    }
+    h->rec_host = !noalloc && !mon_steps.empty();
+    if (h->rec_host)
+        if (int rc = ensure_host_records(h, mon_steps.size())) return rc;
     if (need > 0)
-        HIPCHK(hipMemcpyAsync(h->srcv_cur, src_values, (size_t)need * sizeof(double), hipMemcpyHostToDevice, h->stream));
+        HIPCHK(hipMemcpyAsync(h->srcv_cur, up, (size_t)need * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    if (use_graph) {
+        if (int rc = step_graph_launch(h, n0, nsteps, mon_steps, h->h_pin + need)) return rc;
+        h->pending = true;
+        h->pending_mon = std::move(mon_steps);
+        return FDTD_OK;
+    }
     int slot = 0;
     if (ev) {
         ev->resize((size_t)((nsteps + every - 1) / every) * 4 + 2);
@@ -1453,6 +1679,7 @@ static int step_launch(fdtd_handle h, long n0, int nsteps, const double* src_val
         }
         if (h->prec == FDTD_FP16 && n % h->fp16_interval == 0) fp16_maintain(h, h->stream);
     }
+    if (h->rec_host) enqueue_record_copy(h, mon_steps.size());
     HIPCHK(hipGetLastError());
     if (ev) HIPCHK(hipEventRecord(ev->back(), h->stream));
     h->pending = true;

@@ -944,8 +944,16 @@ struct ProbeSlot {
 };
 struct ProbeArgs {
     int n;
+    // 0: out is the record's address; 1: out holds the address of a device
+    // word that holds it (null: no record this step), so a launch recorded
+    // in a graph takes each replay's slots from a table written before it
+    int ind;
     ProbeSlot p[16];
 };
+__device__ __forceinline__ void probe_store(const ProbeArgs& pa, int t, double v) {
+    double* dst = pa.ind ? *reinterpret_cast<double* const*>(pa.p[t].out) : pa.p[t].out;
+    if (dst) *dst = v;
+}
 
 template <typename T, typename R>
 __device__ __forceinline__ double field_value(const StepArgs<T, R>& a, int comp, long id) {
@@ -1070,7 +1078,7 @@ template <typename T, typename R>
 __global__ void k_probes(StepArgs<T, R> a, ProbeArgs pa) {
     const int t = threadIdx.x;
     if (t >= pa.n) return;
-    *pa.p[t].out = field_value<T, R>(a, pa.p[t].comp, pa.p[t].id);
+    probe_store(pa, t, field_value<T, R>(a, pa.p[t].comp, pa.p[t].id));
 }
 
 struct DftArgs {
@@ -1078,8 +1086,14 @@ struct DftArgs {
     long ncell;
     int nf;
     double wr[8], wi[8];
+    // where non-null, the phase factors are read from device memory
+    // (wtab[f] and wtab[nf + f]) instead of wr/wi: a launch recorded in a
+    // graph takes each replay's phases from a table written before it
+    const double* wtab;
     double* acc;
 };
+__device__ __forceinline__ double dft_wr(const DftArgs& d, int f) { return d.wtab ? d.wtab[f] : d.wr[f]; }
+__device__ __forceinline__ double dft_wi(const DftArgs& d, int f) { return d.wtab ? d.wtab[d.nf + f] : d.wi[f]; }
 
 template <typename T, typename R>
 __device__ __forceinline__ void dft_cell(const StepArgs<T, R>& a, const DftArgs& d, long c) {
@@ -1115,7 +1129,7 @@ __global__ void __launch_bounds__(256) k_observe(StepArgs<T, R> a, DftLaunch L, 
     const unsigned nb_dft = L.per_cell ? L.bpf : L.bpf * (unsigned)L.d.nf;
     if (blockIdx.x >= nb_dft) {
         const int t = threadIdx.x;
-        if (t < pa.n) *pa.p[t].out = field_value<T, R>(a, pa.p[t].comp, pa.p[t].id);
+        if (t < pa.n) probe_store(pa, t, field_value<T, R>(a, pa.p[t].comp, pa.p[t].id));
         return;
     }
     if (L.per_cell) {
@@ -1129,8 +1143,8 @@ __global__ void __launch_bounds__(256) k_observe(StepArgs<T, R> a, DftLaunch L, 
         for (int f = 0; f < L.d.nf; ++f) {
             double* p = L.d.acc + ((long)f * L.d.ncell + c) * 2;
             const double re = p[0], im = p[1];
-            p[0] = re + F * L.d.wr[f];
-            p[1] = im + F * L.d.wi[f];
+            p[0] = re + F * dft_wr(L.d, f);
+            p[1] = im + F * dft_wi(L.d, f);
         }
         return;
     }
@@ -1144,8 +1158,8 @@ __global__ void __launch_bounds__(256) k_observe(StepArgs<T, R> a, DftLaunch L, 
     double* p = L.d.acc + ((long)f * L.d.ncell + c) * 2;
     const double re = p[0], im = p[1];
     const double F = field_value<T, R>(a, L.d.comp, id);
-    p[0] = re + F * L.d.wr[f];
-    p[1] = im + F * L.d.wi[f];
+    p[0] = re + F * dft_wr(L.d, (int)f);
+    p[1] = im + F * dft_wi(L.d, (int)f);
 }
 
 // Stability reduction (StabilityMonitor::check_stability inputs,
@@ -1363,6 +1377,14 @@ __global__ void __launch_bounds__(64) k_monitor_final(const MonOut* __restrict__
         pi += __shfl_down(pi, off);
     }
     if (threadIdx.x == 0) *slot = MonOut{mE, mH, mP, nN, pt, pi};
+}
+
+// Copies n 64-bit words (device records into pinned host memory at the end
+// of a call, so the host reads them once the stream is idle instead of
+// queueing a copy behind other streams' work).
+__global__ void __launch_bounds__(64) k_copy_words(const unsigned long long* __restrict__ src,
+                                                   unsigned long long* __restrict__ dst, long n) {
+    for (long q = threadIdx.x; q < n; q += 64) dst[q] = src[q];
 }
 
 // Zeroes n 64-bit words: a kernel rather than a memset node, so that the

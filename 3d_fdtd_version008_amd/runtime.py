@@ -82,6 +82,7 @@ def load() -> C.CDLL:
     vp, i, l, d = C.c_void_p, C.c_int, C.c_long, C.c_double
     sigs = {
         "fdtd_last_error": (C.c_char_p, []),
+        "fdtd_set_step_graphs": (i, [vp, i]),
         "fdtd_destroy": (i, [vp]),
         "fdtd_device_bytes": (i, [vp, C.POINTER(C.c_size_t)]),
         "fdtd_download_material": (i, [vp, i, C.c_void_p]),
@@ -281,6 +282,12 @@ class Scene:
         r = self.lib.fdtd_scene_step_query(self.sc)
         _check(min(r, 0))
         return r == 1
+
+    def set_step_graphs(self, on: bool = True):
+        """Record each stepping call's launches in a HIP graph replayed for
+        later calls of the same shape (fdtd_set_step_graphs); bitwise the
+        direct launches' results."""
+        _check(self.lib.fdtd_set_step_graphs(self.h, int(on)))
 
     def set_kernel_mode(self, mode: int, param: int = 0):
         """0: separate H/E kernels, `param` = wavefront k-width (16, 32, 64;

@@ -145,7 +145,7 @@ class Server:
     def __init__(self, budget_bytes: float = 200e9, max_active: int = 256, tick: int = 50, graph: bool = True,
                  setup_workers: int = 8, tuning: Optional[tuple] = None, engine: str = "streams",
                  small_cells: int = 2_000_000, enqueue_workers: int = 4, stream_workers: int = 0,
-                 defer_free: bool = False, host_bake: bool = False):
+                 defer_free: bool = False, host_bake: bool = False, step_graphs: bool = True):
         if engine not in ("streams", "batch", "hybrid"):
             raise ValueError("engine must be 'streams', 'batch' or 'hybrid'")
         self.engine = engine
@@ -161,6 +161,10 @@ class Server:
         # bake materials on the host in the setup pool instead of on the
         # device at admission (the device bake frees its temporaries)
         self.host_bake = host_bake
+        # requests stepped on their own streams replay each tick from a HIP
+        # graph (fdtd_set_step_graphs: bitwise the direct launches), so a tick
+        # costs the host one launch instead of a few per step
+        self.step_graphs = step_graphs
         self.budget = int(budget_bytes)
         self.max_active = max_active
         self.tick = tick
@@ -238,6 +242,8 @@ class Server:
                 return self._refused(req, setup, ta, e)
             lv = _Live(req, setup, sc, req.steps, now(), sc.device_bytes())
             try:
+                if self.step_graphs:
+                    sc.set_step_graphs(True)
                 sc.step_async(min(self.tick, req.steps))
                 count("ticks", 1)
             except runtime.FdtdError as e:
@@ -408,6 +414,8 @@ class Server:
                         batched = self.engine == "batch" or (self.engine == "hybrid" and cells < self.small_cells)
                         if batched:
                             self._batch(req.precision).add(sc)
+                        elif self.step_graphs:
+                            sc.set_step_graphs(True)
                         live.append(_Live(req, setup, sc, req.steps, now(), actual, batched))
                         used += actual
                         st["admit_s"] += now() - ta

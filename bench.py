@@ -389,7 +389,7 @@ def serve_form(args):
     reqs = serve.request_stream(args.requests, rate=args.rate, seed=1)
     srv = serve.Server(budget_bytes=200e9, max_active=256, tick=args.tick, graph=True, setup_workers=8,
                        engine=args.engine, stream_workers=args.workers, defer_free=args.defer_free,
-                       host_bake=args.host_bake)
+                       host_bake=args.host_bake, step_graphs=not args.no_step_graphs)
     res = srv.run(reqs)
     summ = serve.summarize(res)
     sample = reqs[:: max(1, len(reqs) // args.solo_sample)][: args.solo_sample] if args.solo_sample else []
@@ -410,6 +410,7 @@ def serve_form(args):
             "config": {"requests": args.requests, "arrival_rate_per_s": args.rate,
                        "fp16_share": 0.5, "budget_gb": 200, "tick_steps": args.tick, "engine": args.engine,
                        "workers": args.workers, "deferred_free": args.defer_free,
+                       "step_graphs": not args.no_step_graphs,
                        "bake": "host" if args.host_bake else "device", "makespan_s": round(summ["makespan_s"], 2),
                        "cell_updates": summ["cell_updates"], "failed": summ["failed"],
                        "gpu_max_hw_queues": os.environ.get("GPU_MAX_HW_QUEUES", "runtime default")},
@@ -455,6 +456,8 @@ def main():
     ap.add_argument("--engine", default="streams", choices=["streams", "batch", "hybrid"], help="serve form: request engine")
     ap.add_argument("--solo-sample", type=int, default=16, help="batch/serve forms: one-at-a-time sample size")
     ap.add_argument("--tick", type=int, default=50, help="serve form: steps per tick")
+    ap.add_argument("--no-step-graphs", action="store_true",
+                    help="serve form: requests on streams launch every step directly instead of replaying graphs")
     ap.add_argument("--workers", type=int, default=0,
                     help="serve form, streams engine: threads making the per-request device calls (0: the loop thread)")
     ap.add_argument("--defer-free", action="store_true",

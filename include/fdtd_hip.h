@@ -212,6 +212,18 @@ int fdtd_get_fp16_scales(fdtd_handle h, double* scales18);
 int fdtd_set_kernel_mode(fdtd_handle h, int mode, int param);
 /* Stability monitor cadence (STABILITY_MONITOR_INTERVAL, default 10; 0 = off). */
 int fdtd_set_monitor(fdtd_handle h, int interval);
+/* on != 0: each stepping call on the two-kernel schedule records its
+ * launches once in a HIP graph and replays it for later calls of the same
+ * shape (step count, step phase in the monitor and FP16 cadences, schedule
+ * and device buffers), so a call costs the host one graph launch and two
+ * small uploads instead of a few launches per step.  The DFT phase factors
+ * and probe record slots of each call's steps come from a device table
+ * written before the launch; results are those of the direct launches, bit
+ * for bit.  Calls with timing events, TLS diagnostics or the fused schedule
+ * launch directly.  Off (the default) frees the recorded graphs.  No
+ * reference counterpart (for serving loops that keep many simulations
+ * stepping). */
+int fdtd_set_step_graphs(fdtd_handle h, int on);
 
 /* ---- time stepping ----
  * Advances nsteps steps starting at global step n0, in the reference's step
