@@ -46,6 +46,9 @@ class FakeScene:
             FakeScene.live += 1
             FakeScene.peak = max(FakeScene.peak, FakeScene.live)
 
+    def set_step_graphs(self, on=True):
+        self.graphs = bool(on)
+
     def step_async(self, k):
         assert not self.closed and self.pending == 0 and k > 0
         self.pending = k
