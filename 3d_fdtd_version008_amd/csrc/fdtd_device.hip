@@ -20,6 +20,7 @@
 #include "../../include/fdtd_hip.h"
 #include "fdtd_kernels.hpp"
 #include "fdtd_fused.hpp"
+#include "fdtd_flow.hpp"
 #include "fdtd_bake.hpp"
 
 using fdtdk::MonOut;
@@ -97,7 +98,13 @@ struct fdtd_sim {
     void* F[6] = {};    // = F2[cur]
     void* psi[12] = {}; // = psi2[cur] (psi_E always psi2[0])
     int cur = 0;
-    int mode = 0;       // 0: H and E kernels (default), 1: fused step kernel
+    int mode = 0;       // 0: H and E kernels (default), 1: fused step kernel, 2: dependency-ordered step
+    // mode 2 (k_step_flow): per (plane, tile) flags of finished H tiles, then
+    // of finished E tiles, and {step epoch, tiles not run << 16 | wait time-outs}
+    unsigned* d_flowflags = nullptr;
+    unsigned* d_flowctl = nullptr;
+    int flow_lag = 0;   // planes between a slot's H tiles and its E tiles (0: 2)
+    unsigned flow_wg = 0;  // work-groups of the persistent flow launch (0: not yet sized)
     int ichunk = 0;     // x-planes per work-group of the fused kernel (0: by grid size)
     int fz_cw = 0;      // fused kernel: cells per lane per row chunk (0: by grid size)
     int fz_jw = 0;      // fused kernel: rows per work-group (0: by grid size)
@@ -230,6 +237,13 @@ std::vector<R> to_r(const double* p, size_t n) {
     return v;
 }
 
+// j-tiles per band of the H/E tile order (StepArgs::bj)
+unsigned tile_band(const fdtd_sim* s, unsigned tj) {
+    unsigned bj = s->band_tiles;
+    if (bj == 0) bj = 6.0 * s->ny * s->nz * (double)s->esz > 1.0 * (1 << 20) ? 16u : tj;
+    return std::min(std::max(bj, 1u), tj);
+}
+
 template <typename T, typename R>
 StepArgs<T, R> make_args(const fdtd_sim* s) {
     StepArgs<T, R> a{};
@@ -269,9 +283,7 @@ StepArgs<T, R> make_args(const fdtd_sim* s) {
     // plane's reads meet the lines this band-plane loaded in the L2.
     // Measured (HBM counters, 512x512x384 FP32): whole-plane order reads
     // 1.31x (H) and 1.27x (E) the algorithmic bytes, bands of 16 tiles 1.00x
-    unsigned bj = s->band_tiles;
-    if (bj == 0) bj = 6.0 * s->ny * s->nz * (double)s->esz > 1.0 * (1 << 20) ? 16u : tj;
-    bj = std::min(std::max(bj, 1u), tj);
+    const unsigned bj = tile_band(s, tj);
     const unsigned nfull = tj / bj, bjl = tj - nfull * bj;
     a.bj = (int)bj;
     a.tbp = fdtdk::make_fastdiv(bj * tk);
@@ -685,6 +697,83 @@ void launch_fused(fdtd_sim* s, StepArgs<T, R>& a) {
     default: launch_march_cw<T, R, 4>(s, a, g); break;
     }
 }
+// Geometry of the dependency-ordered step (k_step_flow): queues of planes
+// per XCD, bands of the two-kernel tile order, slots of H and E tiles.
+// false where a launch would not fit: tickets or flags beyond 2^31.
+bool flow_geom(const fdtd_sim* s, fdtdk::FlowGeom& g, unsigned& nwg, unsigned& nedge) {
+    const unsigned long tk = (unsigned long)((s->nz + 63) / 64),
+                        tj = (unsigned long)((s->ny + fdtdk::TILE_ROWS - 1) / fdtdk::TILE_ROWS);
+    const unsigned long bj = tile_band(s, (unsigned)tj);
+    const unsigned long nx = (unsigned long)s->nx;
+    g = fdtdk::FlowGeom{};
+    g.pbase = (unsigned)(nx / 8);
+    g.prem = (unsigned)(nx % 8);
+    const unsigned long pmax = g.pbase + (g.prem ? 1 : 0);
+    g.lag = s->flow_lag > 0 ? (unsigned)s->flow_lag : 2u;
+    const unsigned long tpp = tk * tj, nbf = tj / bj, bjl = tj - nbf * bj;
+    const unsigned long T = bj * tk, Tl = bjl * tk, slots = pmax + g.lag;
+    const unsigned long per_band = 2 * T * slots, tfull = nbf * per_band, tlast = 2 * Tl * slots;
+    const unsigned long tickets = tfull + tlast;
+    if (tickets >= (1ul << 31) || 2 * nx * tpp >= (1ul << 31)) return false;
+    g.tpp = (unsigned)tpp;
+    g.nbf = (unsigned)nbf;
+    g.tfull = (unsigned)tfull;
+    g.tlast = (unsigned)tlast;
+    g.band_t = fdtdk::make_fastdiv((unsigned)per_band);
+    g.slot_t = fdtdk::make_fastdiv((unsigned)(2 * T));
+    g.slot_l = fdtdk::make_fastdiv(Tl ? (unsigned)(2 * Tl) : 2u);
+    g.tppd = fdtdk::make_fastdiv((unsigned)tpp);
+    g.hflags = s->d_flowflags;
+    g.eflags = s->d_flowflags ? s->d_flowflags + nx * tpp : nullptr;
+    g.ctl = s->d_flowctl;
+    nwg = 0;  // persistent: set by the launch from the occupancy
+    nedge = (unsigned)(std::min<unsigned long>(8, nx) * tpp);
+    return true;
+}
+// The flags (zero: no step's epoch) and the control words ({1, 0}) of the
+// dependency-ordered step.
+int ensure_flow(fdtd_sim* s) {
+    if (s->d_flowctl) return FDTD_OK;
+    fdtdk::FlowGeom g{};
+    unsigned nwg = 0, nedge = 0;
+    if (!flow_geom(s, g, nwg, nedge)) return fail(FDTD_EINVAL, "mode 2: the grid is too large for one flow launch");
+    const size_t nflags = 2 * (size_t)s->nx * g.tpp;
+    unsigned ctl0[fdtdk::FLOW_CTL_WORDS] = {};
+    ctl0[0] = 1u;
+    void* f = nullptr;
+    void* c = nullptr;
+    if (int rc = s->alloc(&f, nflags * sizeof(unsigned))) return rc;
+    if (int rc = s->alloc(&c, sizeof(ctl0))) return rc;
+    HIPCHK(hipMemset(f, 0, nflags * sizeof(unsigned)));
+    HIPCHK(hipMemcpy(c, ctl0, sizeof(ctl0), hipMemcpyHostToDevice));
+    s->d_flowflags = (unsigned*)f;
+    s->d_flowctl = (unsigned*)c;
+    return FDTD_OK;
+}
+template <typename T, typename R, typename IX>
+void launch_flow_ix(fdtd_sim* s, StepArgs<T, R>& a, unsigned long long* fm, hipEvent_t* ev) {
+    fdtdk::FlowGeom g{};
+    unsigned nwg = 0, nedge = 0;
+    flow_geom(s, g, nwg, nedge);
+    // persistent: as many work-groups as stay resident at once
+    if (!s->flow_wg) {
+        int per_cu = 0, cus = 0, dev = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fdtdk::k_step_flow<T, R, 32, IX, true>, 256, 0);
+        s->flow_wg = (unsigned)std::max(8, std::max(per_cu, 1) * std::max(cus, 8));
+    }
+    nwg = s->flow_wg;
+    if (ev) (void)hipEventRecord(ev[0], s->stream);
+    if (fm) hipLaunchKernelGGL((fdtdk::k_step_flow<T, R, 32, IX, true>), dim3(nwg), dim3(256), 0, s->stream, a, g, fm);
+    else hipLaunchKernelGGL((fdtdk::k_step_flow<T, R, 32, IX, false>), dim3(nwg), dim3(256), 0, s->stream, a, g, fm);
+    if (ev) (void)hipEventRecord(ev[1], s->stream);
+    if (ev) (void)hipEventRecord(ev[2], s->stream);
+    hipLaunchKernelGGL((fdtdk::k_step_flow_edge<T, R, 32, IX>), dim3(nedge), dim3(256), 0, s->stream, a, g);
+    hipLaunchKernelGGL(fdtdk::k_flow_advance, dim3(1), dim3(64), 0, s->stream, s->d_flowctl);
+    if (ev) (void)hipEventRecord(ev[3], s->stream);
+}
+
 void set_cur(fdtd_sim* s, int c) {
     s->cur = c;
     for (int q = 0; q < 6; ++q) s->F[q] = s->F2[c][q];
@@ -747,6 +836,11 @@ int launch_step(fdtd_sim* s, long n, int sidx, hipEvent_t* ev) {
         set_cur(s, o);
         a = make_args<T, R>(s);
         a.srcv = s->srcv_cur + (long)sidx * s->nsrc;
+    } else if (s->mode == 2 && s->d_flowctl) {
+        // after a monitor step: the H tiles also take its field maxima
+        unsigned long long* fm = s->mon_defer ? s->d_fmax : nullptr;
+        if (s->off32 && !s->force64) launch_flow_ix<T, R, unsigned>(s, a, fm, ev);
+        else launch_flow_ix<T, R, long>(s, a, fm, ev);
     } else {
         const dim3 blk(64 * fdtdk::TILE_ROWS);
         // narrow wavefronts pay off where instruction issue, not HBM, bounds
@@ -1320,7 +1414,16 @@ int fdtd_get_fp16_scales(fdtd_handle h, double* scales18) {
 }
 
 int fdtd_set_kernel_mode(fdtd_handle h, int mode, int param) {
-    if (!h || mode < 0 || mode > 1) return fail(FDTD_EINVAL, "mode must be 0 (H and E kernels) or 1 (fused step kernel)");
+    if (!h || mode < 0 || mode > 2)
+        return fail(FDTD_EINVAL, "mode must be 0 (H and E kernels), 1 (fused step kernel) or 2 (dependency-ordered step)");
+    if (mode == 2) {
+        if (param < 0 || param > 64) return fail(FDTD_EINVAL, "mode 2: param is the plane lag of the E tiles, 0..64 (0: 2)");
+        if (int rc = step_finish(h)) return rc;
+        if (int rc = ensure_flow(h)) return rc;
+        h->flow_lag = param;
+        h->mode = 2;
+        return FDTD_OK;
+    }
     auto width_ok = [](int w) { return w == 16 || w == 32 || w == 64; };
     if (mode == 0) {
         h->force64 = (param & 0x20000) != 0;  // 64-bit element offsets even where 32-bit ones fit
@@ -1402,6 +1505,25 @@ static int scratch_address(int dev, unsigned char** out) {
 static int step_finish(fdtd_handle h) {
     if (!h->pending) return FDTD_OK;
     h->pending = false;
+    if (h->d_flowctl) {
+        // the dependency-ordered step's counter (read after every call of a
+        // handle that has used that step): a tile the launch did not run, or
+        // a wait that timed out, means the call's results cannot be trusted
+        unsigned c = 0;
+        HIPCHK(hipMemcpyAsync(&c, h->d_flowctl + 1, sizeof(c), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        if (c) {
+            const unsigned zero = 0;
+            HIPCHK(hipMemcpy(h->d_flowctl + 1, &zero, sizeof(zero), hipMemcpyHostToDevice));
+            h->pending_mon.clear();
+            h->mode = 0;
+            return fail(FDTD_EDEVICE, "dependency-ordered step: " + std::to_string(c >> 16) +
+                                          " tiles were not run (work-groups not dealt round-robin to the XCDs) and " +
+                                          std::to_string(c & 0xffff) +
+                                          " dependency waits timed out; this call's results are invalid "
+                                          "(the handle is back on the two-kernel step)");
+        }
+    }
     const std::vector<long> mon_steps = std::move(h->pending_mon);
     h->pending_mon.clear();
     h->checks.clear();
@@ -1530,7 +1652,7 @@ static int step_graph_launch(fdtd_handle h, long n0, int nsteps, const std::vect
                              h->prec == FDTD_FP16 ? n0 % h->fp16_interval : -1,
                              h->mon_interval,
                              h->prec == FDTD_FP16 ? h->fp16_interval : 0,
-                             h->kwid_h, h->kwid_e, h->force64, h->erev, h->mon_fused_h, (long)h->band_tiles, h->cur,
+                             h->kwid_h, h->kwid_e, h->force64, h->erev, h->mon_fused_h, (long)h->band_tiles, h->cur, h->mode, h->flow_lag,
                              (long)h->probes.size(), (long)h->dfts.size(), h->nsrc,
                              (long)(uintptr_t)h->srcv_cur, (long)(uintptr_t)h->mon_cur, (long)(uintptr_t)h->d_tab,
                              (long)(uintptr_t)h->F2[h->cur ^ 1][0], (long)(uintptr_t)h->d_rec, h->rec_host};

@@ -83,6 +83,24 @@ template <typename T, typename R, typename IX>
 __device__ __forceinline__ void stx(T* p, IX i, R v, R inv_scale) {
     st<T, R>(at(p, i), 0L, v, inv_scale);
 }
+// ldx through a load that does not hit the CU's L1 (an agent-scope relaxed
+// atomic load), for a value another work-group stored earlier in the same
+// launch: a line this CU cached before that store would otherwise be read
+// stale (in an ordinary dispatch an L1 invalidate at work-group scope does
+// not reach those lines).  COH = false is ldx.
+template <bool COH, typename T, typename R, typename IX>
+__device__ __forceinline__ R ldxh(const T* p, IX i, R scale) {
+    if constexpr (!COH) {
+        return ldx<T, R>(p, i, scale);
+    } else if constexpr (sizeof(T) == 2) {
+        const unsigned short b = __hip_atomic_load(reinterpret_cast<const unsigned short*>(at(p, i)), __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+        return (R)__half2float(__ushort_as_half(b)) * scale;
+    } else {
+        (void)scale;
+        return (R)__hip_atomic_load(at(p, i), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
 
 // The compiler cannot see through this (no instruction is emitted): a value
 // that passes through it is not a constant, so operations on a variable that
@@ -712,7 +730,7 @@ __device__ __forceinline__ void e_sources(const Tabs<R>& tb, int nsrc, int i, in
 // of its lanes is in the slab.
 // Returns H^{n+1/2} at the cell and at its -x, -y, -z neighbours in h[9]:
 // hx hy hz hx_j hz_j hx_k hy_k hy_i hz_i.
-template <typename T, typename R, int KW = 64, bool CP = true, bool GN = true, typename IX = long>
+template <typename T, typename R, int KW = 64, bool CP = true, bool GN = true, typename IX = long, bool COH = false>
 __device__ __forceinline__ EIn<R> e_gather(const StepArgs<T, R>& a, const Tabs<R>& tb, int i, int j, int k, int kw,
                                            int jw, R* h) {
     constexpr int RW = 64 / KW;  // rows per wavefront
@@ -737,9 +755,10 @@ __device__ __forceinline__ EIn<R> e_gather(const StepArgs<T, R>& a, const Tabs<R
     v.ez0 = ldx<T, R>(a.Ez, id, a.sc[2]);
     // neighbours unscaled (FP16; scale 1 otherwise), applied in e_compute
     const R n1 = 1;
-    h[0] = ldx<T, R>(a.Hx, id, a.sc[3]); h[3] = ldx<T, R>(a.Hx, id - sJ, n1); h[5] = ldx<T, R>(a.Hx, id - km, n1);
-    h[1] = ldx<T, R>(a.Hy, id, a.sc[4]); h[7] = ldx<T, R>(a.Hy, id - sI, n1); h[6] = ldx<T, R>(a.Hy, id - km, n1);
-    h[2] = ldx<T, R>(a.Hz, id, a.sc[5]); h[8] = ldx<T, R>(a.Hz, id - sI, n1); h[4] = ldx<T, R>(a.Hz, id - sJ, n1);
+    // (COH: H^{n+1/2} stored by other work-groups of the same launch)
+    h[0] = ldxh<COH, T, R>(a.Hx, id, a.sc[3]); h[3] = ldxh<COH, T, R>(a.Hx, id - sJ, n1); h[5] = ldxh<COH, T, R>(a.Hx, id - km, n1);
+    h[1] = ldxh<COH, T, R>(a.Hy, id, a.sc[4]); h[7] = ldxh<COH, T, R>(a.Hy, id - sI, n1); h[6] = ldxh<COH, T, R>(a.Hy, id - km, n1);
+    h[2] = ldxh<COH, T, R>(a.Hz, id, a.sc[5]); h[8] = ldxh<COH, T, R>(a.Hz, id - sI, n1); h[4] = ldxh<COH, T, R>(a.Hz, id - sJ, n1);
     for (int q = 0; q < 3; ++q) v.hns[q] = sizeof(T) == 2 ? (R)a.sc[3 + q] : (R)1;
     v.ex0 = ldx<T, R>(a.Ex, id, a.sc[0]);
     v.ey0 = ldx<T, R>(a.Ey, id, a.sc[1]);
@@ -915,11 +934,12 @@ __device__ __forceinline__ H3<R> e_compute(const StepArgs<T, R>& a, const Tabs<R
 }
 
 // E update of one cell (i, kw and jw uniform across the wavefront, as for
-// e_gather).  CP and GN compile the CPML and gain code in (the default) or out.
-template <typename T, typename R, int KW = 64, bool CP = true, bool GN = true, typename IX = long>
+// e_gather).  CP and GN compile the CPML and gain code in (the default) or out;
+// COH reads H through loads that bypass the CU's L1 (e_gather).
+template <typename T, typename R, int KW = 64, bool CP = true, bool GN = true, typename IX = long, bool COH = false>
 __device__ __forceinline__ void e_cell(const StepArgs<T, R>& a, const Tabs<R>& tb, int i, int j, int k, int kw, int jw) {
     R h[9];
-    const EIn<R> v = e_gather<T, R, KW, CP, GN, IX>(a, tb, i, j, k, kw, jw, h);
+    const EIn<R> v = e_gather<T, R, KW, CP, GN, IX, COH>(a, tb, i, j, k, kw, jw, h);
     e_compute<T, R, CP, IX>(a, tb, i, j, k, v, h[0], h[1], h[2], h[3], h[4], h[5], h[6], h[7], h[8]);
 }
 

@@ -291,7 +291,8 @@ class Scene:
 
     def set_kernel_mode(self, mode: int, param: int = 0):
         """0: separate H/E kernels, `param` = wavefront k-width (16, 32, 64;
-        0 keeps it); 1: fused step kernel, `param` = x-planes per work-group."""
+        0 keeps it); 1: fused step kernel, `param` = x-planes per work-group;
+        2: dependency-ordered step, `param` = its E tiles' plane lag (0: 2)."""
         _check(self.lib.fdtd_set_kernel_mode(self.h, mode, param))
 
     def synchronize(self):

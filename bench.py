@@ -188,6 +188,16 @@ def plain(args):
     if args.mode == 1:
         kname, kbytes, kt = "k_step_march", step_bytes, t_h
         kernels = {kname: round(ms_h / n_timed, 4)}
+    elif args.mode == 2:
+        # the dependency-ordered step: one launch updates H everywhere and E
+        # on every plane but the first of each of the 8 plane queues, whose E
+        # tiles run in k_step_flow_edge after it; its algorithmic bytes are
+        # the step's minimum on the planes it steps fully, H's on the others
+        nx = setup.shape[0]
+        ne = min(8, nx)
+        kname, kt = "k_step_flow", t_h
+        kbytes = int(step_bytes * (nx - ne) / nx + h_bytes * ne / nx)
+        kernels = {"k_step_flow": round(ms_h / n_timed, 4), "k_step_flow_edge": round(ms_e / n_timed, 4)}
     else:
         kernels = {"k_update_H": round(ms_h / n_timed, 4), "k_update_E": round(ms_e / n_timed, 4)}
         if t_e >= t_h:
@@ -218,7 +228,7 @@ def plain(args):
         "config": {"workload": spec.name, "grid": list(setup.shape), "cells": N, "npml": setup.npml,
                    "precision": args.precision, "gain_cells": gain_cells, "materials": info["n_materials"],
                    "mat_id_bytes": info["mat_id_bytes"], "setup_s": round(t_setup, 2),
-                   "kernel_mode": ["h_e_kernels", "fused_step"][args.mode], "fused_param": args.ichunk,
+                   "kernel_mode": ["h_e_kernels", "fused_step", "flow_step"][args.mode], "fused_param": args.ichunk,
                    "wavefront_kwidth": {"H": args.kwidth or 32, "E": args.kwidth_e or args.kwidth or 32}
                    if args.mode == 0 else None},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
@@ -440,7 +450,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=80, help="CPU baseline: timed FP64 steps, all threads")
     ap.add_argument("--cpu-steps-1t", type=int, default=12, help="CPU baseline: timed FP64 steps on one thread")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--mode", type=int, default=0, help="0: separate H/E kernels, 1: fused step kernel")
+    ap.add_argument("--mode", type=int, default=0,
+                    help="0: separate H/E kernels, 1: fused step kernel, 2: dependency-ordered step "
+                         "(--ichunk: its E-tile plane lag, 0 = 2)")
     ap.add_argument("--ichunk", type=int, default=0,
                     help="fused kernel parameter (planes per work-group + 4096 * cells per lane + 65536 * rows); "
                          "0 = library default")

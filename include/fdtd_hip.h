@@ -206,7 +206,14 @@ int fdtd_get_fp16_scales(fdtd_handle h, double* scales18);
  * taken as 4), each 0 for the library's choice (8 planes, 2 cells, 2 rows).
  * Returns FDTD_EINVAL (and keeps the schedule) for a grid the fused kernel
  * does not take: per-cell H coefficients, an array of 4 GiB or more, or a
- * row that does not fit 8 wavefronts of that many cells.  Results are
+ * row that does not fit 8 wavefronts of that many cells;
+ * mode 2 = the dependency-ordered step: one launch runs every H tile and
+ * every E tile but those of the first plane of each of 8 plane queues (one
+ * per XCD), each E tile after the four H tiles it reads and overwrites for
+ * have stored their flags; a small launch then updates E on those 8 planes.
+ * `param` = the plane lag of a queue's E tiles behind its H tiles (1..64;
+ * 0 for 2).  A call in which some tile did not run or a wait timed out
+ * returns FDTD_EDEVICE and the handle returns to mode 0.  Results are
  * identical in every schedule; only instruction count and memory traffic
  * differ.  No reference counterpart. */
 int fdtd_set_kernel_mode(fdtd_handle h, int mode, int param);

@@ -109,14 +109,15 @@ def test_monitor_maxima_in_next_h_launch_equal_separate_pass(pkg, precision):
     are taken by the next step's H launch; the records (every field,
     including the population sums) equal bit for bit those of the separate
     pass over the six fields (kernel-mode bit 262144) and those of the fused
-    step schedule (which keeps the separate pass), with a monitor step also
+    step schedule (which keeps the separate pass) and of the dependency-
+    ordered step (whose H tiles take the maxima), with a monitor step also
     as the last step of a call (where the separate pass always runs) and with
     the monitor every step."""
     sp = _case(pkg, 1)
     setup = pkg.Setup(sp)
     _say(f"[case] {sp.name} {precision} monitor maxima in the H launch")
     recs = {}
-    for tag, mode, bit in (("h_launch", 0, 0), ("separate", 0, 262144), ("fused", 1, 0)):
+    for tag, mode, bit in (("h_launch", 0, 0), ("separate", 0, 262144), ("fused", 1, 0), ("flow", 2, 0)):
         for interval in (10, 1):
             sc = pkg.Scene(setup, precision)
             sc.set_kernel_mode(mode, bit)
@@ -132,4 +133,6 @@ def test_monitor_maxima_in_next_h_launch_equal_separate_pass(pkg, precision):
         a, b, c = recs[("h_launch", interval)], recs[("separate", interval)], recs[("fused", interval)]
         assert len(a) == (4 if interval == 10 else 40) and a == b, (interval, a[:3], b[:3])
         assert a == c, (interval, a[:3], c[:3])  # the fused step keeps the separate pass
+        d = recs[("flow", interval)]  # the dependency-ordered step's H tiles take the maxima
+        assert a == d, (interval, a[:3], d[:3])
         assert a[-1][1] > 0 and a[-1][2] > 0 and a[-1][8] > 0

@@ -136,14 +136,15 @@ def test_step_kernels_agree_bitwise(pkg, precision):
     element offsets: 0x20000 + width selects the latter; with the E kernel
     walking its tiles backwards, the default, or forwards: 0x100000) and the
     fused step kernel run the same operations per value, so their results
-    are identical in every storage format."""
+    are identical in every storage format; so does the dependency-ordered
+    step (mode 2, at E-tile lags 2, 1 and 3; 44 planes: queues of 6 and 5)."""
     sp = pkg.scene.gain_slab_case(n_total=44, npml=6, steps=120, fine_len=100e-9, coarse=20e-9, fine=5e-9,
                                   n_cuboids=4, seed=5)
     setup = pkg.Setup(sp)
     runs = {}
     for mode in [(0, 16), (0, 32), (0, 64), (0, 0x20000 + 32), (0, 0x20000 + 16 + 256 * 64), (1, 16), (1, 5),
                  (0, 0x100000 + 32), (0, 0x100000 + 0x20000 + 64), (1, 4096 * 1 + 65536 * 2 + 3),
-                 (1, 4096 * 3 + 65536 * 1 + 9), (1, 4096 * 4 + 7)]:
+                 (1, 4096 * 3 + 65536 * 1 + 9), (1, 4096 * 4 + 7), (2, 0), (2, 1), (2, 3)]:
         _say(f"[case] {sp.name} {precision} mode={mode}")
         sc = pkg.Scene(setup, precision)
         sc.set_kernel_mode(*mode)
@@ -172,7 +173,8 @@ def test_step_kernels_agree_bitwise_ragged(pkg, precision):
     # (dead lanes past NzT in the last chunk), and 1..4 rows per block (7 is
     # taken as 4)
     for mode in [(0, 32), (0, 16), (0, 64), (0, 0x100000 + 32), (1, 3), (1, 20), (1, 4096 * 1 + 65536 * 2 + 4),
-                 (1, 4096 * 2 + 65536 * 7 + 6), (1, 4096 * 3 + 65536 * 1 + 20), (1, 4096 * 4 + 65536 * 3 + 1)]:
+                 (1, 4096 * 2 + 65536 * 7 + 6), (1, 4096 * 3 + 65536 * 1 + 20), (1, 4096 * 4 + 65536 * 3 + 1),
+                 (2, 0), (2, 1), (2, 5)]:
         _say(f"[case] ragged {precision} mode={mode}")
         sc = pkg.Scene(setup, precision)
         sc.set_kernel_mode(*mode)
@@ -186,10 +188,38 @@ def test_step_kernels_agree_bitwise_ragged(pkg, precision):
         assert not diff, f"mode {mode} differs in {diff}"
 
 
+@pytest.mark.parametrize("shape", [(6, 24, 40), (9, 20, 70), (17, 33, 65)], ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "fp16"])
+def test_flow_step_small_and_uneven_queues(pkg, precision, shape):
+    """The dependency-ordered step (mode 2) splits the planes into eight
+    queues: fewer planes than queues (6: every queue one plane, all E tiles in
+    the edge launch), a remainder of one (9) and one queue short (17), each
+    with a partial last band of rows and a partial last k-tile; bitwise the
+    two-kernel step, over calls that end between monitor steps."""
+    sp = pkg.scene.gain_slab_case(shape=shape, npml=2, steps=45, fine_len=60e-9, coarse=20e-9, fine=5e-9,
+                                  n_cuboids=2, seed=11)
+    setup = pkg.Setup(sp)
+    runs = {}
+    for mode in [(0, 32), (2, 0), (2, 2)]:
+        _say(f"[case] flow {shape} {precision} mode={mode}")
+        sc = pkg.Scene(setup, precision)
+        sc.set_kernel_mode(*mode)
+        for n in (20, 13, 12):
+            sc.step(n)
+        runs[mode] = {n: sc.field(n) for n in FIELDS + ("Nu", "Pz") + tuple(pkg.runtime.PSI_NAMES)}
+        runs[mode]["checks"] = [(r.step, r.max_E, r.max_H) for r in sc.checks()]
+        sc.close()
+    base = runs[(0, 32)]
+    assert np.abs(base["Ex"]).max() > 0
+    for mode, got in runs.items():
+        diff = [n for n in base if not (np.array_equal(base[n], got[n]) if n != "checks" else base[n] == got[n])]
+        assert not diff, f"mode {mode} differs in {diff}"
+
+
 GOLDEN = __import__("os").path.join(__import__("os").path.dirname(__file__), "golden")
 
 
-@pytest.mark.parametrize("mode", [(0, 16), (0, 64), (1, 8)], ids=lambda m: f"mode{m[0]}_c{m[1]}")
+@pytest.mark.parametrize("mode", [(0, 16), (0, 64), (1, 8), (2, 0)], ids=lambda m: f"mode{m[0]}_c{m[1]}")
 @pytest.mark.parametrize("name", ["uniform_sphere", "gain_slab", "gain_clamp_dipole", "plane_waves", "pec_cavity",
                                   "graded_cuboids", "auto_mesh_cylinder"])
 def test_fp64_matches_reference_golden(pkg, name, mode):

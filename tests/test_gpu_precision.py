@@ -77,11 +77,11 @@ def test_c3_error_against_fp64(c3_runs, prec, bar_probe, bar_spec):
 @pytest.mark.parametrize("precision", ["fp32", "fp16"])
 def test_c3_schedules_agree_bitwise(pkg, precision):
     """At full size, every schedule (two-kernel step at several wavefront
-    shapes and tile orders, and the fused step kernel) gives the same bits as
-    the default."""
+    shapes and tile orders, the fused step kernel and the dependency-ordered
+    step) gives the same bits as the default."""
     spec = pkg.scene.precision_case(steps=30)
     setup = pkg.Setup(spec)
-    modes = [(0, 32), (0, 64), (0, 0x100000 + 32), (1, 0)]
+    modes = [(0, 32), (0, 64), (0, 0x100000 + 32), (1, 0), (2, 0)]
     scenes = []
     for mode in modes:
         _say(f"[c3 schedules] {precision} mode={mode}")

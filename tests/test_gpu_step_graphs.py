@@ -33,12 +33,16 @@ def _state(sc, setup):
     return st
 
 
+@pytest.mark.parametrize("schedule", [0, 2])
 @pytest.mark.parametrize("precision", ["fp32", "fp16", "fp64"])
 @pytest.mark.parametrize("kind", [1, 2])
-def test_step_graphs_equal_direct_launches(pkg, precision, kind):
+def test_step_graphs_equal_direct_launches(pkg, precision, kind, schedule):
+    """schedule 2 (the dependency-ordered step) keeps its step epoch in device
+    memory, so every replay of a recorded call waits on its own steps' flags."""
     sp = pkg.scene.serve_case(5 + kind, kind=kind, seed=3, lo=36, hi=52, steps=sum(CALLS))
     setup = pkg.Setup(sp)
-    _say(f"[step graphs] kind={kind} {precision} shape={setup.shape} probes={setup.n_channels} dfts={setup.n_dfts}")
+    _say(f"[step graphs] kind={kind} {precision} schedule={schedule} shape={setup.shape} probes={setup.n_channels} "
+         f"dfts={setup.n_dfts}")
     direct = pkg.Scene(setup, precision)
     for k in CALLS:
         direct.step(k)
@@ -47,6 +51,7 @@ def test_step_graphs_equal_direct_launches(pkg, precision, kind):
     direct.close()
     for mode in ("sync", "async"):
         sc = pkg.Scene(setup, precision)
+        sc.set_kernel_mode(schedule, 0)
         sc.set_step_graphs(True)
         for k in CALLS:
             if mode == "sync":
