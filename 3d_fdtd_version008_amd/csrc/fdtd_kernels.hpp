@@ -1480,11 +1480,38 @@ __device__ __forceinline__ int batch_domain(const int* __restrict__ cum, int ndo
 // divisors of each domain's tiling in its table entry (tk: k-tiles per row
 // group, tj: row groups per plane group).  IX is the cell-offset type
 // (unsigned where every member's arrays are below 4 GiB).
+// One tile of a batch pass: member d's tile t of the pass's tile list (its
+// tiles k-block fastest, then rows, then groups of `planes` planes).
+template <typename T, typename R, bool EPASS, int KW, typename IX>
+__device__ __forceinline__ void batch_tile(const StepArgs<T, R>* __restrict__ doms, const char* anchor,
+                                           const int* __restrict__ cum, int d, int t, int lstep, int planes, int ty) {
+    constexpr int G = 64 / KW;  // rows per wavefront
+    StepArgs<T, R> a = doms[d];
+    shift_args(a, anchor, 1);
+    a.srcv += (long)lstep * a.nsrc;  // this step's source values
+    FDTD_SCALES_IN_REGISTERS(a)
+    const Tabs<R> tb = global_tabs(a);
+    const unsigned lt = (unsigned)(t - cum[d]);
+    const unsigned r = fdiv(lt, a.tk), ig = fdiv(r, a.tj);
+    const int kt = (int)(lt - r * a.tk.d), jt = (int)(r - ig * a.tj.d);
+    const int kw = kt * KW, jw = (jt * (int)blockDim.y + ty) * G;
+    const int k = kw + (int)threadIdx.x % KW, j = jw + (int)threadIdx.x / KW;
+    const int i0 = (int)ig * planes, i1 = min(i0 + planes, a.nx);
+    for (int i = i0; i < i1; ++i) {
+        if (!EPASS) {
+            if (k < a.nz - 1 && j < a.ny - 1 && i < a.nx - 1)
+                (void)h_compute<T, R, true, IX>(a, tb, i, j, k, h_gather<T, R, KW, true, IX>(a, tb, i, j, k, kw, jw),
+                                                true);
+        } else if (k < a.nz && j < a.ny) {
+            e_cell<T, R, KW, true, true, IX>(a, tb, i, j, k, kw, jw);
+        }
+    }
+}
+
 template <typename T, typename R, bool EPASS, int KW, typename IX>
 __global__ void __launch_bounds__(256) k_batch_step(const StepArgs<T, R>* __restrict__ doms, const char* anchor,
                                                     const int* __restrict__ cum, const BatchDom* __restrict__ bd,
                                                     int ndom, int chunk, int planes, BatchCtl* ctl) {
-    constexpr int G = 64 / KW;  // rows per wavefront
     __shared__ int first_s;
     const int ty = __builtin_amdgcn_readfirstlane(threadIdx.y);
     const int total = cum[ndom];
@@ -1510,28 +1537,28 @@ __global__ void __launch_bounds__(256) k_batch_step(const StepArgs<T, R>* __rest
         for (int t = first; t < last; ++t) {
             while (t >= cum[d + 1]) ++d;
             if (lstep >= bd[d].steps) continue;
-            StepArgs<T, R> a = doms[d];
-            shift_args(a, anchor, 1);
-            a.srcv += (long)lstep * a.nsrc;  // this step's source values
-            FDTD_SCALES_IN_REGISTERS(a)
-            const Tabs<R> tb = global_tabs(a);
-            const unsigned lt = (unsigned)(t - cum[d]);
-            const unsigned r = fdiv(lt, a.tk), ig = fdiv(r, a.tj);
-            const int kt = (int)(lt - r * a.tk.d), jt = (int)(r - ig * a.tj.d);
-            const int kw = kt * KW, jw = (jt * (int)blockDim.y + ty) * G;
-            const int k = kw + (int)threadIdx.x % KW, j = jw + (int)threadIdx.x / KW;
-            const int i0 = (int)ig * planes, i1 = min(i0 + planes, a.nx);
-            for (int i = i0; i < i1; ++i) {
-                if (!EPASS) {
-                    if (k < a.nz - 1 && j < a.ny - 1 && i < a.nx - 1)
-                        (void)h_compute<T, R, true, IX>(a, tb, i, j, k, h_gather<T, R, KW, true, IX>(a, tb, i, j, k, kw, jw),
-                                                        true);
-                } else if (k < a.nz && j < a.ny) {
-                    e_cell<T, R, KW, true, true, IX>(a, tb, i, j, k, kw, jw);
-                }
-            }
+            batch_tile<T, R, EPASS, KW, IX>(doms, anchor, cum, d, t, lstep, planes, ty);
         }
     }
+}
+
+// The same pass with one work-group per tile (tuning chunk 0): no counter and
+// no loop, so a work-group keeps the plain kernels' short life and register
+// footprint.  Ids are dealt to the XCDs round-robin; the remap gives each XCD
+// one contiguous eighth of the tile list, as the persistent form's segments
+// do, so neighbouring tiles share an L2.
+template <typename T, typename R, bool EPASS, int KW, typename IX = long>
+__global__ void __launch_bounds__(256) k_batch_tiles(const StepArgs<T, R>* __restrict__ doms, const char* anchor,
+                                                     const int* __restrict__ cum, const BatchDom* __restrict__ bd,
+                                                     int ndom, int planes, const BatchCtl* ctl) {
+    const int total = cum[ndom];
+    const int per = (int)(gridDim.x / BATCH_SEGS);
+    const int t = (int)(blockIdx.x % BATCH_SEGS) * per + (int)(blockIdx.x / BATCH_SEGS);
+    if (t >= total) return;
+    const int d = batch_domain(cum, ndom, t);
+    const int lstep = ctl->lstep;
+    if (lstep >= bd[d].steps) return;
+    batch_tile<T, R, EPASS, KW, IX>(doms, anchor, cum, d, t, lstep, planes, __builtin_amdgcn_readfirstlane(threadIdx.y));
 }
 
 template <typename T, typename R>

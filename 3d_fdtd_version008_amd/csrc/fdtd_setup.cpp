@@ -1020,6 +1020,18 @@ int fdtd_scene_batch_autotune(fdtd_scene_batch* sb, int trial_steps, int* tuning
                         win = {r, w, c, p};
                     }
                 }
+    // one work-group per tile (chunk 0; work-groups per CU unused)
+    for (int r : {1, 4})
+        for (int p : {1, 4, 8}) {
+            const int cand[4] = {r, 4, 0, p};
+            if ((rc = fdtd_batch_set_tuning(sb->b, cand))) return rc;
+            if ((rc = fdtd_scene_batch_step(sb, trial_steps, &ms))) return rc;
+            taken += trial_steps;
+            if (ms < best) {
+                best = ms;
+                win = {r, 4, 0, p};
+            }
+        }
     if ((rc = fdtd_batch_set_tuning(sb->b, win.data()))) return rc;
     {
         std::lock_guard<std::mutex> g(tune_mutex());

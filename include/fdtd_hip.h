@@ -343,8 +343,9 @@ int fdtd_batch_add(fdtd_batch_handle b, fdtd_handle h);
 int fdtd_batch_remove(fdtd_batch_handle b, fdtd_handle h);
 int fdtd_batch_count(fdtd_batch_handle b);
 /* Tiling, tuning[4] = {rows per tile (1, 2 or 4), persistent work-groups per
- * CU (1..16), tiles taken per counter increment (1..64), x-planes per tile
- * (1..16)}; a tile is 64 k-values by rows by planes. */
+ * CU (1..16), tiles taken per counter increment (1..64; 0: no persistent
+ * work-groups, one work-group per tile, the second entry unused), x-planes
+ * per tile (1..16)}; a tile is 64 k-values by rows by planes. */
 int fdtd_batch_set_tuning(fdtd_batch_handle b, const int* tuning);
 int fdtd_batch_get_tuning(fdtd_batch_handle b, int* tuning);
 /* on != 0: capture the launches of an fdtd_batch_step call in a HIP graph and

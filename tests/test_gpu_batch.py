@@ -90,7 +90,10 @@ def test_batch_equals_alone(pkg, precision):
     setups = [pkg.Setup(sp) for sp in specs]
     _say(f"[batch] {precision} domains={[st.shape for st in setups]}")
     ref = _alone(pkg, setups, precision, 80)
-    for tuning, graph in [((4, 4, 4, 1), False), ((1, 2, 1, 3), False), ((2, 8, 16, 8), True)]:
+    # (rows, work-groups per CU, tiles per counter increment, planes); chunk 0:
+    # one work-group per tile instead of persistent work-groups
+    for tuning, graph in [((4, 4, 4, 1), False), ((1, 2, 1, 3), False), ((2, 8, 16, 8), True), ((4, 4, 0, 1), False),
+                          ((1, 4, 0, 3), True)]:
         _say(f"  tuning={tuning} graph={graph}")
         b = pkg.Batch(precision)
         scs = [pkg.Scene(st, precision) for st in setups]
@@ -158,7 +161,8 @@ def test_batch_autotune_keeps_results(pkg):
         b.add(sc)
     tun, taken = b.autotune(trial_steps=2)
     _say(f"  autotune -> rows, wg/cu, chunk, planes = {tun} after {taken} steps")
-    assert taken == 2 * 25 and tun == b.tuning()
+    # a warm-up call, 24 persistent tilings and 6 with one work-group per tile
+    assert taken == 2 * (1 + 24 + 6) and tun == b.tuning()
     b.step(4)
     total = taken + 4
     ref = _alone(pkg, setups, "fp32", total)
