@@ -2070,11 +2070,14 @@ struct fdtd_batch {
 };
 
 namespace {
-// Batch tiles are 64 consecutive k of blockDim.y rows: 32- and 16-wide
-// wavefronts (two and four rows each) fill the short rows of small domains
-// better but measured slower on the 256-domain workload (26.5-27.1 and
-// 20.8-22.5 Gcell/s against 28.3-28.4 at 64; same box, graph blocks).
+// Wavefront k-width of the batch passes.  Persistent work-groups: 64
+// consecutive k of blockDim.y rows; 32- and 16-wide wavefronts (two and four
+// rows each) fill the short rows of small domains better but measured slower
+// on the 256-domain workload (26.5-27.1 and 20.8-22.5 Gcell/s against
+// 28.3-28.4 at 64; same box, graph blocks).  One work-group per tile: 32
+// wide, 3.73 against 3.84 ms per step (same box, 4 rows x 4 planes).
 constexpr int BATCH_KW = 64;
+constexpr int BATCH_KW_TILES = 32;
 }  // namespace
 
 namespace {
@@ -2108,7 +2111,7 @@ int batch_tables(fdtd_batch* b, int nsteps, const long* n0, const int* msteps) {
         args[d].srcv = b->d_srcv + b->srcv_off[d];
         // pointers travel as offsets from the table (the kernel adds it back)
         fdtdk::shift_args(args[d], (const char*)b->d_ctl, -1);
-        const int kw = BATCH_KW, rows = b->rows * (64 / kw);
+        const int kw = b->chunk ? BATCH_KW : BATCH_KW_TILES, rows = b->rows * (64 / kw);
         const unsigned nkt = (unsigned)((s->nz + kw - 1) / kw), njt = (unsigned)((s->ny + rows - 1) / rows);
         args[d].tk = fdtdk::make_fastdiv(nkt);
         args[d].tj = fdtdk::make_fastdiv(njt);
@@ -2170,14 +2173,14 @@ void batch_launch_step(fdtd_batch* b, int s, const long* n0, const int* msteps) 
         // one work-group per tile: the grid is the tile list, padded to eighths
         const unsigned per = (unsigned)((total + fdtdk::BATCH_SEGS - 1) / fdtdk::BATCH_SEGS), grd = per * fdtdk::BATCH_SEGS;
         if (o32) {
-            hipLaunchKernelGGL((fdtdk::k_batch_tiles<T, R, false, BATCH_KW, unsigned>), dim3(grd), blk, 0, b->stream,
+            hipLaunchKernelGGL((fdtdk::k_batch_tiles<T, R, false, BATCH_KW_TILES, unsigned>), dim3(grd), blk, 0, b->stream,
                                args, anchor, b->d_cum, b->d_dom, nd, b->planes, b->d_ctl);
-            hipLaunchKernelGGL((fdtdk::k_batch_tiles<T, R, true, BATCH_KW, unsigned>), dim3(grd), blk, 0, b->stream,
+            hipLaunchKernelGGL((fdtdk::k_batch_tiles<T, R, true, BATCH_KW_TILES, unsigned>), dim3(grd), blk, 0, b->stream,
                                args, anchor, b->d_cum, b->d_dom, nd, b->planes, b->d_ctl);
         } else {
-            hipLaunchKernelGGL((fdtdk::k_batch_tiles<T, R, false, BATCH_KW, long>), dim3(grd), blk, 0, b->stream,
+            hipLaunchKernelGGL((fdtdk::k_batch_tiles<T, R, false, BATCH_KW_TILES, long>), dim3(grd), blk, 0, b->stream,
                                args, anchor, b->d_cum, b->d_dom, nd, b->planes, b->d_ctl);
-            hipLaunchKernelGGL((fdtdk::k_batch_tiles<T, R, true, BATCH_KW, long>), dim3(grd), blk, 0, b->stream,
+            hipLaunchKernelGGL((fdtdk::k_batch_tiles<T, R, true, BATCH_KW_TILES, long>), dim3(grd), blk, 0, b->stream,
                                args, anchor, b->d_cum, b->d_dom, nd, b->planes, b->d_ctl);
         }
     } else if (o32) {
