@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <unordered_map>
@@ -75,6 +76,109 @@ struct Key12Hash {
     }
 };
 
+// Device memory cache (fdtd_set_device_pool).  hipFree waits for the whole
+// device, so a serving loop that frees finished requests while others run
+// stalls every stream at each retirement, and hipMalloc is slow at each
+// admission.  With the cache on, a handle's blocks return here when it is
+// destroyed (after its own stream has drained) and a later allocation of the
+// same device reuses a cached block at most 1/8 larger than it asks for;
+// cached bytes above the cap are freed at once.  Blocks are keyed by
+// (device, size); a handle still reports the bytes it asked for.
+struct DevPool {
+    std::mutex mu;
+    size_t cap = 0;     // bytes the cache may hold (0: off, every free is a hipFree)
+    size_t cached = 0;
+    std::multimap<std::pair<int, size_t>, void*> blocks;
+};
+DevPool& dev_pool() {
+    static DevPool p;
+    return p;
+}
+// Allocates at least b bytes on `dev` (the current device); *got = the block's size.
+hipError_t pool_alloc(int dev, size_t b, void** out, size_t* got) {
+    DevPool& P = dev_pool();
+    {
+        std::lock_guard<std::mutex> g(P.mu);
+        if (P.cap) {
+            auto it = P.blocks.lower_bound({dev, b});
+            if (it != P.blocks.end() && it->first.first == dev && it->first.second <= b + b / 8) {
+                *out = it->second;
+                *got = it->first.second;
+                P.cached -= it->first.second;
+                P.blocks.erase(it);
+                return hipSuccess;
+            }
+        }
+    }
+    *got = b;
+    return hipMalloc(out, b);
+}
+// Pinned host blocks share the cache under negative keys: mapped and
+// coherent (records and result copies the device writes), or plain (upload
+// staging); their sizes are powers of two, so blocks match across handles.
+constexpr int PIN_MAPPED = -1, PIN_PLAIN = -2;
+// Evicts cached blocks (largest key first) until at most `keep` bytes remain.
+void pool_trim(DevPool& P, size_t keep) {
+    while (P.cached > keep && !P.blocks.empty()) {
+        auto it = std::prev(P.blocks.end());
+        if (it->first.first < 0) {
+            (void)hipHostFree(it->second);
+        } else {
+            int cur = 0;
+            (void)hipGetDevice(&cur);
+            if (cur != it->first.first) (void)hipSetDevice(it->first.first);
+            (void)hipFree(it->second);
+            if (cur != it->first.first) (void)hipSetDevice(cur);
+        }
+        P.cached -= it->first.second;
+        P.blocks.erase(it);
+    }
+}
+// Pinned host memory of at least b bytes (kind PIN_MAPPED or PIN_PLAIN); *got = its size.
+hipError_t pool_host_alloc(int kind, size_t b, void** out, size_t* got) {
+    size_t r = 4096;
+    while (r < b) r <<= 1;
+    *got = r;
+    DevPool& P = dev_pool();
+    {
+        std::lock_guard<std::mutex> g(P.mu);
+        if (P.cap) {
+            auto it = P.blocks.find({kind, r});
+            if (it != P.blocks.end()) {
+                *out = it->second;
+                P.cached -= r;
+                P.blocks.erase(it);
+                return hipSuccess;
+            }
+        }
+    }
+    return hipHostMalloc(out, r, kind == PIN_MAPPED ? (hipHostMallocMapped | hipHostMallocCoherent) : hipHostMallocDefault);
+}
+// Returns a pinned block no copy or kernel uses any more.
+void pool_host_free(int kind, void* p, size_t size) {
+    DevPool& P = dev_pool();
+    std::lock_guard<std::mutex> g(P.mu);
+    if (P.cap && size <= P.cap) {
+        P.blocks.insert({{kind, size}, p});
+        P.cached += size;
+        pool_trim(P, P.cap);
+        return;
+    }
+    (void)hipHostFree(p);
+}
+// Returns a block no stream uses any more.
+void pool_free(int dev, void* p, size_t size) {
+    DevPool& P = dev_pool();
+    std::lock_guard<std::mutex> g(P.mu);
+    if (P.cap && size <= P.cap) {
+        P.blocks.insert({{dev, size}, p});
+        P.cached += size;
+        pool_trim(P, P.cap);
+        return;
+    }
+    (void)hipFree(p);
+}
+
 }  // namespace
 
 struct fdtd_sim {
@@ -85,6 +189,16 @@ struct fdtd_sim {
     size_t esz = 8;     // bytes per stored field value
     size_t rsz = 8;     // bytes per compute value
     std::vector<void*> allocs;
+    std::vector<size_t> alloc_sz;  // size of each block in allocs (the cache may hand out a larger one)
+    // stepping calls so far (batch steps included): staged results are valid
+    // while this equals res_gen
+    unsigned long gen = 0, res_gen = ~0ul;
+    // pinned, coherent host copy of the probe records and DFT accumulators
+    // (fdtd_stage_results): h_res / d_res, res_words doubles; where each
+    // probe's and DFT's copy starts
+    double *h_res = nullptr, *d_res = nullptr;
+    size_t res_words = 0;
+    std::vector<size_t> res_probe_off, res_dft_off;
     std::vector<Key12> h_tuples;  // the coefficient table in FP64, by material id
     // steps enqueued by fdtd_step_async whose monitor records are not read yet
     bool pending = false;
@@ -205,17 +319,20 @@ struct fdtd_sim {
     ~fdtd_sim() {
         if (stream) hipStreamSynchronize(stream);
         for (auto& g : graphs) hipGraphExecDestroy(g.exec);
-        if (h_pin) hipHostFree(h_pin);
-        if (h_rec) hipHostFree(h_rec);
-        for (void* p : allocs) hipFree(p);
+        if (h_pin) pool_host_free(PIN_PLAIN, h_pin, pin_words * sizeof(double));
+        if (h_rec) pool_host_free(PIN_MAPPED, h_rec, rec_words * sizeof(unsigned long long));
+        if (h_res) pool_host_free(PIN_MAPPED, h_res, res_words * sizeof(double));
+        for (size_t q = 0; q < allocs.size(); ++q) pool_free(device, allocs[q], alloc_sz[q]);
         if (stream) hipStreamDestroy(stream);
     }
 
     int alloc(void** p, size_t b) {
         if (b == 0) b = 8;
-        hipError_t e = hipMalloc(p, b);
+        size_t got = b;
+        hipError_t e = pool_alloc(device, b, p, &got);
         if (e != hipSuccess) return fail(FDTD_ENOMEM, "hipMalloc(" + std::to_string(b) + " B) failed: " + hipGetErrorString(e));
         allocs.push_back(*p);
+        alloc_sz.push_back(got);
         bytes += b;
         return 0;
     }
@@ -340,18 +457,25 @@ int ids_from_arrays(fdtd_sim* s, const fdtd_material_desc* m, std::vector<Key12>
     return 0;
 }
 
-// Device memory that lives for one setup call.
+// Device memory that lives for one setup call, used by kernels on `stream`.
 struct TempBuf {
+    int dev;
+    hipStream_t stream;
     std::vector<void*> p;
+    std::vector<size_t> sz;
+    TempBuf(int d, hipStream_t st) : dev(d), stream(st) {}
     ~TempBuf() {
-        for (void* q : p) (void)hipFree(q);
+        if (!p.empty()) (void)hipStreamSynchronize(stream);
+        for (size_t q = 0; q < p.size(); ++q) pool_free(dev, p[q], sz[q]);
     }
     template <typename X>
     int get(X** out, size_t n) {
         void* q = nullptr;
-        if (hipMalloc(&q, std::max<size_t>(n * sizeof(X), 8)) != hipSuccess)
+        size_t got = 0;
+        if (pool_alloc(dev, std::max<size_t>(n * sizeof(X), 8), &q, &got) != hipSuccess)
             return fail(FDTD_ENOMEM, "hipMalloc of a bake buffer failed");
         p.push_back(q);
+        sz.push_back(got);
         *out = (X*)q;
         return 0;
     }
@@ -411,7 +535,7 @@ int bake_on_device(fdtd_sim* s, const fdtd_bake_desc* bd, std::vector<Key12>& tu
         double* pr = &props[(size_t)4 * (q + 1)];
         pr[0] = b.eps_r; pr[1] = b.mu_r; pr[2] = b.sigma; pr[3] = b.sigma_m;
     }
-    TempBuf tmp;
+    TempBuf tmp(s->device, s->stream);
     fdtdk::BakeArgs a{};
     a.nx = s->nx; a.ny = s->ny; a.nz = s->nz; a.ns = ns; a.nsh = nsh;
     a.dt = bd->dt; a.eps0 = bd->eps0; a.mu0 = bd->mu0;
@@ -437,7 +561,7 @@ int bake_on_device(fdtd_sim* s, const fdtd_bake_desc* bd, std::vector<Key12>& tu
     std::vector<uint8_t> used((size_t)nsh + 1);
     bool done = false;
     for (int pass = 0; pass < 2 && !done; ++pass) {
-        TempBuf lst;
+        TempBuf lst(s->device, s->stream);
         long long* icell;
         double* itup;
         if ((rc = lst.get(&icell, (size_t)cap)) || (rc = lst.get(&itup, (size_t)cap * 12))) return rc;
@@ -1468,6 +1592,23 @@ int fdtd_set_step_graphs(fdtd_handle h, int on) {
     return FDTD_OK;
 }
 
+int fdtd_set_device_pool(long max_cached_bytes) {
+    if (max_cached_bytes < 0) return fail(FDTD_EINVAL, "negative cache size");
+    DevPool& P = dev_pool();
+    std::lock_guard<std::mutex> g(P.mu);
+    P.cap = (size_t)max_cached_bytes;
+    pool_trim(P, P.cap);
+    return FDTD_OK;
+}
+
+int fdtd_device_pool_bytes(long* cached) {
+    if (!cached) return fail(FDTD_EINVAL, "null argument");
+    DevPool& P = dev_pool();
+    std::lock_guard<std::mutex> g(P.mu);
+    *cached = (long)P.cached;
+    return FDTD_OK;
+}
+
 int fdtd_set_monitor(fdtd_handle h, int interval) {
     if (!h || interval < 0) return fail(FDTD_EINVAL, "bad monitor interval");
     h->mon_interval = interval;
@@ -1580,15 +1721,18 @@ constexpr int REC_FLAG_WORDS = 1;  // the FP16 non-finite flags (two 32-bit word
 static int ensure_host_records(fdtd_handle h, size_t nrec) {
     const size_t words = nrec * (sizeof(MonOut) / sizeof(unsigned long long)) + REC_FLAG_WORDS;
     if (words <= h->rec_words) return FDTD_OK;
-    if (h->h_rec) HIPCHK(hipHostFree(h->h_rec));
+    if (h->h_rec) {
+        HIPCHK(hipStreamSynchronize(h->stream));
+        pool_host_free(PIN_MAPPED, h->h_rec, h->rec_words * sizeof(unsigned long long));
+    }
     h->h_rec = h->d_rec = nullptr;
     h->rec_words = 0;
     const size_t cap = std::max<size_t>(words, 64 * (sizeof(MonOut) / sizeof(unsigned long long)) + REC_FLAG_WORDS);
-    if (hipHostMalloc((void**)&h->h_rec, cap * sizeof(unsigned long long), hipHostMallocMapped | hipHostMallocCoherent) !=
-        hipSuccess)
+    size_t got = 0;
+    if (pool_host_alloc(PIN_MAPPED, cap * sizeof(unsigned long long), (void**)&h->h_rec, &got) != hipSuccess)
         return fail(FDTD_ENOMEM, "hipHostMalloc of the monitor record copy failed");
     HIPCHK(hipHostGetDevicePointer((void**)&h->d_rec, h->h_rec, 0));
-    h->rec_words = cap;
+    h->rec_words = got / sizeof(unsigned long long);
     return FDTD_OK;
 }
 // The copy kernels at the end of a call: records, then the FP16 flags.
@@ -1702,6 +1846,7 @@ static int step_graph_launch(fdtd_handle h, long n0, int nsteps, const std::vect
 // plus one at each end of the call.
 static int step_launch(fdtd_handle h, long n0, int nsteps, const double* src_values, std::vector<hipEvent_t>* ev,
                        int every = 1, bool noalloc = false) {
+    if (h) ++h->gen;
     if (!h || nsteps < 0 || n0 < 0) return fail(FDTD_EINVAL, "bad step arguments");
     if (h->batch) return fail(FDTD_EINVAL, "domain is a batch member: step it through its batch");
     if (int rc = step_finish(h)) return rc;  // the monitor buffer is reused
@@ -1763,13 +1908,17 @@ static int step_launch(fdtd_handle h, long n0, int nsteps, const double* src_val
     const double* up = src_values;
     if (!noalloc && need + tab_words > 0) {
         if ((size_t)(need + tab_words) > h->pin_words) {
-            if (h->h_pin) HIPCHK(hipHostFree(h->h_pin));
+            if (h->h_pin) {
+                HIPCHK(hipStreamSynchronize(h->stream));  // an earlier upload may still read it
+                pool_host_free(PIN_PLAIN, h->h_pin, h->pin_words * sizeof(double));
+            }
             h->h_pin = nullptr;
             h->pin_words = 0;
             const size_t words = std::max<size_t>((size_t)(need + tab_words), 4096);
-            if (hipHostMalloc((void**)&h->h_pin, words * sizeof(double), hipHostMallocDefault) != hipSuccess)
+            size_t got = 0;
+            if (pool_host_alloc(PIN_PLAIN, words * sizeof(double), (void**)&h->h_pin, &got) != hipSuccess)
                 return fail(FDTD_ENOMEM, "hipHostMalloc of the upload staging failed");
-            h->pin_words = words;
+            h->pin_words = got / sizeof(double);
         }
         if (need > 0) std::memcpy(h->h_pin, src_values, (size_t)need * sizeof(double));
         up = h->h_pin;
@@ -1960,12 +2109,63 @@ int fdtd_upload(fdtd_handle h, int which, const double* in) {
     return fail(FDTD_EINVAL, "unknown array id");
 }
 
+// The copies fdtd_stage_results enqueued are current (no stepping call since).
+static bool staged(fdtd_handle h) {
+    return h->h_res && h->res_gen == h->gen && h->res_probe_off.size() == h->probes.size() &&
+           h->res_dft_off.size() == h->dfts.size();
+}
+
+int fdtd_stage_results(fdtd_handle h) {
+    if (!h) return fail(FDTD_EINVAL, "null handle");
+    if (h->batch) return fail(FDTD_EINVAL, "domain is a batch member");
+    size_t words = 0;
+    h->res_probe_off.clear();
+    h->res_dft_off.clear();
+    for (const Probe& p : h->probes) {
+        h->res_probe_off.push_back(words);
+        words += (size_t)p.count;
+    }
+    for (const Dft& d : h->dfts) {
+        h->res_dft_off.push_back(words);
+        words += (size_t)d.ncell * d.nf * 2;
+    }
+    if (words > h->res_words) {
+        if (h->h_res) {
+            HIPCHK(hipStreamSynchronize(h->stream));  // an earlier staging may still write the old copy
+            pool_host_free(PIN_MAPPED, h->h_res, h->res_words * sizeof(double));
+        }
+        h->h_res = h->d_res = nullptr;
+        h->res_words = 0;
+        size_t got = 0;
+        if (pool_host_alloc(PIN_MAPPED, words * sizeof(double), (void**)&h->h_res, &got) != hipSuccess)
+            return fail(FDTD_ENOMEM, "hipHostMalloc of the result copy failed");
+        HIPCHK(hipHostGetDevicePointer((void**)&h->d_res, h->h_res, 0));
+        h->res_words = got / sizeof(double);
+    }
+    auto copy = [&](const double* src, size_t off, long n) {
+        if (n <= 0) return;
+        const unsigned blocks = (unsigned)std::min<long>((n + 255) / 256, 64);
+        hipLaunchKernelGGL(fdtdk::k_copy_f64, dim3(blocks), dim3(256), 0, h->stream, src, h->d_res + off, n);
+    };
+    for (size_t q = 0; q < h->probes.size(); ++q) copy(h->probes[q].d_buf, h->res_probe_off[q], h->probes[q].count);
+    for (size_t m = 0; m < h->dfts.size(); ++m)
+        copy(h->dfts[m].d_acc, h->res_dft_off[m], h->dfts[m].ncell * h->dfts[m].nf * 2);
+    HIPCHK(hipGetLastError());
+    h->res_gen = h->gen;
+    return FDTD_OK;
+}
+
 int fdtd_read_probe(fdtd_handle h, int probe, double* out, long max, long* count) {
     if (!h || probe < 0 || probe >= (int)h->probes.size()) return fail(FDTD_EINVAL, "bad probe id");
     const Probe& p = h->probes[(size_t)probe];
     if (count) *count = p.count;
     const long n = std::min(max, p.count);
     if (n > 0 && out) {
+        if (staged(h)) {  // the handle's own stream, then host memory: no copy queued behind other streams
+            HIPCHK(hipStreamSynchronize(h->stream));
+            std::memcpy(out, h->h_res + h->res_probe_off[(size_t)probe], (size_t)n * sizeof(double));
+            return FDTD_OK;
+        }
         HIPCHK(hipMemcpyAsync(out, p.d_buf, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
         HIPCHK(hipStreamSynchronize(h->stream));
     }
@@ -1975,7 +2175,13 @@ int fdtd_read_probe(fdtd_handle h, int probe, double* out, long max, long* count
 int fdtd_read_dft(fdtd_handle h, int m, double* out) {
     if (!h || m < 0 || m >= (int)h->dfts.size() || !out) return fail(FDTD_EINVAL, "bad DFT id");
     const Dft& d = h->dfts[(size_t)m];
-    HIPCHK(hipMemcpyAsync(out, d.d_acc, (size_t)d.ncell * d.nf * 2 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    const size_t n = (size_t)d.ncell * d.nf * 2;
+    if (staged(h)) {
+        HIPCHK(hipStreamSynchronize(h->stream));
+        std::memcpy(out, h->h_res + h->res_dft_off[(size_t)m], n * sizeof(double));
+        return FDTD_OK;
+    }
+    HIPCHK(hipMemcpyAsync(out, d.d_acc, n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     return FDTD_OK;
 }
@@ -2422,6 +2628,7 @@ int fdtd_batch_step(fdtd_batch_handle b, int nsteps, const long* n0, const int* 
         return FDTD_OK;
     }
     std::vector<int> ms_(b->doms.size(), nsteps);
+    for (fdtd_sim* m : b->doms) ++m->gen;
     for (size_t d = 0; d < b->doms.size(); ++d) {
         if (n0[d] < 0) return fail(FDTD_EINVAL, "negative step index");
         if (member_steps) {

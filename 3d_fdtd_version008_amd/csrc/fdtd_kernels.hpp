@@ -1402,6 +1402,10 @@ __global__ void __launch_bounds__(64) k_monitor_final(const MonOut* __restrict__
 // Copies n 64-bit words (device records into pinned host memory at the end
 // of a call, so the host reads them once the stream is idle instead of
 // queueing a copy behind other streams' work).
+// Copies n doubles (grid-stride), e.g. into mapped host memory.
+__global__ void __launch_bounds__(256) k_copy_f64(const double* __restrict__ src, double* __restrict__ dst, long n) {
+    for (long q = (long)blockIdx.x * 256 + threadIdx.x; q < n; q += (long)gridDim.x * 256) dst[q] = src[q];
+}
 __global__ void __launch_bounds__(64) k_copy_words(const unsigned long long* __restrict__ src,
                                                    unsigned long long* __restrict__ dst, long n) {
     for (long q = threadIdx.x; q < n; q += 64) dst[q] = src[q];

@@ -42,7 +42,8 @@ EXPORTS = [
     "fdtd_scene_steps_done", "fdtd_scene_handle", "fdtd_scene_step_timed", "fdtd_scene_step_async",
     "fdtd_scene_step_wait", "fdtd_scene_step_query", "fdtd_step_async", "fdtd_step_wait", "fdtd_step_query",
     "fdtd_set_fp16_scales",
-    "fdtd_get_fp16_scales", "fdtd_set_kernel_mode",
+    "fdtd_get_fp16_scales", "fdtd_set_kernel_mode", "fdtd_set_device_pool", "fdtd_device_pool_bytes",
+    "fdtd_stage_results",
     "fdtd_batch_create", "fdtd_batch_destroy", "fdtd_batch_add", "fdtd_batch_remove", "fdtd_batch_count",
     "fdtd_batch_set_tuning", "fdtd_batch_get_tuning", "fdtd_batch_set_graph", "fdtd_batch_step",
     "fdtd_scene_batch_create", "fdtd_scene_batch_destroy", "fdtd_scene_batch_add", "fdtd_scene_batch_remove",
@@ -123,6 +124,9 @@ def load() -> C.CDLL:
         "fdtd_step_timed": (i, [vp, l, i, vp, vp]),
         "fdtd_set_fp16_scales": (i, [vp, vp]),
         "fdtd_set_kernel_mode": (i, [vp, i, i]),
+        "fdtd_set_device_pool": (i, [l]),
+        "fdtd_stage_results": (i, [vp]),
+        "fdtd_device_pool_bytes": (i, [C.POINTER(l)]),
         "fdtd_get_fp16_scales": (i, [vp, vp]),
         "fdtd_scene_steps_done": (l, [vp]),
         "fdtd_scene_handle": (vp, [vp]),
@@ -153,6 +157,20 @@ def _check(rc: int) -> int:
     if rc < 0:
         raise FdtdError(rc, load().fdtd_last_error().decode(errors="replace"))
     return rc
+
+
+def set_device_pool(max_cached_bytes: int):
+    """Process-wide device memory cache (fdtd_set_device_pool): destroyed
+    handles' blocks are kept for reuse, up to that many bytes; 0 turns it off
+    and frees what it holds."""
+    _check(load().fdtd_set_device_pool(int(max_cached_bytes)))
+
+
+def device_pool_bytes() -> int:
+    """Bytes the device memory cache holds now."""
+    v = C.c_long(0)
+    _check(load().fdtd_device_pool_bytes(C.byref(v)))
+    return v.value
 
 
 class Setup:
@@ -330,6 +348,12 @@ class Scene:
         if cnt.value:
             _check(self.lib.fdtd_read_tls_monitor(self.h, out.ctypes.data_as(C.c_void_p), cnt.value, C.byref(cnt)))
         return out
+
+    def stage_results(self):
+        """Queue a host copy of the probe records and DFT accumulators after
+        the steps enqueued so far; probe() and dft() read it until the next
+        step (fdtd_stage_results)."""
+        _check(self.lib.fdtd_stage_results(self.h))
 
     def dft(self, m: int) -> np.ndarray:
         d = self.setup.spec.dfts[m]

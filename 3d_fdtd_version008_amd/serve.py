@@ -145,11 +145,15 @@ class Server:
     def __init__(self, budget_bytes: float = 200e9, max_active: int = 256, tick: int = 50, graph: bool = True,
                  setup_workers: int = 8, tuning: Optional[tuple] = None, engine: str = "streams",
                  small_cells: int = 2_000_000, enqueue_workers: int = 4, stream_workers: int = 0,
+                 pool_bytes: int = 24 << 30,
                  defer_free: bool = False, host_bake: bool = False, step_graphs: bool = True):
         if engine not in ("streams", "batch", "hybrid"):
             raise ValueError("engine must be 'streams', 'batch' or 'hybrid'")
         self.engine = engine
         self.small_cells = small_cells
+        # device memory cache while serving: freed requests' blocks go to the
+        # next admissions instead of hipFree (which waits for the whole device)
+        self.pool_bytes = pool_bytes
         self.enqueue_workers = enqueue_workers
         # streams engine: threads that make the per-request device calls
         # (0: the loop thread makes them itself)
@@ -188,9 +192,13 @@ class Server:
     def run(self, requests: List[Request]) -> List[Result]:
         """Replays the arrival times in real time and serves every request;
         returns the results in completion order."""
-        if self.engine == "streams":
-            return self._run_streams(requests)
-        return self._run_ticks(requests)
+        runtime.set_device_pool(self.pool_bytes)
+        try:
+            if self.engine == "streams":
+                return self._run_streams(requests)
+            return self._run_ticks(requests)
+        finally:
+            runtime.set_device_pool(0)
 
     def _run_streams(self, requests: List[Request]) -> List[Result]:
         """Streams engine.  The loop polls the resident requests' streams
@@ -245,6 +253,8 @@ class Server:
                 if self.step_graphs:
                     sc.set_step_graphs(True)
                 sc.step_async(min(self.tick, req.steps))
+                if req.steps <= self.tick:
+                    sc.stage_results()
                 count("ticks", 1)
             except runtime.FdtdError as e:
                 return self._retire(lv, now(), e)
@@ -260,6 +270,10 @@ class Server:
                 if left <= 0:
                     return True
                 lv.scene.step_async(min(self.tick, left))
+                if left <= self.tick:
+                    # the last tick: its results reach host memory with it,
+                    # so retirement queues no copy behind other requests
+                    lv.scene.stage_results()
                 count("ticks", 1)
                 return None
             except runtime.FdtdError as e:

@@ -399,6 +399,7 @@ def serve_form(args):
     reqs = serve.request_stream(args.requests, rate=args.rate, seed=1)
     srv = serve.Server(budget_bytes=200e9, max_active=256, tick=args.tick, graph=True, setup_workers=8,
                        engine=args.engine, stream_workers=args.workers, defer_free=args.defer_free,
+                       pool_bytes=int(args.pool_gb * (1 << 30)),
                        host_bake=args.host_bake, step_graphs=not args.no_step_graphs)
     res = srv.run(reqs)
     summ = serve.summarize(res)
@@ -418,7 +419,7 @@ def serve_form(args):
             "form": "serve", "value": round(summ["gcells_per_s"], 3), "unit": "Gcell-updates/s",
             "latency_p50_s": round(summ["latency_p50_s"], 3), "latency_p99_s": round(summ["latency_p99_s"], 3),
             "config": {"requests": args.requests, "arrival_rate_per_s": args.rate,
-                       "fp16_share": 0.5, "budget_gb": 200, "tick_steps": args.tick, "engine": args.engine,
+                       "fp16_share": 0.5, "budget_gb": 200, "pool_gb": args.pool_gb, "tick_steps": args.tick, "engine": args.engine,
                        "workers": args.workers, "deferred_free": args.defer_free,
                        "step_graphs": not args.no_step_graphs,
                        "bake": "host" if args.host_bake else "device", "makespan_s": round(summ["makespan_s"], 2),
@@ -466,6 +467,8 @@ def main():
     ap.add_argument("--requests", type=int, default=64, help="serve form: requests")
     ap.add_argument("--rate", type=float, default=20.0, help="serve form: arrivals per second")
     ap.add_argument("--engine", default="streams", choices=["streams", "batch", "hybrid"], help="serve form: request engine")
+    ap.add_argument("--pool-gb", type=float, default=24.0,
+                    help="serve form: device memory cache for freed requests' blocks (GiB; 0: hipFree at each retirement)")
     ap.add_argument("--solo-sample", type=int, default=16, help="batch/serve forms: one-at-a-time sample size")
     ap.add_argument("--tick", type=int, default=50, help="serve form: steps per tick")
     ap.add_argument("--no-step-graphs", action="store_true",

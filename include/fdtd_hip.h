@@ -219,6 +219,20 @@ int fdtd_get_fp16_scales(fdtd_handle h, double* scales18);
 int fdtd_set_kernel_mode(fdtd_handle h, int mode, int param);
 /* Stability monitor cadence (STABILITY_MONITOR_INTERVAL, default 10; 0 = off). */
 int fdtd_set_monitor(fdtd_handle h, int interval);
+/* Process-wide cache of device memory: with max_cached_bytes > 0, the
+ * blocks of a destroyed handle (and the device bake's temporary buffers) are
+ * kept, up to that many bytes per process, and a later allocation on the same
+ * device reuses a cached block at most 1/8 larger than it asks for; a
+ * handle's pinned host buffers (upload staging, record and result copies)
+ * are kept the same way, in power-of-two sizes.  So a serving loop that
+ * creates and destroys handles while others run makes neither hipMalloc nor
+ * hipFree (which waits for the whole device), nor the slow pinned host
+ * allocations, on its common path.  0 (the default) turns it off and frees what it holds.  A
+ * handle's fdtd_device_bytes stays the bytes it asked for.  No reference
+ * counterpart (the reference allocates once per run). */
+int fdtd_set_device_pool(long max_cached_bytes);
+/* Bytes the cache holds now. */
+int fdtd_device_pool_bytes(long* cached);
 /* on != 0: each stepping call on the two-kernel schedule records its
  * launches once in a HIP graph and replays it for later calls of the same
  * shape (step count, step phase in the monitor and FP16 cadences, schedule
@@ -294,6 +308,15 @@ int fdtd_read_probe(fdtd_handle h, int probe, double* out, long max, long* count
 /* Accumulators of DFT monitor m, layout [freq][cell][re, im], cell in box
  * order (i, j, k with k fastest). */
 int fdtd_read_dft(fdtd_handle h, int m, double* out);
+/* Enqueues, on the handle's stream after the steps enqueued so far, a copy
+ * of every probe's records and every DFT monitor's accumulators into pinned
+ * host memory; until the next stepping call, fdtd_read_probe and
+ * fdtd_read_dft read that copy (after the handle's own stream) instead of
+ * queueing a device-to-host copy, which waits behind other streams' kernels
+ * on the shared hardware queues.  For a request loop: stage with a
+ * request's last steps, read at retirement.  Not for batch members.  No
+ * reference counterpart. */
+int fdtd_stage_results(fdtd_handle h);
 /* TLS diagnostics as device reductions: every save_every steps (after the
  * population update, where the reference records its TLS detectors,
  * src/3D_FDTD_version008.cpp:330-336) one record of 5 values is kept, up to

@@ -155,3 +155,72 @@ def test_served_requests_match_oracle(pkg, oracle_mod):
         bad = {k: v for k, v in errs.items() if not v <= (bp if k.startswith("probe") else bd)}
         assert not bad, (r.rid, r.precision, bad)
         ref.close()
+
+
+def test_device_pool_reuse_is_bitwise(pkg):
+    """With the device memory cache on, a destroyed scene's blocks (and the
+    device bake's temporary buffers) are handed to the next scenes: results
+    equal those of scenes on fresh allocations, the cache holds the freed
+    bytes, and turning it off frees them."""
+    rt = pkg.runtime
+    serve = importlib.import_module("3d_fdtd_version008_amd.serve")
+    reqs = serve.request_stream(4, rate=0.0, seed=3, lo=32, hi=48, steps=(60, 60))
+    setups = [pkg.Setup(r.spec) for r in reqs]
+
+    def run(st, prec):
+        sc = pkg.Scene(st, prec)
+        sc.step(60)
+        out = {f: sc.field(f) for f in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")}
+        out["probe0"] = sc.probe(0)
+        sc.close()
+        return out
+
+    rt.set_device_pool(0)
+    ref = [run(st, r.precision) for st, r in zip(setups, reqs)]
+    try:
+        rt.set_device_pool(4 << 30)
+        for rep in range(2):  # the second pass runs on blocks the first one freed
+            got = [run(st, r.precision) for st, r in zip(setups, reqs)]
+            for a, b in zip(ref, got):
+                assert all(np.array_equal(a[k], b[k]) for k in a), f"pass {rep}"
+            assert rt.device_pool_bytes() > 0
+    finally:
+        rt.set_device_pool(0)
+    assert rt.device_pool_bytes() == 0
+
+
+def test_staged_results_equal_device_reads(pkg):
+    """fdtd_stage_results: the host copy enqueued after a call's steps equals
+    what the device buffers hold; the next step makes reads go to the device
+    again (and a new staging picks the new records up); a batch member is
+    refused."""
+    serve = importlib.import_module("3d_fdtd_version008_amd.serve")
+    req = serve.request_stream(1, rate=0.0, seed=7, lo=36, hi=44, steps=(90, 90))[0]
+    st = pkg.Setup(req.spec)
+    ref = pkg.Scene(st, req.precision)
+    sc = pkg.Scene(st, req.precision)
+    sc.set_step_graphs(True)
+    ref.step(60)
+    sc.step_async(60)
+    sc.stage_results()
+    sc.step_wait()
+    for c in range(st.n_channels):
+        assert np.array_equal(sc.probe(c), ref.probe(c))
+    for m in range(st.n_dfts):
+        assert np.array_equal(sc.dft(m), ref.dft(m))
+    ref.step(30)
+    sc.step(30)  # the staged copy is stale now: reads go to the device
+    for c in range(st.n_channels):
+        assert len(sc.probe(c)) == len(ref.probe(c)) and np.array_equal(sc.probe(c), ref.probe(c))
+    for m in range(st.n_dfts):
+        assert np.array_equal(sc.dft(m), ref.dft(m))
+    sc.stage_results()
+    for c in range(st.n_channels):
+        assert np.array_equal(sc.probe(c), ref.probe(c))
+    b = pkg.Batch(req.precision)
+    b.add(ref)
+    with pytest.raises(pkg.runtime.FdtdError):
+        ref.stage_results()
+    b.close()
+    ref.close()
+    sc.close()

@@ -46,6 +46,9 @@ class FakeScene:
             FakeScene.live += 1
             FakeScene.peak = max(FakeScene.peak, FakeScene.live)
 
+    def stage_results(self):
+        self.staged = True
+
     def set_step_graphs(self, on=True):
         self.graphs = bool(on)
 
