@@ -167,8 +167,12 @@ class Server:
         self.host_bake = host_bake
         # requests stepped on their own streams replay each tick from a HIP
         # graph (fdtd_set_step_graphs: bitwise the direct launches), so a tick
-        # costs the host one launch instead of a few per step
-        self.step_graphs = step_graphs
+        # costs the host one launch instead of a few per step.  Not with
+        # worker threads: a tick recorded on one thread while another thread
+        # makes device calls failed to record ("operation failed due to a
+        # previous error during capture", 4 workers, 64 requests), so those
+        # requests launch their kernels directly
+        self.step_graphs = step_graphs and stream_workers == 0
         self.budget = int(budget_bytes)
         self.max_active = max_active
         self.tick = tick

@@ -473,8 +473,8 @@ def main():
     ap.add_argument("--tick", type=int, default=50, help="serve form: steps per tick")
     ap.add_argument("--no-step-graphs", action="store_true",
                     help="serve form: requests on streams launch every step directly instead of replaying graphs")
-    ap.add_argument("--workers", type=int, default=0,
-                    help="serve form, streams engine: threads making the per-request device calls (0: the loop thread)")
+    ap.add_argument("--workers", type=int, default=4,
+                    help="serve form, streams engine: threads for the per-request device calls (0: the loop thread; with threads the requests launch directly instead of replaying recorded ticks)")
     ap.add_argument("--defer-free", action="store_true",
                     help="serve form: free finished requests' device memory only when admission needs it")
     ap.add_argument("--host-bake", action="store_true", help="serve form: bake materials on the host during setup")

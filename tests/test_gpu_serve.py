@@ -21,8 +21,10 @@ def _say(msg):
     sys.stdout.flush()
 
 
-@pytest.mark.parametrize("engine", ["streams", "batch", "hybrid"])
+@pytest.mark.parametrize("engine", ["streams", "batch", "hybrid", "streams-workers"])
 def test_served_results_equal_alone(pkg, engine):
+    workers = 3 if engine == "streams-workers" else 0
+    engine = engine.split("-")[0]
     serve = importlib.import_module("3d_fdtd_version008_amd.serve")
     reqs = serve.request_stream(10, rate=40.0, seed=2, lo=32, hi=48, steps=(60, 140))
     for r in reqs:
@@ -32,7 +34,7 @@ def test_served_results_equal_alone(pkg, engine):
     # hybrid: requests below the median size go to the batches
     small = int(np.median([pkg.Setup(r.spec).shape[0] ** 3 for r in reqs]))
     srv = serve.Server(budget_bytes=3.2 * max(est), max_active=8, tick=20, graph=True, setup_workers=4,
-                       engine=engine, small_cells=small)
+                       engine=engine, small_cells=small, stream_workers=workers)
     served = {r.rid: r for r in srv.run(reqs)}
     assert sorted(served) == [r.rid for r in reqs]
     s = serve.summarize(list(served.values()))
