@@ -1601,11 +1601,12 @@ int fdtd_set_device_pool(long max_cached_bytes) {
     return FDTD_OK;
 }
 
-int fdtd_device_pool_bytes(long* cached) {
-    if (!cached) return fail(FDTD_EINVAL, "null argument");
+int fdtd_device_pool_bytes(long* cached, long* max_cached_bytes) {
+    if (!cached && !max_cached_bytes) return fail(FDTD_EINVAL, "null argument");
     DevPool& P = dev_pool();
     std::lock_guard<std::mutex> g(P.mu);
-    *cached = (long)P.cached;
+    if (cached) *cached = (long)P.cached;
+    if (max_cached_bytes) *max_cached_bytes = (long)P.cap;
     return FDTD_OK;
 }
 

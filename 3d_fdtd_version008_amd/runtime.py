@@ -126,7 +126,7 @@ def load() -> C.CDLL:
         "fdtd_set_kernel_mode": (i, [vp, i, i]),
         "fdtd_set_device_pool": (i, [l]),
         "fdtd_stage_results": (i, [vp]),
-        "fdtd_device_pool_bytes": (i, [C.POINTER(l)]),
+        "fdtd_device_pool_bytes": (i, [C.POINTER(l), C.POINTER(l)]),
         "fdtd_get_fp16_scales": (i, [vp, vp]),
         "fdtd_scene_steps_done": (l, [vp]),
         "fdtd_scene_handle": (vp, [vp]),
@@ -169,7 +169,14 @@ def set_device_pool(max_cached_bytes: int):
 def device_pool_bytes() -> int:
     """Bytes the device memory cache holds now."""
     v = C.c_long(0)
-    _check(load().fdtd_device_pool_bytes(C.byref(v)))
+    _check(load().fdtd_device_pool_bytes(C.byref(v), None))
+    return v.value
+
+
+def device_pool_cap() -> int:
+    """The device memory cache's cap (0: off)."""
+    v = C.c_long(0)
+    _check(load().fdtd_device_pool_bytes(None, C.byref(v)))
     return v.value
 
 

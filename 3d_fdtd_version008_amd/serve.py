@@ -196,13 +196,16 @@ class Server:
     def run(self, requests: List[Request]) -> List[Result]:
         """Replays the arrival times in real time and serves every request;
         returns the results in completion order."""
+        # the cache is process-wide: the caller's setting comes back afterwards
+        # (one server at a time per process sets it)
+        before = runtime.device_pool_cap()
         runtime.set_device_pool(self.pool_bytes)
         try:
             if self.engine == "streams":
                 return self._run_streams(requests)
             return self._run_ticks(requests)
         finally:
-            runtime.set_device_pool(0)
+            runtime.set_device_pool(before)
 
     def _run_streams(self, requests: List[Request]) -> List[Result]:
         """Streams engine.  The loop polls the resident requests' streams

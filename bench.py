@@ -421,7 +421,7 @@ def serve_form(args):
             "config": {"requests": args.requests, "arrival_rate_per_s": args.rate,
                        "fp16_share": 0.5, "budget_gb": 200, "pool_gb": args.pool_gb, "tick_steps": args.tick, "engine": args.engine,
                        "workers": args.workers, "deferred_free": args.defer_free,
-                       "step_graphs": not args.no_step_graphs,
+                       "step_graphs": srv.step_graphs,
                        "bake": "host" if args.host_bake else "device", "makespan_s": round(summ["makespan_s"], 2),
                        "cell_updates": summ["cell_updates"], "failed": summ["failed"],
                        "gpu_max_hw_queues": os.environ.get("GPU_MAX_HW_QUEUES", "runtime default")},

@@ -231,8 +231,8 @@ int fdtd_set_monitor(fdtd_handle h, int interval);
  * handle's fdtd_device_bytes stays the bytes it asked for.  No reference
  * counterpart (the reference allocates once per run). */
 int fdtd_set_device_pool(long max_cached_bytes);
-/* Bytes the cache holds now. */
-int fdtd_device_pool_bytes(long* cached);
+/* Bytes the cache holds now, and its cap (either pointer may be null). */
+int fdtd_device_pool_bytes(long* cached, long* max_cached_bytes);
 /* on != 0: each stepping call on the two-kernel schedule records its
  * launches once in a HIP graph and replays it for later calls of the same
  * shape (step count, step phase in the monitor and FP16 cadences, schedule

@@ -63,3 +63,21 @@ def test_no_cpu_fallback_without_device(pkg):
         pkg.Scene(setup, "fp64")
     assert e.value.code == -2
     assert lib is not None
+
+
+def test_device_pool_setting_round_trip(pkg):
+    """The device memory cache's cap is process-wide state: set, read back,
+    refused when negative, and back to off; nothing is cached without blocks."""
+    rt = pkg.runtime
+    before = rt.device_pool_cap()
+    try:
+        rt.set_device_pool(1 << 30)
+        assert rt.device_pool_cap() == 1 << 30
+        assert rt.device_pool_bytes() == 0
+        with pytest.raises(rt.FdtdError):
+            rt.set_device_pool(-1)
+        assert rt.device_pool_cap() == 1 << 30
+        rt.set_device_pool(0)
+        assert rt.device_pool_cap() == 0
+    finally:
+        rt.set_device_pool(before)
